@@ -1,0 +1,193 @@
+"""Benchmark of the daily transmission step on one MI355X.
+
+A step is one simulated day of the plain hybrid population (1M agents; layers
+h/s/w/c with betas 3.0/0.6/0.6/0.3, community resampled daily) run through the
+C-ABI day loop captured as one hipGraph. ``--steps K`` days are timed from a
+fresh state after ``--warmup W`` untimed days (replays of the same graph).
+
+Prints one JSON line: agent-days/s for the whole job, the edge-pass roofline
+(algorithmic bytes per launch over its HIP-event-measured duration, against the
+8 TB/s HBM3E peak) and the CPU baseline (the oracle restatement timed on a bounded
+sample of the same workload on this host).
+"""
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = 'agent-days/sec at 1M hybrid; batched vs plain sims/sec; HBM GB/s vs peak'
+HBM_PEAK_GBS = 8000.0
+
+WORKLOADS = {
+    'hybrid1m': dict(pop_size=1_000_000, pop_type='hybrid', n_days=180, households='categorical',
+                     pars=dict(beta=0.016, use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
+                     desc='hybrid 1M agents, 180 days, households 1..7 (mean 2.6), s Poisson(20) ages 6-22, '
+                          'w Poisson(16) ages 22-65, c Poisson(20) resampled daily; layer betas h3.0 s0.6 w0.6 '
+                          'c0.3; beta 0.016; float32 state'),
+    'random20k': dict(pop_size=20_000, pop_type='random', n_days=60, households='poisson',
+                      pars=dict(beta=0.016, use_waning=True, pop_infected=20),
+                      desc='random 20k agents, one layer a with Poisson(20) contacts, 60 days, beta 0.016'),
+}
+
+
+def build_population(w, seed):
+    from covasim_amd import parameters as cvpar
+    from covasim_amd import population as cvpop
+    pars = cvpar.make_pars(pop_size=w['pop_size'], pop_type=w['pop_type'])
+    pars['prognoses'] = cvpar.get_prognoses()
+    rng = np.random.default_rng(seed)
+    pop = cvpop.make_randpop(pars, rng, households=w['households'])
+    pop.update(cvpop.make_people_arrays(pop['age'], pars, rng))
+    for lay in pop['contacts'].values():
+        lay['beta'] = np.ones(len(lay['p1']), np.float32)
+    return pop
+
+
+def cpu_baseline(w, pop, seed, budget_s):
+    """The oracle (numpy restatement, one thread) on the same population until budget_s elapses."""
+    from oracle import covasim_oracle as co
+    opars = co.make_pars(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=seed,
+                         **w['pars'])
+    keys = ('symp_prob', 'severe_prob', 'crit_prob', 'death_prob', 'rel_sus', 'rel_trans')
+    init = dict(age=pop['age'], **{k: pop[k] for k in keys})
+    layers = {k: (pop['contacts'][k]['p1'], pop['contacts'][k]['p2'], pop['contacts'][k]['beta'])
+              for k in opars['beta_layer']}
+    sim = co.Sim(opars, init, layers, co.PhiloxRNG(seed))
+    t0 = time.perf_counter()
+    days = 0
+    while days < sim.npts and time.perf_counter() - t0 < budget_s:
+        sim.step()
+        days += 1
+    el = time.perf_counter() - t0
+    return dict(value=w['pop_size'] * days / el, unit='agent-days/s', cores=1, kind='port',
+                sample=f'oracle (numpy float32 restatement, PhiloxRNG) days 0..{days - 1} of the same workload and '
+                       f'population, {el:.1f} s on one host thread')
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=None, help='timed days (default: the full run)')
+    ap.add_argument('--warmup', type=int, default=None)
+    ap.add_argument('--workload', default='hybrid1m', choices=sorted(WORKLOADS))
+    ap.add_argument('--seed', type=int, default=1)
+    ap.add_argument('--cpu-budget', type=float, default=15.0)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus above 1 is not supported (the sharded '
+                                   'design is described in DESIGN.md)'}))
+        sys.exit(2)
+
+    import torch
+    import covasim_amd as cv
+    from covasim_amd import _lib
+
+    w = WORKLOADS[args.workload]
+    steps = args.steps or w['n_days']
+    warmup = steps if args.warmup is None else args.warmup
+    n_days = max(w['n_days'], steps)
+    t_build = time.perf_counter()
+    pop = build_population(w, args.seed)
+    n_edges = sum(len(l['p1']) for l in pop['contacts'].values())
+    dyn_edges = sum(len(pop['contacts'][k]['p1']) for k, v in w['pars'].get('dynam_layer', {}).items() if v)
+    sim = cv.Sim(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=n_days, rand_seed=args.seed,
+                 popdict=pop, households=w['households'], **w['pars'])
+    sim.initialize()
+    build_s = time.perf_counter() - t_build
+    L = _lib.lib()
+    stream = torch.cuda.Stream()
+    people = sim.people
+    init_state = {k: v.clone() for k, v in people._d.items()}
+    init_layers = {lk: {c: lay.tensor(c).clone() for c in ('p1', 'p2', 'beta')} for lk, lay in people.contacts.items()}
+
+    def reset():
+        for k, v in init_state.items():
+            people._d[k].copy_(v)
+        for lk, lay in people.contacts.items():
+            for c in ('p1', 'p2', 'beta'):
+                lay.tensor(c).copy_(init_layers[lk][c])
+        sim._w['winner'].fill_(-1)
+        for k in ('touched', 'bitmap', 'ctl', 'results', 'results_f', 'select_hist'):
+            sim._w[k].zero_()
+
+    P = people.struct()
+    with torch.cuda.stream(stream):
+        reset()
+        torch.cuda.synchronize()
+        gexec = C.c_void_p()
+        _lib.check(L.cvk_graph_create(P, sim._layers, sim._params, sim._work, 0, steps,
+                                      int(w['pars']['pop_infected']), sim._imports, C.c_void_p(stream.cuda_stream),
+                                      C.byref(gexec)), 'graph create')
+        for _ in range(max(0, -(-warmup // steps))):  # warmup days, as whole-graph replays
+            reset()
+            _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'graph launch')
+        torch.cuda.synchronize()
+        reset()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'graph launch')
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res = sim._w['results'].cpu().numpy().view(np.uint32)
+        cum_inf = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
+        _lib.check(L.cvk_graph_destroy(gexec), 'graph destroy')
+
+        # Edge-pass duration: the same days again, eager, with HIP events on this stream around each edge pass.
+        reset()
+        torch.cuda.synchronize()
+        edge_ms = []
+        st = C.c_void_p(stream.cuda_stream)
+        pr, wk = sim._params, sim._work
+        for t in range(steps):
+            if t == 0:
+                _lib.check(L.cvk_choose(P, pr, wk, 0, _lib.STREAM_CHOOSE, int(w['pars']['pop_infected']), 0,
+                                        _lib.LAYER_SEED, 0, st), 'seed')
+                _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, 0, 0, st), 'seed')
+            _lib.check(L.cvk_agent_pass(P, pr, wk, t, st), 'agent')
+            for li in range(pr.n_layers):
+                if sim._layers[li].dynamic:
+                    _lib.check(L.cvk_resample_layer(C.byref(sim._layers[li]), li, pr, t, st), 'resample')
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(stream)
+            _lib.check(L.cvk_edge_pass(P, sim._layers, pr, wk, t, None, st), 'edge')
+            e1.record(stream)
+            _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, t, 1, st), 'infect')
+            _lib.check(L.cvk_count_pass(P, pr, wk, t, st), 'count')
+            edge_ms.append((e0, e1))
+        torch.cuda.synchronize()
+        edge_ms = [a.elapsed_time(b) for a, b in edge_ms]
+        res2 = sim._w['results'].cpu().numpy().view(np.uint32)
+        eager_matches_graph = bool(np.array_equal(res[:steps], res2[:steps]))
+
+    n = w['pop_size']
+    avg_edge_s = float(np.mean(edge_ms)) / 1e3
+    edge_bytes = 8 * n_edges + n / 8  # p1+p2 int32 streams + the infectious bitmap (per launch)
+    achieved = edge_bytes / avg_edge_s / 1e9
+    value = n * steps / wall
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': steps, 'warmup': warmup,
+        'ms_per_step': wall / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': w['desc'], 'agents': n, 'days': steps, 'edges': int(n_edges),
+                   'dynamic_edges': int(dyn_edges), 'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
+                   'setup_s': round(build_s, 2)},
+        'roofline': {'bound': 'hbm', 'kernel': 'edge_pass_kernel', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes},
+    }
+    if not args.no_cpu_baseline:
+        out['cpu_baseline'] = cpu_baseline(w, pop, args.seed, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == '__main__':
+    main()

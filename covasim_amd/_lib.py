@@ -1,0 +1,192 @@
+"""ctypes binding of libcovasim_amd.so (the C ABI declared in include/covasim_amd.h).
+
+The library is loaded from the package folder; there is no fallback: if it is
+missing, every entry point raises. Structs below mirror the C layout and are
+checked against the library's own ``cvk_abi_layout`` report on load.
+"""
+
+import ctypes as C
+import os
+
+MAX_LAYERS = 8
+MAX_VARIANTS = 8
+N_DURS = 9
+
+OK, E_INVALID, E_CAPACITY, E_HIP, E_ALIGN = 0, -1, -2, -3, -4
+
+# state bits (flags word)
+SUSCEPTIBLE, NAIVE, EXPOSED, INFECTIOUS, SYMPTOMATIC, SEVERE, CRITICAL, TESTED = (1 << k for k in range(8))
+DIAGNOSED, RECOVERED, KNOWN_DEAD, DEAD, KNOWN_CONTACT, QUARANTINED, ISOLATED, VACCINATED = (1 << k for k in range(8, 16))
+EARLY = 1 << 16
+INFVAR_SHIFT = 20
+VAR_NONE = 0xF
+STATE_BITS = {
+    'susceptible': SUSCEPTIBLE, 'naive': NAIVE, 'exposed': EXPOSED, 'infectious': INFECTIOUS,
+    'symptomatic': SYMPTOMATIC, 'severe': SEVERE, 'critical': CRITICAL, 'tested': TESTED,
+    'diagnosed': DIAGNOSED, 'recovered': RECOVERED, 'known_dead': KNOWN_DEAD, 'dead': DEAD,
+    'known_contact': KNOWN_CONTACT, 'quarantined': QUARANTINED, 'isolated': ISOLATED, 'vaccinated': VACCINATED,
+}
+
+DIST_CODES = {'uniform': 0, 'unif': 0, 'normal': 1, 'norm': 1, 'normal_pos': 2, 'normal_int': 3,
+              'lognormal': 4, 'lognorm': 4, 'lognormal_int': 5, 'lognorm_int': 5}
+DUR_ORDER = ['exp2inf', 'inf2sym', 'sym2sev', 'sev2crit', 'asym2rec', 'mild2rec', 'sev2rec', 'crit2rec', 'crit2die']
+
+LAYER_SEED = 14
+LAYER_IMPORT = 15
+
+STREAM_TX = 0x10000000
+STREAM_INFECT = 0x20000000
+STREAM_CHOOSE = 0x30000000
+STREAM_IMPORT = 0x40000000
+STREAM_RESAMPLE = 0x50000000
+
+# result columns
+R_NAMES = ['new_infections', 'new_reinfections', 'new_infectious', 'new_symptomatic', 'new_severe',
+           'new_critical', 'new_recoveries', 'new_deaths', 'new_known_deaths',
+           'n_susceptible', 'n_exposed', 'n_infectious', 'n_symptomatic', 'n_severe', 'n_critical',
+           'n_recovered', 'n_dead', 'n_diagnosed', 'n_known_dead', 'n_quarantined', 'n_isolated',
+           'n_vaccinated', 'n_imports', 'mid_severe', 'mid_critical', 'n_alive']
+R_COL = {k: i for i, k in enumerate(R_NAMES)}
+R_BYVAR = len(R_NAMES)
+RV_NAMES = ['new_infections_by_variant', 'new_symptomatic_by_variant', 'new_severe_by_variant',
+            'new_infectious_by_variant', 'n_exposed_by_variant', 'n_infectious_by_variant']
+R_COLS = R_BYVAR + len(RV_NAMES) * MAX_VARIANTS
+RF_COLS = 4
+CTL_WORDS = 64
+CTL_TOUCHED, CTL_LOG_COUNT, CTL_LOG_OVERFLOW, CTL_SEL_SHORT = 0, 2, 4, 10
+
+
+class Dur(C.Structure):
+    _fields_ = [('dist', C.c_int32), ('pad', C.c_int32), ('par1', C.c_double), ('par2', C.c_double),
+                ('mu', C.c_double), ('sigma', C.c_double)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ('n_agents', C.c_int64), ('n_variants', C.c_int32), ('n_layers', C.c_int32), ('seed', C.c_uint64),
+        ('frac_time', C.c_float), ('load_ratio', C.c_float), ('high_cap', C.c_float),
+        ('vl_hi', C.c_float), ('vl_lo', C.c_float), ('asymp_factor', C.c_float),
+        ('beta_v', C.c_float * MAX_VARIANTS),
+        ('beta_layer', C.c_float * MAX_LAYERS), ('iso_factor', C.c_float * MAX_LAYERS),
+        ('quar_factor', C.c_float * MAX_LAYERS),
+        ('rel_symp_prob', C.c_float * MAX_VARIANTS), ('rel_severe_prob', C.c_float * MAX_VARIANTS),
+        ('rel_crit_prob', C.c_float * MAX_VARIANTS), ('rel_death_prob', C.c_float * MAX_VARIANTS),
+        ('no_hosp_factor', C.c_float), ('no_icu_factor', C.c_float), ('trans_redux', C.c_float),
+        ('nab_boost', C.c_float),
+        ('n_beds_hosp', C.c_int64), ('n_beds_icu', C.c_int64),
+        ('dur', Dur * N_DURS), ('nab_init', Dur),
+        ('use_waning', C.c_int32), ('nab_kin_len', C.c_int32),
+        ('rel_imm_symp', C.c_double * 3), ('nab_norm', C.c_double),
+        ('ve_exp_alpha', C.c_double * 3), ('ve_beta', C.c_double * 3),
+        ('immunity', (C.c_float * MAX_VARIANTS) * MAX_VARIANTS),
+        ('nab_kin', C.c_void_p),
+    ]
+
+
+PEOPLE_PTR_FIELDS = ['flags', 'vstate', 'date_exposed', 'date_infectious', 'date_symptomatic', 'date_severe',
+                     'date_critical', 'date_recovered', 'date_dead', 'date_diagnosed', 'date_end_isolation',
+                     'dur_exp2inf', 'dur_inf2sym', 'dur_sym2sev', 'dur_sev2crit', 'dur_disease', 'rel_trans',
+                     'rel_sus', 'symp_prob', 'severe_prob', 'crit_prob', 'death_prob', 'n_infections',
+                     'n_breakthroughs', 'nab', 'peak_nab', 't_nab_event', 'sus_imm', 'symp_imm', 'sev_imm']
+
+
+class PeopleS(C.Structure):
+    _fields_ = [('n', C.c_int64)] + [(k, C.c_void_p) for k in PEOPLE_PTR_FIELDS]
+
+
+class LayerS(C.Structure):
+    _fields_ = [('p1', C.c_void_p), ('p2', C.c_void_p), ('beta', C.c_void_p), ('n_edges', C.c_int64),
+                ('dynamic', C.c_int32), ('pad', C.c_int32)]
+
+
+class WorkS(C.Structure):
+    _fields_ = [('winner', C.c_void_p), ('touched', C.c_void_p), ('bitmap', C.c_void_p), ('ctl', C.c_void_p),
+                ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
+                ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('npts', C.c_int32), ('pad', C.c_int32)]
+
+
+class ImportsS(C.Structure):
+    _fields_ = [('n', C.c_int32), ('days', C.POINTER(C.c_int32)), ('variants', C.POINTER(C.c_int32)),
+                ('n_imports', C.POINTER(C.c_int64))]
+
+
+_LIB = None
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libcovasim_amd.so')
+
+
+class CovasimAMDError(RuntimeError):
+    """Raised when a library call returns a failure status."""
+
+
+def lib():
+    """Load libcovasim_amd.so (once); raises if it was not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f'covasim_amd: HIP library not built ({LIB_PATH} missing); run __graft_entry__.build()')
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
+    L.cvk_last_error.restype = C.c_char_p
+    L.cvk_version.restype = C.c_int
+    L.cvk_abi_layout.argtypes = [C.POINTER(C.c_int64), i32]
+    L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.cvk_workspace_sizes.argtypes = [i64, i32] + [C.POINTER(C.c_int64)] * 5
+    L.cvk_viral_load.argtypes = [i32, vp, vp, vp, i64, f32, f32, f32, vp, vp]
+    L.cvk_trans_sus.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp, vp, f32, f32, f32, vp, i64, vp, vp, vp]
+    L.cvk_infections.argtypes = [f32, vp, vp, vp, i64, vp, vp, i64, vp, u64, i32, i32, i32, i32, vp, vp, vp, i64,
+                                 C.POINTER(C.c_int64), vp]
+    P, Pa, W, Ly = C.POINTER(PeopleS), C.POINTER(Params), C.POINTER(WorkS), C.POINTER(LayerS)
+    L.cvk_choose.argtypes = [P, Pa, W, i32, u32, i64, i32, i32, i32, vp]
+    L.cvk_agent_pass.argtypes = [P, Pa, W, i32, vp]
+    L.cvk_resample_layer.argtypes = [Ly, i32, Pa, i32, vp]
+    L.cvk_edge_pass.argtypes = [P, Ly, Pa, W, i32, C.POINTER(C.c_void_p), vp]
+    L.cvk_infect_pass.argtypes = [P, Ly, Pa, W, i32, i32, vp]
+    L.cvk_count_pass.argtypes = [P, Pa, W, i32, vp]
+    L.cvk_run_days.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), i32, vp]
+    L.cvk_graph_create.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), vp, C.POINTER(C.c_void_p)]
+    L.cvk_graph_launch.argtypes = [vp, vp]
+    L.cvk_graph_destroy.argtypes = [vp]
+    for name in ['cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes', 'cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_choose',
+                 'cvk_agent_pass', 'cvk_resample_layer', 'cvk_edge_pass', 'cvk_infect_pass', 'cvk_count_pass',
+                 'cvk_run_days', 'cvk_abi_layout', 'cvk_device_info']:
+        getattr(L, name).restype = C.c_int
+    _check_layout(L)
+    _LIB = L
+    return L
+
+
+def _check_layout(L):
+    out = (C.c_int64 * 32)()
+    m = L.cvk_abi_layout(out, 32)
+    got = list(out[:m])
+    want = [C.sizeof(Dur), C.sizeof(Params), C.sizeof(PeopleS), C.sizeof(LayerS), C.sizeof(WorkS),
+            C.sizeof(ImportsS), Params.dur.offset, Params.nab_init.offset, Params.rel_imm_symp.offset,
+            Params.immunity.offset, Params.nab_kin.offset, PeopleS.sev_imm.offset, WorkS.log_capacity.offset,
+            WorkS.npts.offset, R_COLS, RF_COLS]
+    if got != want:
+        raise RuntimeError(f'covasim_amd: ctypes struct layout mismatch with the library: {got} != {want}')
+
+
+def check(rc, what=''):
+    """Raise on a negative status, mapping codes to Python exception types like the reference's."""
+    if rc >= 0:
+        return rc
+    msg = lib().cvk_last_error().decode(errors='replace')
+    if rc == E_INVALID:
+        raise ValueError(f'{what}: {msg}')
+    if rc == E_ALIGN:
+        raise ValueError(f'{what}: {msg}')
+    if rc == E_CAPACITY:
+        raise OverflowError(f'{what}: {msg}')
+    raise CovasimAMDError(f'{what}: {msg}')
+
+
+def stream_ptr(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)

@@ -1,0 +1,14 @@
+#!/bin/bash
+# Builds libcovasim_amd.so for gfx950 from the HIP sources in this folder.
+set -euo pipefail
+here="$(cd "$(dirname "${BASH_SOURCE[0]}")" && pwd)"
+out="${here}/../libcovasim_amd.so"
+HIPCC="${HIPCC:-/opt/rocm/bin/hipcc}"
+"${HIPCC}" --offload-arch=gfx950 -O3 -std=c++17 -fPIC -shared \
+  -ffp-contract=off -fhip-fp32-correctly-rounded-divide-sqrt -fno-gpu-rdc \
+  -Xclang -target-feature -Xclang -packed-fp32-ops \
+  -Wall -Wno-unused-function \
+  "${here}/cvk_mirror.hip" "${here}/cvk_day.hip" \
+  -o "${out}.tmp"
+mv "${out}.tmp" "${out}"
+echo "built ${out}"

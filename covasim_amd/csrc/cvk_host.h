@@ -1,0 +1,28 @@
+// Host-side error plumbing shared by the covasim_amd translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+
+#include "../../include/covasim_amd.h"
+
+namespace cvk {
+
+// Thread-local message of the last failure (returned by cvk_last_error()).
+char* last_error_buffer();
+
+inline int fail(int code, const char* msg) {
+  snprintf(last_error_buffer(), 512, "%s", msg);
+  return code;
+}
+
+inline int check_launch(const char* where) {
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    snprintf(last_error_buffer(), 512, "%s: %s", where, hipGetErrorString(e));
+    return CVK_E_HIP;
+  }
+  return CVK_OK;
+}
+
+}  // namespace cvk

@@ -1,0 +1,287 @@
+/*
+ * covasim_amd.h -- C ABI of the MI355X (gfx950) daily-transmission-step library.
+ *
+ * Plain extern "C" entry points over plain device pointers and sizes; no C++
+ * or torch types cross the boundary. All pointers are device pointers unless
+ * a parameter name ends in _host. Every call is asynchronous on the caller's
+ * HIP stream (hipStream_t passed as void*), returns 0 on success and a
+ * negative CVK_E* code otherwise; cvk_last_error() gives the message of the
+ * most recent failure on the calling thread.
+ *
+ * Reference interfaces replaced (paths relative to the Covasim 3.1.7 checkout):
+ *   cvk_viral_load     covasim/utils.py:39-79   compute_viral_load
+ *   cvk_trans_sus      covasim/utils.py:82-90   compute_trans_sus
+ *   cvk_infections     covasim/utils.py:93-128  compute_infections
+ *   cvk_agent_pass     covasim/people.py:164-186 People.update_states_pre (+ check_* :211-312, :368-374)
+ *                      covasim/immunity.py:303-350 check_immunity, utils.py:39-79 (viral-load phase)
+ *   cvk_resample_layer covasim/base.py:1849-1876 Layer.update (frac = 1)
+ *   cvk_choose         covasim/utils.py:430-443 choose / immunity.py:117-130 variant import choice
+ *   cvk_edge_pass      covasim/sim.py:611-649   variant x layer loop of compute_trans_sus + compute_infections,
+ *                      with People.infect's first-occurrence dedup (people.py:465) fused as a min-key
+ *   cvk_infect_pass    covasim/people.py:435-586 People.infect (+ immunity.py:138-202 update_peak_nab)
+ *   cvk_count_pass     covasim/sim.py:651-674   stock counts (base.py:1133-1145) + immunity.py:205-213 update_nab
+ *   cvk_run_days       covasim/sim.py:558-685   Sim.step, repeated (the day loop of Sim.run, sim.py:733-755)
+ */
+#ifndef COVASIM_AMD_H
+#define COVASIM_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CVK_MAX_LAYERS 8
+#define CVK_MAX_VARIANTS 8
+#define CVK_N_DURS 9
+
+/* status codes */
+#define CVK_OK 0
+#define CVK_E_INVALID (-1)   /* invalid argument (null pointer, bad size, bad enum) */
+#define CVK_E_CAPACITY (-2)  /* an output buffer is too small */
+#define CVK_E_HIP (-3)       /* a HIP runtime call failed */
+#define CVK_E_ALIGN (-4)     /* a pointer is not aligned as required */
+
+/* state bits of cvk_people.flags (defaults.py:54-71 order) */
+#define CVK_SUSCEPTIBLE (1u << 0)
+#define CVK_NAIVE (1u << 1)
+#define CVK_EXPOSED (1u << 2)
+#define CVK_INFECTIOUS (1u << 3)
+#define CVK_SYMPTOMATIC (1u << 4)
+#define CVK_SEVERE (1u << 5)
+#define CVK_CRITICAL (1u << 6)
+#define CVK_TESTED (1u << 7)
+#define CVK_DIAGNOSED (1u << 8)
+#define CVK_RECOVERED (1u << 9)
+#define CVK_KNOWN_DEAD (1u << 10)
+#define CVK_DEAD (1u << 11)
+#define CVK_KNOWN_CONTACT (1u << 12)
+#define CVK_QUARANTINED (1u << 13)
+#define CVK_ISOLATED (1u << 14)
+#define CVK_VACCINATED (1u << 15)
+#define CVK_EARLY (1u << 16)             /* derived daily: in the high viral-load phase */
+#define CVK_INFVAR_SHIFT 20              /* bits 20..23: infectious_variant, 0xF = undefined */
+/* cvk_people.vstate: bits 0..3 exposed_variant, 4..7 recovered_variant (0xF = undefined),
+ * 8..15 exposed_by_variant bitmask, 16..23 infectious_by_variant bitmask */
+#define CVK_VAR_NONE 0xFu
+
+/* duration distribution codes (utils.py:156-236 names) */
+#define CVK_DIST_UNIFORM 0
+#define CVK_DIST_NORMAL 1
+#define CVK_DIST_NORMAL_POS 2
+#define CVK_DIST_NORMAL_INT 3
+#define CVK_DIST_LOGNORMAL 4
+#define CVK_DIST_LOGNORMAL_INT 5
+
+/* duration slots of cvk_params.dur (parameters.py:85-95) */
+#define CVK_DUR_EXP2INF 0
+#define CVK_DUR_INF2SYM 1
+#define CVK_DUR_SYM2SEV 2
+#define CVK_DUR_SEV2CRIT 3
+#define CVK_DUR_ASYM2REC 4
+#define CVK_DUR_MILD2REC 5
+#define CVK_DUR_SEV2REC 6
+#define CVK_DUR_CRIT2REC 7
+#define CVK_DUR_CRIT2DIE 8
+
+/* infection-log layer codes besides 0..n_layers-1 */
+#define CVK_LAYER_SEED 14
+#define CVK_LAYER_IMPORT 15
+
+/* per-day result columns (counts, uint32) */
+enum {
+  CVK_R_NEW_INFECTIONS = 0, CVK_R_NEW_REINFECTIONS, CVK_R_NEW_INFECTIOUS, CVK_R_NEW_SYMPTOMATIC,
+  CVK_R_NEW_SEVERE, CVK_R_NEW_CRITICAL, CVK_R_NEW_RECOVERIES, CVK_R_NEW_DEATHS, CVK_R_NEW_KNOWN_DEATHS,
+  CVK_R_N_SUSCEPTIBLE, CVK_R_N_EXPOSED, CVK_R_N_INFECTIOUS, CVK_R_N_SYMPTOMATIC, CVK_R_N_SEVERE,
+  CVK_R_N_CRITICAL, CVK_R_N_RECOVERED, CVK_R_N_DEAD, CVK_R_N_DIAGNOSED, CVK_R_N_KNOWN_DEAD,
+  CVK_R_N_QUARANTINED, CVK_R_N_ISOLATED, CVK_R_N_VACCINATED, CVK_R_N_IMPORTS,
+  CVK_R_MID_SEVERE, CVK_R_MID_CRITICAL,   /* counts after update_states_pre, for bed limits */
+  CVK_R_N_ALIVE_NAB,                      /* alive agents (denominator of pop_nabs) */
+  CVK_R_BYVAR,                            /* start of 6 x CVK_MAX_VARIANTS by-variant columns */
+  CVK_R_COLS = CVK_R_BYVAR + 6 * CVK_MAX_VARIANTS
+};
+/* by-variant column offsets: CVK_R_BYVAR + k*CVK_MAX_VARIANTS + v */
+#define CVK_RV_NEW_INFECTIONS 0
+#define CVK_RV_NEW_SYMPTOMATIC 1
+#define CVK_RV_NEW_SEVERE 2
+#define CVK_RV_NEW_INFECTIOUS 3
+#define CVK_RV_N_EXPOSED 4
+#define CVK_RV_N_INFECTIOUS 5
+/* per-day float columns (double): pop_nabs numerator, sus_imm sum, symp_imm sum */
+#define CVK_RF_NAB_SUM 0
+#define CVK_RF_SUS_IMM_SUM 1
+#define CVK_RF_SYMP_IMM_SUM 2
+#define CVK_RF_COLS 4
+
+typedef struct {
+  int32_t dist;   /* CVK_DIST_* */
+  int32_t pad;
+  double par1, par2;   /* as given to sample() */
+  double mu, sigma;    /* lognormal: parameters of the underlying normal (utils.py:223-227) */
+} cvk_dur;
+
+/* Scalar parameters of one sim (parameters.py:15-151). Plain old data, copied by value. */
+typedef struct {
+  int64_t n_agents;
+  int32_t n_variants, n_layers;
+  uint64_t seed;                 /* Philox key */
+  float frac_time, load_ratio, high_cap;   /* viral_dist */
+  float vl_hi, vl_lo;            /* viral load in the early / late phase (utils.py:78), float32 */
+  float asymp_factor;
+  float beta_v[CVK_MAX_VARIANTS];          /* default_float(beta*rel_beta*variant rel_beta), sim.py:627 */
+  float beta_layer[CVK_MAX_LAYERS], iso_factor[CVK_MAX_LAYERS], quar_factor[CVK_MAX_LAYERS];
+  float rel_symp_prob[CVK_MAX_VARIANTS], rel_severe_prob[CVK_MAX_VARIANTS];
+  float rel_crit_prob[CVK_MAX_VARIANTS], rel_death_prob[CVK_MAX_VARIANTS];
+  float no_hosp_factor, no_icu_factor, trans_redux, nab_boost;
+  int64_t n_beds_hosp, n_beds_icu;         /* -1 = no constraint */
+  cvk_dur dur[CVK_N_DURS];
+  cvk_dur nab_init;
+  int32_t use_waning, nab_kin_len;
+  double rel_imm_symp[3];        /* asymptomatic, mild, severe */
+  double nab_norm;               /* 1 + alpha_inf_diff */
+  double ve_exp_alpha[3], ve_beta[3];   /* sus, symp, sev: exp(alpha), beta (immunity.py:216-247) */
+  float immunity[CVK_MAX_VARIANTS][CVK_MAX_VARIANTS];
+  const double* nab_kin;         /* device, length nab_kin_len (immunity.py:356-401) */
+} cvk_params;
+
+/* Agent state, structure of arrays of length n (device pointers). */
+typedef struct {
+  int64_t n;
+  uint32_t* flags;
+  uint32_t* vstate;
+  float *date_exposed, *date_infectious, *date_symptomatic, *date_severe, *date_critical;
+  float *date_recovered, *date_dead, *date_diagnosed, *date_end_isolation;
+  float *dur_exp2inf, *dur_inf2sym, *dur_sym2sev, *dur_sev2crit, *dur_disease;
+  float *rel_trans, *rel_sus, *symp_prob, *severe_prob, *crit_prob, *death_prob;
+  int32_t *n_infections, *n_breakthroughs;
+  float *nab, *peak_nab;
+  int32_t* t_nab_event;
+  float *sus_imm, *symp_imm, *sev_imm;   /* [n_variants][n] */
+} cvk_people;
+
+/* One contact layer: int32 endpoints and float32 per-edge beta (base.py:1610-1676). */
+typedef struct {
+  int32_t* p1;
+  int32_t* p2;
+  float* beta;
+  int64_t n_edges;
+  int32_t dynamic;     /* resampled each day (dynam_layer) */
+  int32_t pad;
+} cvk_layer;
+
+/* Device workspace (sizes from cvk_workspace_sizes). */
+typedef struct {
+  uint64_t* winner;      /* [n] min transmission key per target; all-ones = none */
+  int32_t* touched;      /* [n] targets with a winner key this day */
+  uint32_t* bitmap;      /* [bitmap_words] infectious-with-nonzero-rel_trans bits */
+  uint32_t* ctl;         /* [64] control words: touched count, choose state, flags */
+  uint32_t* results;     /* [npts][CVK_R_COLS] */
+  double* results_f;     /* [npts][CVK_RF_COLS] */
+  int32_t* log;          /* [log_capacity][4] infection records: source, target, day, layer | variant<<8 */
+  int64_t log_capacity;
+  uint32_t* select_hist; /* [256] histogram scratch of cvk_choose */
+  int32_t npts;
+  int32_t pad;
+} cvk_work;
+
+/* control word indices of cvk_work.ctl */
+#define CVK_CTL_TOUCHED 0
+#define CVK_CTL_LOG_COUNT 2       /* 64-bit counter in words 2 (lo) and 3 (hi) */
+#define CVK_CTL_LOG_OVERFLOW 4
+#define CVK_CTL_SEL_PREFIX 6      /* cvk_choose radix-select state: 64-bit prefix in words 6, 7 */
+#define CVK_CTL_SEL_REMAIN 8
+#define CVK_CTL_SEL_ALL 9
+#define CVK_CTL_SEL_SHORT 10      /* set when fewer agents were eligible than requested */
+#define CVK_CTL_WORDS 64
+
+/* ---- library ---- */
+const char* cvk_last_error(void);
+int cvk_version(void);
+/* No reference counterpart: ABI self-description for bindings. Writes sizeof/offsetof values of
+ * the structs above into out[0..m) and returns m (see cvk_day.hip for the order). */
+int cvk_abi_layout(int64_t* out, int32_t n);
+/* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
+int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);
+/* Sizes in bytes of the workspace arrays for n agents and npts days. */
+int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winner_bytes, int64_t* touched_bytes,
+                        int64_t* bitmap_bytes, int64_t* results_bytes, int64_t* results_f_bytes);
+
+/* ---- 1:1 mirrors of the reference's numba kernels (parity boundary) ---- */
+
+/* utils.py:39-79. out[i] = viral load of agent i on day t. */
+int cvk_viral_load(int32_t t, const float* time_start, const float* time_recovered, const float* time_dead,
+                   int64_t n, float frac_time, float load_ratio, float high_cap, float* out, void* stream);
+
+/* utils.py:82-90. Boolean inputs are uint8 (0/1) arrays of length n. */
+int cvk_trans_sus(const float* rel_trans, const float* rel_sus, const uint8_t* inf, const uint8_t* sus,
+                  float beta_layer, const float* viral_load, const uint8_t* symp, const uint8_t* iso,
+                  const uint8_t* quar, float asymp_factor, float iso_factor, float quar_factor,
+                  const float* immunity_factors, int64_t n, float* rel_trans_out, float* rel_sus_out,
+                  void* stream);
+
+/* utils.py:93-128. Writes the transmitting (source, target) pairs in the reference's order:
+ * direction (p1->p2, then p2->p1, or only p1->p2 if legacy), then ascending edge index.
+ * uniforms: NULL for counter-based Philox draws keyed by (seed, t, variant, layer), else a
+ * device float64 array [2][n_edges] of per-(direction, edge) uniforms. prob_out (may be NULL)
+ * receives the float32 probability of each emitted pair. *count_host receives the number of
+ * pairs; returns CVK_E_CAPACITY (after writing count) if it exceeds capacity. Synchronises
+ * the stream to read the count. */
+int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
+                   int64_t n_edges, const float* rel_trans, const float* rel_sus, int64_t n_agents,
+                   const double* uniforms, uint64_t seed, int32_t t, int32_t variant, int32_t layer,
+                   int32_t legacy, int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity,
+                   int64_t* count_host, void* stream);
+
+/* ---- production path: one day is agent_pass -> resample -> choose(imports) -> edge_pass ->
+ *      infect_pass -> count_pass; cvk_run_days chains them for a range of days ---- */
+
+/* Seed infections at t = 0 (sim.py:505-532): choose k distinct agents (eligible = all, or
+ * susceptible only when susceptible_only != 0) with Philox selection keys on stream
+ * `stream_id`, and register them as infections of class `layer_code` / `variant` in
+ * work->winner / work->touched. */
+int cvk_choose(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
+               uint32_t stream_id, int64_t k, int32_t susceptible_only, int32_t layer_code,
+               int32_t variant, void* stream);
+
+int cvk_agent_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
+                   void* stream);
+int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
+                       void* stream);
+/* uniforms: NULL (Philox) or per layer a device float64 array [n_variants][2][n_edges]. */
+int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
+                  cvk_work* work, int32_t t, const double* const* uniforms, void* stream);
+int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
+                    cvk_work* work, int32_t t, int32_t count_flows, void* stream);
+int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
+                   void* stream);
+
+/* Import schedule for cvk_run_days: on day days[i], n_imports[i] susceptible agents are
+ * infected with variant variants[i] (immunity.py:117-130). Host arrays. */
+typedef struct {
+  int32_t n;
+  const int32_t* days;
+  const int32_t* variants;
+  const int64_t* n_imports;
+} cvk_imports;
+
+/* Runs days [t0, t1) (t0 == 0 also seeds pop_infected infections first). use_graph != 0
+ * captures the launches into one hipGraph and launches it once. */
+int cvk_run_days(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
+                 int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports,
+                 int32_t use_graph, void* stream);
+
+/* Capture days [t0, t1) (the cvk_run_days sequence) into an instantiated hipGraph; replay it with
+ * cvk_graph_launch (each replay re-runs the captured days on the same buffers, so the caller
+ * resets the state in between); release with cvk_graph_destroy. The day loop of Sim.run
+ * (sim.py:733-755) as one launch. */
+int cvk_graph_create(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
+                     int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, void* stream,
+                     void** graph_exec_out);
+int cvk_graph_launch(void* graph_exec, void* stream);
+int cvk_graph_destroy(void* graph_exec);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* COVASIM_AMD_H */

@@ -1,0 +1,61 @@
+"""The C-ABI library loads without a GPU and exports every entry point include/covasim_amd.h declares."""
+
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, 'include', 'covasim_amd.h')
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+    return sorted(set(re.findall(r'\b(cvk_[a-z0-9_]+)\s*\(', text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ['cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_agent_pass', 'cvk_edge_pass',
+                 'cvk_infect_pass', 'cvk_count_pass', 'cvk_run_days', 'cvk_choose', 'cvk_resample_layer']:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from covasim_amd import _lib
+    L = _lib.lib()  # also verifies the ctypes struct layouts against cvk_abi_layout
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_struct_layout_report():
+    from covasim_amd import _lib
+    L = _lib.lib()
+    out = (C.c_int64 * 32)()
+    m = L.cvk_abi_layout(out, 32)
+    assert m == 16
+    assert out[0] == C.sizeof(_lib.Dur) == 40
+    assert L.cvk_abi_layout(out, 3) < 0  # too-small output is an error, not an overflow
+
+
+def test_host_side_argument_errors():
+    """Invalid arguments fail before any device work (no GPU needed)."""
+    from covasim_amd import _lib
+    L = _lib.lib()
+    assert L.cvk_viral_load(0, None, None, None, 10, 0.3, 2.0, 4.0, None, None) == _lib.E_INVALID
+    assert b'null' in L.cvk_last_error()
+    cnt = C.c_int64(7)
+    assert L.cvk_infections(0.1, None, None, None, 5, None, None, 5, None, 0, 0, 0, 0, 0, None, None, None, 0,
+                            C.byref(cnt), None) == _lib.E_INVALID
+    assert cnt.value == 0
+    assert L.cvk_agent_pass(None, None, None, 0, None) == _lib.E_INVALID
+    w = _lib.WorkS()
+    sizes = [C.c_int64() for _ in range(5)]
+    assert L.cvk_workspace_sizes(1000, 61, *[C.byref(s) for s in sizes]) == 0
+    assert [s.value for s in sizes] == [8000, 4000, 128, 61 * _lib.R_COLS * 4, 61 * _lib.RF_COLS * 8]
+    assert L.cvk_workspace_sizes(0, 61, *[C.byref(s) for s in sizes]) == _lib.E_INVALID
+    assert L.cvk_graph_launch(None, None) == _lib.E_INVALID
+    with pytest.raises(ValueError):
+        _lib.check(L.cvk_agent_pass(None, None, None, 0, None), 'agent pass')
