@@ -32,6 +32,11 @@ WORKLOADS = {
                      desc='hybrid 1M agents, 180 days, households 1..7 (mean 2.6), s Poisson(20) ages 6-22, '
                           'w Poisson(16) ages 22-65, c Poisson(20) resampled daily; layer betas h3.0 s0.6 w0.6 '
                           'c0.3; beta 0.016; float32 state'),
+    'hybrid10m_variants': dict(pop_size=10_000_000, pop_type='hybrid', n_days=120, households='categorical',
+                               pars=dict(beta=0.016, use_waning=True, pop_infected=20, dynam_layer=dict(c=1)),
+                               variants=[dict(rel_beta=1.7, days=[30], n_imports=100)],
+                               desc='hybrid 10M agents, 120 days, waning NAbs, 2 variants (second rel_beta 1.7, '
+                                    '100 imports on day 30); layers as hybrid1m'),
     'random20k': dict(pop_size=20_000, pop_type='random', n_days=60, households='poisson',
                       pars=dict(beta=0.016, use_waning=True, pop_infected=20),
                       desc='random 20k agents, one layer a with Poisson(20) contacts, 60 days, beta 0.016'),
@@ -51,25 +56,31 @@ def build_population(w, seed):
     return pop
 
 
-def cpu_baseline(w, pop, seed, budget_s):
-    """The oracle (numpy restatement, one thread) on the same population until budget_s elapses."""
+def cpu_baseline(w, pop, seed, budget_s, omp=False):
+    """The oracle's compiled C restatement on the same population until budget_s elapses.
+
+    One host thread (numba's default single-threaded mode) unless omp: then OpenMP over the
+    host's cores. Days are run from day 0, so the sample covers the epidemic's growth phase.
+    """
     from oracle import covasim_oracle as co
+    from oracle.cport.port import PortSim, load
     opars = co.make_pars(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=seed,
-                         **w['pars'])
+                         variants=list(w.get('variants', [])), **w['pars'])
     keys = ('symp_prob', 'severe_prob', 'crit_prob', 'death_prob', 'rel_sus', 'rel_trans')
     init = dict(age=pop['age'], **{k: pop[k] for k in keys})
     layers = {k: (pop['contacts'][k]['p1'], pop['contacts'][k]['p2'], pop['contacts'][k]['beta'])
               for k in opars['beta_layer']}
-    sim = co.Sim(opars, init, layers, co.PhiloxRNG(seed))
+    sim = PortSim(opars, init, layers, seed, omp=omp)
+    threads = load(omp).port_threads()
     t0 = time.perf_counter()
     days = 0
     while days < sim.npts and time.perf_counter() - t0 < budget_s:
         sim.step()
         days += 1
     el = time.perf_counter() - t0
-    return dict(value=w['pop_size'] * days / el, unit='agent-days/s', cores=1, kind='port',
-                sample=f'oracle (numpy float32 restatement, PhiloxRNG) days 0..{days - 1} of the same workload and '
-                       f'population, {el:.1f} s on one host thread')
+    return dict(value=w['pop_size'] * days / el, unit='agent-days/s', cores=threads, kind='port',
+                sample=f'oracle C restatement (gcc -O3{", OpenMP" if omp else ""}, Philox draws) on the same '
+                       f'population, days 0..{days - 1} of the workload, {el:.1f} s on {threads} host thread(s)')
 
 
 def main():
@@ -81,6 +92,7 @@ def main():
     ap.add_argument('--seed', type=int, default=1)
     ap.add_argument('--cpu-budget', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus above 1 is not supported (the sharded '
@@ -96,11 +108,14 @@ def main():
     warmup = steps if args.warmup is None else args.warmup
     n_days = max(w['n_days'], steps)
     t_build = time.perf_counter()
+    print(f'building population ({args.workload})', file=sys.stderr, flush=True)
     pop = build_population(w, args.seed)
     n_edges = sum(len(l['p1']) for l in pop['contacts'].values())
     dyn_edges = sum(len(pop['contacts'][k]['p1']) for k, v in w['pars'].get('dynam_layer', {}).items() if v)
+    variants = [cv.variant(dict(rel_beta=v['rel_beta']), days=v['days'], n_imports=v['n_imports'])
+                for v in w.get('variants', [])]
     sim = cv.Sim(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=n_days, rand_seed=args.seed,
-                 popdict=pop, households=w['households'], **w['pars'])
+                 popdict=pop, households=w['households'], variants=variants, **w['pars'])
     sim.initialize()
     build_s = time.perf_counter() - t_build
     L = _lib.lib()
@@ -115,9 +130,7 @@ def main():
         for lk, lay in people.contacts.items():
             for c in ('p1', 'p2', 'beta'):
                 lay.tensor(c).copy_(init_layers[lk][c])
-        sim._w['winner'].fill_(-1)
-        for k in ('touched', 'bitmap', 'ctl', 'results', 'results_f', 'select_hist'):
-            sim._w[k].zero_()
+        _lib.reset_workspace(sim._w)
 
     P = people.struct()
     with torch.cuda.stream(stream):
@@ -137,6 +150,7 @@ def main():
         _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'graph launch')
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
+        print(f'timed {steps} days in {wall * 1e3:.1f} ms', file=sys.stderr, flush=True)
         res = sim._w['results'].cpu().numpy().view(np.uint32)
         cum_inf = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
         _lib.check(L.cvk_graph_destroy(gexec), 'graph destroy')
@@ -156,6 +170,11 @@ def main():
             for li in range(pr.n_layers):
                 if sim._layers[li].dynamic:
                     _lib.check(L.cvk_resample_layer(C.byref(sim._layers[li]), li, pr, t, st), 'resample')
+            days_i, vars_i, nimp_i = sim._imp_arrays
+            for j in range(len(days_i)):
+                if days_i[j] == t:
+                    _lib.check(L.cvk_choose(P, pr, wk, t, _lib.STREAM_IMPORT | int(vars_i[j]), int(nimp_i[j]), 1,
+                                            _lib.LAYER_IMPORT, int(vars_i[j]), st), 'imports')
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
             _lib.check(L.cvk_edge_pass(P, sim._layers, pr, wk, t, None, st), 'edge')
@@ -182,10 +201,15 @@ def main():
                    'setup_s': round(build_s, 2)},
         'roofline': {'bound': 'hbm', 'kernel': 'edge_pass_kernel', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes},
+                     'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes,
+                     'launch_us_p10_p50_p90_max': [round(float(x) * 1e3, 1) for x in
+                                                   np.percentile(edge_ms, [10, 50, 90, 100])]},
     }
+    print('edge pass per-day us: ' + ' '.join(f'{x * 1e3:.0f}' for x in edge_ms), file=sys.stderr, flush=True)
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(w, pop, args.seed, args.cpu_budget)
+        if args.cpu_omp:
+            out['cpu_baseline_omp'] = cpu_baseline(w, pop, args.seed, args.cpu_budget, omp=True)
     print(json.dumps(out), flush=True)
 
 

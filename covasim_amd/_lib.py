@@ -102,7 +102,48 @@ class LayerS(C.Structure):
 class WorkS(C.Structure):
     _fields_ = [('winner', C.c_void_p), ('touched', C.c_void_p), ('bitmap', C.c_void_p), ('ctl', C.c_void_p),
                 ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
-                ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('npts', C.c_int32), ('pad', C.c_int32)]
+                ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('susmap', C.c_void_p),
+                ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p),
+                ('npts', C.c_int32), ('pad', C.c_int32)]
+
+
+MAX_REDUCE_BLOCKS = 1024
+PARTIAL_COLS = 64
+WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'susmap',
+                   'rec', 'partials', 'partials_f']
+
+
+def make_workspace(n, npts, log_capacity, device):
+    """Device workspace tensors for n agents and npts days, and the cvk_work struct over them."""
+    import torch
+    words = ((n + 127) // 128) * 4
+    w = dict(
+        winner=torch.full((n,), -1, dtype=torch.int64, device=device),
+        touched=torch.zeros(n, dtype=torch.int32, device=device),
+        bitmap=torch.zeros(words, dtype=torch.int32, device=device),
+        susmap=torch.zeros(words, dtype=torch.int32, device=device),
+        rec=torch.zeros(n, dtype=torch.int64, device=device),
+        ctl=torch.zeros(CTL_WORDS, dtype=torch.int32, device=device),
+        results=torch.zeros((npts, R_COLS), dtype=torch.int32, device=device),
+        results_f=torch.zeros((npts, RF_COLS), dtype=torch.float64, device=device),
+        log=torch.zeros((max(1, log_capacity), 4), dtype=torch.int32, device=device),
+        select_hist=torch.zeros(256, dtype=torch.int32, device=device),
+        partials=torch.zeros((MAX_REDUCE_BLOCKS, PARTIAL_COLS), dtype=torch.int32, device=device),
+        partials_f=torch.zeros((MAX_REDUCE_BLOCKS, RF_COLS), dtype=torch.float64, device=device),
+    )
+    s = WorkS()
+    for k in WORK_PTR_FIELDS:
+        setattr(s, k, w[k].data_ptr())
+    s.log_capacity = log_capacity
+    s.npts = npts
+    return w, s
+
+
+def reset_workspace(w):
+    """Clear a workspace for a fresh run (device-side fills, no host copies)."""
+    w['winner'].fill_(-1)
+    for k in ('touched', 'bitmap', 'susmap', 'rec', 'ctl', 'results', 'results_f', 'select_hist'):
+        w[k].zero_()
 
 
 class ImportsS(C.Structure):
@@ -132,6 +173,7 @@ def lib():
     L.cvk_abi_layout.argtypes = [C.POINTER(C.c_int64), i32]
     L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cvk_workspace_sizes.argtypes = [i64, i32] + [C.POINTER(C.c_int64)] * 5
+    L.cvk_workspace_sizes2.argtypes = [i64] + [C.POINTER(C.c_int64)] * 3
     L.cvk_viral_load.argtypes = [i32, vp, vp, vp, i64, f32, f32, f32, vp, vp]
     L.cvk_trans_sus.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp, vp, f32, f32, f32, vp, i64, vp, vp, vp]
     L.cvk_infections.argtypes = [f32, vp, vp, vp, i64, vp, vp, i64, vp, u64, i32, i32, i32, i32, vp, vp, vp, i64,
@@ -147,7 +189,8 @@ def lib():
     L.cvk_graph_create.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), vp, C.POINTER(C.c_void_p)]
     L.cvk_graph_launch.argtypes = [vp, vp]
     L.cvk_graph_destroy.argtypes = [vp]
-    for name in ['cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes', 'cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_choose',
+    for name in ['cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes2',
+                 'cvk_workspace_sizes', 'cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_choose',
                  'cvk_agent_pass', 'cvk_resample_layer', 'cvk_edge_pass', 'cvk_infect_pass', 'cvk_count_pass',
                  'cvk_run_days', 'cvk_abi_layout', 'cvk_device_info']:
         getattr(L, name).restype = C.c_int

@@ -1,14 +1,18 @@
 // Production day loop of the covasim_amd library (gfx950, wave64).
 //
-// One simulated day (covasim/sim.py:558-685) is six stages on one HIP stream:
-//   agent_pass   update_states_pre transitions + check_immunity + viral-load phase bit +
-//                infectious bitmap + transition flows           (people.py:164-312, immunity.py:303-350)
-//   resample     dynamic layers: iid uniform endpoints per edge   (base.py:1849-1876)
-//   choose       variant imports among susceptibles (radix select of Philox keys; immunity.py:117-130)
-//   edge_pass    all layers x variants x both directions, fused; LDS-staged infectious bitmap;
-//                per-target atomicMin of the (variant, layer, direction, edge) key (sim.py:611-649)
-//   infect_pass  outcome tree, durations, NAb peak, infection log  (people.py:435-586, immunity.py:138-202)
-//   count_pass   stocks + update_nab + population immunity sums   (sim.py:651-674, immunity.py:205-213)
+// One simulated day (covasim/sim.py:558-685) is three launches on one HIP stream
+// (plus a radix-select choose on variant-import days):
+//   day_begin    previous day's stocks + update_nab (sim.py:651-674), then this day's
+//                update_states_pre transitions + check_immunity + viral-load phase bit
+//                (people.py:164-312, immunity.py:303-350, utils.py:58-78), the two agent
+//                bitmaps and edge records; extra blocks resample dynamic layers (base.py:1849-1876)
+//   edge_pass    all layers x variants x both directions in one pass; infectious bitmap in LDS,
+//                susceptible bitmap from L2, wave-local candidate queues, per-target atomicMin of
+//                the (variant, layer, direction, edge) key (sim.py:611-649, people.py:465)
+//   infect_pass  outcome tree, durations, NAb peak, infection log (people.py:435-586,
+//                immunity.py:138-202)
+// A count-only finish pass closes the last day of a run. Per-block counts are combined by the
+// last block to finish (ticket in cvk_work.ctl), so no same-address atomics pile up.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -30,14 +34,29 @@ constexpr uint32_t STATE_CLEAR_ON_DEATH = CVK_SUSCEPTIBLE | CVK_EXPOSED | CVK_IN
                                           CVK_SEVERE | CVK_CRITICAL | CVK_KNOWN_CONTACT | CVK_QUARANTINED |
                                           CVK_RECOVERED;
 constexpr uint32_t INFVAR_MASK = 0xFu << CVK_INFVAR_SHIFT;
+constexpr int MAXV = CVK_MAX_VARIANTS;
+constexpr int AGENT_BLOCK = 256;
+constexpr int AGENT_WAVES_MAX = 16;
 
 __device__ inline uint32_t* day_row(const cvk_work& w, int32_t t) { return w.results + (int64_t)t * CVK_R_COLS; }
 __device__ inline double* day_row_f(const cvk_work& w, int32_t t) { return w.results_f + (int64_t)t * CVK_RF_COLS; }
 
-// Wave-aggregated count: every lane of the wave must reach this call.
-__device__ inline void wave_count(uint32_t* s_cnt, int col, bool pred) {
-  const uint64_t m = __ballot(pred);
-  if ((threadIdx.x & 63) == 0 && m) atomicAdd(&s_cnt[col], (uint32_t)__popcll(m));
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
 }
 
 // calc_VE, immunity.py:216-247, in float64 then stored as float32.
@@ -46,155 +65,324 @@ __device__ inline float calc_ve(double nab, double exp_alpha, double beta) {
   return (float)(exp_lo / (1.0 + exp_lo));
 }
 
-// ===========================================================================
-// agent_pass
-// ===========================================================================
-constexpr int AGENT_BLOCK = 256;
-constexpr int AGENT_NCOUNT = 9 + CVK_MAX_VARIANTS;  // 9 scalar flows + infectious-by-variant
-
-__global__ __launch_bounds__(AGENT_BLOCK) void agent_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
-                                                                 int32_t t) {
-  __shared__ uint32_t s_cnt[AGENT_NCOUNT];
-  if (threadIdx.x < AGENT_NCOUNT) s_cnt[threadIdx.x] = 0;
+// ---------------------------------------------------------------------------
+// Reductions without device-scope fences: every block of a reducing kernel stores its
+// partial row (no fence, no ticket); the rows are summed after the kernel boundary by
+// finalize_partials (block 0 of the next kernel, or a one-block reduce kernel). Integer
+// columns are order-free; float columns are summed in a fixed order (deterministic).
+// ---------------------------------------------------------------------------
+template <int NC, int ND>
+__device__ void write_partials(const uint32_t (&c)[NC], const double (&d)[ND], uint32_t* s_red, double* s_redf,
+                               uint32_t* partials, double* partials_f) {
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) s_red[k] = 0;
+  for (int k = threadIdx.x; k < ND; k += blockDim.x) s_redf[k] = 0.0;
   __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NC; ++k) {
+    const uint32_t v = wave_sum(c[k]);
+    if (lane_id() == 0 && v) atomicAdd(&s_red[k], v);
+  }
+  // float sums: per-wave totals in LDS, combined by thread 0 in wave order
+  __shared__ double s_wave[AGENT_WAVES_MAX][3];
+#pragma unroll
+  for (int k = 0; k < ND; ++k) {
+    const double v = wave_sum_d(d[k]);
+    if (lane_id() == 0) s_wave[threadIdx.x >> 6][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < ND; ++k) {
+      double acc = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) acc += s_wave[w][k];
+      s_redf[k] = acc;
+    }
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) partials[(size_t)k * CVK_MAX_REDUCE_BLOCKS + blockIdx.x] = s_red[k];
+  for (int k = threadIdx.x; k < ND; k += blockDim.x) partials_f[(size_t)k * CVK_MAX_REDUCE_BLOCKS + blockIdx.x] = s_redf[k];
+}
 
-  const int64_t i = (int64_t)blockIdx.x * AGENT_BLOCK + threadIdx.x;
-  const bool valid = i < P.n;
-  const float tf = (float)t;
-  bool c_inf = false, c_sym = false, c_sev = false, c_crit = false, c_rec = false, c_death = false, c_kdeath = false;
-  uint32_t inf_v = CVK_VAR_NONE;
-  bool bit = false, mid_sev = false, mid_crit = false;
+// Sums the n_blocks partial values of every column (column-major: one wave per column,
+// coalesced loads); totals land in s_tot[NC] / s_totf[ND]. Float columns are summed in a
+// fixed lane/shuffle order, so results are deterministic.
+template <int NC, int ND>
+__device__ void finalize_partials(const uint32_t* partials, const double* partials_f, uint32_t n_blocks,
+                                  uint32_t* s_tot, double* s_totf) {
+  const int n_waves = (int)(blockDim.x >> 6);
+  const int w = threadIdx.x >> 6;
+  for (int k = w; k < NC + ND; k += n_waves) {
+    if (k < NC) {
+      const uint32_t* col = partials + (size_t)k * CVK_MAX_REDUCE_BLOCKS;
+      uint32_t acc = 0;
+      for (uint32_t b = lane_id(); b < n_blocks; b += 64) acc += col[b];
+      acc = wave_sum(acc);
+      if (lane_id() == 0) s_tot[k] = acc;
+    } else {
+      const double* col = partials_f + (size_t)(k - NC) * CVK_MAX_REDUCE_BLOCKS;
+      double acc = 0.0;
+      for (uint32_t b = lane_id(); b < n_blocks; b += 64) acc += col[b];
+      acc = wave_sum_d(acc);
+      if (lane_id() == 0) s_totf[k - NC] = acc;
+    }
+  }
+  __syncthreads();
+}
 
-  if (valid) {
-    uint32_t f = P.flags[i];
-    uint32_t vs = P.vstate[i];
-    const uint32_t f0 = f, vs0 = vs;
-    const bool is_exp = (f & CVK_EXPOSED) != 0;  // people.py:169 snapshot
-    const float d_inf = P.date_infectious[i];
-    const float d_rec = P.date_recovered[i];
-    const float d_dead = P.date_dead[i];
-    if (is_exp) {
-      // check_infectious, people.py:222-232
-      if (!(f & CVK_INFECTIOUS) && !isnan(d_inf) && tf >= d_inf) {
-        const uint32_t ev = exp_var(vs);
-        f = (f | CVK_INFECTIOUS) & ~INFVAR_MASK;
-        f |= ev << CVK_INFVAR_SHIFT;
-        if (ev < CVK_MAX_VARIANTS) vs |= 1u << (16 + ev);
-        c_inf = true;
-        inf_v = ev;
-      }
-      // check_symptomatic / severe / critical, people.py:235-253
-      if (!(f & CVK_SYMPTOMATIC)) {
-        const float d = P.date_symptomatic[i];
-        if (!isnan(d) && tf >= d) { f |= CVK_SYMPTOMATIC; c_sym = true; }
-      }
-      if (!(f & CVK_SEVERE)) {
-        const float d = P.date_severe[i];
-        if (!isnan(d) && tf >= d) { f |= CVK_SEVERE; c_sev = true; }
-      }
-      if (!(f & CVK_CRITICAL)) {
-        const float d = P.date_critical[i];
-        if (!isnan(d) && tf >= d) { f |= CVK_CRITICAL; c_crit = true; }
-      }
-      // check_recovery, people.py:256-291
-      if (!(f & CVK_RECOVERED) && !isnan(d_rec) && tf >= d_rec) {
-        const uint32_t ev = exp_var(vs);
-        f = (f & ~STATE_CLEAR_ON_RECOVERY) | CVK_RECOVERED;
-        f |= INFVAR_MASK;  // infectious_variant = nan
-        vs = (vs & ~0xFFFFFFu) | (ev << 4) | CVK_VAR_NONE;  // recovered_variant = exposed_variant; clear by-variant
-        if (prm.use_waning) f = (f | CVK_SUSCEPTIBLE) & ~CVK_DIAGNOSED;
-        c_rec = true;
-      }
-    }
-    // check_exit_iso, people.py:368-374 (not filtered by exposure)
-    if (f & CVK_ISOLATED) {
-      const float d = P.date_end_isolation[i];
-      if (!isnan(d) && tf >= d) f &= ~CVK_ISOLATED;
-    }
-    // check_death, people.py:294-312
-    if (is_exp && !(f & CVK_DEAD) && !isnan(d_dead) && tf >= d_dead) {
-      f |= CVK_DEAD;
-      if (f & CVK_DIAGNOSED) { f |= CVK_KNOWN_DEAD; c_kdeath = true; }
-      f &= ~STATE_CLEAR_ON_DEATH;
-      f |= INFVAR_MASK;
-      vs = (vs & ~0xFFu) | CVK_VAR_NONE | (CVK_VAR_NONE << 4);
-      c_death = true;
-    }
+// ===========================================================================
+// day_begin: [previous-day stocks + update_nab] + transitions + bitmaps + records,
+// with extra blocks resampling the dynamic layers.
+// ===========================================================================
+// stock columns (0..29) and day columns (30..46) of the partial rows
+constexpr int SC_ALIVE = 13, SC_EXPV = 14, SC_INFV = 14 + MAXV, N_STOCK_COLS = 14 + 2 * MAXV;
+constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV, N_AGENT_COLS = N_STOCK_COLS + N_DAY_COLS;
+static_assert(N_AGENT_COLS <= CVK_PARTIAL_COLS, "partial row too small");
 
-    // check_immunity, immunity.py:303-350 (no vaccine sources on this path)
-    if (prm.use_waning) {
-      const float peak = P.peak_nab[i];
-      if (peak != 0.0f) {  // agents never infected keep sus/symp/sev_imm == 0
-        const double nab = (double)P.nab[i];
-        const bool was_inf = tf >= d_rec;
-        const uint32_t rv = rec_var(vs);
-        for (int v = 0; v < prm.n_variants; ++v) {
-          double imm = 0.0;
-          if (was_inf && rv < CVK_MAX_VARIANTS) imm = (double)prm.immunity[v][rv];
-          const double eff = nab * imm;
-          const int64_t off = (int64_t)v * P.n + i;
-          P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
-          P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
-          P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
-        }
+struct ResampleLayer {
+  int32_t* p1;
+  int32_t* p2;
+  float* beta;
+  int64_t n;
+  uint32_t layer;
+  uint32_t pad;
+};
+
+struct DayBeginArgs {
+  int32_t t;
+  int32_t prologue;     // count stocks and update NAbs of day t-1 first
+  int32_t transitions;  // 0: count-only finish pass for day t
+  int32_t n_resample;
+  uint32_t agent_blocks;
+  uint32_t pad;
+  ResampleLayer rs[CVK_MAX_LAYERS];
+};
+
+__device__ inline void stock_counts(uint32_t f, uint32_t vs, int nv, uint32_t (&c)[N_AGENT_COLS]) {
+  const uint32_t stock_bits[13] = {CVK_SUSCEPTIBLE, CVK_EXPOSED, CVK_INFECTIOUS, CVK_SYMPTOMATIC, CVK_SEVERE,
+                                   CVK_CRITICAL, CVK_RECOVERED, CVK_DEAD, CVK_DIAGNOSED, CVK_KNOWN_DEAD,
+                                   CVK_QUARANTINED, CVK_ISOLATED, CVK_VACCINATED};
+#pragma unroll
+  for (int k = 0; k < 13; ++k) c[k] += (f & stock_bits[k]) ? 1u : 0u;
+  c[SC_ALIVE] += (f & CVK_DEAD) ? 0u : 1u;
+#pragma unroll
+  for (int v = 0; v < MAXV; ++v) {
+    if (v < nv) {
+      c[SC_EXPV + v] += (vs >> (8 + v)) & 1u;
+      c[SC_INFV + v] += (vs >> (16 + v)) & 1u;
+    }
+  }
+}
+
+__global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cvk_params prm, cvk_work W,
+                                                                DayBeginArgs a) {
+  __shared__ uint32_t s_red[N_AGENT_COLS];
+  __shared__ double s_redf[3];
+
+  if (blockIdx.x >= a.agent_blocks) {
+    // ---- dynamic-layer resample: iid uniform endpoints per edge, beta = 1 (base.py:1849-1876)
+    const int64_t stride = (int64_t)(gridDim.x - a.agent_blocks) * AGENT_BLOCK;
+    const int64_t start = (int64_t)(blockIdx.x - a.agent_blocks) * AGENT_BLOCK + threadIdx.x;
+    for (int r = 0; r < a.n_resample; ++r) {
+      const ResampleLayer& R = a.rs[r];
+      for (int64_t e = start; e < R.n; e += stride) {
+        const uint64_t ue = (uint64_t)e;
+        const u32x4 w = philox_seeded(prm.seed, (uint32_t)ue, (uint32_t)a.t, STREAM_RESAMPLE | R.layer,
+                                      (uint32_t)(ue >> 32));
+        R.p1[e] = (int32_t)below(w.x, w.y, (uint32_t)P.n);
+        R.p2[e] = (int32_t)below(w.z, w.w, (uint32_t)P.n);
+        R.beta[e] = 1.0f;
       }
     }
-
-    // viral-load phase (utils.py:58-78): early iff (t - start)/total < switch point
-    {
-      const float stop = isnan(d_dead) ? d_rec : d_dead;
-      const float total = stop - d_inf;
-      const float trans_day = prm.frac_time * total;
-      const float trans_point = trans_day > prm.high_cap ? prm.high_cap / total : prm.frac_time;
-      const bool early = (tf - d_inf) / total < trans_point;
-      f = early ? (f | CVK_EARLY) : (f & ~CVK_EARLY);
-    }
-    if (f != f0) P.flags[i] = f;
-    if (vs != vs0) P.vstate[i] = vs;
-    bit = (f & CVK_INFECTIOUS) && P.rel_trans[i] != 0.0f;
-    mid_sev = (f & CVK_SEVERE) != 0;
-    mid_crit = (f & CVK_CRITICAL) != 0;
+    return;
   }
 
-  // infectious bitmap: one 64-bit ballot per wave covers 64 consecutive agents
-  const uint64_t bits = __ballot(bit);
-  const int64_t wave = i >> 6;
-  if ((threadIdx.x & 63) == 0 && wave < (P.n + 63) / 64) ((uint64_t*)W.bitmap)[wave] = bits;
+  uint32_t c[N_AGENT_COLS];
+#pragma unroll
+  for (int k = 0; k < N_AGENT_COLS; ++k) c[k] = 0;
+  double d[3] = {0.0, 0.0, 0.0};
+  const int nv = prm.n_variants;
+  const int32_t t = a.t;
+  const float tf = (float)t;
+  const int64_t stride = (int64_t)a.agent_blocks * AGENT_BLOCK;
+  const int64_t n_waves_bits = (P.n + 63) / 64;
 
-  wave_count(s_cnt, 0, c_inf);
-  wave_count(s_cnt, 1, c_sym);
-  wave_count(s_cnt, 2, c_sev);
-  wave_count(s_cnt, 3, c_crit);
-  wave_count(s_cnt, 4, c_rec);
-  wave_count(s_cnt, 5, c_death);
-  wave_count(s_cnt, 6, c_kdeath);
-  wave_count(s_cnt, 7, mid_sev);
-  wave_count(s_cnt, 8, mid_crit);
-  for (int v = 0; v < prm.n_variants; ++v) wave_count(s_cnt, 9 + v, c_inf && inf_v == (uint32_t)v);
-  __syncthreads();
-  if (threadIdx.x < AGENT_NCOUNT && s_cnt[threadIdx.x]) {
+  for (int64_t base = (int64_t)blockIdx.x * AGENT_BLOCK; base < P.n; base += stride) {
+    const int64_t i = base + threadIdx.x;
+    const bool valid = i < P.n;
+    bool inf_bit = false, sus_bit = false;
+    if (valid) {
+      uint32_t f = P.flags[i];
+      uint32_t vs = P.vstate[i];
+      const uint32_t f0 = f, vs0 = vs;
+      if (a.prologue || !a.transitions) {
+        // ---- end of the previous day (or of day t for the finish pass): update_nab + stocks
+        const int32_t tc = a.transitions ? t - 1 : t;
+        if (prm.use_waning) {
+          const float peak = P.peak_nab[i];
+          float nab = P.nab[i];
+          if (peak != 0.0f) {  // has_nabs = true(peak_nab), sim.py:666-669; immunity.py:205-213
+            int32_t ts = tc - P.t_nab_event[i];
+            ts = ts < 0 ? 0 : (ts >= prm.nab_kin_len ? prm.nab_kin_len - 1 : ts);
+            nab = (float)((double)nab + prm.nab_kin[ts] * (double)peak);
+            nab = nab < 0.0f ? 0.0f : nab;
+            nab = nab > peak ? peak : nab;
+            P.nab[i] = nab;
+            for (int v = 0; v < nv; ++v) {  // population means of sus/symp_imm (sim.py:673-674)
+              d[1] += (double)P.sus_imm[(int64_t)v * P.n + i];
+              d[2] += (double)P.symp_imm[(int64_t)v * P.n + i];
+            }
+          }
+          if (!(f & CVK_DEAD)) d[0] += (double)nab;
+        }
+        stock_counts(f, vs, nv, c);
+      }
+      if (a.transitions) {
+        const bool is_exp = (f & CVK_EXPOSED) != 0;  // people.py:169 snapshot
+        const float d_inf = P.date_infectious[i];
+        const float d_rec = P.date_recovered[i];
+        const float d_dead = P.date_dead[i];
+        if (is_exp) {
+          // check_infectious, people.py:222-232
+          if (!(f & CVK_INFECTIOUS) && !isnan(d_inf) && tf >= d_inf) {
+            const uint32_t ev = exp_var(vs);
+            f = ((f | CVK_INFECTIOUS) & ~INFVAR_MASK) | (ev << CVK_INFVAR_SHIFT);
+            if (ev < MAXV) {
+              vs |= 1u << (16 + ev);
+#pragma unroll
+              for (int v = 0; v < MAXV; ++v) c[DC_BASE + 9 + v] += (ev == (uint32_t)v) ? 1u : 0u;
+            }
+            c[DC_BASE + 0] += 1;
+          }
+          // check_symptomatic / severe / critical, people.py:235-253
+          if (!(f & CVK_SYMPTOMATIC)) {
+            const float dd = P.date_symptomatic[i];
+            if (!isnan(dd) && tf >= dd) { f |= CVK_SYMPTOMATIC; c[DC_BASE + 1] += 1; }
+          }
+          if (!(f & CVK_SEVERE)) {
+            const float dd = P.date_severe[i];
+            if (!isnan(dd) && tf >= dd) { f |= CVK_SEVERE; c[DC_BASE + 2] += 1; }
+          }
+          if (!(f & CVK_CRITICAL)) {
+            const float dd = P.date_critical[i];
+            if (!isnan(dd) && tf >= dd) { f |= CVK_CRITICAL; c[DC_BASE + 3] += 1; }
+          }
+          // check_recovery, people.py:256-291
+          if (!(f & CVK_RECOVERED) && !isnan(d_rec) && tf >= d_rec) {
+            const uint32_t ev = exp_var(vs);
+            f = ((f & ~STATE_CLEAR_ON_RECOVERY) | CVK_RECOVERED) | INFVAR_MASK;
+            vs = (vs & ~0xFFFFFFu) | (ev << 4) | CVK_VAR_NONE;  // recovered_variant = exposed_variant
+            if (prm.use_waning) f = (f | CVK_SUSCEPTIBLE) & ~CVK_DIAGNOSED;
+            c[DC_BASE + 4] += 1;
+          }
+        }
+        // check_exit_iso, people.py:368-374 (not filtered by exposure)
+        if (f & CVK_ISOLATED) {
+          const float dd = P.date_end_isolation[i];
+          if (!isnan(dd) && tf >= dd) f &= ~CVK_ISOLATED;
+        }
+        // check_death, people.py:294-312
+        if (is_exp && !(f & CVK_DEAD) && !isnan(d_dead) && tf >= d_dead) {
+          f |= CVK_DEAD;
+          if (f & CVK_DIAGNOSED) { f |= CVK_KNOWN_DEAD; c[DC_BASE + 6] += 1; }
+          f = (f & ~STATE_CLEAR_ON_DEATH) | INFVAR_MASK;
+          vs = (vs & ~0xFFu) | CVK_VAR_NONE | (CVK_VAR_NONE << 4);
+          c[DC_BASE + 5] += 1;
+        }
+        // check_immunity, immunity.py:303-350 (no vaccine sources on this path)
+        if (prm.use_waning) {
+          const float peak = P.peak_nab[i];
+          if (peak != 0.0f) {  // agents never infected keep sus/symp/sev_imm == 0
+            const double nab = (double)P.nab[i];
+            const bool was_inf = tf >= d_rec;
+            const uint32_t rv = rec_var(vs);
+            for (int v = 0; v < nv; ++v) {
+              const double imm = (was_inf && rv < MAXV) ? (double)prm.immunity[v][rv] : 0.0;
+              const double eff = nab * imm;
+              const int64_t off = (int64_t)v * P.n + i;
+              P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
+              P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
+              P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
+            }
+          }
+        }
+        // viral-load phase (utils.py:58-78): early iff (t - start)/total < switch point
+        {
+          const float stop = isnan(d_dead) ? d_rec : d_dead;
+          const float total = stop - d_inf;
+          const float trans_day = prm.frac_time * total;
+          const float trans_point = trans_day > prm.high_cap ? prm.high_cap / total : prm.frac_time;
+          const bool early = (tf - d_inf) / total < trans_point;
+          f = early ? (f | CVK_EARLY) : (f & ~CVK_EARLY);
+        }
+        if (f != f0) P.flags[i] = f;
+        if (vs != vs0) P.vstate[i] = vs;
+        const float rt = P.rel_trans[i];
+        const float rs = P.rel_sus[i];
+        inf_bit = (f & CVK_INFECTIOUS) && rt != 0.0f;
+        sus_bit = (f & CVK_SUSCEPTIBLE) && rs != 0.0f;
+        const float val = (f & CVK_INFECTIOUS) ? rt : rs;
+        W.rec[i] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
+        c[DC_BASE + 7] += (f & CVK_SEVERE) ? 1u : 0u;
+        c[DC_BASE + 8] += (f & CVK_CRITICAL) ? 1u : 0u;
+      }
+    }
+    if (a.transitions) {
+      // one 64-bit ballot per wave covers 64 consecutive agents
+      const uint64_t ib = __ballot(inf_bit);
+      const uint64_t sb = __ballot(sus_bit);
+      const int64_t wave = i >> 6;
+      if (lane_id() == 0 && wave < n_waves_bits) {
+        ((uint64_t*)W.bitmap)[wave] = ib;
+        ((uint64_t*)W.susmap)[wave] = sb;
+      }
+    }
+  }
+
+  if (blockIdx.x == 0 && threadIdx.x == 0 && a.transitions) {
+    W.ctl[CVK_CTL_TOUCHED] = 0;       // the new day's target list
+    W.ctl[CVK_CTL_EDGE_TILES] = 0;    // the edge pass's tile counter
+  }
+  write_partials<N_AGENT_COLS, 3>(c, d, s_red, s_redf, W.partials, W.partials_f);
+}
+
+// Writes the day rows from the summed partial rows of day_begin (block-wide).
+struct DayReduce {
+  uint32_t n_blocks;  // 0: nothing to reduce
+  int32_t t;
+  int32_t prologue;
+  int32_t transitions;
+};
+
+__device__ void day_reduce(const cvk_work& W, const DayReduce& r) {
+  __shared__ uint32_t s_tot[N_AGENT_COLS];
+  __shared__ double s_totf[3];
+  finalize_partials<N_AGENT_COLS, 3>(W.partials, W.partials_f, r.n_blocks, s_tot, s_totf);
+  if (r.prologue || !r.transitions) {
+    const int32_t tc = r.transitions ? r.t - 1 : r.t;
+    uint32_t* row = day_row(W, tc);
+    for (int k = threadIdx.x; k < N_STOCK_COLS; k += blockDim.x) {
+      int col;
+      if (k < 13) col = CVK_R_N_SUSCEPTIBLE + k;
+      else if (k == SC_ALIVE) col = CVK_R_N_ALIVE_NAB;
+      else if (k < SC_INFV) col = CVK_R_BYVAR + CVK_RV_N_EXPOSED * MAXV + (k - SC_EXPV);
+      else col = CVK_R_BYVAR + CVK_RV_N_INFECTIOUS * MAXV + (k - SC_INFV);
+      row[col] = s_tot[k];
+    }
+    if (threadIdx.x < 3) day_row_f(W, tc)[threadIdx.x] = s_totf[threadIdx.x];
+  }
+  if (r.transitions) {
+    uint32_t* row = day_row(W, r.t);
     static constexpr int cols[9] = {CVK_R_NEW_INFECTIOUS, CVK_R_NEW_SYMPTOMATIC, CVK_R_NEW_SEVERE,
                                     CVK_R_NEW_CRITICAL, CVK_R_NEW_RECOVERIES, CVK_R_NEW_DEATHS,
                                     CVK_R_NEW_KNOWN_DEATHS, CVK_R_MID_SEVERE, CVK_R_MID_CRITICAL};
-    const int k = threadIdx.x;
-    const int col = k < 9 ? cols[k] : CVK_R_BYVAR + CVK_RV_NEW_INFECTIOUS * CVK_MAX_VARIANTS + (k - 9);
-    atomicAdd(&day_row(W, t)[col], s_cnt[k]);
+    for (int k = threadIdx.x; k < N_DAY_COLS; k += blockDim.x) {
+      const int col = k < 9 ? cols[k] : CVK_R_BYVAR + CVK_RV_NEW_INFECTIOUS * MAXV + (k - 9);
+      row[col] = s_tot[DC_BASE + k];
+    }
   }
+  __syncthreads();
 }
 
-// ===========================================================================
-// resample (Layer.update with frac = 1)
-// ===========================================================================
-__global__ void resample_kernel(int32_t* __restrict__ p1, int32_t* __restrict__ p2, float* __restrict__ beta,
-                                int64_t n_edges, uint32_t n_agents, uint64_t seed, int32_t t, uint32_t layer) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_edges) return;
-  const uint64_t ue = (uint64_t)e;
-  const u32x4 w = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(ue >> 32));
-  p1[e] = (int32_t)below(w.x, w.y, n_agents);
-  p2[e] = (int32_t)below(w.z, w.w, n_agents);
-  beta[e] = 1.0f;
-}
+__global__ __launch_bounds__(1024) void day_reduce_kernel(cvk_work W, DayReduce r) { day_reduce(W, r); }
 
 // ===========================================================================
 // choose: the k smallest 64-bit selection keys (philox word << 32 | agent) among
@@ -242,11 +430,11 @@ __global__ void choose_select_kernel(cvk_work W, int pass) {
   uint32_t remain = W.ctl[CVK_CTL_SEL_REMAIN];
   if (pass == 0) {
     uint64_t total = 0;
-    for (int d = 0; d < 256; ++d) total += W.select_hist[d];
+    for (int dd = 0; dd < 256; ++dd) total += W.select_hist[dd];
     if (remain >= total) {  // everyone eligible is chosen
       W.ctl[CVK_CTL_SEL_ALL] = 1;
       if (remain > total) W.ctl[CVK_CTL_SEL_SHORT] = 1;
-      for (int d = 0; d < 256; ++d) W.select_hist[d] = 0;
+      for (int dd = 0; dd < 256; ++dd) W.select_hist[dd] = 0;
       return;
     }
   }
@@ -254,9 +442,9 @@ __global__ void choose_select_kernel(cvk_work W, int pass) {
   const int shift = 56 - 8 * pass;
   uint32_t cum = 0;
   int digit = 255;
-  for (int d = 0; d < 256; ++d) {
-    const uint32_t h = W.select_hist[d];
-    if (cum + h >= remain) { digit = d; break; }
+  for (int dd = 0; dd < 256; ++dd) {
+    const uint32_t h = W.select_hist[dd];
+    if (cum + h >= remain) { digit = dd; break; }
     cum += h;
   }
   remain -= cum;
@@ -264,15 +452,14 @@ __global__ void choose_select_kernel(cvk_work W, int pass) {
   W.ctl[CVK_CTL_SEL_PREFIX] = (uint32_t)prefix;
   W.ctl[CVK_CTL_SEL_PREFIX + 1] = (uint32_t)(prefix >> 32);
   W.ctl[CVK_CTL_SEL_REMAIN] = remain;
-  for (int d = 0; d < 256; ++d) W.select_hist[d] = 0;
+  for (int dd = 0; dd < 256; ++dd) W.select_hist[dd] = 0;
 }
 
 __global__ __launch_bounds__(256) void choose_mark_kernel(cvk_people P, cvk_work W, uint64_t seed, int32_t t,
                                                           uint32_t stream_id, int32_t sus_only, uint32_t layer_code,
-                                                          uint32_t variant, int32_t count_imports, uint32_t k) {
+                                                          uint32_t variant) {
   const uint64_t thr = W.ctl[CVK_CTL_SEL_ALL] ? ~0ull
                                               : ((uint64_t)W.ctl[CVK_CTL_SEL_PREFIX + 1] << 32) | W.ctl[CVK_CTL_SEL_PREFIX];
-  if (count_imports && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&day_row(W, t)[CVK_R_N_IMPORTS], k);
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * 256) {
     if (sus_only && !(P.flags[i] & CVK_SUSCEPTIBLE)) continue;
     const uint64_t key = selection_key(seed, t, stream_id, (uint32_t)i);
@@ -300,43 +487,51 @@ struct EdgeArgs {
   EdgeLayer layer[CVK_MAX_LAYERS];
   int64_t tile_start[CVK_MAX_LAYERS + 1];
   int32_t n_layers, n_variants;
-  float beta_v[CVK_MAX_VARIANTS];
+  float beta_v[MAXV];
   float asymp_factor, vl_hi, vl_lo;
   int32_t use_sus_imm;
   uint64_t seed;
   int32_t t;
   int32_t bitmap_words;
-  const uint32_t* flags;
-  const float* rel_trans;
-  const float* rel_sus;
+  const uint64_t* rec;
   const float* sus_imm;
   int64_t n_agents;
   const uint32_t* bitmap;
+  const uint32_t* susmap;
   uint64_t* winner;
   int32_t* touched;
   uint32_t* touched_count;
+  uint32_t* tile_counter;  // zeroed by the agent pass of the day
+  cvk_work work;        // for the fused day reduction
+  DayReduce reduce;     // day_begin partial rows to reduce in block 0 (n_blocks == 0: none)
 };
 
-constexpr int EDGE_EPT = 8;  // edges per thread per tile (two 16-byte loads per endpoint array)
+typedef int v4i __attribute__((ext_vector_type(4)));
 
-// One directed slot (source -> target) of edge e of layer l: the float32 products of
-// compute_trans_sus (utils.py:88-89) and compute_infections (utils.py:117), then the
-// float64 uniform test (utils.py:123).
-__device__ inline void edge_slot(const EdgeArgs& a, const EdgeLayer& L, int l, int dir, int64_t e, int32_t src,
-                                 int32_t tgt) {
-  const uint32_t fs = a.flags[src];
+constexpr int EDGE_BLOCK = 1024;
+constexpr int EDGE_CHUNKS = 2;                        // 4-edge chunks per thread per tile
+constexpr int EDGE_TILE = EDGE_BLOCK * 4 * EDGE_CHUNKS;
+constexpr int EDGE_QCAP = 64 * 8;                     // per-wave queue: 4 edges x 2 directions x 64 lanes
+
+// A candidate slot (infectious source, susceptible target) of edge e of layer l: the float32
+// products of compute_trans_sus (utils.py:88-89) and compute_infections (utils.py:117), the
+// float64 uniform test (utils.py:123), and the first-occurrence dedup as an atomicMin.
+__device__ inline void edge_candidate(const EdgeArgs& a, const EdgeLayer& L, int l, int dir, int64_t e, int32_t src,
+                                      int32_t tgt) {
+  const uint64_t rsrc = a.rec[src];
+  const uint64_t rtgt = a.rec[tgt];
+  const uint32_t fs = (uint32_t)(rsrc >> 32);
+  const uint32_t ft = (uint32_t)(rtgt >> 32);
   const uint32_t v = inf_var(fs);
-  if (!(fs & CVK_INFECTIOUS) || v >= (uint32_t)a.n_variants) return;
-  float rt = a.rel_trans[src];                                   // * inf (== 1)
+  if (!(fs & CVK_INFECTIOUS) || v >= (uint32_t)a.n_variants || !(ft & CVK_SUSCEPTIBLE)) return;
+  float rt = __uint_as_float((uint32_t)rsrc);                   // rel_trans * inf (== 1)
   if (fs & CVK_QUARANTINED) rt = rt * L.quar_factor;             // * f_quar
   if (!(fs & CVK_SYMPTOMATIC)) rt = rt * a.asymp_factor;         // * f_asymp
   if (fs & CVK_ISOLATED) rt = rt * L.iso_factor;                 // * f_iso
   rt = rt * L.beta_layer;                                        // * beta_layer
   rt = rt * ((fs & CVK_EARLY) ? a.vl_hi : a.vl_lo);              // * viral_load
   if (rt == 0.0f) return;                                        // utils.py:116
-  const uint32_t ft = a.flags[tgt];
-  if (!(ft & CVK_SUSCEPTIBLE)) return;                            // rel_sus * sus == 0
-  float rs = a.rel_sus[tgt];
+  float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
   if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
   if (a.use_sus_imm) rs = rs * (1.0f - a.sus_imm[(int64_t)v * a.n_agents + tgt]);
   const float p = ((a.beta_v[v] * L.beta[e]) * rt) * rs;         // utils.py:117
@@ -357,63 +552,96 @@ __device__ inline void edge_slot(const EdgeArgs& a, const EdgeLayer& L, int l, i
 }
 
 template <bool LDS>
-__device__ inline bool infectious_bit(const EdgeArgs& a, const uint32_t* s_bits, int32_t i) {
+__device__ inline bool inf_bit(const EdgeArgs& a, const uint32_t* s_bits, uint32_t i) {
   const uint32_t w = LDS ? s_bits[i >> 5] : a.bitmap[i >> 5];
   return (w >> (i & 31)) & 1u;
 }
 
-template <bool LDS>
-__device__ inline void edge_quad(const EdgeArgs& a, const uint32_t* s_bits, const EdgeLayer& L, int l, int64_t e,
-                                 const int32_t (&x)[4], const int32_t (&y)[4], int cnt) {
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-    if (q < cnt && (uint32_t)x[q] < (uint32_t)a.n_agents && (uint32_t)y[q] < (uint32_t)a.n_agents) {
-      const bool bx = infectious_bit<LDS>(a, s_bits, x[q]);
-      const bool by = infectious_bit<LDS>(a, s_bits, y[q]);
-      if (bx) edge_slot(a, L, l, 0, e + q, x[q], y[q]);
-      if (by) edge_slot(a, L, l, 1, e + q, y[q], x[q]);
-    }
-  }
-}
+__device__ inline bool sus_bit(const EdgeArgs& a, uint32_t i) { return (a.susmap[i >> 5] >> (i & 31)) & 1u; }
 
-template <bool LDS, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void edge_pass_kernel(EdgeArgs a) {
-  extern __shared__ uint32_t s_bits[];
+template <bool LDS>
+__global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
+  extern __shared__ uint32_t s_mem[];
+  const int nbw = LDS ? a.bitmap_words : 0;
+  uint32_t* s_bits = s_mem;
+  uint32_t* s_q = s_mem + nbw + (threadIdx.x >> 6) * EDGE_QCAP;  // this wave's candidate queue
+  __shared__ uint32_t s_tile;
+  if (blockIdx.x == 0 && a.reduce.n_blocks) day_reduce(a.work, a.reduce);  // the agent pass's counts
   if (LDS) {
-    const int nw4 = a.bitmap_words >> 2;  // bitmap_words is a multiple of 4
     const uint4* src = (const uint4*)a.bitmap;
     uint4* dst = (uint4*)s_bits;
-    for (int j = threadIdx.x; j < nw4; j += BLOCK) dst[j] = src[j];
-    __syncthreads();
+    for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
-  const int64_t tile_elems = (int64_t)BLOCK * EDGE_EPT;
+  const uint32_t lane = lane_id();
+  const uint32_t n_agents = (uint32_t)a.n_agents;
   const int64_t n_tiles = a.tile_start[a.n_layers];
-  for (int64_t tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+  // tiles are handed out dynamically: blocks that finish early (or start late) take more
+  for (;;) {
+    __syncthreads();
+    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= n_tiles) break;
     int l = 0;
     while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
     const EdgeLayer& L = a.layer[l];
-    const int64_t e_tile = (tile - a.tile_start[l]) * tile_elems;
+    const int64_t e_tile = (tile - a.tile_start[l]) * EDGE_TILE;
+    int32_t x[EDGE_CHUNKS][4], y[EDGE_CHUNKS][4];
+    int cnt[EDGE_CHUNKS];
+    // stream the tile's endpoints: 16-byte loads, all issued before use
 #pragma unroll
-    for (int c = 0; c < EDGE_EPT / 4; ++c) {
-      const int64_t e = e_tile + ((int64_t)c * BLOCK + threadIdx.x) * 4;
-      int32_t x[4], y[4];
-      int cnt = 0;
+    for (int c = 0; c < EDGE_CHUNKS; ++c) {
+      const int64_t e = e_tile + ((int64_t)c * EDGE_BLOCK + threadIdx.x) * 4;
       if (e + 3 < L.n) {
-        const int4 vx = *(const int4*)(L.p1 + e);
-        const int4 vy = *(const int4*)(L.p2 + e);
-        x[0] = vx.x; x[1] = vx.y; x[2] = vx.z; x[3] = vx.w;
-        y[0] = vy.x; y[1] = vy.y; y[2] = vy.z; y[3] = vy.w;
-        cnt = 4;
-      } else if (e < L.n) {
-        cnt = (int)(L.n - e);
+        const v4i vx = __builtin_nontemporal_load((const v4i*)(L.p1 + e));
+        const v4i vy = __builtin_nontemporal_load((const v4i*)(L.p2 + e));
+        x[c][0] = vx[0]; x[c][1] = vx[1]; x[c][2] = vx[2]; x[c][3] = vx[3];
+        y[c][0] = vy[0]; y[c][1] = vy[1]; y[c][2] = vy[2]; y[c][3] = vy[3];
+        cnt[c] = 4;
+      } else {
+        cnt[c] = e < L.n ? (int)(L.n - e) : 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          x[q] = q < cnt ? L.p1[e + q] : 0;
-          y[q] = q < cnt ? L.p2[e + q] : 0;
+          x[c][q] = q < cnt[c] ? L.p1[e + q] : 0;
+          y[c][q] = q < cnt[c] ? L.p2[e + q] : 0;
         }
       }
-      if (cnt) edge_quad<LDS>(a, s_bits, L, l, e, x, y, cnt);
+    }
+#pragma unroll
+    for (int c = 0; c < EDGE_CHUNKS; ++c) {
+      // candidate bits (q*2 + dir): infectious source and susceptible target
+      uint32_t cand = 0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const uint32_t xi = (uint32_t)x[c][q], yi = (uint32_t)y[c][q];
+        // endpoints outside [0, n) are ignored (the reference leaves them undefined)
+        if (q < cnt[c] && xi < n_agents && yi < n_agents) {
+          if (inf_bit<LDS>(a, s_bits, xi) && sus_bit(a, yi)) cand |= 1u << (2 * q);
+          if (inf_bit<LDS>(a, s_bits, yi) && sus_bit(a, xi)) cand |= 2u << (2 * q);
+        }
+      }
+      // wave-local compaction of the candidates into the LDS queue (ballot + mbcnt)
+      uint32_t total = 0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const bool has = (cand >> b) & 1u;
+        const uint64_t m = __ballot(has);
+        if (has) s_q[total + lanes_below(m)] = ((uint32_t)(c * EDGE_BLOCK + threadIdx.x) * 4 + (b >> 1)) * 2 + (b & 1);
+        total += (uint32_t)__popcll(m);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      for (uint32_t j = lane; j < total; j += 64) {
+        const uint32_t code = s_q[j];
+        const int dir = code & 1;
+        const int64_t e = e_tile + (code >> 1);
+        const int32_t p1v = L.p1[e], p2v = L.p2[e];
+        edge_candidate(a, L, l, dir, e, dir ? p2v : p1v, dir ? p1v : p2v);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
   }
 }
@@ -431,12 +659,12 @@ struct InfectArgs {
 };
 
 constexpr int INFECT_BLOCK = 256;
-constexpr int INFECT_NCOUNT = 2 + 3 * CVK_MAX_VARIANTS;
+constexpr int N_INFECT_COLS = 2 + 3 * MAXV;
 
 __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
                                                                    InfectArgs ia) {
-  __shared__ uint32_t s_cnt[INFECT_NCOUNT];
-  if (threadIdx.x < INFECT_NCOUNT) s_cnt[threadIdx.x] = 0;
+  __shared__ uint32_t s_cnt[N_INFECT_COLS];
+  if (threadIdx.x < N_INFECT_COLS) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const int32_t t = ia.t;
   const uint32_t n_touched = W.ctl[CVK_CTL_TOUCHED];
@@ -444,29 +672,28 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
   const bool hosp_max = ia.allow_beds && prm.n_beds_hosp >= 0 && (int64_t)row[CVK_R_MID_SEVERE] > prm.n_beds_hosp;
   const bool icu_max = ia.allow_beds && prm.n_beds_icu >= 0 && (int64_t)row[CVK_R_MID_CRITICAL] > prm.n_beds_icu;
   const uint64_t seed = prm.seed;
-  const uint32_t n_rounds = (n_touched + gridDim.x * INFECT_BLOCK - 1) / (gridDim.x * INFECT_BLOCK);
+  const uint32_t stride = gridDim.x * INFECT_BLOCK;
+  const uint32_t n_rounds = (n_touched + stride - 1) / stride;
   for (uint32_t r = 0; r < n_rounds; ++r) {
-    const uint32_t j = (r * gridDim.x + blockIdx.x) * INFECT_BLOCK + threadIdx.x;
+    const uint32_t j = r * stride + blockIdx.x * INFECT_BLOCK + threadIdx.x;
     bool inf = false, reinf = false, symp = false, sev = false;
-    uint32_t v = 0;
+    uint32_t v = 0, l = 0;
+    int32_t source = -1, i = 0;
     if (j < n_touched) {
-      const int32_t i = W.touched[j];
+      i = W.touched[j];
       const uint64_t key = W.winner[i];
       W.winner[i] = KEY_NONE;
       uint32_t f = P.flags[i];
       if (key != KEY_NONE && (f & CVK_SUSCEPTIBLE)) {  // people.py:470 keeps susceptibles only
         v = key_variant(key);
-        const uint32_t cls = key_class(key);
-        const uint32_t l = key_layer(key);
-        int32_t source = -1;
-        if (cls == 1) {
+        l = key_layer(key);
+        if (key_class(key) == 1) {
           const int64_t e = (int64_t)key_index(key);
           source = key_dir(key) ? ia.p2[l][e] : ia.p1[l][e];
         }
         inf = true;
         // breakthrough, people.py:486-491
-        const bool breakthrough = P.peak_nab[i] != 0.0f;
-        if (breakthrough) {
+        if (P.peak_nab[i] != 0.0f) {
           const int32_t nb = P.n_breakthroughs[i];
           if (nb == 0) P.rel_trans[i] = P.rel_trans[i] * prm.trans_redux;
           P.n_breakthroughs[i] = nb + 1;
@@ -477,18 +704,6 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
         P.vstate[i] = (P.vstate[i] & ~0xFu) | v | (1u << (8 + v));
         P.n_infections[i] += 1;
         reinf = !isnan(P.date_recovered[i]);
-        // infection log, people.py:509-511
-        const unsigned long long pos = atomicAdd((unsigned long long*)&W.ctl[CVK_CTL_LOG_COUNT], 1ull);
-        if ((int64_t)pos < W.log_capacity) {
-          int4 rec;
-          rec.x = source;
-          rec.y = i;
-          rec.z = t;
-          rec.w = (int32_t)(l | (v << 8));
-          ((int4*)W.log)[pos] = rec;
-        } else {
-          W.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
-        }
         // outcome tree, people.py:513-579
         const float d_e2i = (float)draw_dur(prm.dur[CVK_DUR_EXP2INF], seed, (uint32_t)i, t, SLOT_EXP2INF);
         P.dur_exp2inf[i] = d_e2i;
@@ -507,9 +722,9 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
         symp = u_symp < (double)sp;
         int imm_class = 0;  // 0 asymptomatic, 1 mild, 2 severe (rel_imm_symp)
         if (!symp) {
-          const double d = draw_dur(prm.dur[CVK_DUR_ASYM2REC], seed, (uint32_t)i, t, SLOT_DUR_A);
-          date_rec = (float)((double)date_inf + d);
-          P.dur_disease[i] = (float)((double)d_e2i + d);
+          const double dd = draw_dur(prm.dur[CVK_DUR_ASYM2REC], seed, (uint32_t)i, t, SLOT_DUR_A);
+          date_rec = (float)((double)date_inf + dd);
+          P.dur_disease[i] = (float)((double)d_e2i + dd);
         } else {
           const float d_i2s = (float)draw_dur(prm.dur[CVK_DUR_INF2SYM], seed, (uint32_t)i, t, SLOT_DUR_A);
           P.dur_inf2sym[i] = d_i2s;
@@ -519,9 +734,9 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
           sev = u_sev < (double)sevp;
           if (!sev) {
             imm_class = 1;
-            const double d = draw_dur(prm.dur[CVK_DUR_MILD2REC], seed, (uint32_t)i, t, SLOT_DUR_B);
-            date_rec = (float)((double)date_sym + d);
-            P.dur_disease[i] = (float)((double)(d_e2i + d_i2s) + d);
+            const double dd = draw_dur(prm.dur[CVK_DUR_MILD2REC], seed, (uint32_t)i, t, SLOT_DUR_B);
+            date_rec = (float)((double)date_sym + dd);
+            P.dur_disease[i] = (float)((double)(d_e2i + d_i2s) + dd);
           } else {
             imm_class = 2;
             const float d_s2s = (float)draw_dur(prm.dur[CVK_DUR_SYM2SEV], seed, (uint32_t)i, t, SLOT_DUR_B);
@@ -531,9 +746,9 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
             draw_pair(seed, (uint32_t)i, t, SLOT_BERN_CRIT_DEATH, u_crit, u_death);
             const float critp = (prm.rel_crit_prob[v] * P.crit_prob[i]) * (hosp_max ? prm.no_hosp_factor : 1.0f);
             if (!(u_crit < (double)critp)) {
-              const double d = draw_dur(prm.dur[CVK_DUR_SEV2REC], seed, (uint32_t)i, t, SLOT_DUR_C);
-              date_rec = (float)((double)date_sev + d);
-              P.dur_disease[i] = (float)((double)((d_e2i + d_i2s) + d_s2s) + d);
+              const double dd = draw_dur(prm.dur[CVK_DUR_SEV2REC], seed, (uint32_t)i, t, SLOT_DUR_C);
+              date_rec = (float)((double)date_sev + dd);
+              P.dur_disease[i] = (float)((double)((d_e2i + d_i2s) + d_s2s) + dd);
             } else {
               const float d_s2c = (float)draw_dur(prm.dur[CVK_DUR_SEV2CRIT], seed, (uint32_t)i, t, SLOT_DUR_C);
               P.dur_sev2crit[i] = d_s2c;
@@ -542,13 +757,13 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
               const float deathp = (prm.rel_death_prob[v] * P.death_prob[i]) * (icu_max ? prm.no_icu_factor : 1.0f);
               const float dsum = ((d_e2i + d_i2s) + d_s2s) + d_s2c;
               if (!(u_death < (double)deathp)) {
-                const double d = draw_dur(prm.dur[CVK_DUR_CRIT2REC], seed, (uint32_t)i, t, SLOT_DUR_D);
-                date_rec = (float)((double)date_crit + d);
-                P.dur_disease[i] = (float)((double)dsum + d);
+                const double dd = draw_dur(prm.dur[CVK_DUR_CRIT2REC], seed, (uint32_t)i, t, SLOT_DUR_D);
+                date_rec = (float)((double)date_crit + dd);
+                P.dur_disease[i] = (float)((double)dsum + dd);
               } else {
-                const double d = draw_dur(prm.dur[CVK_DUR_CRIT2DIE], seed, (uint32_t)i, t, SLOT_DUR_D);
-                P.date_dead[i] = (float)((double)date_crit + d);
-                P.dur_disease[i] = (float)((double)dsum + d);
+                const double dd = draw_dur(prm.dur[CVK_DUR_CRIT2DIE], seed, (uint32_t)i, t, SLOT_DUR_D);
+                P.date_dead[i] = (float)((double)date_crit + dd);
+                P.dur_disease[i] = (float)((double)dsum + dd);
               }
             }
           }
@@ -560,118 +775,59 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
             P.peak_nab[i] = P.peak_nab[i] * prm.nab_boost;
           } else {
             const double init_nab = draw_dur(prm.nab_init, seed, (uint32_t)i, t, SLOT_NAB);
-            const double peak = (pow(2.0, init_nab) * prm.rel_imm_symp[imm_class]) * prm.nab_norm;
-            P.peak_nab[i] = (float)peak;
+            P.peak_nab[i] = (float)((pow(2.0, init_nab) * prm.rel_imm_symp[imm_class]) * prm.nab_norm);
           }
           P.t_nab_event[i] = t;
         }
       }
     }
-    if (ia.count_flows) {
-      wave_count(s_cnt, 0, inf);
-      wave_count(s_cnt, 1, inf && reinf);
+    // infection log, people.py:509-511: one wave-aggregated append per round
+    const uint64_t m = __ballot(inf);
+    if (m) {
+      unsigned long long base = 0;
+      if (lane_id() == 0) base = atomicAdd((unsigned long long*)&W.ctl[CVK_CTL_LOG_COUNT], (unsigned long long)__popcll(m));
+      base = __shfl(base, 0, 64);
+      if (inf) {
+        const unsigned long long pos = base + lanes_below(m);
+        if ((int64_t)pos < W.log_capacity) {
+          int4 rec;
+          rec.x = source;
+          rec.y = i;
+          rec.z = t;
+          rec.w = (int32_t)(l | (v << 8));
+          ((int4*)W.log)[pos] = rec;
+        } else {
+          W.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
+        }
+      }
+    }
+    if (ia.count_flows && m) {
+      const uint32_t n_inf = (uint32_t)__popcll(m);
+      const uint32_t n_reinf = (uint32_t)__popcll(__ballot(inf && reinf));
+      if (lane_id() == 0) {
+        atomicAdd(&s_cnt[0], n_inf);
+        if (n_reinf) atomicAdd(&s_cnt[1], n_reinf);
+      }
       for (int vv = 0; vv < prm.n_variants; ++vv) {
-        wave_count(s_cnt, 2 + vv, inf && v == (uint32_t)vv);
-        wave_count(s_cnt, 2 + CVK_MAX_VARIANTS + vv, inf && symp && v == (uint32_t)vv);
-        wave_count(s_cnt, 2 + 2 * CVK_MAX_VARIANTS + vv, inf && sev && v == (uint32_t)vv);
+        const uint32_t a0 = (uint32_t)__popcll(__ballot(inf && v == (uint32_t)vv));
+        const uint32_t a1 = (uint32_t)__popcll(__ballot(inf && symp && v == (uint32_t)vv));
+        const uint32_t a2 = (uint32_t)__popcll(__ballot(inf && sev && v == (uint32_t)vv));
+        if (lane_id() == 0) {
+          if (a0) atomicAdd(&s_cnt[2 + vv], a0);
+          if (a1) atomicAdd(&s_cnt[2 + MAXV + vv], a1);
+          if (a2) atomicAdd(&s_cnt[2 + 2 * MAXV + vv], a2);
+        }
       }
     }
   }
   __syncthreads();
-  if (ia.count_flows && threadIdx.x < INFECT_NCOUNT && s_cnt[threadIdx.x]) {
+  if (ia.count_flows && threadIdx.x < N_INFECT_COLS && s_cnt[threadIdx.x]) {
     const int k = threadIdx.x;
     int col;
     if (k == 0) col = CVK_R_NEW_INFECTIONS;
     else if (k == 1) col = CVK_R_NEW_REINFECTIONS;
-    else {
-      const int kk = k - 2;
-      const int which = kk / CVK_MAX_VARIANTS;  // 0 infections, 1 symptomatic, 2 severe
-      const int vv = kk % CVK_MAX_VARIANTS;
-      col = CVK_R_BYVAR + which * CVK_MAX_VARIANTS + vv;
-    }
+    else col = CVK_R_BYVAR + ((k - 2) / MAXV) * MAXV + (k - 2) % MAXV;  // infections, symptomatic, severe
     atomicAdd(&day_row(W, t)[col], s_cnt[k]);
-  }
-}
-
-__global__ void reset_touched_kernel(cvk_work W) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) W.ctl[CVK_CTL_TOUCHED] = 0;
-}
-
-// ===========================================================================
-// count_pass (end of day): stocks, update_nab, population immunity sums
-// ===========================================================================
-constexpr int COUNT_BLOCK = 256;
-constexpr int COUNT_NSTOCK = 14;  // 13 stocks + alive
-constexpr int COUNT_NCOUNT = COUNT_NSTOCK + 2 * CVK_MAX_VARIANTS;
-
-__global__ __launch_bounds__(COUNT_BLOCK) void count_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
-                                                                 int32_t t) {
-  __shared__ uint32_t s_cnt[COUNT_NCOUNT];
-  __shared__ double s_sum[3];
-  if (threadIdx.x < COUNT_NCOUNT) s_cnt[threadIdx.x] = 0;
-  if (threadIdx.x < 3) s_sum[threadIdx.x] = 0.0;
-  __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * COUNT_BLOCK + threadIdx.x;
-  const bool valid = i < P.n;
-  uint32_t f = 0, vs = 0;
-  double nab_sum = 0.0, sus_sum = 0.0, symp_sum = 0.0;
-  if (valid) {
-    f = P.flags[i];
-    vs = P.vstate[i];
-    if (prm.use_waning) {
-      const float peak = P.peak_nab[i];
-      float nab = P.nab[i];
-      if (peak != 0.0f) {  // update_nab over has_nabs = true(peak_nab), sim.py:666-669
-        int32_t ts = t - P.t_nab_event[i];
-        ts = ts < 0 ? 0 : (ts >= prm.nab_kin_len ? prm.nab_kin_len - 1 : ts);
-        nab = (float)((double)nab + prm.nab_kin[ts] * (double)peak);
-        nab = nab < 0.0f ? 0.0f : nab;
-        nab = nab > peak ? peak : nab;
-        P.nab[i] = nab;
-        for (int v = 0; v < prm.n_variants; ++v) {
-          sus_sum += (double)P.sus_imm[(int64_t)v * P.n + i];
-          symp_sum += (double)P.symp_imm[(int64_t)v * P.n + i];
-        }
-      }
-      if (!(f & CVK_DEAD)) nab_sum = (double)nab;
-    }
-  }
-  const uint32_t stock_bits[13] = {CVK_SUSCEPTIBLE, CVK_EXPOSED, CVK_INFECTIOUS, CVK_SYMPTOMATIC, CVK_SEVERE,
-                                   CVK_CRITICAL, CVK_RECOVERED, CVK_DEAD, CVK_DIAGNOSED, CVK_KNOWN_DEAD,
-                                   CVK_QUARANTINED, CVK_ISOLATED, CVK_VACCINATED};
-#pragma unroll
-  for (int k = 0; k < 13; ++k) wave_count(s_cnt, k, valid && (f & stock_bits[k]));
-  wave_count(s_cnt, 13, valid && !(f & CVK_DEAD));
-  for (int v = 0; v < prm.n_variants; ++v) {
-    wave_count(s_cnt, COUNT_NSTOCK + v, valid && ((vs >> (8 + v)) & 1u));
-    wave_count(s_cnt, COUNT_NSTOCK + CVK_MAX_VARIANTS + v, valid && ((vs >> (16 + v)) & 1u));
-  }
-  if (prm.use_waning) {
-    // wave reduction of the three sums, then one LDS atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-      nab_sum += __shfl_down(nab_sum, off, 64);
-      sus_sum += __shfl_down(sus_sum, off, 64);
-      symp_sum += __shfl_down(symp_sum, off, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      atomicAdd(&s_sum[0], nab_sum);
-      atomicAdd(&s_sum[1], sus_sum);
-      atomicAdd(&s_sum[2], symp_sum);
-    }
-  }
-  __syncthreads();
-  uint32_t* row = day_row(W, t);
-  if (threadIdx.x < COUNT_NCOUNT && s_cnt[threadIdx.x]) {
-    const int k = threadIdx.x;
-    int col;
-    if (k < 13) col = CVK_R_N_SUSCEPTIBLE + k;
-    else if (k == 13) col = CVK_R_N_ALIVE_NAB;
-    else if (k < COUNT_NSTOCK + CVK_MAX_VARIANTS) col = CVK_R_BYVAR + CVK_RV_N_EXPOSED * CVK_MAX_VARIANTS + (k - COUNT_NSTOCK);
-    else col = CVK_R_BYVAR + CVK_RV_N_INFECTIOUS * CVK_MAX_VARIANTS + (k - COUNT_NSTOCK - CVK_MAX_VARIANTS);
-    atomicAdd(&row[col], s_cnt[k]);
-  }
-  if (prm.use_waning && threadIdx.x < 3 && s_sum[threadIdx.x] != 0.0) {
-    atomicAdd(&day_row_f(W, t)[threadIdx.x], s_sum[threadIdx.x]);
   }
 }
 
@@ -711,10 +867,10 @@ static bool people_ok(const cvk_people* P) {
 
 static bool work_ok(const cvk_work* W) {
   return W && W->winner && W->touched && W->bitmap && W->ctl && W->results && W->results_f && W->select_hist &&
-         W->npts > 0 && (W->log_capacity == 0 || W->log);
+         W->susmap && W->rec && W->partials && W->partials_f && W->npts > 0 && (W->log_capacity == 0 || W->log);
 }
 
-static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* W, int32_t t, const char* who) {
+static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* W, int32_t t) {
   if (!people_ok(P)) return fail(CVK_E_INVALID, "people struct has a null pointer or n <= 0");
   if (!prm || prm->n_agents != P->n) return fail(CVK_E_INVALID, "params.n_agents does not match people.n");
   if (P->n >= (1ll << 31) - 1) return fail(CVK_E_INVALID, "more than 2^31-2 agents");
@@ -724,8 +880,9 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
   if (prm->use_waning && (!prm->nab_kin || prm->nab_kin_len <= 0))
     return fail(CVK_E_INVALID, "use_waning requires a nab_kin table");
   if (!work_ok(W)) return fail(CVK_E_INVALID, "work struct has a null pointer");
+  if (((uintptr_t)W->bitmap | (uintptr_t)W->susmap | (uintptr_t)W->rec) & 15)
+    return fail(CVK_E_ALIGN, "bitmap, susmap and rec must be 16-byte aligned");
   if (t < 0 || t >= W->npts) return fail(CVK_E_INVALID, "day index outside the results table");
-  (void)who;
   return CVK_OK;
 }
 
@@ -734,12 +891,58 @@ static int grid_for(int64_t n, int block) {
   return (int)(g < 1 ? 1 : g);
 }
 
+static uint32_t agent_blocks_for(int64_t n) {
+  return (uint32_t)std::min<int64_t>(grid_for(n, AGENT_BLOCK),
+                                     std::min<int64_t>(CVK_MAX_REDUCE_BLOCKS, 4 * std::max(caps().n_cu, 1)));
+}
+
+static int day_reduce_launch(cvk_work* work, const DayReduce& r, hipStream_t st) {
+  if (!r.n_blocks) return CVK_OK;
+  hipLaunchKernelGGL(day_reduce_kernel, dim3(1), dim3(1024), 0, st, *work, r);
+  return check_launch("day_reduce");
+}
+
+static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
+                            cvk_work* work, int32_t t, int prologue, int transitions, int resample, hipStream_t st,
+                            DayReduce* reduce_out) {
+  DayBeginArgs a;
+  memset(&a, 0, sizeof(a));
+  a.t = t;
+  a.prologue = prologue;
+  a.transitions = transitions;
+  a.agent_blocks = agent_blocks_for(people->n);
+  int64_t max_dyn = 0;
+  if (resample) {
+    for (int l = 0; l < params->n_layers; ++l) {
+      const cvk_layer& L = layers[l];
+      if (!L.dynamic || L.n_edges <= 0) continue;
+      if (!L.p1 || !L.p2 || !L.beta) return fail(CVK_E_INVALID, "dynamic layer has a null pointer");
+      ResampleLayer& R = a.rs[a.n_resample++];
+      R.p1 = L.p1;
+      R.p2 = L.p2;
+      R.beta = L.beta;
+      R.n = L.n_edges;
+      R.layer = (uint32_t)l;
+      max_dyn = std::max(max_dyn, L.n_edges);
+    }
+  }
+  const uint32_t rs_blocks =
+      a.n_resample ? (uint32_t)std::min<int64_t>(grid_for(max_dyn, AGENT_BLOCK), 4 * std::max(caps().n_cu, 1)) : 0u;
+  hipLaunchKernelGGL(day_begin_kernel, dim3(a.agent_blocks + rs_blocks), dim3(AGENT_BLOCK), 0, st, *people, *params,
+                     *work, a);
+  reduce_out->n_blocks = a.agent_blocks;
+  reduce_out->t = t;
+  reduce_out->prologue = prologue;
+  reduce_out->transitions = transitions;
+  return check_launch("day_begin");
+}
+
 }  // namespace cvk
 
 using namespace cvk;
 
 extern "C" const char* cvk_last_error(void) { return last_error_buffer(); }
-extern "C" int cvk_version(void) { return 1; }
+extern "C" int cvk_version(void) { return 2; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
 extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
@@ -777,13 +980,33 @@ extern "C" int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winn
   return CVK_OK;
 }
 
+extern "C" int cvk_workspace_sizes2(int64_t n_agents, int64_t* rec_bytes, int64_t* partials_bytes,
+                                    int64_t* partials_f_bytes) {
+  if (n_agents <= 0) return fail(CVK_E_INVALID, "cvk_workspace_sizes2: n_agents must be positive");
+  if (rec_bytes) *rec_bytes = n_agents * 8;
+  if (partials_bytes) *partials_bytes = (int64_t)CVK_MAX_REDUCE_BLOCKS * CVK_PARTIAL_COLS * 4;
+  if (partials_f_bytes) *partials_f_bytes = (int64_t)CVK_MAX_REDUCE_BLOCKS * CVK_RF_COLS * 8;
+  return CVK_OK;
+}
+
 extern "C" int cvk_agent_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
                               void* stream) {
-  int rc = validate(people, params, work, t, "cvk_agent_pass");
+  int rc = validate(people, params, work, t);
   if (rc) return rc;
-  hipLaunchKernelGGL(agent_pass_kernel, dim3(grid_for(people->n, AGENT_BLOCK)), dim3(AGENT_BLOCK), 0,
-                     (hipStream_t)stream, *people, *params, *work, t);
-  return check_launch("cvk_agent_pass");
+  DayReduce r;
+  rc = day_begin_launch(people, nullptr, params, work, t, 0, 1, 0, (hipStream_t)stream, &r);
+  if (rc) return rc;
+  return day_reduce_launch(work, r, (hipStream_t)stream);
+}
+
+extern "C" int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
+                              void* stream) {
+  int rc = validate(people, params, work, t);
+  if (rc) return rc;
+  DayReduce r;
+  rc = day_begin_launch(people, nullptr, params, work, t, 0, 0, 0, (hipStream_t)stream, &r);
+  if (rc) return rc;
+  return day_reduce_launch(work, r, (hipStream_t)stream);
 }
 
 extern "C" int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
@@ -794,20 +1017,35 @@ extern "C" int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, c
   if (params->n_agents <= 0 || params->n_agents >= (1ll << 31))
     return fail(CVK_E_INVALID, "cvk_resample_layer: bad n_agents");
   if (layer->n_edges == 0) return CVK_OK;
-  hipLaunchKernelGGL(resample_kernel, dim3(grid_for(layer->n_edges, 256)), dim3(256), 0, (hipStream_t)stream,
-                     layer->p1, layer->p2, layer->beta, layer->n_edges, (uint32_t)params->n_agents, params->seed, t,
-                     (uint32_t)layer_index);
+  // only resample blocks: agent_blocks = 0
+  DayBeginArgs a;
+  memset(&a, 0, sizeof(a));
+  a.t = t;
+  a.n_resample = 1;
+  a.agent_blocks = 0;
+  a.rs[0].p1 = layer->p1;
+  a.rs[0].p2 = layer->p2;
+  a.rs[0].beta = layer->beta;
+  a.rs[0].n = layer->n_edges;
+  a.rs[0].layer = (uint32_t)layer_index;
+  cvk_people P;
+  memset(&P, 0, sizeof(P));
+  P.n = params->n_agents;
+  cvk_work W;
+  memset(&W, 0, sizeof(W));
+  const int blocks = (int)std::min<int64_t>(grid_for(layer->n_edges, AGENT_BLOCK), 4 * std::max(caps().n_cu, 1));
+  hipLaunchKernelGGL(day_begin_kernel, dim3(blocks), dim3(AGENT_BLOCK), 0, (hipStream_t)stream, P, *params, W, a);
   return check_launch("cvk_resample_layer");
 }
 
 static int choose_impl(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
                        uint32_t stream_id, int64_t k, int32_t susceptible_only, int32_t layer_code, int32_t variant,
-                       int32_t count_imports, hipStream_t st) {
+                       hipStream_t st) {
   if (k < 0 || k > people->n) return fail(CVK_E_INVALID, "cvk_choose: k outside [0, n]");
   if (layer_code < 0 || layer_code > 15 || variant < 0 || variant >= params->n_variants)
     return fail(CVK_E_INVALID, "cvk_choose: bad layer code or variant");
   if (k == 0) return CVK_OK;
-  const int grid = (int)std::min<int64_t>(grid_for(people->n, 256), 4 * std::max(caps().n_cu, 1) * 8);
+  const int grid = (int)std::min<int64_t>(grid_for(people->n, 256), 8 * std::max(caps().n_cu, 1));
   hipLaunchKernelGGL(choose_init_kernel, dim3(1), dim3(256), 0, st, *work, (uint32_t)k);
   for (int pass = 0; pass < 8; ++pass) {
     hipLaunchKernelGGL(choose_hist_kernel, dim3(grid), dim3(256), 0, st, *people, *work, params->seed, t, stream_id,
@@ -815,48 +1053,74 @@ static int choose_impl(const cvk_people* people, const cvk_params* params, cvk_w
     hipLaunchKernelGGL(choose_select_kernel, dim3(1), dim3(64), 0, st, *work, pass);
   }
   hipLaunchKernelGGL(choose_mark_kernel, dim3(grid), dim3(256), 0, st, *people, *work, params->seed, t, stream_id,
-                     susceptible_only, (uint32_t)layer_code, (uint32_t)variant, count_imports, (uint32_t)k);
+                     susceptible_only, (uint32_t)layer_code, (uint32_t)variant);
   return check_launch("cvk_choose");
 }
 
 extern "C" int cvk_choose(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
                           uint32_t stream_id, int64_t k, int32_t susceptible_only, int32_t layer_code,
                           int32_t variant, void* stream) {
-  int rc = validate(people, params, work, t, "cvk_choose");
+  int rc = validate(people, params, work, t);
   if (rc) return rc;
-  return choose_impl(people, params, work, t, stream_id, k, susceptible_only, layer_code, variant, 0,
+  return choose_impl(people, params, work, t, stream_id, k, susceptible_only, layer_code, variant,
                      (hipStream_t)stream);
 }
 
+// Dynamic LDS budget of the edge pass: the device's per-workgroup LDS minus the kernel's static LDS.
+static int edge_dynamic_lds(bool lds_variant, int64_t* budget) {
+  DeviceCaps& c = caps();
+  static thread_local int configured_device = -1;
+  static thread_local int64_t budgets[2] = {0, 0};
+  if (configured_device != c.device) {
+    const void* fns[2] = {(const void*)edge_pass_kernel<false>, (const void*)edge_pass_kernel<true>};
+    for (int j = 0; j < 2; ++j) {
+      hipFuncAttributes attr;
+      if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
+        return fail(CVK_E_HIP, "cannot query the edge pass attributes");
+      const int64_t b = (int64_t)c.max_lds - (int64_t)attr.sharedSizeBytes;
+      if (b <= 0 || hipFuncSetAttribute(fns[j], hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess)
+        return fail(CVK_E_HIP, "cannot raise the LDS limit of the edge pass");
+      budgets[j] = b;
+    }
+    configured_device = c.device;
+  }
+  *budget = budgets[lds_variant ? 1 : 0];
+  return CVK_OK;
+}
+
 static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
-                          int32_t t, const double* const* uniforms, hipStream_t st) {
+                          int32_t t, const double* const* uniforms, hipStream_t st, const DayReduce* reduce) {
   EdgeArgs a;
   memset(&a, 0, sizeof(a));
+  a.work = *work;
+  if (reduce) a.reduce = *reduce;
   a.n_layers = params->n_layers;
   a.n_variants = params->n_variants;
-  for (int v = 0; v < CVK_MAX_VARIANTS; ++v) a.beta_v[v] = params->beta_v[v];
+  for (int v = 0; v < MAXV; ++v) a.beta_v[v] = params->beta_v[v];
   a.asymp_factor = params->asymp_factor;
   a.vl_hi = params->vl_hi;
   a.vl_lo = params->vl_lo;
   a.use_sus_imm = params->use_waning;
   a.seed = params->seed;
   a.t = t;
-  a.flags = people->flags;
-  a.rel_trans = people->rel_trans;
-  a.rel_sus = people->rel_sus;
+  a.rec = work->rec;
   a.sus_imm = people->sus_imm;
   a.n_agents = people->n;
   a.bitmap = work->bitmap;
+  a.susmap = work->susmap;
   a.bitmap_words = (int32_t)bitmap_words_for(people->n);
   a.winner = work->winner;
   a.touched = work->touched;
   a.touched_count = work->ctl + CVK_CTL_TOUCHED;
+  a.tile_counter = work->ctl + CVK_CTL_EDGE_TILES;
 
   DeviceCaps& c = caps();
-  const int64_t lds_bytes = (int64_t)a.bitmap_words * 4;
-  const bool use_lds = lds_bytes <= (int64_t)c.max_lds - 1024;
-  const int block = use_lds ? 1024 : 256;
-  const int64_t tile_elems = (int64_t)block * EDGE_EPT;
+  const int64_t queue_bytes = (int64_t)(EDGE_BLOCK / 64) * EDGE_QCAP * 4;
+  const int64_t lds_bytes = (int64_t)a.bitmap_words * 4 + queue_bytes;
+  int64_t budget = 0;
+  int rc0 = edge_dynamic_lds(true, &budget);
+  if (rc0) return rc0;
+  const bool use_lds = lds_bytes <= budget;
   int64_t tiles = 0;
   for (int l = 0; l < params->n_layers; ++l) {
     const cvk_layer& L = layers[l];
@@ -872,34 +1136,26 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     a.layer[l].iso_factor = params->iso_factor[l];
     a.layer[l].quar_factor = params->quar_factor[l];
     a.tile_start[l] = tiles;
-    tiles += (L.n_edges + tile_elems - 1) / tile_elems;
+    tiles += (L.n_edges + EDGE_TILE - 1) / EDGE_TILE;
   }
   a.tile_start[params->n_layers] = tiles;
   for (int l = params->n_layers + 1; l <= CVK_MAX_LAYERS; ++l) a.tile_start[l] = tiles;
-  if (tiles == 0) return CVK_OK;
+  if (tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
+  const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1));
   if (use_lds) {
-    const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1));
-    static thread_local int configured_device = -1;
-    if (lds_bytes > 65536 && configured_device != c.device) {
-      if (hipFuncSetAttribute((const void*)edge_pass_kernel<true, 1024>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                              c.max_lds) != hipSuccess)
-        return fail(CVK_E_HIP, "cvk_edge_pass: cannot raise the LDS limit");
-      configured_device = c.device;
-    }
-    hipLaunchKernelGGL((edge_pass_kernel<true, 1024>), dim3(grid), dim3(1024), (size_t)lds_bytes, st, a);
+    hipLaunchKernelGGL((edge_pass_kernel<true>), dim3(grid), dim3(EDGE_BLOCK), (size_t)lds_bytes, st, a);
   } else {
-    const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1) * 8);
-    hipLaunchKernelGGL((edge_pass_kernel<false, 256>), dim3(grid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((edge_pass_kernel<false>), dim3(grid), dim3(EDGE_BLOCK), (size_t)queue_bytes, st, a);
   }
   return check_launch("cvk_edge_pass");
 }
 
 extern "C" int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                              cvk_work* work, int32_t t, const double* const* uniforms, void* stream) {
-  int rc = validate(people, params, work, t, "cvk_edge_pass");
+  int rc = validate(people, params, work, t);
   if (rc) return rc;
   if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "cvk_edge_pass: layers is null");
-  return edge_pass_impl(people, layers, params, work, t, uniforms, (hipStream_t)stream);
+  return edge_pass_impl(people, layers, params, work, t, uniforms, (hipStream_t)stream, nullptr);
 }
 
 static int infect_impl(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
@@ -913,27 +1169,17 @@ static int infect_impl(const cvk_people* people, const cvk_layer* layers, const 
   ia.t = t;
   ia.count_flows = count_flows;
   ia.allow_beds = allow_beds;
-  const int grid = std::max(caps().n_cu, 1) * 4;
+  const int grid = std::max(caps().n_cu, 1);
   hipLaunchKernelGGL(infect_pass_kernel, dim3(grid), dim3(INFECT_BLOCK), 0, st, *people, *params, *work, ia);
-  hipLaunchKernelGGL(reset_touched_kernel, dim3(1), dim3(64), 0, st, *work);
   return check_launch("cvk_infect_pass");
 }
 
 extern "C" int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                                cvk_work* work, int32_t t, int32_t count_flows, void* stream) {
-  int rc = validate(people, params, work, t, "cvk_infect_pass");
+  int rc = validate(people, params, work, t);
   if (rc) return rc;
   if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "cvk_infect_pass: layers is null");
   return infect_impl(people, layers, params, work, t, count_flows, count_flows, (hipStream_t)stream);
-}
-
-extern "C" int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
-                              void* stream) {
-  int rc = validate(people, params, work, t, "cvk_count_pass");
-  if (rc) return rc;
-  hipLaunchKernelGGL(count_pass_kernel, dim3(grid_for(people->n, COUNT_BLOCK)), dim3(COUNT_BLOCK), 0,
-                     (hipStream_t)stream, *people, *params, *work, t);
-  return check_launch("cvk_count_pass");
 }
 
 static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
@@ -941,72 +1187,66 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
   int rc;
   for (int32_t t = t0; t < t1; ++t) {
     if (t == 0 && pop_infected > 0) {  // init_infections, sim.py:505-532 (flows not counted)
-      rc = choose_impl(people, params, work, 0, STREAM_CHOOSE, pop_infected, 0, CVK_LAYER_SEED, 0, 0, st);
+      rc = choose_impl(people, params, work, 0, STREAM_CHOOSE, pop_infected, 0, CVK_LAYER_SEED, 0, st);
       if (rc) return rc;
       rc = infect_impl(people, layers, params, work, 0, 0, 0, st);
       if (rc) return rc;
     }
-    hipLaunchKernelGGL(agent_pass_kernel, dim3(grid_for(people->n, AGENT_BLOCK)), dim3(AGENT_BLOCK), 0, st, *people,
-                       *params, *work, t);
-    for (int l = 0; l < params->n_layers; ++l) {
-      if (layers[l].dynamic) {
-        rc = cvk_resample_layer(&layers[l], l, params, t, st);
-        if (rc) return rc;
-      }
-    }
+    // day t begins with the stocks of day t-1 (already closed if t == t0)
+    DayReduce r;
+    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, 1, st, &r);
+    if (rc) return rc;
     if (imports) {
       for (int j = 0; j < imports->n; ++j) {
         if (imports->days[j] != t) continue;
         const int32_t v = imports->variants[j];
         rc = choose_impl(people, params, work, t, STREAM_IMPORT | (uint32_t)v, imports->n_imports[j], 1,
-                         CVK_LAYER_IMPORT, v, 0, st);
+                         CVK_LAYER_IMPORT, v, st);
         if (rc) return rc;
       }
     }
-    rc = edge_pass_impl(people, layers, params, work, t, nullptr, st);
+    rc = edge_pass_impl(people, layers, params, work, t, nullptr, st, &r);  // block 0 reduces day_begin's rows
     if (rc) return rc;
     rc = infect_impl(people, layers, params, work, t, 1, 1, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(count_pass_kernel, dim3(grid_for(people->n, COUNT_BLOCK)), dim3(COUNT_BLOCK), 0, st, *people,
-                       *params, *work, t);
-    rc = check_launch("cvk_run_days");
-    if (rc) return rc;
   }
+  // close the last day: stocks and NAb update
+  DayReduce r;
+  rc = day_begin_launch(people, layers, params, work, t1 - 1, 0, 0, 0, st, &r);
+  if (rc) return rc;
+  return day_reduce_launch(work, r, st);
+}
+
+static int check_run_args(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
+                          int32_t t0, int32_t t1) {
+  int rc = validate(people, params, work, t0);
+  if (rc) return rc;
+  if (t1 < t0 || t1 > work->npts) return fail(CVK_E_INVALID, "bad day range");
+  if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "layers is null");
   return CVK_OK;
 }
+
+extern "C" int cvk_graph_create(const cvk_people* people, cvk_layer* layers, const cvk_params* params,
+                                cvk_work* work, int32_t t0, int32_t t1, int64_t pop_infected,
+                                const cvk_imports* imports, void* stream, void** graph_exec_out);
+extern "C" int cvk_graph_launch(void* graph_exec, void* stream);
+extern "C" int cvk_graph_destroy(void* graph_exec);
 
 extern "C" int cvk_run_days(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
                             int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports,
                             int32_t use_graph, void* stream) {
-  int rc = validate(people, params, work, t0, "cvk_run_days");
+  int rc = check_run_args(people, layers, params, work, t0, t1);
   if (rc) return rc;
-  if (t1 < t0 || t1 > work->npts) return fail(CVK_E_INVALID, "cvk_run_days: bad day range");
-  if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "cvk_run_days: layers is null");
   if (t1 == t0) return CVK_OK;
   hipStream_t st = (hipStream_t)stream;
   if (!use_graph) return run_days_impl(people, layers, params, work, t0, t1, pop_infected, imports, st);
-  if (st == nullptr) return fail(CVK_E_INVALID, "cvk_run_days: graph capture needs a non-default stream");
-  // Configure the LDS limit outside the capture (hipFuncSetAttribute is not a stream operation).
-  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
-    return fail(CVK_E_HIP, "cvk_run_days: hipStreamBeginCapture failed");
-  rc = run_days_impl(people, layers, params, work, t0, t1, pop_infected, imports, st);
-  hipGraph_t graph = nullptr;
-  const hipError_t ec = hipStreamEndCapture(st, &graph);
-  if (rc) {
-    if (graph) (void)hipGraphDestroy(graph);
-    return rc;
-  }
-  if (ec != hipSuccess || !graph) return fail(CVK_E_HIP, "cvk_run_days: hipStreamEndCapture failed");
-  hipGraphExec_t exec = nullptr;
-  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
-    (void)hipGraphDestroy(graph);
-    return fail(CVK_E_HIP, "cvk_run_days: hipGraphInstantiate failed");
-  }
-  const hipError_t el = hipGraphLaunch(exec, st);
+  void* exec = nullptr;
+  rc = cvk_graph_create(people, layers, params, work, t0, t1, pop_infected, imports, stream, &exec);
+  if (rc) return rc;
+  rc = cvk_graph_launch(exec, stream);
   (void)hipStreamSynchronize(st);
-  (void)hipGraphExecDestroy(exec);
-  (void)hipGraphDestroy(graph);
-  return el == hipSuccess ? CVK_OK : fail(CVK_E_HIP, "cvk_run_days: hipGraphLaunch failed");
+  (void)cvk_graph_destroy(exec);
+  return rc;
 }
 
 // Two-phase graph API: capture days [t0, t1) once, replay many times (inputs must be reset
@@ -1016,12 +1256,14 @@ extern "C" int cvk_graph_create(const cvk_people* people, cvk_layer* layers, con
                                 const cvk_imports* imports, void* stream, void** graph_exec_out) {
   if (!graph_exec_out) return fail(CVK_E_INVALID, "cvk_graph_create: null output handle");
   *graph_exec_out = nullptr;
-  int rc = validate(people, params, work, t0, "cvk_graph_create");
+  int rc = check_run_args(people, layers, params, work, t0, t1);
   if (rc) return rc;
-  if (t1 <= t0 || t1 > work->npts) return fail(CVK_E_INVALID, "cvk_graph_create: bad day range");
-  if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "cvk_graph_create: layers is null");
+  if (t1 <= t0) return fail(CVK_E_INVALID, "cvk_graph_create: empty day range");
   hipStream_t st = (hipStream_t)stream;
   if (st == nullptr) return fail(CVK_E_INVALID, "cvk_graph_create: graph capture needs a non-default stream");
+  int64_t budget = 0;
+  rc = edge_dynamic_lds(true, &budget);  // not a stream operation: done before capture
+  if (rc) return rc;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(CVK_E_HIP, "cvk_graph_create: hipStreamBeginCapture failed");
   rc = run_days_impl(people, layers, params, work, t0, t1, pop_infected, imports, st);

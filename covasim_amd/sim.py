@@ -164,29 +164,11 @@ class Sim:
         self.popdict = None
 
     def init_workspace(self):
-        import torch
         n, npts = self.people.n, self.npts
-        dev = self.people.device
-        words = ((n + 127) // 128) * 4
         cap = self.log_capacity
         if cap is None:
             cap = max(1024, 2 * n + int(self.pars['pop_infected']))
-        self._w = dict(
-            winner=torch.full((n,), -1, dtype=torch.int64, device=dev),
-            touched=torch.zeros(n, dtype=torch.int32, device=dev),
-            bitmap=torch.zeros(words, dtype=torch.int32, device=dev),
-            ctl=torch.zeros(_lib.CTL_WORDS, dtype=torch.int32, device=dev),
-            results=torch.zeros((npts, _lib.R_COLS), dtype=torch.int32, device=dev),
-            results_f=torch.zeros((npts, _lib.RF_COLS), dtype=torch.float64, device=dev),
-            log=torch.zeros((cap, 4), dtype=torch.int32, device=dev),
-            select_hist=torch.zeros(256, dtype=torch.int32, device=dev),
-        )
-        w = _lib.WorkS()
-        for k in ('winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist'):
-            setattr(w, k, self._w[k].data_ptr())
-        w.log_capacity = cap
-        w.npts = npts
-        self._work = w
+        self._w, self._work = _lib.make_workspace(n, npts, int(cap), self.people.device)
 
     def layer_keys(self):
         return list(self.people.contacts.keys())
@@ -388,8 +370,9 @@ class Sim:
         for j in range(len(days)):
             imp[days[j]] += nimp[j]
         R['n_imports'] = Result('n_imports', values=imp)
-        n_alive_agents = np.maximum(res[:, col['n_alive']], 1)
-        R['pop_nabs'] = Result('pop_nabs', scale=False, values=resf[:, 0] / n_alive_agents)
+        n_alive_agents = res[:, col['n_alive']]
+        with np.errstate(divide='ignore', invalid='ignore'):  # 0/0 -> nan when everyone died, as sim.py:672
+            R['pop_nabs'] = Result('pop_nabs', scale=False, values=resf[:, 0] / n_alive_agents)
         R['pop_protection'] = Result('pop_protection', scale=False, values=resf[:, 1] / (nv * self.people.n))
         R['pop_symp_protection'] = Result('pop_symp_protection', scale=False, values=resf[:, 2] / (nv * self.people.n))
         V = {}

@@ -173,16 +173,24 @@ typedef struct {
 typedef struct {
   uint64_t* winner;      /* [n] min transmission key per target; all-ones = none */
   int32_t* touched;      /* [n] targets with a winner key this day */
-  uint32_t* bitmap;      /* [bitmap_words] infectious-with-nonzero-rel_trans bits */
-  uint32_t* ctl;         /* [64] control words: touched count, choose state, flags */
+  uint32_t* bitmap;      /* [bitmap_words] bit i: agent i infectious with rel_trans != 0 */
+  uint32_t* ctl;         /* [64] control words: touched count, choose state, tickets, flags */
   uint32_t* results;     /* [npts][CVK_R_COLS] */
   double* results_f;     /* [npts][CVK_RF_COLS] */
   int32_t* log;          /* [log_capacity][4] infection records: source, target, day, layer | variant<<8 */
   int64_t log_capacity;
   uint32_t* select_hist; /* [256] histogram scratch of cvk_choose */
+  uint32_t* susmap;      /* [bitmap_words] bit i: agent i susceptible with rel_sus != 0 */
+  uint64_t* rec;         /* [n] per-agent edge-pass record: low word float (rel_trans if infectious,
+                            else rel_sus), high word the flags */
+  uint32_t* partials;    /* [CVK_PARTIAL_COLS][CVK_MAX_REDUCE_BLOCKS] per-block counts (column-major) */
+  double* partials_f;    /* [CVK_RF_COLS][CVK_MAX_REDUCE_BLOCKS] per-block sums (column-major) */
   int32_t npts;
   int32_t pad;
 } cvk_work;
+
+#define CVK_MAX_REDUCE_BLOCKS 1024
+#define CVK_PARTIAL_COLS 64
 
 /* control word indices of cvk_work.ctl */
 #define CVK_CTL_TOUCHED 0
@@ -192,6 +200,7 @@ typedef struct {
 #define CVK_CTL_SEL_REMAIN 8
 #define CVK_CTL_SEL_ALL 9
 #define CVK_CTL_SEL_SHORT 10      /* set when fewer agents were eligible than requested */
+#define CVK_CTL_EDGE_TILES 11     /* dynamic tile counter of the edge pass (zeroed by the agent pass) */
 #define CVK_CTL_WORDS 64
 
 /* ---- library ---- */
@@ -205,6 +214,8 @@ int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);
 /* Sizes in bytes of the workspace arrays for n agents and npts days. */
 int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winner_bytes, int64_t* touched_bytes,
                         int64_t* bitmap_bytes, int64_t* results_bytes, int64_t* results_f_bytes);
+/* Bytes of the remaining workspace arrays: rec (8 per agent), partials and partials_f. */
+int cvk_workspace_sizes2(int64_t n_agents, int64_t* rec_bytes, int64_t* partials_bytes, int64_t* partials_f_bytes);
 
 /* ---- 1:1 mirrors of the reference's numba kernels (parity boundary) ---- */
 
@@ -247,7 +258,8 @@ int cvk_agent_pass(const cvk_people* people, const cvk_params* params, cvk_work*
                    void* stream);
 int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
                        void* stream);
-/* uniforms: NULL (Philox) or per layer a device float64 array [n_variants][2][n_edges]. */
+/* uniforms: NULL (Philox) or per layer a device float64 array [n_variants][2][n_edges].
+ * Must follow cvk_agent_pass of the same day (it reads that pass's bitmaps, records and tile counter). */
 int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                   cvk_work* work, int32_t t, const double* const* uniforms, void* stream);
 int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,

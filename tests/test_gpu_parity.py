@@ -193,7 +193,7 @@ def test_graph_and_eager_and_step_agree(gpu):
     snaps = [r.people.snapshot() for r in runs]
     for other in snaps[1:]:
         for k in snaps[0]:
-            assert np.array_equal(snaps[0][k], other[k]), k
+            assert np.array_equal(snaps[0][k].view(np.uint8), other[k].view(np.uint8)), k  # NaN-safe, bitwise
 
 
 def test_infection_log_attribution(gpu):
