@@ -31,6 +31,8 @@ DIST_CODES = {'uniform': 0, 'unif': 0, 'normal': 1, 'norm': 1, 'normal_pos': 2, 
               'lognormal': 4, 'lognorm': 4, 'lognormal_int': 5, 'lognorm_int': 5}
 DUR_ORDER = ['exp2inf', 'inf2sym', 'sym2sev', 'sev2crit', 'asym2rec', 'mild2rec', 'sev2rec', 'crit2rec', 'crit2die']
 
+TUNE_EDGE_LDS = 1
+
 LAYER_SEED = 14
 LAYER_IMPORT = 15
 
@@ -96,7 +98,7 @@ class PeopleS(C.Structure):
 
 class LayerS(C.Structure):
     _fields_ = [('p1', C.c_void_p), ('p2', C.c_void_p), ('beta', C.c_void_p), ('n_edges', C.c_int64),
-                ('dynamic', C.c_int32), ('pad', C.c_int32)]
+                ('dynamic', C.c_int32), ('beta_ones', C.c_int32)]
 
 
 class WorkS(C.Structure):
@@ -107,7 +109,7 @@ class WorkS(C.Structure):
                 ('npts', C.c_int32), ('pad', C.c_int32)]
 
 
-MAX_REDUCE_BLOCKS = 1024
+MAX_REDUCE_BLOCKS = 65536
 PARTIAL_COLS = 64
 WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'susmap',
                    'rec', 'partials', 'partials_f']
@@ -172,6 +174,7 @@ def lib():
     L.cvk_version.restype = C.c_int
     L.cvk_abi_layout.argtypes = [C.POINTER(C.c_int64), i32]
     L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.cvk_set_tuning.argtypes = [i32, i32]
     L.cvk_workspace_sizes.argtypes = [i64, i32] + [C.POINTER(C.c_int64)] * 5
     L.cvk_workspace_sizes2.argtypes = [i64] + [C.POINTER(C.c_int64)] * 3
     L.cvk_viral_load.argtypes = [i32, vp, vp, vp, i64, f32, f32, f32, vp, vp]
@@ -189,7 +192,7 @@ def lib():
     L.cvk_graph_create.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), vp, C.POINTER(C.c_void_p)]
     L.cvk_graph_launch.argtypes = [vp, vp]
     L.cvk_graph_destroy.argtypes = [vp]
-    for name in ['cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes2',
+    for name in ['cvk_set_tuning', 'cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes2',
                  'cvk_workspace_sizes', 'cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_choose',
                  'cvk_agent_pass', 'cvk_resample_layer', 'cvk_edge_pass', 'cvk_infect_pass', 'cvk_count_pass',
                  'cvk_run_days', 'cvk_abi_layout', 'cvk_device_info']:

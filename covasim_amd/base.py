@@ -97,6 +97,15 @@ class Layer:
         """The device tensor backing a column."""
         return self._d[key]
 
+    def beta_all_ones(self):
+        """True when every edge weight is exactly 1.0 (checked on the device, cached per column tensor)."""
+        t = self._d['beta']
+        key = (t.data_ptr(), t.numel(), t._version)
+        if getattr(self, '_ones_key', None) != key:
+            self._ones = bool((t == 1.0).all()) if t.numel() else True
+            self._ones_key = key
+        return self._ones
+
     def validate(self):
         """Equal lengths for every column (base.py:1716-1736)."""
         n = len(self)

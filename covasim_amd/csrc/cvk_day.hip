@@ -111,18 +111,33 @@ __device__ void finalize_partials(const uint32_t* partials, const double* partia
   const int n_waves = (int)(blockDim.x >> 6);
   const int w = threadIdx.x >> 6;
   for (int k = w; k < NC + ND; k += n_waves) {
+    // 8 independent loads in flight per lane per step (the loop is latency-bound otherwise)
     if (k < NC) {
       const uint32_t* col = partials + (size_t)k * CVK_MAX_REDUCE_BLOCKS;
-      uint32_t acc = 0;
-      for (uint32_t b = lane_id(); b < n_blocks; b += 64) acc += col[b];
-      acc = wave_sum(acc);
-      if (lane_id() == 0) s_tot[k] = acc;
+      uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t b0 = 0; b0 < n_blocks; b0 += 512) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t b = b0 + j * 64 + lane_id();
+          acc[j] += b < n_blocks ? col[b] : 0u;
+        }
+      }
+      uint32_t tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+      tot = wave_sum(tot);
+      if (lane_id() == 0) s_tot[k] = tot;
     } else {
       const double* col = partials_f + (size_t)(k - NC) * CVK_MAX_REDUCE_BLOCKS;
-      double acc = 0.0;
-      for (uint32_t b = lane_id(); b < n_blocks; b += 64) acc += col[b];
-      acc = wave_sum_d(acc);
-      if (lane_id() == 0) s_totf[k - NC] = acc;
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t b0 = 0; b0 < n_blocks; b0 += 512) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t b = b0 + j * 64 + lane_id();
+          acc[j] += b < n_blocks ? col[b] : 0.0;
+        }
+      }
+      double tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+      tot = wave_sum_d(tot);
+      if (lane_id() == 0) s_totf[k - NC] = tot;
     }
   }
   __syncthreads();
@@ -172,11 +187,40 @@ __device__ inline void stock_counts(uint32_t f, uint32_t vs, int nv, uint32_t (&
   }
 }
 
+// Per-agent inputs of day_begin, all loaded before any is used (one memory round trip).
+struct AgentIn {
+  uint32_t f, vs;
+  float d_inf, d_sym, d_sev, d_crit, d_rec, d_dead, d_iso, rt, rs, peak, nab;
+  int32_t tne;
+};
+
+__device__ inline void load_agent(const cvk_people& P, int64_t i, bool waning, AgentIn& x) {
+  x.f = P.flags[i];
+  x.vs = P.vstate[i];
+  x.d_inf = P.date_infectious[i];
+  x.d_sym = P.date_symptomatic[i];
+  x.d_sev = P.date_severe[i];
+  x.d_crit = P.date_critical[i];
+  x.d_rec = P.date_recovered[i];
+  x.d_dead = P.date_dead[i];
+  x.d_iso = P.date_end_isolation[i];
+  x.rt = P.rel_trans[i];
+  x.rs = P.rel_sus[i];
+  if (waning) {
+    x.peak = P.peak_nab[i];
+    x.nab = P.nab[i];
+    x.tne = P.t_nab_event[i];
+  } else {
+    x.peak = 0.0f;
+    x.nab = 0.0f;
+    x.tne = 0;
+  }
+}
+
+constexpr int AGENTS_PER_THREAD = 2;
+
 __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cvk_params prm, cvk_work W,
                                                                 DayBeginArgs a) {
-  __shared__ uint32_t s_red[N_AGENT_COLS];
-  __shared__ double s_redf[3];
-
   if (blockIdx.x >= a.agent_blocks) {
     // ---- dynamic-layer resample: iid uniform endpoints per edge, beta = 1 (base.py:1849-1876)
     const int64_t stride = (int64_t)(gridDim.x - a.agent_blocks) * AGENT_BLOCK;
@@ -195,113 +239,105 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
     return;
   }
 
+  // per-wave counters: sums of per-agent booleans via ballot (wave-uniform, no shuffles)
   uint32_t c[N_AGENT_COLS];
 #pragma unroll
   for (int k = 0; k < N_AGENT_COLS; ++k) c[k] = 0;
   double d[3] = {0.0, 0.0, 0.0};
   const int nv = prm.n_variants;
+  const bool waning = prm.use_waning != 0;
   const int32_t t = a.t;
   const float tf = (float)t;
-  const int64_t stride = (int64_t)a.agent_blocks * AGENT_BLOCK;
   const int64_t n_waves_bits = (P.n + 63) / 64;
+  const bool closing = a.prologue || !a.transitions;  // stocks + update_nab of the previous (or this) day
+  const int32_t tc = a.transitions ? t - 1 : t;
 
-  for (int64_t base = (int64_t)blockIdx.x * AGENT_BLOCK; base < P.n; base += stride) {
-    const int64_t i = base + threadIdx.x;
+  int64_t idx[AGENTS_PER_THREAD];
+  AgentIn in[AGENTS_PER_THREAD];
+#pragma unroll
+  for (int r = 0; r < AGENTS_PER_THREAD; ++r) {
+    idx[r] = ((int64_t)blockIdx.x * AGENTS_PER_THREAD + r) * AGENT_BLOCK + threadIdx.x;
+    if (idx[r] < P.n) load_agent(P, idx[r], waning, in[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < AGENTS_PER_THREAD; ++r) {
+    const int64_t i = idx[r];
     const bool valid = i < P.n;
     bool inf_bit = false, sus_bit = false;
+    uint32_t f_close = 0, vs_close = 0;  // state counted as the previous day's stocks
+    uint32_t ev = 0;                     // this day's transition events (bits as the day columns)
+    uint32_t ev_var = CVK_VAR_NONE;      // variant of a new infectious agent
     if (valid) {
-      uint32_t f = P.flags[i];
-      uint32_t vs = P.vstate[i];
-      const uint32_t f0 = f, vs0 = vs;
-      if (a.prologue || !a.transitions) {
-        // ---- end of the previous day (or of day t for the finish pass): update_nab + stocks
-        const int32_t tc = a.transitions ? t - 1 : t;
-        if (prm.use_waning) {
-          const float peak = P.peak_nab[i];
-          float nab = P.nab[i];
-          if (peak != 0.0f) {  // has_nabs = true(peak_nab), sim.py:666-669; immunity.py:205-213
-            int32_t ts = tc - P.t_nab_event[i];
-            ts = ts < 0 ? 0 : (ts >= prm.nab_kin_len ? prm.nab_kin_len - 1 : ts);
-            nab = (float)((double)nab + prm.nab_kin[ts] * (double)peak);
-            nab = nab < 0.0f ? 0.0f : nab;
-            nab = nab > peak ? peak : nab;
-            P.nab[i] = nab;
-            for (int v = 0; v < nv; ++v) {  // population means of sus/symp_imm (sim.py:673-674)
-              d[1] += (double)P.sus_imm[(int64_t)v * P.n + i];
-              d[2] += (double)P.symp_imm[(int64_t)v * P.n + i];
-            }
+      const AgentIn& x = in[r];
+      uint32_t f = x.f;
+      uint32_t vs = x.vs;
+      f_close = f;
+      vs_close = vs;
+      if (closing && waning) {
+        float nab = x.nab;
+        if (x.peak != 0.0f) {  // has_nabs = true(peak_nab), sim.py:666-669; immunity.py:205-213
+          int32_t ts = tc - x.tne;
+          ts = ts < 0 ? 0 : (ts >= prm.nab_kin_len ? prm.nab_kin_len - 1 : ts);
+          nab = (float)((double)nab + prm.nab_kin[ts] * (double)x.peak);
+          nab = nab < 0.0f ? 0.0f : nab;
+          nab = nab > x.peak ? x.peak : nab;
+          P.nab[i] = nab;
+          for (int v = 0; v < nv; ++v) {  // population means of sus/symp_imm (sim.py:673-674)
+            d[1] += (double)P.sus_imm[(int64_t)v * P.n + i];
+            d[2] += (double)P.symp_imm[(int64_t)v * P.n + i];
           }
-          if (!(f & CVK_DEAD)) d[0] += (double)nab;
+          in[r].nab = nab;
         }
-        stock_counts(f, vs, nv, c);
+        if (!(f & CVK_DEAD)) d[0] += (double)nab;
       }
       if (a.transitions) {
+        const uint32_t f0 = f, vs0 = vs;
         const bool is_exp = (f & CVK_EXPOSED) != 0;  // people.py:169 snapshot
-        const float d_inf = P.date_infectious[i];
-        const float d_rec = P.date_recovered[i];
-        const float d_dead = P.date_dead[i];
+        const float d_inf = x.d_inf, d_rec = x.d_rec, d_dead = x.d_dead;
         if (is_exp) {
           // check_infectious, people.py:222-232
           if (!(f & CVK_INFECTIOUS) && !isnan(d_inf) && tf >= d_inf) {
-            const uint32_t ev = exp_var(vs);
-            f = ((f | CVK_INFECTIOUS) & ~INFVAR_MASK) | (ev << CVK_INFVAR_SHIFT);
-            if (ev < MAXV) {
-              vs |= 1u << (16 + ev);
-#pragma unroll
-              for (int v = 0; v < MAXV; ++v) c[DC_BASE + 9 + v] += (ev == (uint32_t)v) ? 1u : 0u;
-            }
-            c[DC_BASE + 0] += 1;
+            const uint32_t evv = exp_var(vs);
+            f = ((f | CVK_INFECTIOUS) & ~INFVAR_MASK) | (evv << CVK_INFVAR_SHIFT);
+            if (evv < MAXV) vs |= 1u << (16 + evv);
+            ev |= 1u << 0;
+            ev_var = evv;
           }
           // check_symptomatic / severe / critical, people.py:235-253
-          if (!(f & CVK_SYMPTOMATIC)) {
-            const float dd = P.date_symptomatic[i];
-            if (!isnan(dd) && tf >= dd) { f |= CVK_SYMPTOMATIC; c[DC_BASE + 1] += 1; }
-          }
-          if (!(f & CVK_SEVERE)) {
-            const float dd = P.date_severe[i];
-            if (!isnan(dd) && tf >= dd) { f |= CVK_SEVERE; c[DC_BASE + 2] += 1; }
-          }
-          if (!(f & CVK_CRITICAL)) {
-            const float dd = P.date_critical[i];
-            if (!isnan(dd) && tf >= dd) { f |= CVK_CRITICAL; c[DC_BASE + 3] += 1; }
-          }
+          if (!(f & CVK_SYMPTOMATIC) && !isnan(x.d_sym) && tf >= x.d_sym) { f |= CVK_SYMPTOMATIC; ev |= 1u << 1; }
+          if (!(f & CVK_SEVERE) && !isnan(x.d_sev) && tf >= x.d_sev) { f |= CVK_SEVERE; ev |= 1u << 2; }
+          if (!(f & CVK_CRITICAL) && !isnan(x.d_crit) && tf >= x.d_crit) { f |= CVK_CRITICAL; ev |= 1u << 3; }
           // check_recovery, people.py:256-291
           if (!(f & CVK_RECOVERED) && !isnan(d_rec) && tf >= d_rec) {
-            const uint32_t ev = exp_var(vs);
+            const uint32_t evv = exp_var(vs);
             f = ((f & ~STATE_CLEAR_ON_RECOVERY) | CVK_RECOVERED) | INFVAR_MASK;
-            vs = (vs & ~0xFFFFFFu) | (ev << 4) | CVK_VAR_NONE;  // recovered_variant = exposed_variant
-            if (prm.use_waning) f = (f | CVK_SUSCEPTIBLE) & ~CVK_DIAGNOSED;
-            c[DC_BASE + 4] += 1;
+            vs = (vs & ~0xFFFFFFu) | (evv << 4) | CVK_VAR_NONE;  // recovered_variant = exposed_variant
+            if (waning) f = (f | CVK_SUSCEPTIBLE) & ~CVK_DIAGNOSED;
+            ev |= 1u << 4;
           }
         }
         // check_exit_iso, people.py:368-374 (not filtered by exposure)
-        if (f & CVK_ISOLATED) {
-          const float dd = P.date_end_isolation[i];
-          if (!isnan(dd) && tf >= dd) f &= ~CVK_ISOLATED;
-        }
+        if ((f & CVK_ISOLATED) && !isnan(x.d_iso) && tf >= x.d_iso) f &= ~CVK_ISOLATED;
         // check_death, people.py:294-312
         if (is_exp && !(f & CVK_DEAD) && !isnan(d_dead) && tf >= d_dead) {
           f |= CVK_DEAD;
-          if (f & CVK_DIAGNOSED) { f |= CVK_KNOWN_DEAD; c[DC_BASE + 6] += 1; }
+          if (f & CVK_DIAGNOSED) { f |= CVK_KNOWN_DEAD; ev |= 1u << 6; }
           f = (f & ~STATE_CLEAR_ON_DEATH) | INFVAR_MASK;
           vs = (vs & ~0xFFu) | CVK_VAR_NONE | (CVK_VAR_NONE << 4);
-          c[DC_BASE + 5] += 1;
+          ev |= 1u << 5;
         }
         // check_immunity, immunity.py:303-350 (no vaccine sources on this path)
-        if (prm.use_waning) {
-          const float peak = P.peak_nab[i];
-          if (peak != 0.0f) {  // agents never infected keep sus/symp/sev_imm == 0
-            const double nab = (double)P.nab[i];
-            const bool was_inf = tf >= d_rec;
-            const uint32_t rv = rec_var(vs);
-            for (int v = 0; v < nv; ++v) {
-              const double imm = (was_inf && rv < MAXV) ? (double)prm.immunity[v][rv] : 0.0;
-              const double eff = nab * imm;
-              const int64_t off = (int64_t)v * P.n + i;
-              P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
-              P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
-              P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
-            }
+        if (waning && x.peak != 0.0f) {  // agents never infected keep sus/symp/sev_imm == 0
+          const double nab = (double)in[r].nab;
+          const bool was_inf = tf >= d_rec;
+          const uint32_t rv = rec_var(vs);
+          for (int v = 0; v < nv; ++v) {
+            const double imm = (was_inf && rv < MAXV) ? (double)prm.immunity[v][rv] : 0.0;
+            const double eff = nab * imm;
+            const int64_t off = (int64_t)v * P.n + i;
+            P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
+            P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
+            P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
           }
         }
         // viral-load phase (utils.py:58-78): early iff (t - start)/total < switch point
@@ -315,17 +351,36 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
         }
         if (f != f0) P.flags[i] = f;
         if (vs != vs0) P.vstate[i] = vs;
-        const float rt = P.rel_trans[i];
-        const float rs = P.rel_sus[i];
-        inf_bit = (f & CVK_INFECTIOUS) && rt != 0.0f;
-        sus_bit = (f & CVK_SUSCEPTIBLE) && rs != 0.0f;
-        const float val = (f & CVK_INFECTIOUS) ? rt : rs;
+        inf_bit = (f & CVK_INFECTIOUS) && x.rt != 0.0f;
+        sus_bit = (f & CVK_SUSCEPTIBLE) && x.rs != 0.0f;
+        const float val = (f & CVK_INFECTIOUS) ? x.rt : x.rs;
         W.rec[i] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
-        c[DC_BASE + 7] += (f & CVK_SEVERE) ? 1u : 0u;
-        c[DC_BASE + 8] += (f & CVK_CRITICAL) ? 1u : 0u;
+        if (f & CVK_SEVERE) ev |= 1u << 7;
+        if (f & CVK_CRITICAL) ev |= 1u << 8;
+      }
+    }
+    // wave counts (every lane reaches these ballots)
+    if (closing) {
+      const uint32_t stock_bits[13] = {CVK_SUSCEPTIBLE, CVK_EXPOSED, CVK_INFECTIOUS, CVK_SYMPTOMATIC, CVK_SEVERE,
+                                       CVK_CRITICAL, CVK_RECOVERED, CVK_DEAD, CVK_DIAGNOSED, CVK_KNOWN_DEAD,
+                                       CVK_QUARANTINED, CVK_ISOLATED, CVK_VACCINATED};
+#pragma unroll
+      for (int k = 0; k < 13; ++k) c[k] += (uint32_t)__popcll(__ballot(f_close & stock_bits[k]));
+      c[SC_ALIVE] += (uint32_t)__popcll(__ballot(valid && !(f_close & CVK_DEAD)));
+#pragma unroll
+      for (int v = 0; v < MAXV; ++v) {
+        if (v < nv) {  // nv is uniform: the ballots stay wave-wide
+          c[SC_EXPV + v] += (uint32_t)__popcll(__ballot((vs_close >> (8 + v)) & 1u));
+          c[SC_INFV + v] += (uint32_t)__popcll(__ballot((vs_close >> (16 + v)) & 1u));
+        }
       }
     }
     if (a.transitions) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c[DC_BASE + k] += (uint32_t)__popcll(__ballot((ev >> k) & 1u));
+#pragma unroll
+      for (int v = 0; v < MAXV; ++v)
+        if (v < nv) c[DC_BASE + 9 + v] += (uint32_t)__popcll(__ballot(ev_var == (uint32_t)v));
       // one 64-bit ballot per wave covers 64 consecutive agents
       const uint64_t ib = __ballot(inf_bit);
       const uint64_t sb = __ballot(sus_bit);
@@ -341,7 +396,34 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
     W.ctl[CVK_CTL_TOUCHED] = 0;       // the new day's target list
     W.ctl[CVK_CTL_EDGE_TILES] = 0;    // the edge pass's tile counter
   }
-  write_partials<N_AGENT_COLS, 3>(c, d, s_red, s_redf, W.partials, W.partials_f);
+  // block partial row: per-wave counts through LDS (no atomics), float sums in wave order
+  __shared__ uint32_t s_wc[AGENT_BLOCK / 64][N_AGENT_COLS];
+  __shared__ double s_wd[AGENT_BLOCK / 64][3];
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < N_AGENT_COLS; ++k) s_wc[w][k] = c[k];
+  }
+  if (waning) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double v = wave_sum_d(d[k]);
+      if (lane_id() == 0) s_wd[w][k] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < N_AGENT_COLS) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wc[j][threadIdx.x];
+    W.partials[(size_t)threadIdx.x * CVK_MAX_REDUCE_BLOCKS + blockIdx.x] = tot;
+  } else if (threadIdx.x >= 64 && threadIdx.x < 67) {
+    const int k = threadIdx.x - 64;
+    double tot = 0.0;
+    if (waning)
+      for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wd[j][k];
+    W.partials_f[(size_t)k * CVK_MAX_REDUCE_BLOCKS + blockIdx.x] = tot;
+  }
 }
 
 // Writes the day rows from the summed partial rows of day_begin (block-wide).
@@ -480,7 +562,7 @@ struct EdgeLayer {
   const double* unif;  // [n_variants][2][n] or null
   int64_t n;
   float beta_layer, iso_factor, quar_factor;
-  int32_t pad;
+  int32_t beta_ones;   // every per-edge beta is 1.0f: no gather needed
 };
 
 struct EdgeArgs {
@@ -534,7 +616,8 @@ __device__ inline void edge_candidate(const EdgeArgs& a, const EdgeLayer& L, int
   float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
   if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
   if (a.use_sus_imm) rs = rs * (1.0f - a.sus_imm[(int64_t)v * a.n_agents + tgt]);
-  const float p = ((a.beta_v[v] * L.beta[e]) * rt) * rs;         // utils.py:117
+  const float eb = L.beta_ones ? 1.0f : L.beta[e];
+  const float p = ((a.beta_v[v] * eb) * rt) * rs;                // utils.py:117
   if (p == 0.0f) return;                                         // utils.py:118
   double u;
   if (L.unif) {
@@ -559,90 +642,133 @@ __device__ inline bool inf_bit(const EdgeArgs& a, const uint32_t* s_bits, uint32
 
 __device__ inline bool sus_bit(const EdgeArgs& a, uint32_t i) { return (a.susmap[i >> 5] >> (i & 31)) & 1u; }
 
+static_assert(EDGE_CHUNKS == 2, "EdgeTile holds two 4-edge chunks per thread");
+
+// One tile's endpoints for this thread, in vector registers (two 4-edge chunks).
+struct EdgeTile {
+  int64_t tile;
+  int64_t e0;       // first edge of the tile in its layer
+  int32_t l;
+  int32_t n_valid;  // edges of the tile that exist (the last tile of a layer is partial)
+  v4i x0, y0, x1, y1;
+};
+
+__device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
+  if (e + 3 < n) return __builtin_nontemporal_load((const v4i*)(p + e));
+  v4i r = {0, 0, 0, 0};
+  if (e < n) r[0] = p[e];
+  if (e + 1 < n) r[1] = p[e + 1];
+  if (e + 2 < n) r[2] = p[e + 2];
+  return r;
+}
+
+// Locates a tile and issues its 16-byte endpoint loads (non-temporal: streamed once).
+__device__ inline void load_tile(const EdgeArgs& a, int64_t tile, EdgeTile& T) {
+  T.tile = tile;
+  if (tile >= a.tile_start[a.n_layers]) return;
+  int l = 0;
+  while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
+  T.l = l;
+  const EdgeLayer& L = a.layer[l];
+  T.e0 = (tile - a.tile_start[l]) * EDGE_TILE;
+  const int64_t rem = L.n - T.e0;
+  T.n_valid = (int32_t)(rem < EDGE_TILE ? rem : EDGE_TILE);
+  const int64_t ea = T.e0 + (int64_t)threadIdx.x * 4;
+  const int64_t eb = T.e0 + ((int64_t)EDGE_BLOCK + threadIdx.x) * 4;
+  T.x0 = load4(L.p1, ea, L.n);
+  T.y0 = load4(L.p2, ea, L.n);
+  T.x1 = load4(L.p1, eb, L.n);
+  T.y1 = load4(L.p2, eb, L.n);
+}
+
+template <bool LDS>
+__device__ inline void process_chunk(const EdgeArgs& a, const uint32_t* s_bits, uint32_t* s_q, const EdgeLayer& L,
+                                     const EdgeTile& T, int c, const v4i xs, const v4i ys) {
+  const uint32_t lane = lane_id();
+  const uint32_t n_agents = (uint32_t)a.n_agents;
+  const uint32_t wave_base = threadIdx.x & ~63u;
+  const int32_t local0 = (c * EDGE_BLOCK + (int32_t)threadIdx.x) * 4;  // tile-local index of edge q = 0
+  // candidate bits (q*2 + dir): infectious source and susceptible target
+  uint32_t cand = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t xi = (uint32_t)xs[q], yi = (uint32_t)ys[q];
+    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
+    if (local0 + q < T.n_valid && xi < n_agents && yi < n_agents) {
+      if (inf_bit<LDS>(a, s_bits, xi) && sus_bit(a, yi)) cand |= 1u << (2 * q);
+      if (inf_bit<LDS>(a, s_bits, yi) && sus_bit(a, xi)) cand |= 2u << (2 * q);
+    }
+  }
+  // wave-local compaction of the candidates into the LDS queue (ballot + mbcnt): code = lane << 3 | slot
+  uint32_t total = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool has = (cand >> b) & 1u;
+    const uint64_t m = __ballot(has);
+    if (has) s_q[total + lanes_below(m)] = (lane << 3) | (uint32_t)b;
+    total += (uint32_t)__popcll(m);
+  }
+  if (total == 0) return;  // wave-uniform
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // every lane takes one queued slot per round; endpoints come from the owning lane's registers
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t j = base + lane;
+    const uint32_t code = j < total ? s_q[j] : 0u;
+    const int src_lane = (int)(code >> 3);
+    const int q = (int)((code >> 1) & 3u);
+    const int dir = (int)(code & 1u);
+    const int x0 = __shfl(xs[0], src_lane, 64), x1 = __shfl(xs[1], src_lane, 64);
+    const int x2 = __shfl(xs[2], src_lane, 64), x3 = __shfl(xs[3], src_lane, 64);
+    const int y0 = __shfl(ys[0], src_lane, 64), y1 = __shfl(ys[1], src_lane, 64);
+    const int y2 = __shfl(ys[2], src_lane, 64), y3 = __shfl(ys[3], src_lane, 64);
+    if (j < total) {
+      const int32_t xv = q == 0 ? x0 : (q == 1 ? x1 : (q == 2 ? x2 : x3));
+      const int32_t yv = q == 0 ? y0 : (q == 1 ? y1 : (q == 2 ? y2 : y3));
+      const int64_t e = T.e0 + ((int64_t)c * EDGE_BLOCK + wave_base + (uint32_t)src_lane) * 4 + q;
+      edge_candidate(a, L, T.l, dir, e, dir ? yv : xv, dir ? xv : yv);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool LDS>
+__device__ inline void process_tile(const EdgeArgs& a, const uint32_t* s_bits, uint32_t* s_q, const EdgeTile& T) {
+  const EdgeLayer& L = a.layer[T.l];
+  process_chunk<LDS>(a, s_bits, s_q, L, T, 0, T.x0, T.y0);
+  process_chunk<LDS>(a, s_bits, s_q, L, T, 1, T.x1, T.y1);
+}
+
 template <bool LDS>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
+  __shared__ int64_t s_tiles[2];
   const int nbw = LDS ? a.bitmap_words : 0;
   uint32_t* s_bits = s_mem;
   uint32_t* s_q = s_mem + nbw + (threadIdx.x >> 6) * EDGE_QCAP;  // this wave's candidate queue
-  __shared__ uint32_t s_tile;
   if (blockIdx.x == 0 && a.reduce.n_blocks) day_reduce(a.work, a.reduce);  // the agent pass's counts
+  if (threadIdx.x == 0) s_tiles[0] = atomicAdd(a.tile_counter, 1u);
   if (LDS) {
     const uint4* src = (const uint4*)a.bitmap;
     uint4* dst = (uint4*)s_bits;
     for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
-  const uint32_t lane = lane_id();
-  const uint32_t n_agents = (uint32_t)a.n_agents;
+  __syncthreads();
   const int64_t n_tiles = a.tile_start[a.n_layers];
-  // tiles are handed out dynamically: blocks that finish early (or start late) take more
-  for (;;) {
+  // tiles are handed out dynamically; the next tile's endpoints are in flight while this one is processed
+  EdgeTile cur, nxt;
+  load_tile(a, s_tiles[0], cur);
+  int par = 1;
+  while (cur.tile < n_tiles) {
+    if (threadIdx.x == 0) s_tiles[par] = atomicAdd(a.tile_counter, 1u);
     __syncthreads();
-    if (threadIdx.x == 0) s_tile = atomicAdd(a.tile_counter, 1u);
-    __syncthreads();
-    const int64_t tile = s_tile;
-    if (tile >= n_tiles) break;
-    int l = 0;
-    while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
-    const EdgeLayer& L = a.layer[l];
-    const int64_t e_tile = (tile - a.tile_start[l]) * EDGE_TILE;
-    int32_t x[EDGE_CHUNKS][4], y[EDGE_CHUNKS][4];
-    int cnt[EDGE_CHUNKS];
-    // stream the tile's endpoints: 16-byte loads, all issued before use
-#pragma unroll
-    for (int c = 0; c < EDGE_CHUNKS; ++c) {
-      const int64_t e = e_tile + ((int64_t)c * EDGE_BLOCK + threadIdx.x) * 4;
-      if (e + 3 < L.n) {
-        const v4i vx = __builtin_nontemporal_load((const v4i*)(L.p1 + e));
-        const v4i vy = __builtin_nontemporal_load((const v4i*)(L.p2 + e));
-        x[c][0] = vx[0]; x[c][1] = vx[1]; x[c][2] = vx[2]; x[c][3] = vx[3];
-        y[c][0] = vy[0]; y[c][1] = vy[1]; y[c][2] = vy[2]; y[c][3] = vy[3];
-        cnt[c] = 4;
-      } else {
-        cnt[c] = e < L.n ? (int)(L.n - e) : 0;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          x[c][q] = q < cnt[c] ? L.p1[e + q] : 0;
-          y[c][q] = q < cnt[c] ? L.p2[e + q] : 0;
-        }
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < EDGE_CHUNKS; ++c) {
-      // candidate bits (q*2 + dir): infectious source and susceptible target
-      uint32_t cand = 0;
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint32_t xi = (uint32_t)x[c][q], yi = (uint32_t)y[c][q];
-        // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-        if (q < cnt[c] && xi < n_agents && yi < n_agents) {
-          if (inf_bit<LDS>(a, s_bits, xi) && sus_bit(a, yi)) cand |= 1u << (2 * q);
-          if (inf_bit<LDS>(a, s_bits, yi) && sus_bit(a, xi)) cand |= 2u << (2 * q);
-        }
-      }
-      // wave-local compaction of the candidates into the LDS queue (ballot + mbcnt)
-      uint32_t total = 0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const bool has = (cand >> b) & 1u;
-        const uint64_t m = __ballot(has);
-        if (has) s_q[total + lanes_below(m)] = ((uint32_t)(c * EDGE_BLOCK + threadIdx.x) * 4 + (b >> 1)) * 2 + (b & 1);
-        total += (uint32_t)__popcll(m);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      for (uint32_t j = lane; j < total; j += 64) {
-        const uint32_t code = s_q[j];
-        const int dir = code & 1;
-        const int64_t e = e_tile + (code >> 1);
-        const int32_t p1v = L.p1[e], p2v = L.p2[e];
-        edge_candidate(a, L, l, dir, e, dir ? p2v : p1v, dir ? p1v : p2v);
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+    load_tile(a, s_tiles[par], nxt);
+    par ^= 1;
+    process_tile<LDS>(a, s_bits, s_q, cur);
+    cur = nxt;
   }
 }
 
@@ -834,6 +960,8 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
 // ===========================================================================
 // host side
 // ===========================================================================
+static int g_edge_lds = 1;  // tunable: stage the infectious bitmap in LDS when it fits
+
 struct DeviceCaps {
   int device = -1;
   int n_cu = 0;
@@ -852,6 +980,11 @@ static DeviceCaps& caps() {
     c.max_lds = lds;
   }
   return c;
+}
+
+static int grid_for(int64_t n, int block) {
+  const int64_t g = (n + block - 1) / block;
+  return (int)(g < 1 ? 1 : g);
 }
 
 static int64_t bitmap_words_for(int64_t n) { return ((n + 127) / 128) * 4; }  // multiple of 4 words
@@ -874,6 +1007,8 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
   if (!people_ok(P)) return fail(CVK_E_INVALID, "people struct has a null pointer or n <= 0");
   if (!prm || prm->n_agents != P->n) return fail(CVK_E_INVALID, "params.n_agents does not match people.n");
   if (P->n >= (1ll << 31) - 1) return fail(CVK_E_INVALID, "more than 2^31-2 agents");
+  if ((int64_t)grid_for(P->n, AGENT_BLOCK * AGENTS_PER_THREAD) > CVK_MAX_REDUCE_BLOCKS)
+    return fail(CVK_E_CAPACITY, "too many agents for the reduction workspace (CVK_MAX_REDUCE_BLOCKS)");
   if (prm->n_variants < 1 || prm->n_variants > CVK_MAX_VARIANTS)
     return fail(CVK_E_INVALID, "n_variants must be in [1, 8]");
   if (prm->n_layers < 0 || prm->n_layers > CVK_MAX_LAYERS) return fail(CVK_E_INVALID, "n_layers must be in [0, 8]");
@@ -886,15 +1021,7 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
   return CVK_OK;
 }
 
-static int grid_for(int64_t n, int block) {
-  const int64_t g = (n + block - 1) / block;
-  return (int)(g < 1 ? 1 : g);
-}
-
-static uint32_t agent_blocks_for(int64_t n) {
-  return (uint32_t)std::min<int64_t>(grid_for(n, AGENT_BLOCK),
-                                     std::min<int64_t>(CVK_MAX_REDUCE_BLOCKS, 4 * std::max(caps().n_cu, 1)));
-}
+static uint32_t agent_blocks_for(int64_t n) { return (uint32_t)grid_for(n, AGENT_BLOCK * AGENTS_PER_THREAD); }
 
 static int day_reduce_launch(cvk_work* work, const DayReduce& r, hipStream_t st) {
   if (!r.n_blocks) return CVK_OK;
@@ -942,6 +1069,16 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
 using namespace cvk;
 
 extern "C" const char* cvk_last_error(void) { return last_error_buffer(); }
+
+extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
+  switch (key) {
+    case CVK_TUNE_EDGE_LDS:
+      g_edge_lds = value ? 1 : 0;
+      return CVK_OK;
+    default:
+      return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
+  }
+}
 extern "C" int cvk_version(void) { return 2; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
@@ -1120,7 +1257,7 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   int64_t budget = 0;
   int rc0 = edge_dynamic_lds(true, &budget);
   if (rc0) return rc0;
-  const bool use_lds = lds_bytes <= budget;
+  const bool use_lds = lds_bytes <= budget && g_edge_lds != 0;
   int64_t tiles = 0;
   for (int l = 0; l < params->n_layers; ++l) {
     const cvk_layer& L = layers[l];
@@ -1135,6 +1272,7 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     a.layer[l].beta_layer = params->beta_layer[l];
     a.layer[l].iso_factor = params->iso_factor[l];
     a.layer[l].quar_factor = params->quar_factor[l];
+    a.layer[l].beta_ones = L.beta_ones;
     a.tile_start[l] = tiles;
     tiles += (L.n_edges + EDGE_TILE - 1) / EDGE_TILE;
   }

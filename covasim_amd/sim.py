@@ -244,6 +244,7 @@ class Sim:
             layers[li].beta = lay.tensor('beta').data_ptr()
             layers[li].n_edges = len(lay)
             layers[li].dynamic = 1 if (dyn.get(lk, 0) and type(lay).update is Layer.update) else 0
+            layers[li].beta_ones = 1 if (layers[li].dynamic or lay.beta_all_ones()) else 0
         self._layers = layers
         # import schedule of the variants (immunity.py:117-130)
         days, vars_, nimp = [], [], []
@@ -270,6 +271,7 @@ class Sim:
             self._layers[li].p2 = lay.tensor('p2').data_ptr()
             self._layers[li].beta = lay.tensor('beta').data_ptr()
             self._layers[li].n_edges = len(lay)
+            self._layers[li].beta_ones = 1 if lay.beta_all_ones() else 0
 
     def _manual_day(self, t):
         """One day through the individual entry points (used by step() and custom layer updates)."""
@@ -302,6 +304,8 @@ class Sim:
             self.initialize()
         if self.complete:
             raise AlreadyRunError('Simulation already complete (call sim.initialize() to re-run)')
+        # parameters edited in place between days (e.g. beta_layer[k] *= f, interventions.py:575-586) apply
+        self._build_params()
         self._manual_day(self.t)
         self.t += 1
         self.people.t = self.t

@@ -165,8 +165,8 @@ typedef struct {
   int32_t* p2;
   float* beta;
   int64_t n_edges;
-  int32_t dynamic;     /* resampled each day (dynam_layer) */
-  int32_t pad;
+  int32_t dynamic;     /* resampled each day (dynam_layer); the resample sets every beta to 1 */
+  int32_t beta_ones;   /* nonzero when every beta[e] is 1.0f (the edge pass then skips the beta gather) */
 } cvk_layer;
 
 /* Device workspace (sizes from cvk_workspace_sizes). */
@@ -189,7 +189,7 @@ typedef struct {
   int32_t pad;
 } cvk_work;
 
-#define CVK_MAX_REDUCE_BLOCKS 1024
+#define CVK_MAX_REDUCE_BLOCKS 65536
 #define CVK_PARTIAL_COLS 64
 
 /* control word indices of cvk_work.ctl */
@@ -209,6 +209,10 @@ int cvk_version(void);
 /* No reference counterpart: ABI self-description for bindings. Writes sizeof/offsetof values of
  * the structs above into out[0..m) and returns m (see cvk_day.hip for the order). */
 int cvk_abi_layout(int64_t* out, int32_t n);
+/* No reference counterpart: launch tunables (process-wide). CVK_TUNE_EDGE_LDS = 0 keeps the edge
+ * pass's infectious bitmap in global memory (L2) even when it fits in LDS; 1 (default) stages it. */
+#define CVK_TUNE_EDGE_LDS 1
+int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);
 /* Sizes in bytes of the workspace arrays for n agents and npts days. */

@@ -9,14 +9,17 @@ from oracle import covasim_oracle as co
 PERSON_KEYS = ('symp_prob', 'severe_prob', 'crit_prob', 'death_prob', 'rel_sus', 'rel_trans')
 
 
-def make_population(n=2000, seed=1, pop_type='random', households='poisson', contacts=None):
+def make_population(n=2000, seed=1, pop_type='random', households='poisson', contacts=None, weighted=False):
     pars = cvpar.make_pars(pop_size=n, pop_type=pop_type, contacts=contacts or {})
     pars['prognoses'] = cvpar.get_prognoses()
     rng = np.random.default_rng(seed + 12345)
     pop = cvpop.make_randpop(pars, rng, households=households)
     pop.update(cvpop.make_people_arrays(pop['age'], pars, rng))
     for lay in pop['contacts'].values():
-        lay['beta'] = np.ones(len(lay['p1']), np.float32)
+        if weighted:  # per-edge weights other than 1 (base.py:1671-1674 allows any float32)
+            lay['beta'] = rng.choice(np.array([0.5, 1.0, 2.0, 0.0], np.float32), len(lay['p1']))
+        else:
+            lay['beta'] = np.ones(len(lay['p1']), np.float32)
     return pop
 
 

@@ -149,6 +149,8 @@ CONFIGS = {
     'variants_waning': dict(pop=dict(n=4000, pop_type='hybrid'),
                             pars=dict(use_waning=True, beta=0.02,
                                       variants=[dict(rel_beta=1.7, days=[10], n_imports=30)])),
+    'weighted_edges_l2_bitmap': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
+                                     pars=dict(use_waning=True, beta=0.04), edge_lds=0),
     'beds_limited': dict(pop=dict(n=3000, pop_type='random'),
                          pars=dict(use_waning=False, beta=0.08, n_beds_hosp=2, n_beds_icu=1,
                                    rel_severe_prob=20.0, rel_crit_prob=5.0)),
@@ -157,12 +159,17 @@ CONFIGS = {
 
 @pytest.mark.parametrize('name', list(CONFIGS))
 def test_full_sim_matches_oracle(gpu, name):
+    from covasim_amd import _lib
     cfg = CONFIGS[name]
     pop = make_population(**cfg['pop'])
     pars = dict(cfg['pars'])
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
-    sim, osim, ores = run_pair(pop, n_days=45, seed=11, variants=variants, **pars)
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, cfg.get('edge_lds', 1)), 'tuning')
+    try:
+        sim, osim, ores = run_pair(pop, n_days=45, seed=11, variants=variants, **pars)
+    finally:
+        _lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, 1)
     assert sim.results['cum_infections'].values[-1] > 50  # the epidemic took off
     assert_results_equal(sim, ores, nv=1 + len(variants))
     assert_people_equal(sim, osim)

@@ -50,6 +50,84 @@ def test_uniform_and_integer_transforms():
     assert abs(z.mean()) < 0.01 and abs(z.std() - 1) < 0.01
 
 
+def _chi2_pvalue(observed, expected):
+    from scipy import stats
+    keep = expected > 5
+    obs = np.append(observed[keep], observed[~keep].sum())
+    exp = np.append(expected[keep], expected[~keep].sum())
+    if exp[-1] == 0:
+        obs, exp = obs[:-1], exp[:-1]
+    exp = exp * obs.sum() / exp.sum()
+    return stats.chisquare(obs, exp).pvalue
+
+
+@pytest.mark.parametrize('rng_cls', [co.PhiloxRNG, co.RefRNG])
+def test_lognormal_int_durations_match_scipy(rng_cls):
+    """sample('lognormal_int', mean, std) (utils.py:223-231): the mean/std are those of the lognormal itself,
+    mapped to (mu, sigma) of the underlying normal, then rounded. Compared with scipy's lognorm."""
+    from scipy import stats
+    rng = rng_cls(3)
+    for par1, par2 in [(4.5, 1.5), (8.0, 2.0), (18.1, 6.3)]:  # exp2inf, sym2sev, crit2die defaults
+        x = rng.sample(dict(dist='lognormal_int', par1=par1, par2=par2), np.arange(100_000), 4, 3)
+        mu, sigma = co.lognormal_params(par1, par2)
+        ref = stats.lognorm(s=sigma, scale=np.exp(mu))
+        assert abs(ref.mean() - par1) < 1e-9 and abs(ref.std() - par2) < 1e-9
+        k = np.arange(0, int(x.max()) + 2)
+        expected = (ref.cdf(k + 0.5) - ref.cdf(k - 0.5)) * len(x)
+        observed = np.bincount(x.astype(np.int64), minlength=len(k))[:len(k)].astype(float)
+        assert _chi2_pvalue(observed, expected) > 1e-4, (par1, par2)
+        xc = rng.sample(dict(dist='lognormal', par1=par1, par2=par2), np.arange(50_000), 5, 2)
+        assert stats.kstest(xc, ref.cdf).pvalue > 1e-4
+
+
+def test_normal_family_durations_match_scipy():
+    from scipy import stats
+    rng = co.PhiloxRNG(11)
+    n = 60_000
+    x = rng.sample(dict(dist='normal', par1=2.0, par2=3.0), np.arange(n), 1, 0)
+    assert stats.kstest(x, stats.norm(2.0, 3.0).cdf).pvalue > 1e-4
+    x = rng.sample(dict(dist='normal_pos', par1=1.0, par2=2.0), np.arange(n), 1, 1)
+    assert stats.kstest(x, stats.foldnorm(0.5, scale=2.0).cdf).pvalue > 1e-4
+    x = rng.sample(dict(dist='uniform', par1=2.0, par2=5.0), np.arange(n), 1, 2)
+    assert stats.kstest(x, stats.uniform(2.0, 3.0).cdf).pvalue > 1e-4
+    # par1 = 0 lognormal: all zeros (utils.py:228-229)
+    assert np.all(rng.sample(dict(dist='lognormal_int', par1=0.0, par2=1.0), np.arange(10), 1, 3) == 0)
+
+
+def test_bernoulli_rates_and_independent_halves():
+    """binomial_arr (utils.py:343-357) is rand < p; each slot's two halves are independent draws."""
+    from scipy import stats
+    rng = co.PhiloxRNG(21)
+    n = 200_000
+    for p in (0.02, 0.3, 0.9):
+        for half in (0, 1):
+            k = int(rng.bernoulli(np.full(n, p), np.arange(n), 7, 1, half).sum())
+            assert stats.binomtest(k, n, p).pvalue > 1e-4
+    u0, u1 = px.slot_uniforms(21, 7, np.arange(n), 1)
+    assert abs(np.corrcoef(u0, u1)[0, 1]) < 0.01
+    # p above 1 always succeeds and p = 0 never does (test_mortality.py:21-22 uses p = 1e6)
+    assert rng.bernoulli(np.full(100, 1e6), np.arange(100), 7, 2, 0).all()
+    assert not rng.bernoulli(np.zeros(100), np.arange(100), 7, 2, 1).any()
+
+
+def test_resample_endpoints_uniform():
+    """Layer.update with frac = 1 (base.py:1849-1876): every edge gets iid uniform endpoints, self-loops allowed."""
+    n_agents, n_edges = 1000, 300_000
+    p1, p2 = px.resample_edges(4, 9, 3, n_edges, n_agents)
+    assert p1.dtype == np.int32 and len(p1) == n_edges and len(p2) == n_edges
+    assert p1.min() >= 0 and p2.max() < n_agents
+    exp = np.full(n_agents, n_edges / n_agents)
+    assert _chi2_pvalue(np.bincount(p1, minlength=n_agents).astype(float), exp) > 1e-4
+    assert _chi2_pvalue(np.bincount(p2, minlength=n_agents).astype(float), exp) > 1e-4
+    # pairs independent: joint counts over 10 x 10 coarse cells
+    joint = np.bincount((p1 // 100) * 10 + p2 // 100, minlength=100).astype(float)
+    assert _chi2_pvalue(joint, np.full(100, n_edges / 100)) > 1e-4
+    loops = int((p1 == p2).sum())
+    assert abs(loops - n_edges / n_agents) < 5 * np.sqrt(n_edges / n_agents)
+    q1, _ = px.resample_edges(4, 10, 3, n_edges, n_agents)
+    assert (q1 != p1).mean() > 0.99  # a fresh draw each day
+
+
 def test_choose_k_is_uniform_subset():
     counts = np.zeros(50)
     for s in range(400):
