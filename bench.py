@@ -37,6 +37,11 @@ WORKLOADS = {
                                variants=[dict(rel_beta=1.7, days=[30], n_imports=100)],
                                desc='hybrid 10M agents, 120 days, waning NAbs, 2 variants (second rel_beta 1.7, '
                                     '100 imports on day 30); layers as hybrid1m'),
+    'batch256': dict(n_sims=256, pop_lo=10_000, pop_hi=500_000, beta_lo=0.010, beta_hi=0.025, n_days=120,
+                     pop_type='hybrid', households='categorical',
+                     pars=dict(use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
+                     desc='256 independent hybrid sims, pop sizes log-uniform over 10k-500k, beta uniform over '
+                          '0.010-0.025, per-sim seeds, 120 days, layers as hybrid1m, one launch per phase per day'),
     'random20k': dict(pop_size=20_000, pop_type='random', n_days=60, households='poisson',
                       pars=dict(beta=0.016, use_waning=True, pop_infected=20),
                       desc='random 20k agents, one layer a with Poisson(20) contacts, 60 days, beta 0.016'),
@@ -83,6 +88,83 @@ def cpu_baseline(w, pop, seed, budget_s, omp=False):
                        f'population, days 0..{days - 1} of the workload, {el:.1f} s on {threads} host thread(s)')
 
 
+def run_batch(w, args):
+    """Batched many-sim workload: all sims per launch vs the same sims run one at a time."""
+    import torch
+    import covasim_amd as cv
+    from covasim_amd import _lib
+    rng = np.random.default_rng(args.seed)
+    S = w['n_sims']
+    sizes = np.exp(rng.uniform(np.log(w['pop_lo']), np.log(w['pop_hi']), S)).astype(np.int64)
+    betas = rng.uniform(w['beta_lo'], w['beta_hi'], S)
+    t0 = time.perf_counter()
+    sims = []
+    for i in range(S):
+        pop = build_population(dict(w, pop_size=int(sizes[i])), args.seed * 100003 + i)
+        sim = cv.Sim(pop_size=int(sizes[i]), pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=args.seed + i,
+                     beta=float(betas[i]), popdict=pop, households=w['households'], log_capacity=1024, **w['pars'])
+        sim.initialize()
+        sims.append(sim)
+        if (i + 1) % 32 == 0:
+            print(f'built {i + 1}/{S} sims ({time.perf_counter() - t0:.1f} s)', file=sys.stderr, flush=True)
+    batch = cv.BatchSim(sims)
+    setup_s = time.perf_counter() - t0
+    n_agents = int(sizes.sum())
+    n_edges = int(batch.edge_counts.sum())
+    steps = w['n_days'] + 1
+    tuned = batch.autotune(days=8)
+    print(f'autotune: {batch.tuning}', file=sys.stderr, flush=True)
+    stream = torch.cuda.Stream()
+    L = _lib.lib()
+    snap = batch.snapshot_state()
+    with torch.cuda.stream(stream):
+        gexec = batch.graph_create(stream)
+        batch.restore_state(snap)  # warmup replay
+        _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'batch graph launch')
+        torch.cuda.synchronize()
+        batch.restore_state(snap)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'batch graph launch')
+        torch.cuda.synchronize()
+        wall_batch = time.perf_counter() - t1
+        _lib.check(L.cvk_graph_destroy(gexec), 'graph destroy')
+        res, _ = batch.raw_results()
+        batch_inf = res[:, :, _lib.R_COL['new_infections']].sum(axis=1).astype(np.int64)
+        # the same sims one at a time: one captured graph per sim, launched back to back
+        execs = []
+        for sim in sims:
+            e = C.c_void_p()
+            _lib.check(L.cvk_graph_create(sim.people.struct(), sim._layers, sim._params, sim._work, 0, sim.npts,
+                                          int(sim.pars['pop_infected']), sim._imports,
+                                          C.c_void_p(stream.cuda_stream), C.byref(e)), 'graph create')
+            execs.append(e)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        for e in execs:
+            _lib.check(L.cvk_graph_launch(e, C.c_void_p(stream.cuda_stream)), 'graph launch')
+        torch.cuda.synchronize()
+        wall_single = time.perf_counter() - t2
+        for e in execs:
+            L.cvk_graph_destroy(e)
+    single_inf = np.array([int(sim._w['results'][:, _lib.R_COL['new_infections']].sum().item()) for sim in sims])
+    value = n_agents * steps / wall_batch
+    single = n_agents * steps / wall_single
+    out = {
+        'metric': METRIC, 'value': value, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': steps, 'warmup': steps,
+        'ms_per_step': wall_batch / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': w['desc'], 'sims': S, 'agents': n_agents, 'edges': n_edges, 'days': steps,
+                   'setup_s': round(setup_s, 1)},
+        'batched': {'sims_per_s': S / wall_batch, 'agent_days_per_s': value, 'wall_s': wall_batch,
+                    'one_at_a_time_sims_per_s': S / wall_single, 'one_at_a_time_agent_days_per_s': single,
+                    'one_at_a_time_wall_s': wall_single, 'speedup': wall_single / wall_batch,
+                    'identical_infection_totals': bool(np.array_equal(batch_inf, single_inf)),
+                    'autotuned_shape': list(tuned), 'autotune_trials_ms': batch.tuning['trials_ms']},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -104,6 +186,8 @@ def main():
     from covasim_amd import _lib
 
     w = WORKLOADS[args.workload]
+    if 'n_sims' in w:
+        return run_batch(w, args)
     steps = args.steps or w['n_days']
     warmup = steps if args.warmup is None else args.warmup
     n_days = max(w['n_days'], steps)

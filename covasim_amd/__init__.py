@@ -12,9 +12,10 @@ from .immunity import variant
 from .base import Result, Layer, Contacts
 from .people import People
 from .sim import Sim, AlreadyRunError
+from .batch import BatchSim
 from . import utils
 from .utils import set_seed
 
 __all__ = ['__version__', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
-           'get_cross_immunity', 'variant', 'Result', 'Layer', 'Contacts', 'People', 'Sim', 'AlreadyRunError',
+           'get_cross_immunity', 'variant', 'Result', 'Layer', 'Contacts', 'People', 'Sim', 'AlreadyRunError', 'BatchSim',
            'utils', 'set_seed']

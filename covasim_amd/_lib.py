@@ -153,6 +153,36 @@ class ImportsS(C.Structure):
                 ('n_imports', C.POINTER(C.c_int64))]
 
 
+BATCH_MAX_SIMS = 4096
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [('n_sims', C.c_int32), ('n_layers', C.c_int32), ('n_variants', C.c_int32), ('use_waning', C.c_int32),
+                ('n_agents_total', C.c_int64), ('agent_offsets_host', C.c_void_p), ('edge_starts_host', C.c_void_p),
+                ('edge_counts_host', C.c_void_p), ('params_host', C.c_void_p), ('dynamic_host', C.c_void_p),
+                ('people', PeopleS), ('p1', C.c_void_p * MAX_LAYERS), ('p2', C.c_void_p * MAX_LAYERS),
+                ('beta', C.c_void_p * MAX_LAYERS), ('beta_ones', C.c_int32 * MAX_LAYERS)]
+
+
+class BatchImports(C.Structure):
+    _fields_ = [('n_events', C.c_int32), ('days', C.c_void_p), ('variants', C.c_void_p), ('counts', C.c_void_p)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [('edge_block', C.c_int32), ('edge_chunks', C.c_int32), ('edge_blocks_per_cu', C.c_int32),
+                ('agents_per_thread', C.c_int32)]
+
+    def as_tuple(self):
+        return (self.edge_block, self.edge_chunks, self.edge_blocks_per_cu, self.agents_per_thread)
+
+
+class RelaxDesc(C.Structure):
+    _fields_ = [('n_agents', C.c_int64), ('n_layers', C.c_int32), ('n_days', C.c_int32), ('kernel_len', C.c_int32),
+                ('checkpoint_every', C.c_int32), ('precision', C.c_int32), ('pad', C.c_int32),
+                ('seed_frac', C.c_double), ('indptr', C.c_void_p * MAX_LAYERS), ('nbr', C.c_void_p * MAX_LAYERS),
+                ('age_bin', C.c_void_p), ('rel_trans', C.c_void_p), ('w_host', C.c_void_p)]
+
+
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libcovasim_amd.so')
 
@@ -192,6 +222,26 @@ def lib():
     L.cvk_graph_create.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), vp, C.POINTER(C.c_void_p)]
     L.cvk_graph_launch.argtypes = [vp, vp]
     L.cvk_graph_destroy.argtypes = [vp]
+    BD, BI, BC = C.POINTER(BatchDesc), C.POINTER(BatchImports), C.POINTER(BatchConfig)
+    L.cvk_batch_create.argtypes = [BD, i32, i64, C.POINTER(C.c_void_p)]
+    L.cvk_batch_destroy.argtypes = [vp]
+    L.cvk_batch_set_config.argtypes = [vp, BC]
+    L.cvk_batch_get_config.argtypes = [vp, BC]
+    L.cvk_batch_reset.argtypes = [vp, vp]
+    L.cvk_batch_run_days.argtypes = [vp, i32, i32, vp, BI, i32, vp]
+    L.cvk_batch_graph_create.argtypes = [vp, i32, i32, vp, BI, vp, C.POINTER(C.c_void_p)]
+    L.cvk_batch_copy_results.argtypes = [vp, vp, vp]
+    L.cvk_batch_copy_log.argtypes = [vp, vp, i64, C.POINTER(C.c_int64)]
+    L.cvk_relax_create.argtypes = [C.POINTER(RelaxDesc), C.POINTER(C.c_void_p)]
+    L.cvk_relax_destroy.argtypes = [vp]
+    L.cvk_relax_forward.argtypes = [vp, vp, vp, vp]
+    L.cvk_relax_backward.argtypes = [vp, vp, vp, vp, vp]
+    for name in ['cvk_relax_create', 'cvk_relax_destroy', 'cvk_relax_forward', 'cvk_relax_backward']:
+        getattr(L, name).restype = C.c_int
+    for name in ['cvk_batch_create', 'cvk_batch_destroy', 'cvk_batch_set_config', 'cvk_batch_get_config',
+                 'cvk_batch_reset', 'cvk_batch_run_days', 'cvk_batch_graph_create', 'cvk_batch_copy_results',
+                 'cvk_batch_copy_log']:
+        getattr(L, name).restype = C.c_int
     for name in ['cvk_set_tuning', 'cvk_graph_create', 'cvk_graph_launch', 'cvk_graph_destroy', 'cvk_workspace_sizes2',
                  'cvk_workspace_sizes', 'cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_choose',
                  'cvk_agent_pass', 'cvk_resample_layer', 'cvk_edge_pass', 'cvk_infect_pass', 'cvk_count_pass',
@@ -209,7 +259,8 @@ def _check_layout(L):
     want = [C.sizeof(Dur), C.sizeof(Params), C.sizeof(PeopleS), C.sizeof(LayerS), C.sizeof(WorkS),
             C.sizeof(ImportsS), Params.dur.offset, Params.nab_init.offset, Params.rel_imm_symp.offset,
             Params.immunity.offset, Params.nab_kin.offset, PeopleS.sev_imm.offset, WorkS.log_capacity.offset,
-            WorkS.npts.offset, R_COLS, RF_COLS]
+            WorkS.npts.offset, R_COLS, RF_COLS, C.sizeof(BatchDesc), BatchDesc.people.offset, C.sizeof(BatchImports),
+            C.sizeof(BatchConfig), C.sizeof(RelaxDesc)]
     if got != want:
         raise RuntimeError(f'covasim_amd: ctypes struct layout mismatch with the library: {got} != {want}')
 

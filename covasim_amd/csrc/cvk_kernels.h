@@ -1,0 +1,682 @@
+// Device building blocks of the day loop, shared by the single-sim kernels (cvk_day.hip) and
+// the batched many-sim kernels (cvk_batch.hip). Each block works on one sim at a time through a
+// view whose pointers are already offset to that sim's slice, so everything below indexes agents
+// and edges by their index within the sim: counter-based draws and transmission keys are
+// therefore identical whether a sim runs alone or inside a batch.
+#pragma once
+
+#include "cvk_device.h"
+
+namespace cvk {
+
+constexpr uint32_t STATE_CLEAR_ON_RECOVERY = CVK_EXPOSED | CVK_INFECTIOUS | CVK_SYMPTOMATIC | CVK_SEVERE | CVK_CRITICAL;
+constexpr uint32_t STATE_CLEAR_ON_DEATH = CVK_SUSCEPTIBLE | CVK_EXPOSED | CVK_INFECTIOUS | CVK_SYMPTOMATIC |
+                                          CVK_SEVERE | CVK_CRITICAL | CVK_KNOWN_CONTACT | CVK_QUARANTINED |
+                                          CVK_RECOVERED;
+constexpr uint32_t INFVAR_MASK = 0xFu << CVK_INFVAR_SHIFT;
+constexpr int MAXV = CVK_MAX_VARIANTS;
+constexpr int AGENT_BLOCK = 256;
+
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ inline uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_sum_d(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// calc_VE, immunity.py:216-247, in float64 then stored as float32.
+__device__ inline float calc_ve(double nab, double exp_alpha, double beta) {
+  const double exp_lo = exp_alpha * pow(nab, beta);
+  return (float)(exp_lo / (1.0 + exp_lo));
+}
+
+// ---------------------------------------------------------------------------
+// Reductions without device-scope fences: every block of a reducing kernel stores its partial
+// row (column-major, `stride` entries per column); the rows are summed after the kernel
+// boundary. Integer columns are order-free; float columns are summed in a fixed order.
+// Sums blocks [b0, b0 + nb) of every column (one wave per column, 8 loads in flight per lane);
+// totals land in s_tot[NC] / s_totf[ND]. Block-wide (contains a barrier).
+// ---------------------------------------------------------------------------
+template <int NC, int ND>
+__device__ void sum_partials(const uint32_t* partials, const double* partials_f, int64_t stride, uint32_t b0,
+                             uint32_t nb, uint32_t* s_tot, double* s_totf) {
+  const int n_waves = (int)(blockDim.x >> 6);
+  const int w = threadIdx.x >> 6;
+  for (int k = w; k < NC + ND; k += n_waves) {
+    if (k < NC) {
+      const uint32_t* col = partials + (size_t)k * stride + b0;
+      uint32_t acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t c0 = 0; c0 < nb; c0 += 512) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t b = c0 + j * 64 + lane_id();
+          acc[j] += b < nb ? col[b] : 0u;
+        }
+      }
+      uint32_t tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+      tot = wave_sum(tot);
+      if (lane_id() == 0) s_tot[k] = tot;
+    } else {
+      const double* col = partials_f + (size_t)(k - NC) * stride + b0;
+      double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      for (uint32_t c0 = 0; c0 < nb; c0 += 512) {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const uint32_t b = c0 + j * 64 + lane_id();
+          acc[j] += b < nb ? col[b] : 0.0;
+        }
+      }
+      double tot = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+      tot = wave_sum_d(tot);
+      if (lane_id() == 0) s_totf[k - NC] = tot;
+    }
+  }
+  __syncthreads();
+}
+
+// ===========================================================================
+// Agent pass: [previous-day stocks + update_nab] + transitions + bitmaps + edge records
+// ===========================================================================
+// stock columns (0..29) and day columns (30..46) of the partial rows
+constexpr int SC_ALIVE = 13, SC_EXPV = 14, SC_INFV = 14 + MAXV, N_STOCK_COLS = 14 + 2 * MAXV;
+constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV, N_AGENT_COLS = N_STOCK_COLS + N_DAY_COLS;
+constexpr int N_AGENT_FCOLS = 3;
+static_assert(N_AGENT_COLS <= CVK_PARTIAL_COLS, "partial row too small");
+
+// Per-agent inputs, all loaded before any is used (one memory round trip).
+struct AgentIn {
+  uint32_t f, vs;
+  float d_inf, d_sym, d_sev, d_crit, d_rec, d_dead, d_iso, rt, rs, peak, nab;
+  int32_t tne;
+};
+
+__device__ inline void load_agent(const cvk_people& P, int64_t g, bool waning, AgentIn& x) {
+  x.f = P.flags[g];
+  x.vs = P.vstate[g];
+  x.d_inf = P.date_infectious[g];
+  x.d_sym = P.date_symptomatic[g];
+  x.d_sev = P.date_severe[g];
+  x.d_crit = P.date_critical[g];
+  x.d_rec = P.date_recovered[g];
+  x.d_dead = P.date_dead[g];
+  x.d_iso = P.date_end_isolation[g];
+  x.rt = P.rel_trans[g];
+  x.rs = P.rel_sus[g];
+  if (waning) {
+    x.peak = P.peak_nab[g];
+    x.nab = P.nab[g];
+    x.tne = P.t_nab_event[g];
+  } else {
+    x.peak = 0.0f;
+    x.nab = 0.0f;
+    x.tne = 0;
+  }
+}
+
+// The agents one block of the agent pass handles: AGENT_BLOCK * APT consecutive agents of one sim.
+struct AgentSpan {
+  int64_t g0;          // storage index of the sim's agent 0 (the block's agents are g0 + li)
+  int64_t li0;         // first local index of the block
+  int64_t n_local;     // agents in the sim
+  int64_t n_stride;    // row stride of the [variant][agent] arrays (all agents in storage)
+  uint64_t* bitmap;    // the sim's infectious bitmap (64 agents per word, local indexing)
+  uint64_t* susmap;    // the sim's susceptible bitmap
+  uint64_t* rec;       // the sim's edge records (local indexing)
+  uint32_t* partials;  // column-major partial rows; this block writes entry `part_col` of each column
+  double* partials_f;
+  int64_t part_stride;
+  int64_t part_col;
+};
+
+// update_states_pre (people.py:164-312, 368-374) + check_immunity (immunity.py:303-350) + the
+// viral-load phase (utils.py:58-78) for day t, preceded (closing != 0) by update_nab and the
+// stock counts of day tc = transitions ? t - 1 : t (sim.py:651-674). Block-wide.
+template <int APT>
+__device__ void agent_block(const cvk_people& P, const cvk_params& prm, const AgentSpan& S, int32_t t,
+                            int32_t prologue, int32_t transitions) {
+  // per-wave counters: sums of per-agent booleans via ballot (wave-uniform, no shuffles)
+  uint32_t c[N_AGENT_COLS];
+#pragma unroll
+  for (int k = 0; k < N_AGENT_COLS; ++k) c[k] = 0;
+  double d[N_AGENT_FCOLS] = {0.0, 0.0, 0.0};
+  const int nv = prm.n_variants;
+  const bool waning = prm.use_waning != 0;
+  const float tf = (float)t;
+  const int64_t n_words = (S.n_local + 63) / 64;
+  const bool closing = prologue || !transitions;  // stocks + update_nab of the previous (or this) day
+  const int32_t tc = transitions ? t - 1 : t;
+
+  int64_t idx[APT];
+  AgentIn in[APT];
+#pragma unroll
+  for (int r = 0; r < APT; ++r) {
+    idx[r] = S.li0 + (int64_t)r * AGENT_BLOCK + threadIdx.x;
+    if (idx[r] < S.n_local) load_agent(P, S.g0 + idx[r], waning, in[r]);
+  }
+#pragma unroll
+  for (int r = 0; r < APT; ++r) {
+    const int64_t li = idx[r];
+    const int64_t g = S.g0 + li;
+    const bool valid = li < S.n_local;
+    bool inf_bit = false, sus_bit = false;
+    uint32_t f_close = 0, vs_close = 0;  // state counted as the previous day's stocks
+    uint32_t ev = 0;                     // this day's transition events (bits as the day columns)
+    uint32_t ev_var = CVK_VAR_NONE;      // variant of a new infectious agent
+    if (valid) {
+      const AgentIn& x = in[r];
+      uint32_t f = x.f;
+      uint32_t vs = x.vs;
+      f_close = f;
+      vs_close = vs;
+      if (closing && waning) {
+        float nab = x.nab;
+        if (x.peak != 0.0f) {  // has_nabs = true(peak_nab), sim.py:666-669; immunity.py:205-213
+          int32_t ts = tc - x.tne;
+          ts = ts < 0 ? 0 : (ts >= prm.nab_kin_len ? prm.nab_kin_len - 1 : ts);
+          nab = (float)((double)nab + prm.nab_kin[ts] * (double)x.peak);
+          nab = nab < 0.0f ? 0.0f : nab;
+          nab = nab > x.peak ? x.peak : nab;
+          P.nab[g] = nab;
+          for (int v = 0; v < nv; ++v) {  // population means of sus/symp_imm (sim.py:673-674)
+            d[1] += (double)P.sus_imm[(int64_t)v * S.n_stride + g];
+            d[2] += (double)P.symp_imm[(int64_t)v * S.n_stride + g];
+          }
+          in[r].nab = nab;
+        }
+        if (!(f & CVK_DEAD)) d[0] += (double)nab;
+      }
+      if (transitions) {
+        const uint32_t f0 = f, vs0 = vs;
+        const bool is_exp = (f & CVK_EXPOSED) != 0;  // people.py:169 snapshot
+        const float d_inf = x.d_inf, d_rec = x.d_rec, d_dead = x.d_dead;
+        if (is_exp) {
+          // check_infectious, people.py:222-232
+          if (!(f & CVK_INFECTIOUS) && !isnan(d_inf) && tf >= d_inf) {
+            const uint32_t evv = exp_var(vs);
+            f = ((f | CVK_INFECTIOUS) & ~INFVAR_MASK) | (evv << CVK_INFVAR_SHIFT);
+            if (evv < MAXV) vs |= 1u << (16 + evv);
+            ev |= 1u << 0;
+            ev_var = evv;
+          }
+          // check_symptomatic / severe / critical, people.py:235-253
+          if (!(f & CVK_SYMPTOMATIC) && !isnan(x.d_sym) && tf >= x.d_sym) { f |= CVK_SYMPTOMATIC; ev |= 1u << 1; }
+          if (!(f & CVK_SEVERE) && !isnan(x.d_sev) && tf >= x.d_sev) { f |= CVK_SEVERE; ev |= 1u << 2; }
+          if (!(f & CVK_CRITICAL) && !isnan(x.d_crit) && tf >= x.d_crit) { f |= CVK_CRITICAL; ev |= 1u << 3; }
+          // check_recovery, people.py:256-291
+          if (!(f & CVK_RECOVERED) && !isnan(d_rec) && tf >= d_rec) {
+            const uint32_t evv = exp_var(vs);
+            f = ((f & ~STATE_CLEAR_ON_RECOVERY) | CVK_RECOVERED) | INFVAR_MASK;
+            vs = (vs & ~0xFFFFFFu) | (evv << 4) | CVK_VAR_NONE;  // recovered_variant = exposed_variant
+            if (waning) f = (f | CVK_SUSCEPTIBLE) & ~CVK_DIAGNOSED;
+            ev |= 1u << 4;
+          }
+        }
+        // check_exit_iso, people.py:368-374 (not filtered by exposure)
+        if ((f & CVK_ISOLATED) && !isnan(x.d_iso) && tf >= x.d_iso) f &= ~CVK_ISOLATED;
+        // check_death, people.py:294-312
+        if (is_exp && !(f & CVK_DEAD) && !isnan(d_dead) && tf >= d_dead) {
+          f |= CVK_DEAD;
+          if (f & CVK_DIAGNOSED) { f |= CVK_KNOWN_DEAD; ev |= 1u << 6; }
+          f = (f & ~STATE_CLEAR_ON_DEATH) | INFVAR_MASK;
+          vs = (vs & ~0xFFu) | CVK_VAR_NONE | (CVK_VAR_NONE << 4);
+          ev |= 1u << 5;
+        }
+        // check_immunity, immunity.py:303-350 (no vaccine sources on this path)
+        if (waning && x.peak != 0.0f) {  // agents never infected keep sus/symp/sev_imm == 0
+          const double nab = (double)in[r].nab;
+          const bool was_inf = tf >= d_rec;
+          const uint32_t rv = rec_var(vs);
+          for (int v = 0; v < nv; ++v) {
+            const double imm = (was_inf && rv < MAXV) ? (double)prm.immunity[v][rv] : 0.0;
+            const double eff = nab * imm;
+            const int64_t off = (int64_t)v * S.n_stride + g;
+            P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
+            P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
+            P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
+          }
+        }
+        // viral-load phase (utils.py:58-78): early iff (t - start)/total < switch point
+        {
+          const float stop = isnan(d_dead) ? d_rec : d_dead;
+          const float total = stop - d_inf;
+          const float trans_day = prm.frac_time * total;
+          const float trans_point = trans_day > prm.high_cap ? prm.high_cap / total : prm.frac_time;
+          const bool early = (tf - d_inf) / total < trans_point;
+          f = early ? (f | CVK_EARLY) : (f & ~CVK_EARLY);
+        }
+        if (f != f0) P.flags[g] = f;
+        if (vs != vs0) P.vstate[g] = vs;
+        inf_bit = (f & CVK_INFECTIOUS) && x.rt != 0.0f;
+        sus_bit = (f & CVK_SUSCEPTIBLE) && x.rs != 0.0f;
+        const float val = (f & CVK_INFECTIOUS) ? x.rt : x.rs;
+        S.rec[li] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
+        if (f & CVK_SEVERE) ev |= 1u << 7;
+        if (f & CVK_CRITICAL) ev |= 1u << 8;
+      }
+    }
+    // wave counts (every lane reaches these ballots)
+    if (closing) {
+      const uint32_t stock_bits[13] = {CVK_SUSCEPTIBLE, CVK_EXPOSED, CVK_INFECTIOUS, CVK_SYMPTOMATIC, CVK_SEVERE,
+                                       CVK_CRITICAL, CVK_RECOVERED, CVK_DEAD, CVK_DIAGNOSED, CVK_KNOWN_DEAD,
+                                       CVK_QUARANTINED, CVK_ISOLATED, CVK_VACCINATED};
+#pragma unroll
+      for (int k = 0; k < 13; ++k) c[k] += (uint32_t)__popcll(__ballot(f_close & stock_bits[k]));
+      c[SC_ALIVE] += (uint32_t)__popcll(__ballot(valid && !(f_close & CVK_DEAD)));
+#pragma unroll
+      for (int v = 0; v < MAXV; ++v) {
+        if (v < nv) {  // nv is uniform: the ballots stay wave-wide
+          c[SC_EXPV + v] += (uint32_t)__popcll(__ballot((vs_close >> (8 + v)) & 1u));
+          c[SC_INFV + v] += (uint32_t)__popcll(__ballot((vs_close >> (16 + v)) & 1u));
+        }
+      }
+    }
+    if (transitions) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) c[DC_BASE + k] += (uint32_t)__popcll(__ballot((ev >> k) & 1u));
+#pragma unroll
+      for (int v = 0; v < MAXV; ++v)
+        if (v < nv) c[DC_BASE + 9 + v] += (uint32_t)__popcll(__ballot(ev_var == (uint32_t)v));
+      // one 64-bit ballot per wave covers 64 consecutive agents
+      const uint64_t ib = __ballot(inf_bit);
+      const uint64_t sb = __ballot(sus_bit);
+      const int64_t word = li >> 6;
+      if (lane_id() == 0 && word < n_words) {
+        S.bitmap[word] = ib;
+        S.susmap[word] = sb;
+      }
+    }
+  }
+
+  // block partial row: per-wave counts through LDS (no atomics), float sums in wave order
+  __shared__ uint32_t s_wc[AGENT_BLOCK / 64][N_AGENT_COLS];
+  __shared__ double s_wd[AGENT_BLOCK / 64][N_AGENT_FCOLS];
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < N_AGENT_COLS; ++k) s_wc[w][k] = c[k];
+  }
+  if (waning) {
+#pragma unroll
+    for (int k = 0; k < N_AGENT_FCOLS; ++k) {
+      const double v = wave_sum_d(d[k]);
+      if (lane_id() == 0) s_wd[w][k] = v;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < N_AGENT_COLS) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wc[j][threadIdx.x];
+    S.partials[(size_t)threadIdx.x * S.part_stride + S.part_col] = tot;
+  } else if (threadIdx.x >= 64 && threadIdx.x < 64 + N_AGENT_FCOLS) {
+    const int k = threadIdx.x - 64;
+    double tot = 0.0;
+    if (waning)
+      for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wd[j][k];
+    S.partials_f[(size_t)k * S.part_stride + S.part_col] = tot;
+  }
+}
+
+// Writes one sim's day rows from the summed agent-pass partial rows (block-wide; s_tot/s_totf
+// from sum_partials). results / results_f are the sim's [npts][cols] tables.
+__device__ inline void write_agent_rows(uint32_t* results, double* results_f, const uint32_t* s_tot,
+                                        const double* s_totf, int32_t t, int32_t prologue, int32_t transitions) {
+  if (prologue || !transitions) {
+    const int32_t tc = transitions ? t - 1 : t;
+    uint32_t* row = results + (int64_t)tc * CVK_R_COLS;
+    for (int k = threadIdx.x; k < N_STOCK_COLS; k += blockDim.x) {
+      int col;
+      if (k < 13) col = CVK_R_N_SUSCEPTIBLE + k;
+      else if (k == SC_ALIVE) col = CVK_R_N_ALIVE_NAB;
+      else if (k < SC_INFV) col = CVK_R_BYVAR + CVK_RV_N_EXPOSED * MAXV + (k - SC_EXPV);
+      else col = CVK_R_BYVAR + CVK_RV_N_INFECTIOUS * MAXV + (k - SC_INFV);
+      row[col] = s_tot[k];
+    }
+    if (threadIdx.x < N_AGENT_FCOLS) results_f[(int64_t)tc * CVK_RF_COLS + threadIdx.x] = s_totf[threadIdx.x];
+  }
+  if (transitions) {
+    uint32_t* row = results + (int64_t)t * CVK_R_COLS;
+    static constexpr int cols[9] = {CVK_R_NEW_INFECTIOUS, CVK_R_NEW_SYMPTOMATIC, CVK_R_NEW_SEVERE,
+                                    CVK_R_NEW_CRITICAL, CVK_R_NEW_RECOVERIES, CVK_R_NEW_DEATHS,
+                                    CVK_R_NEW_KNOWN_DEATHS, CVK_R_MID_SEVERE, CVK_R_MID_CRITICAL};
+    for (int k = threadIdx.x; k < N_DAY_COLS; k += blockDim.x) {
+      const int col = k < 9 ? cols[k] : CVK_R_BYVAR + CVK_RV_NEW_INFECTIOUS * MAXV + (k - 9);
+      row[col] = s_tot[DC_BASE + k];
+    }
+  }
+  __syncthreads();
+}
+
+// Dynamic-layer resample of one edge: iid uniform endpoints, beta = 1 (base.py:1849-1876).
+__device__ inline void resample_edge(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e,
+                                     int32_t* p1, int32_t* p2, float* beta) {
+  const uint64_t ue = (uint64_t)e;
+  const u32x4 w = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(ue >> 32));
+  p1[e] = (int32_t)below(w.x, w.y, n_agents);
+  p2[e] = (int32_t)below(w.z, w.w, n_agents);
+  beta[e] = 1.0f;
+}
+
+// Selection key of cvk_choose: philox word << 32 | local agent index (distinct per agent).
+__device__ inline uint64_t selection_key(uint64_t seed, int32_t t, uint32_t stream_id, uint32_t i) {
+  const u32x4 w = philox_seeded(seed, i, (uint32_t)t, stream_id, 0u);
+  return ((uint64_t)w.x << 32) | i;
+}
+
+// ===========================================================================
+// Edge pass: compute_trans_sus + compute_infections + first-occurrence dedup
+// ===========================================================================
+// One sim as the edge pass sees it (pointers offset to the sim's slices; local indexing).
+struct SimView {
+  const uint64_t* rec;
+  uint64_t* winner;
+  const float* sus_imm;     // row v at sus_imm + v * sus_stride
+  int64_t sus_stride;
+  const uint32_t* bitmap;
+  const uint32_t* susmap;
+  int32_t* touched;         // storage indices (a_off + local) of targets with a first key
+  uint32_t* touched_count;
+  int64_t a_off;
+  uint64_t seed;
+  uint32_t n_agents;
+  int32_t n_variants;
+  int32_t use_sus_imm;
+  float asymp_factor, vl_hi, vl_lo;
+  float beta_v[MAXV];
+};
+
+// One layer of one sim.
+struct SegView {
+  const int32_t* p1;
+  const int32_t* p2;
+  const float* beta;
+  const double* unif;  // [n_variants][2][n] supplied uniforms, or null
+  int64_t n;
+  float beta_layer, iso_factor, quar_factor;
+  int32_t beta_ones;   // every per-edge beta is 1.0f: no gather needed
+  int32_t l;           // layer index (key and counter field)
+  int32_t pad;
+};
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+constexpr int EDGE_QCAP = 64 * 8;  // per-wave candidate queue: 4 edges x 2 directions x 64 lanes
+
+// A candidate slot (infectious source, susceptible target) of edge e: the float32 products of
+// compute_trans_sus (utils.py:88-89) and compute_infections (utils.py:117), the float64 uniform
+// test (utils.py:123), and the first-occurrence dedup as an atomicMin.
+__device__ inline void edge_candidate(const SimView& S, const SegView& L, int32_t t, int dir, int64_t e, int32_t src,
+                                      int32_t tgt) {
+  const uint64_t rsrc = S.rec[src];
+  const uint64_t rtgt = S.rec[tgt];
+  const uint32_t fs = (uint32_t)(rsrc >> 32);
+  const uint32_t ft = (uint32_t)(rtgt >> 32);
+  const uint32_t v = inf_var(fs);
+  if (!(fs & CVK_INFECTIOUS) || v >= (uint32_t)S.n_variants || !(ft & CVK_SUSCEPTIBLE)) return;
+  float rt = __uint_as_float((uint32_t)rsrc);                   // rel_trans * inf (== 1)
+  if (fs & CVK_QUARANTINED) rt = rt * L.quar_factor;             // * f_quar
+  if (!(fs & CVK_SYMPTOMATIC)) rt = rt * S.asymp_factor;         // * f_asymp
+  if (fs & CVK_ISOLATED) rt = rt * L.iso_factor;                 // * f_iso
+  rt = rt * L.beta_layer;                                        // * beta_layer
+  rt = rt * ((fs & CVK_EARLY) ? S.vl_hi : S.vl_lo);              // * viral_load
+  if (rt == 0.0f) return;                                        // utils.py:116
+  float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
+  if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
+  if (S.use_sus_imm) rs = rs * (1.0f - S.sus_imm[(int64_t)v * S.sus_stride + tgt]);
+  const float eb = L.beta_ones ? 1.0f : L.beta[e];
+  const float p = ((S.beta_v[v] * eb) * rt) * rs;                // utils.py:117
+  if (p == 0.0f) return;                                         // utils.py:118
+  double u;
+  if (L.unif) {
+    u = L.unif[((int64_t)v * 2 + dir) * L.n + e];
+  } else {
+    const uint64_t ue = (uint64_t)e;
+    const u32x4 w = philox_seeded(S.seed, (uint32_t)ue, (uint32_t)t, STREAM_TX | (v << 8) | (uint32_t)L.l,
+                                  (uint32_t)(ue >> 32));
+    u = dir ? u01(w.z, w.w) : u01(w.x, w.y);
+  }
+  if (!(u < (double)p)) return;                                  // utils.py:123
+  const uint64_t key = make_key(1, v, (uint32_t)L.l, (uint32_t)dir, (uint64_t)e);
+  const uint64_t old = atomicMin((unsigned long long*)&S.winner[tgt], (unsigned long long)key);
+  if (old == KEY_NONE) S.touched[atomicAdd(S.touched_count, 1u)] = (int32_t)(S.a_off + tgt);
+}
+
+// One tile's endpoints for this thread, in vector registers (up to two 4-edge chunks).
+struct EdgeTile {
+  int64_t tile;     // tile number (>= the tile count: no tile)
+  int64_t e0;       // first edge of the tile in its layer
+  int32_t seg;      // segment (sim x layer) of the tile
+  int32_t n_valid;  // edges of the tile that exist (the last tile of a layer is partial)
+  v4i x0, y0, x1, y1;
+};
+
+__device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
+  if (e + 3 < n) return __builtin_nontemporal_load((const v4i*)(p + e));
+  v4i r = {0, 0, 0, 0};
+  if (e < n) r[0] = p[e];
+  if (e + 1 < n) r[1] = p[e + 1];
+  if (e + 2 < n) r[2] = p[e + 2];
+  return r;
+}
+
+// Issues the 16-byte endpoint loads of tile T (non-temporal: streamed once). T.e0/seg set by the caller.
+template <int BLOCK, int CHUNKS>
+__device__ inline void load_tile_edges(const SegView& L, EdgeTile& T) {
+  constexpr int TILE = BLOCK * 4 * CHUNKS;
+  const int64_t rem = L.n - T.e0;
+  T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
+  const int64_t ea = T.e0 + (int64_t)threadIdx.x * 4;
+  T.x0 = load4(L.p1, ea, L.n);
+  T.y0 = load4(L.p2, ea, L.n);
+  if (CHUNKS > 1) {
+    const int64_t eb = T.e0 + ((int64_t)BLOCK + threadIdx.x) * 4;
+    T.x1 = load4(L.p1, eb, L.n);
+    T.y1 = load4(L.p2, eb, L.n);
+  }
+}
+
+// Candidate bits from the bitmaps, wave-local compaction into the LDS queue, then one queued slot
+// per lane per round (endpoints fetched from the owning lane's registers).
+template <bool LDS, int BLOCK>
+__device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
+                                     uint32_t* s_q, const EdgeTile& T, int c, const v4i xs, const v4i ys) {
+  const uint32_t lane = lane_id();
+  const uint32_t wave_base = threadIdx.x & ~63u;
+  const int32_t local0 = (c * BLOCK + (int32_t)threadIdx.x) * 4;  // tile-local index of edge q = 0
+  const uint32_t* bits = LDS ? s_bits : S.bitmap;
+  uint32_t cand = 0;  // bit q*2 + dir: infectious source and susceptible target
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t xi = (uint32_t)xs[q], yi = (uint32_t)ys[q];
+    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
+    if (local0 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents) {
+      const bool ix = (bits[xi >> 5] >> (xi & 31)) & 1u;
+      const bool iy = (bits[yi >> 5] >> (yi & 31)) & 1u;
+      if (ix && ((S.susmap[yi >> 5] >> (yi & 31)) & 1u)) cand |= 1u << (2 * q);
+      if (iy && ((S.susmap[xi >> 5] >> (xi & 31)) & 1u)) cand |= 2u << (2 * q);
+    }
+  }
+  uint32_t total = 0;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool has = (cand >> b) & 1u;
+    const uint64_t m = __ballot(has);
+    if (has) s_q[total + lanes_below(m)] = (lane << 3) | (uint32_t)b;
+    total += (uint32_t)__popcll(m);
+  }
+  if (total == 0) return;  // wave-uniform
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  for (uint32_t base = 0; base < total; base += 64) {
+    const uint32_t j = base + lane;
+    const uint32_t code = j < total ? s_q[j] : 0u;
+    const int src_lane = (int)(code >> 3);
+    const int q = (int)((code >> 1) & 3u);
+    const int dir = (int)(code & 1u);
+    const int x0 = __shfl(xs[0], src_lane, 64), x1 = __shfl(xs[1], src_lane, 64);
+    const int x2 = __shfl(xs[2], src_lane, 64), x3 = __shfl(xs[3], src_lane, 64);
+    const int y0 = __shfl(ys[0], src_lane, 64), y1 = __shfl(ys[1], src_lane, 64);
+    const int y2 = __shfl(ys[2], src_lane, 64), y3 = __shfl(ys[3], src_lane, 64);
+    if (j < total) {
+      const int32_t xv = q == 0 ? x0 : (q == 1 ? x1 : (q == 2 ? x2 : x3));
+      const int32_t yv = q == 0 ? y0 : (q == 1 ? y1 : (q == 2 ? y2 : y3));
+      const int64_t e = T.e0 + ((int64_t)c * BLOCK + wave_base + (uint32_t)src_lane) * 4 + q;
+      edge_candidate(S, L, t, dir, e, dir ? yv : xv, dir ? xv : yv);
+    }
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool LDS, int BLOCK, int CHUNKS>
+__device__ inline void process_tile(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
+                                    uint32_t* s_q, const EdgeTile& T) {
+  process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T, 0, T.x0, T.y0);
+  if (CHUNKS > 1) process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T, 1, T.x1, T.y1);
+}
+
+// ===========================================================================
+// People.infect for one target (people.py:470-579 + immunity.py:138-202)
+// ===========================================================================
+struct InfectOut {
+  uint64_t key;  // the target's winning key (KEY_NONE: no infection)
+  bool inf, reinf, symp, sev;
+  uint32_t v, l;
+};
+
+// g: storage index; il: index within the sim (counter field of the draws); stride: row stride of
+// the [variant][agent] arrays. Resets the target's winner key. The caller resolves the source.
+__device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params& prm, uint64_t* winner, int64_t g,
+                                          uint32_t il, int64_t stride, int32_t t, bool hosp_max, bool icu_max) {
+  InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
+  const uint64_t key = winner[g];
+  winner[g] = KEY_NONE;
+  o.key = key;
+  uint32_t f = P.flags[g];
+  if (key == KEY_NONE || !(f & CVK_SUSCEPTIBLE)) return o;  // people.py:470 keeps susceptibles only
+  const uint64_t seed = prm.seed;
+  const uint32_t v = key_variant(key);
+  o.v = v;
+  o.l = key_layer(key);
+  o.inf = true;
+  // breakthrough, people.py:486-491
+  if (P.peak_nab[g] != 0.0f) {
+    const int32_t nb = P.n_breakthroughs[g];
+    if (nb == 0) P.rel_trans[g] = P.rel_trans[g] * prm.trans_redux;
+    P.n_breakthroughs[g] = nb + 1;
+  }
+  // states, people.py:494-506
+  f = (f & ~(CVK_SUSCEPTIBLE | CVK_NAIVE | CVK_RECOVERED | CVK_DIAGNOSED)) | CVK_EXPOSED;
+  P.flags[g] = f;
+  P.vstate[g] = (P.vstate[g] & ~0xFu) | v | (1u << (8 + v));
+  P.n_infections[g] += 1;
+  o.reinf = !isnan(P.date_recovered[g]);
+  // outcome tree, people.py:513-579
+  const float d_e2i = (float)draw_dur(prm.dur[CVK_DUR_EXP2INF], seed, il, t, SLOT_EXP2INF);
+  P.dur_exp2inf[g] = d_e2i;
+  P.date_exposed[g] = (float)t;
+  const float date_inf = d_e2i + (float)t;
+  P.date_infectious[g] = date_inf;
+  const float fnan = __builtin_nanf("");
+  P.date_symptomatic[g] = fnan;
+  P.date_severe[g] = fnan;
+  P.date_critical[g] = fnan;
+  P.date_diagnosed[g] = fnan;
+  float date_rec = fnan;
+  double u_symp, u_sev, u_crit, u_death;
+  draw_pair(seed, il, t, SLOT_BERN_SYMP_SEV, u_symp, u_sev);
+  const float sp = (prm.rel_symp_prob[v] * P.symp_prob[g]) * (1.0f - P.symp_imm[(int64_t)v * stride + g]);
+  o.symp = u_symp < (double)sp;
+  int imm_class = 0;  // 0 asymptomatic, 1 mild, 2 severe (rel_imm_symp)
+  if (!o.symp) {
+    const double dd = draw_dur(prm.dur[CVK_DUR_ASYM2REC], seed, il, t, SLOT_DUR_A);
+    date_rec = (float)((double)date_inf + dd);
+    P.dur_disease[g] = (float)((double)d_e2i + dd);
+  } else {
+    const float d_i2s = (float)draw_dur(prm.dur[CVK_DUR_INF2SYM], seed, il, t, SLOT_DUR_A);
+    P.dur_inf2sym[g] = d_i2s;
+    const float date_sym = date_inf + d_i2s;
+    P.date_symptomatic[g] = date_sym;
+    const float sevp = (prm.rel_severe_prob[v] * P.severe_prob[g]) * (1.0f - P.sev_imm[(int64_t)v * stride + g]);
+    o.sev = u_sev < (double)sevp;
+    if (!o.sev) {
+      imm_class = 1;
+      const double dd = draw_dur(prm.dur[CVK_DUR_MILD2REC], seed, il, t, SLOT_DUR_B);
+      date_rec = (float)((double)date_sym + dd);
+      P.dur_disease[g] = (float)((double)(d_e2i + d_i2s) + dd);
+    } else {
+      imm_class = 2;
+      const float d_s2s = (float)draw_dur(prm.dur[CVK_DUR_SYM2SEV], seed, il, t, SLOT_DUR_B);
+      P.dur_sym2sev[g] = d_s2s;
+      const float date_sev = date_sym + d_s2s;
+      P.date_severe[g] = date_sev;
+      draw_pair(seed, il, t, SLOT_BERN_CRIT_DEATH, u_crit, u_death);
+      const float critp = (prm.rel_crit_prob[v] * P.crit_prob[g]) * (hosp_max ? prm.no_hosp_factor : 1.0f);
+      if (!(u_crit < (double)critp)) {
+        const double dd = draw_dur(prm.dur[CVK_DUR_SEV2REC], seed, il, t, SLOT_DUR_C);
+        date_rec = (float)((double)date_sev + dd);
+        P.dur_disease[g] = (float)((double)((d_e2i + d_i2s) + d_s2s) + dd);
+      } else {
+        const float d_s2c = (float)draw_dur(prm.dur[CVK_DUR_SEV2CRIT], seed, il, t, SLOT_DUR_C);
+        P.dur_sev2crit[g] = d_s2c;
+        const float date_crit = date_sev + d_s2c;
+        P.date_critical[g] = date_crit;
+        const float deathp = (prm.rel_death_prob[v] * P.death_prob[g]) * (icu_max ? prm.no_icu_factor : 1.0f);
+        const float dsum = ((d_e2i + d_i2s) + d_s2s) + d_s2c;
+        if (!(u_death < (double)deathp)) {
+          const double dd = draw_dur(prm.dur[CVK_DUR_CRIT2REC], seed, il, t, SLOT_DUR_D);
+          date_rec = (float)((double)date_crit + dd);
+          P.dur_disease[g] = (float)((double)dsum + dd);
+        } else {
+          const double dd = draw_dur(prm.dur[CVK_DUR_CRIT2DIE], seed, il, t, SLOT_DUR_D);
+          P.date_dead[g] = (float)((double)date_crit + dd);
+          P.dur_disease[g] = (float)((double)dsum + dd);
+        }
+      }
+    }
+  }
+  P.date_recovered[g] = date_rec;
+  // update_peak_nab, immunity.py:138-202
+  if (prm.use_waning) {
+    if (P.nab[g] > 0.0f) {
+      P.peak_nab[g] = P.peak_nab[g] * prm.nab_boost;
+    } else {
+      const double init_nab = draw_dur(prm.nab_init, seed, il, t, SLOT_NAB);
+      P.peak_nab[g] = (float)((pow(2.0, init_nab) * prm.rel_imm_symp[imm_class]) * prm.nab_norm);
+    }
+    P.t_nab_event[g] = t;
+  }
+  return o;
+}
+
+// Source of an infection key (-1 for seeds and imports); p1/p2 are the sim's arrays of the key's layer.
+__device__ inline int32_t key_source(uint64_t key, const int32_t* p1, const int32_t* p2) {
+  if (key_class(key) != 1) return -1;
+  const int64_t e = (int64_t)key_index(key);
+  return key_dir(key) ? p2[e] : p1[e];
+}
+
+constexpr int INFECT_BLOCK = 256;
+constexpr int N_INFECT_COLS = 2 + 3 * MAXV;
+
+// Column of the day row for infect-pass counter k.
+__device__ inline int infect_col(int k) {
+  if (k == 0) return CVK_R_NEW_INFECTIONS;
+  if (k == 1) return CVK_R_NEW_REINFECTIONS;
+  return CVK_R_BYVAR + ((k - 2) / MAXV) * MAXV + (k - 2) % MAXV;  // infections, symptomatic, severe by variant
+}
+
+}  // namespace cvk

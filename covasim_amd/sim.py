@@ -353,11 +353,14 @@ class Sim:
         resf = self._w['results_f'].cpu().numpy()
         return res, resf
 
-    def finalize(self):
-        """Per-day results, cumulative flows and derived states (sim.py:764-850)."""
+    def finalize(self, raw=None):
+        """Per-day results, cumulative flows and derived states (sim.py:764-850).
+
+        raw: optional (counts [npts][cols], float sums [npts][4]) from another runner (BatchSim).
+        """
         if self.results_ready:
             raise AlreadyRunError('Simulation has already been finalized')
-        res, resf = self.raw_results()
+        res, resf = raw if raw is not None else self.raw_results()
         npts, nv = self.npts, int(self.pars['n_variants'])
         R = {}
         col = _lib.R_COL
@@ -423,7 +426,7 @@ class Sim:
         names = {_lib.LAYER_SEED: 'seed_infection', _lib.LAYER_IMPORT: 'importation'}
         out = []
         for s, tg, d, lv in recs:
-            l, v = lv & 0xFF, lv >> 8
+            l, v = lv & 0xFF, (lv >> 8) & 0xF
             out.append(dict(source=None if s < 0 else int(s), target=int(tg), date=int(d),
                             layer=names.get(l, lkeys[l] if l < len(lkeys) else l),
                             variant=self.pars['variant_map'][v]))

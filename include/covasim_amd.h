@@ -296,6 +296,103 @@ int cvk_graph_create(const cvk_people* people, cvk_layer* layers, const cvk_para
 int cvk_graph_launch(void* graph_exec, void* stream);
 int cvk_graph_destroy(void* graph_exec);
 
+/* ---- batched many-sim path ----
+ * No reference counterpart: the reference runs independent sims one per process
+ * (MultiSim.run -> run.py:1477 sc.parallelize over Sim.run, sim.py:688-761). Here S sims share
+ * one launch per phase per day. Agents of all sims are concatenated (sim s owns storage agents
+ * [agent_offsets[s], agent_offsets[s+1])); each layer's edge lists are concatenated over sims
+ * (each sim's run starting on a 4-edge boundary) with endpoints given as indices within the sim. Counter-based draws and transmission keys use
+ * indices within the sim, so every sim of a batch produces exactly what cvk_run_days produces
+ * for it alone. All sims share n_layers, n_variants, use_waning and the dynamic-layer flags. */
+#define CVK_BATCH_MAX_SIMS 4096
+
+typedef struct {
+  int32_t n_sims, n_layers, n_variants, use_waning;
+  int64_t n_agents_total;
+  const int64_t* agent_offsets_host;   /* [n_sims + 1] */
+  const int64_t* edge_starts_host;     /* [n_layers][n_sims]: first edge of sim s in layer l (a multiple of 4) */
+  const int64_t* edge_counts_host;     /* [n_layers][n_sims]: edges of sim s in layer l */
+  const cvk_params* params_host;       /* [n_sims]; params[s].n_agents is sim s's agent count */
+  const int32_t* dynamic_host;         /* [n_layers]: nonzero = resampled every day */
+  cvk_people people;                   /* device SoA over all agents; [variant][agent] rows have stride n_agents_total */
+  int32_t* p1[CVK_MAX_LAYERS];         /* device, 16-byte aligned, concatenated over sims */
+  int32_t* p2[CVK_MAX_LAYERS];
+  float* beta[CVK_MAX_LAYERS];
+  int32_t beta_ones[CVK_MAX_LAYERS];   /* nonzero when every weight of the layer is 1.0f */
+} cvk_batch_desc;
+
+/* Import events of a batch: on day days[j], counts[j * n_sims + s] susceptible agents of sim s are
+ * infected with variant variants[j] (immunity.py:117-130). Host arrays. */
+typedef struct {
+  int32_t n_events;
+  const int32_t* days;
+  const int32_t* variants;
+  const int64_t* counts;
+} cvk_batch_imports;
+
+/* Launch shape of the batch kernels (chosen by an autotuner; see cvk_batch_set_config). */
+typedef struct {
+  int32_t edge_block;       /* 512 or 1024 threads per edge-pass workgroup */
+  int32_t edge_chunks;      /* 1 or 2 four-edge chunks per thread per tile */
+  int32_t edge_blocks_per_cu;  /* 1 or 2 */
+  int32_t agents_per_thread;   /* 1, 2 or 4 in the agent pass */
+} cvk_batch_config;
+
+/* Copies the descriptor's tables to the device and allocates the batch workspace (device
+ * buffers owned by the handle: bitmaps, records, keys, partial rows, results, log). */
+int cvk_batch_create(const cvk_batch_desc* desc, int32_t npts, int64_t log_capacity, void** handle_out);
+int cvk_batch_destroy(void* handle);
+int cvk_batch_set_config(void* handle, const cvk_batch_config* config);
+int cvk_batch_get_config(void* handle, cvk_batch_config* config_out);
+/* Clears keys, counters, results and the log (graph-safe fill kernels on `stream`). */
+int cvk_batch_reset(void* handle, void* stream);
+/* Runs days [t0, t1) of every sim; t0 == 0 first seeds pop_infected[s] agents of sim s (host array). */
+int cvk_batch_run_days(void* handle, int32_t t0, int32_t t1, const int64_t* pop_infected_host,
+                       const cvk_batch_imports* imports, int32_t use_graph, void* stream);
+/* Captures cvk_batch_run_days(t0, t1) into an instantiated hipGraph (launch/destroy with
+ * cvk_graph_launch / cvk_graph_destroy). */
+int cvk_batch_graph_create(void* handle, int32_t t0, int32_t t1, const int64_t* pop_infected_host,
+                           const cvk_batch_imports* imports, void* stream, void** graph_exec_out);
+/* Synchronous copies of the result tables: results [n_sims][npts][CVK_R_COLS] uint32 and
+ * results_f [n_sims][npts][CVK_RF_COLS] (either may be NULL). */
+int cvk_batch_copy_results(void* handle, uint32_t* results_host, double* results_f_host);
+/* Synchronous copy of up to capacity infection records (source, target, day, layer | variant << 8
+ * | sim << 12; source and target are indices within the sim); *count_out = records written. */
+int cvk_batch_copy_log(void* handle, int32_t* log_host, int64_t capacity, int64_t* count_out);
+
+/* ---- differentiable relaxation ----
+ * No reference counterpart: an expected-infection relaxation of the transmission step
+ * (utils.py:88-123 with indicator states replaced by probabilities) for gradient-based
+ * calibration. Per agent i and day t: susceptible probability s_i(t), infection probability
+ * D_i(t) (D_i(0) = seed_frac), expected infectiousness tr_j(t) = rel_trans_j * sum_{d>=1}
+ * D_j(t-d) w[bin_j][d], log-survival L_i(t) = sum over layers l and neighbours j of
+ * log1p(-min(beta * beta_l * sigma[bin_i] * tr_j, 0.999999)), D_i(t) = s_i(t)(1 - e^{L_i}),
+ * s_i(t+1) = s_i(t) e^{L_i}; E_t = sum_i s_i(t)(1 - e^{L_i}). Parameters theta (host doubles):
+ * [beta, beta_l for each layer, sigma_b for the 10 age bins]. */
+typedef struct {
+  int64_t n_agents;
+  int32_t n_layers, n_days;
+  int32_t kernel_len;          /* lags 0..kernel_len-1 of w_host per age bin */
+  int32_t checkpoint_every;    /* days between stored states (the backward re-runs each segment) */
+  int32_t precision;           /* 32 (float arithmetic) or 64 (double) */
+  int32_t pad;
+  double seed_frac;            /* pop_infected / n_agents */
+  const int64_t* indptr[CVK_MAX_LAYERS];  /* device [n_agents + 1]: neighbour list offsets per layer */
+  const int32_t* nbr[CVK_MAX_LAYERS];     /* device: both directions of every edge, no self-loops */
+  const uint8_t* age_bin;      /* device [n_agents]: prognosis age bin 0..9 */
+  const float* rel_trans;      /* device [n_agents] */
+  const double* w_host;        /* host [10][kernel_len]: mean infectiousness d days after infection */
+} cvk_relax_desc;
+
+int cvk_relax_create(const cvk_relax_desc* desc, void** handle_out);
+int cvk_relax_destroy(void* handle);
+/* expected_host[n_days] = E_t. Synchronous on `stream`. */
+int cvk_relax_forward(void* handle, const double* theta_host, double* expected_host, void* stream);
+/* grad_theta_host[1 + n_layers + 10] = sum_t grad_expected_host[t] * dE_t/dtheta (adjoint, after a
+ * forward with the same theta). Synchronous on `stream`. */
+int cvk_relax_backward(void* handle, const double* theta_host, const double* grad_expected_host,
+                       double* grad_theta_host, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
