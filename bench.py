@@ -42,6 +42,11 @@ WORKLOADS = {
                      pars=dict(use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
                      desc='256 independent hybrid sims, pop sizes log-uniform over 10k-500k, beta uniform over '
                           '0.010-0.025, per-sim seeds, 120 days, layers as hybrid1m, one launch per phase per day'),
+    'relax1m': dict(pop_size=1_000_000, pop_type='hybrid', n_days=60, households='categorical', relax=True,
+                    pars=dict(beta=0.016, use_waning=False, pop_infected=20),
+                    desc='differentiable relaxation on the hybrid 1M population (static layers), 60 days: '
+                         'float32 forward + hand-written adjoint with checkpoints every 10 days, gradient wrt '
+                         '4 layer betas, beta and 10 age-bin rel_sus; calibration to a stochastic incidence curve'),
     'random20k': dict(pop_size=20_000, pop_type='random', n_days=60, households='poisson',
                       pars=dict(beta=0.016, use_waning=True, pop_infected=20),
                       desc='random 20k agents, one layer a with Poisson(20) contacts, 60 days, beta 0.016'),
@@ -165,6 +170,70 @@ def run_batch(w, args):
     print(json.dumps(out), flush=True)
 
 
+def run_relax(w, args):
+    """Relaxation workload: forward + adjoint timing, float64 finite-difference check, calibration."""
+    import torch
+    import covasim_amd as cv
+    from covasim_amd.relax import Relaxation, calibrate, log_incidence_loss
+    t0 = time.perf_counter()
+    pop = build_population(w, args.seed)
+
+    def make_sim(seed):
+        return cv.Sim(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=seed,
+                      popdict=pop, households=w['households'], **w['pars']).initialize()
+
+    truth_sim = make_sim(args.seed)
+    truth_sim.run()
+    observed = truth_sim.results['new_infections'].values.copy()
+    r32 = Relaxation(make_sim(args.seed), precision=32, checkpoint_every=10)
+    r64 = Relaxation(r32.sim, precision=64, checkpoint_every=10)
+    setup_s = time.perf_counter() - t0
+    theta = r32.theta0()
+    c = np.random.default_rng(args.seed).uniform(0.5, 1.5, r32.n_days) / 1e4
+    for _ in range(2):  # warmup
+        r32.forward_np(theta)
+        r32.backward_np(theta, c)
+    reps = 5
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    for _ in range(reps):
+        r32.forward_np(theta)
+    fwd_s = (time.perf_counter() - t1) / reps
+    t2 = time.perf_counter()
+    for _ in range(reps):
+        r32.forward_np(theta)
+        g = r32.backward_np(theta, c)
+    fb_s = (time.perf_counter() - t2) / reps
+    h = 1e-4
+    fd = np.zeros_like(theta)
+    for k in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd[k] = (c @ r64.forward_np(tp) - c @ r64.forward_np(tm)) / (2 * h * theta[k])
+    grad_err = float(np.linalg.norm(g - fd) / np.linalg.norm(fd))
+    start = theta * np.random.default_rng(args.seed + 1).uniform(0.6, 1.5, len(theta))
+    t3 = time.perf_counter()
+    fitted, hist = calibrate(r32, observed, start, iters=100, lr=0.05)
+    cal_s = time.perf_counter() - t3
+    obs_t = torch.as_tensor(observed[:r32.n_days])
+    loss_truth = float(log_incidence_loss(torch.as_tensor(r32.forward_np(theta)), obs_t))
+    n, T = w['pop_size'], r32.n_days
+    out = {
+        'metric': METRIC, 'value': n * T / fb_s, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': T, 'warmup': 2 * T,
+        'ms_per_step': fb_s / T * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': w['desc'], 'agents': n, 'days': T, 'params': int(len(theta)),
+                   'setup_s': round(setup_s, 1)},
+        'relaxation': {'forward_ms': fwd_s * 1e3, 'forward_adjoint_ms': fb_s * 1e3,
+                       'grad_rel_err_vs_fp64_central_fd': grad_err, 'fd_step_relative': h,
+                       'calibration_iters': len(hist), 'calibration_s': cal_s,
+                       'loss_start': hist[0], 'loss_end': hist[-1], 'loss_at_true_params': loss_truth,
+                       'fitted_over_true': [round(float(x), 4) for x in fitted / theta]},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -175,6 +244,10 @@ def main():
     ap.add_argument('--cpu-budget', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
+    ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass tile depth (2 or 4 chunks per thread)')
+    ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
+    ap.add_argument('--copy-probe', type=int, default=0,
+                    help='MiB of one device-to-device copy after the run (a known-size kernel for counter scaling)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({'error': 'this build runs on one GPU; --gpus above 1 is not supported (the sharded '
@@ -186,8 +259,17 @@ def main():
     from covasim_amd import _lib
 
     w = WORKLOADS[args.workload]
+    tuning = {}
+    if args.edge_chunks is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_CHUNKS, args.edge_chunks), 'tuning')
+        tuning['edge_chunks'] = args.edge_chunks
+    if args.edge_lds is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, args.edge_lds), 'tuning')
+        tuning['edge_lds'] = args.edge_lds
     if 'n_sims' in w:
         return run_batch(w, args)
+    if w.get('relax'):
+        return run_relax(w, args)
     steps = args.steps or w['n_days']
     warmup = steps if args.warmup is None else args.warmup
     n_days = max(w['n_days'], steps)
@@ -282,7 +364,7 @@ def main():
         'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': w['desc'], 'agents': n, 'days': steps, 'edges': int(n_edges),
                    'dynamic_edges': int(dyn_edges), 'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
-                   'setup_s': round(build_s, 2)},
+                   'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {})},
         'roofline': {'bound': 'hbm', 'kernel': 'edge_pass_kernel', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
                      'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes,
@@ -290,6 +372,14 @@ def main():
                                                    np.percentile(edge_ms, [10, 50, 90, 100])]},
     }
     print('edge pass per-day us: ' + ' '.join(f'{x * 1e3:.0f}' for x in edge_ms), file=sys.stderr, flush=True)
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        out['copy_probe_bytes'] = nbytes
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(w, pop, args.seed, args.cpu_budget)
         if args.cpu_omp:

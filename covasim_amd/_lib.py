@@ -32,6 +32,7 @@ DIST_CODES = {'uniform': 0, 'unif': 0, 'normal': 1, 'norm': 1, 'normal_pos': 2, 
 DUR_ORDER = ['exp2inf', 'inf2sym', 'sym2sev', 'sev2crit', 'asym2rec', 'mild2rec', 'sev2rec', 'crit2rec', 'crit2die']
 
 TUNE_EDGE_LDS = 1
+TUNE_EDGE_CHUNKS = 2
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15

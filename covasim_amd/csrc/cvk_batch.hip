@@ -137,7 +137,7 @@ struct BatchEdgeArgs {
 };
 
 template <int BLOCK, int CHUNKS>
-__device__ inline void batch_load_tile(const BatchEdgeArgs& a, int64_t tile, EdgeTile& T) {
+__device__ inline void batch_load_tile(const BatchEdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
   T.tile = tile;
   if (tile >= a.n_tiles) return;
   const int32_t seg = a.tile_seg[tile];
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
   uint32_t* tile_counter = C.ctl + CVK_CTL_EDGE_TILES;
   if (threadIdx.x == 0) s_tiles[0] = atomicAdd(tile_counter, 1u);
   __syncthreads();
-  EdgeTile cur, nxt;
+  EdgeTile<CHUNKS> cur, nxt;
   batch_load_tile<BLOCK, CHUNKS>(a, s_tiles[0], cur);
   int32_t loaded = -1;
   bool in_lds = false;

@@ -190,8 +190,8 @@ __global__ __launch_bounds__(256) void choose_mark_kernel(cvk_people P, cvk_work
 // edge_pass
 // ===========================================================================
 constexpr int EDGE_BLOCK = 1024;
-constexpr int EDGE_CHUNKS = 2;                        // 4-edge chunks per thread per tile
-constexpr int EDGE_TILE = EDGE_BLOCK * 4 * EDGE_CHUNKS;
+// 4-edge chunks per thread per tile: 2 (8192-edge tiles) or 4 (16384), chosen by cvk_set_tuning
+constexpr int edge_tile(int chunks) { return EDGE_BLOCK * 4 * chunks; }
 
 struct EdgeArgs {
   SimView sim;
@@ -207,17 +207,18 @@ struct EdgeArgs {
 };
 
 // Locates tile `tile` (layer by tile_start) and issues its endpoint loads.
-__device__ inline void load_tile(const EdgeArgs& a, int64_t tile, EdgeTile& T) {
+template <int CHUNKS>
+__device__ inline void load_tile(const EdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
   T.tile = tile;
   if (tile >= a.tile_start[a.n_layers]) return;
   int l = 0;
   while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
   T.seg = l;
-  T.e0 = (tile - a.tile_start[l]) * EDGE_TILE;
-  load_tile_edges<EDGE_BLOCK, EDGE_CHUNKS>(a.seg[l], T);
+  T.e0 = (tile - a.tile_start[l]) * edge_tile(CHUNKS);
+  load_tile_edges<EDGE_BLOCK, CHUNKS>(a.seg[l], T);
 }
 
-template <bool LDS>
+template <bool LDS, int CHUNKS>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ int64_t s_tiles[2];
@@ -234,7 +235,7 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   __syncthreads();
   const int64_t n_tiles = a.tile_start[a.n_layers];
   // tiles are handed out dynamically; the next tile's endpoints are in flight while this one is processed
-  EdgeTile cur, nxt;
+  EdgeTile<CHUNKS> cur, nxt;
   load_tile(a, s_tiles[0], cur);
   int par = 1;
   while (cur.tile < n_tiles) {
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     __syncthreads();
     load_tile(a, s_tiles[par], nxt);
     par ^= 1;
-    process_tile<LDS, EDGE_BLOCK, EDGE_CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur);
+    process_tile<LDS, EDGE_BLOCK, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur);
     cur = nxt;
   }
 }
@@ -327,7 +328,8 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
 // ===========================================================================
 // host side
 // ===========================================================================
-static int g_edge_lds = 1;  // tunable: stage the infectious bitmap in LDS when it fits
+static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
+static int g_edge_chunks = 2;  // tunable: 4-edge chunks per thread per edge tile (2 or 4)
 
 struct DeviceCaps {
   int device = -1;
@@ -441,6 +443,10 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
   switch (key) {
     case CVK_TUNE_EDGE_LDS:
       g_edge_lds = value ? 1 : 0;
+      return CVK_OK;
+    case CVK_TUNE_EDGE_CHUNKS:
+      if (value != 2 && value != 4) return fail(CVK_E_INVALID, "cvk_set_tuning: edge chunks must be 2 or 4");
+      g_edge_chunks = value;
       return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
@@ -572,25 +578,29 @@ extern "C" int cvk_choose(const cvk_people* people, const cvk_params* params, cv
                      (hipStream_t)stream);
 }
 
-// Dynamic LDS budget of the edge pass: the device's per-workgroup LDS minus the kernel's static LDS.
-static int edge_dynamic_lds(bool lds_variant, int64_t* budget) {
+// Dynamic LDS budget of the edge pass: the device's per-workgroup LDS minus the kernel's static LDS
+// (the same for every instantiation; each gets the raised limit).
+static int edge_dynamic_lds(int64_t* budget) {
   DeviceCaps& c = caps();
   static thread_local int configured_device = -1;
-  static thread_local int64_t budgets[2] = {0, 0};
+  static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* fns[2] = {(const void*)edge_pass_kernel<false>, (const void*)edge_pass_kernel<true>};
-    for (int j = 0; j < 2; ++j) {
+    const void* fns[4] = {(const void*)edge_pass_kernel<false, 2>, (const void*)edge_pass_kernel<true, 2>,
+                          (const void*)edge_pass_kernel<false, 4>, (const void*)edge_pass_kernel<true, 4>};
+    int64_t b_min = -1;
+    for (int j = 0; j < 4; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
       const int64_t b = (int64_t)c.max_lds - (int64_t)attr.sharedSizeBytes;
       if (b <= 0 || hipFuncSetAttribute(fns[j], hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess)
         return fail(CVK_E_HIP, "cannot raise the LDS limit of the edge pass");
-      budgets[j] = b;
+      b_min = b_min < 0 ? b : std::min(b_min, b);
     }
+    value = b_min;
     configured_device = c.device;
   }
-  *budget = budgets[lds_variant ? 1 : 0];
+  *budget = value;
   return CVK_OK;
 }
 
@@ -627,7 +637,7 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   const int64_t queue_bytes = (int64_t)(EDGE_BLOCK / 64) * EDGE_QCAP * 4;
   const int64_t lds_bytes = (int64_t)a.bitmap_words * 4 + queue_bytes;
   int64_t budget = 0;
-  int rc0 = edge_dynamic_lds(true, &budget);
+  int rc0 = edge_dynamic_lds(&budget);
   if (rc0) return rc0;
   const bool use_lds = lds_bytes <= budget && g_edge_lds != 0;
   int64_t tiles = 0;
@@ -648,16 +658,19 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     G.beta_ones = L.beta_ones;
     G.l = l;
     a.tile_start[l] = tiles;
-    tiles += (L.n_edges + EDGE_TILE - 1) / EDGE_TILE;
+    tiles += (L.n_edges + edge_tile(g_edge_chunks) - 1) / edge_tile(g_edge_chunks);
   }
   a.tile_start[params->n_layers] = tiles;
   for (int l = params->n_layers + 1; l <= CVK_MAX_LAYERS; ++l) a.tile_start[l] = tiles;
   if (tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
   const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1));
-  if (use_lds) {
-    hipLaunchKernelGGL((edge_pass_kernel<true>), dim3(grid), dim3(EDGE_BLOCK), (size_t)lds_bytes, st, a);
+  const size_t shmem = (size_t)(use_lds ? lds_bytes : queue_bytes);
+  if (g_edge_chunks == 4) {
+    if (use_lds) hipLaunchKernelGGL((edge_pass_kernel<true, 4>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
+    else hipLaunchKernelGGL((edge_pass_kernel<false, 4>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
   } else {
-    hipLaunchKernelGGL((edge_pass_kernel<false>), dim3(grid), dim3(EDGE_BLOCK), (size_t)queue_bytes, st, a);
+    if (use_lds) hipLaunchKernelGGL((edge_pass_kernel<true, 2>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
+    else hipLaunchKernelGGL((edge_pass_kernel<false, 2>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
   }
   return check_launch("cvk_edge_pass");
 }
@@ -775,7 +788,7 @@ extern "C" int cvk_graph_create(const cvk_people* people, cvk_layer* layers, con
   hipStream_t st = (hipStream_t)stream;
   if (st == nullptr) return fail(CVK_E_INVALID, "cvk_graph_create: graph capture needs a non-default stream");
   int64_t budget = 0;
-  rc = edge_dynamic_lds(true, &budget);  // not a stream operation: done before capture
+  rc = edge_dynamic_lds(&budget);  // not a stream operation: done before capture
   if (rc) return rc;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(CVK_E_HIP, "cvk_graph_create: hipStreamBeginCapture failed");

@@ -452,13 +452,15 @@ __device__ inline void edge_candidate(const SimView& S, const SegView& L, int32_
   if (old == KEY_NONE) S.touched[atomicAdd(S.touched_count, 1u)] = (int32_t)(S.a_off + tgt);
 }
 
-// One tile's endpoints for this thread, in vector registers (up to two 4-edge chunks).
+// One tile's endpoints for this thread, in vector registers (CHUNKS 4-edge chunks; every index into
+// x/y is a compile-time constant after unrolling, so the arrays stay in registers).
+template <int CHUNKS>
 struct EdgeTile {
   int64_t tile;     // tile number (>= the tile count: no tile)
   int64_t e0;       // first edge of the tile in its layer
   int32_t seg;      // segment (sim x layer) of the tile
   int32_t n_valid;  // edges of the tile that exist (the last tile of a layer is partial)
-  v4i x0, y0, x1, y1;
+  v4i x[CHUNKS], y[CHUNKS];
 };
 
 __device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
@@ -472,17 +474,15 @@ __device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
 
 // Issues the 16-byte endpoint loads of tile T (non-temporal: streamed once). T.e0/seg set by the caller.
 template <int BLOCK, int CHUNKS>
-__device__ inline void load_tile_edges(const SegView& L, EdgeTile& T) {
+__device__ inline void load_tile_edges(const SegView& L, EdgeTile<CHUNKS>& T) {
   constexpr int TILE = BLOCK * 4 * CHUNKS;
   const int64_t rem = L.n - T.e0;
   T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
-  const int64_t ea = T.e0 + (int64_t)threadIdx.x * 4;
-  T.x0 = load4(L.p1, ea, L.n);
-  T.y0 = load4(L.p2, ea, L.n);
-  if (CHUNKS > 1) {
-    const int64_t eb = T.e0 + ((int64_t)BLOCK + threadIdx.x) * 4;
-    T.x1 = load4(L.p1, eb, L.n);
-    T.y1 = load4(L.p2, eb, L.n);
+#pragma unroll
+  for (int c = 0; c < CHUNKS; ++c) {
+    const int64_t e = T.e0 + ((int64_t)c * BLOCK + threadIdx.x) * 4;
+    T.x[c] = load4(L.p1, e, L.n);
+    T.y[c] = load4(L.p2, e, L.n);
   }
 }
 
@@ -490,7 +490,7 @@ __device__ inline void load_tile_edges(const SegView& L, EdgeTile& T) {
 // per lane per round (endpoints fetched from the owning lane's registers).
 template <bool LDS, int BLOCK>
 __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                     uint32_t* s_q, const EdgeTile& T, int c, const v4i xs, const v4i ys) {
+                                     uint32_t* s_q, int64_t e0, int32_t n_valid, int c, const v4i xs, const v4i ys) {
   const uint32_t lane = lane_id();
   const uint32_t wave_base = threadIdx.x & ~63u;
   const int32_t local0 = (c * BLOCK + (int32_t)threadIdx.x) * 4;  // tile-local index of edge q = 0
@@ -500,7 +500,7 @@ __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t
   for (int q = 0; q < 4; ++q) {
     const uint32_t xi = (uint32_t)xs[q], yi = (uint32_t)ys[q];
     // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-    if (local0 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents) {
+    if (local0 + q < n_valid && xi < S.n_agents && yi < S.n_agents) {
       const bool ix = (bits[xi >> 5] >> (xi & 31)) & 1u;
       const bool iy = (bits[yi >> 5] >> (yi & 31)) & 1u;
       if (ix && ((S.susmap[yi >> 5] >> (yi & 31)) & 1u)) cand |= 1u << (2 * q);
@@ -532,7 +532,7 @@ __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t
     if (j < total) {
       const int32_t xv = q == 0 ? x0 : (q == 1 ? x1 : (q == 2 ? x2 : x3));
       const int32_t yv = q == 0 ? y0 : (q == 1 ? y1 : (q == 2 ? y2 : y3));
-      const int64_t e = T.e0 + ((int64_t)c * BLOCK + wave_base + (uint32_t)src_lane) * 4 + q;
+      const int64_t e = e0 + ((int64_t)c * BLOCK + wave_base + (uint32_t)src_lane) * 4 + q;
       edge_candidate(S, L, t, dir, e, dir ? yv : xv, dir ? xv : yv);
     }
   }
@@ -543,9 +543,9 @@ __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t
 
 template <bool LDS, int BLOCK, int CHUNKS>
 __device__ inline void process_tile(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                    uint32_t* s_q, const EdgeTile& T) {
-  process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T, 0, T.x0, T.y0);
-  if (CHUNKS > 1) process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T, 1, T.x1, T.y1);
+                                    uint32_t* s_q, const EdgeTile<CHUNKS>& T) {
+#pragma unroll
+  for (int c = 0; c < CHUNKS; ++c) process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T.e0, T.n_valid, c, T.x[c], T.y[c]);
 }
 
 // ===========================================================================

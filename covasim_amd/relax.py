@@ -193,6 +193,7 @@ class Relaxation:
         out = np.zeros(self.n_days, dtype=np.float64)
         _lib.check(_lib.lib().cvk_relax_forward(self._handle, theta.ctypes.data, out.ctypes.data, _lib.stream_ptr()),
                    'cvk_relax_forward')
+        self._last_theta = theta.copy()
         return out
 
     def backward_np(self, theta, grad_expected):
@@ -228,7 +229,9 @@ def _make_function():
             ge = grad_e.detach().cpu().to(torch.float64).numpy()
             relax = ctx.relax
             # the adjoint needs the forward state of this theta (a later forward may have replaced it)
-            relax.forward_np(ctx.theta)
+            last = getattr(relax, '_last_theta', None)
+            if last is None or not np.array_equal(last, ctx.theta):
+                relax.forward_np(ctx.theta)
             g = relax.backward_np(ctx.theta, ge)
             return torch.from_numpy(g).to(device=ctx.device, dtype=ctx.dtype), None
 

@@ -212,6 +212,8 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 /* No reference counterpart: launch tunables (process-wide). CVK_TUNE_EDGE_LDS = 0 keeps the edge
  * pass's infectious bitmap in global memory (L2) even when it fits in LDS; 1 (default) stages it. */
 #define CVK_TUNE_EDGE_LDS 1
+/* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks each edge-pass thread loads per tile, 2 (default) or 4. */
+#define CVK_TUNE_EDGE_CHUNKS 2
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

@@ -1,0 +1,243 @@
+"""The expected-infection relaxation: CPU pins of the float64 reference, GPU forward/adjoint parity.
+
+CPU: the infectiousness table on fixed-duration cases (exact), the reference model on a
+two-agent graph computed by hand, beta = 0, saturation, and autograd against central
+differences. GPU: the HIP forward (float64 and float32) and the hand-written adjoint
+against the CPU float64 autograd reference; checkpoint-interval invariance; at 1M agents
+and 60 days, the float32 adjoint against float64 central finite differences (step
+h = 1e-4 relative to each parameter, error <= 1e-4 relative); calibration.
+"""
+
+import numpy as np
+import pytest
+
+from covasim_amd import parameters as cvpar
+from covasim_amd.relax import infectiousness_table
+from oracle import relax_ref
+
+XMAX = relax_ref.XMAX
+
+
+def fixed_pars(**dur):
+    pars = cvpar.make_pars()
+    pars['prognoses'] = cvpar.get_prognoses()
+    for k in list(pars['dur']):
+        pars['dur'][k] = dict(dist='lognormal_int', par1=1.0, par2=0.0)
+    for k, v in dur.items():
+        pars['dur'][k] = dict(dist='lognormal_int', par1=float(v), par2=0.0)
+    return pars
+
+
+def test_infectiousness_table_fixed_durations():
+    hi, lo = 2 / 1.3, 1 / 1.3  # load_ratio 2, frac_time 0.3 (parameters.py:61)
+    # asymptomatic: infectious on lags [3, 8); early while (d - 3) / 5 < 0.3
+    pars = fixed_pars(exp2inf=3, asym2rec=5)
+    pars['prognoses']['symp_probs'][:] = 0.0
+    w = infectiousness_table(pars, 12, n_samples=1000)
+    want = np.array([0, 0, 0, hi, hi, lo, lo, lo, 0, 0, 0, 0])
+    np.testing.assert_allclose(w, np.tile(want, (10, 1)), rtol=1e-12)
+    # mild, 13 days infectious (frac_time * 13 < high_cap 4): early on lags 2..5
+    pars = fixed_pars(exp2inf=2, inf2sym=1, mild2rec=12)
+    pars['prognoses']['symp_probs'][:] = 1.0
+    pars['prognoses']['severe_probs'][:] = 0.0
+    w = infectiousness_table(pars, 18, n_samples=1000)
+    want = np.array([0, 0] + [hi] * 4 + [lo] * 9 + [0, 0, 0])
+    np.testing.assert_allclose(w[4], want, rtol=1e-12)
+    # 20 days infectious: the switch point is high_cap / 20 = 0.2 -> early on lags 2..5 again;
+    # asymp_factor 0.5 applies before symptoms (lag 2 only)
+    pars = fixed_pars(exp2inf=2, inf2sym=1, mild2rec=19)
+    pars['prognoses']['symp_probs'][:] = 1.0
+    pars['prognoses']['severe_probs'][:] = 0.0
+    pars['asymp_factor'] = 0.5
+    w = infectiousness_table(pars, 24, n_samples=1000)
+    want = np.array([0, 0, hi * 0.5] + [hi] * 3 + [lo] * 16 + [0, 0])
+    np.testing.assert_allclose(w[0], want, rtol=1e-12)
+    # death ends infectiousness: critical branch with crit2die 2 after sev2crit 1, sym2sev 1, inf2sym 1
+    pars = fixed_pars(exp2inf=1, inf2sym=1, sym2sev=1, sev2crit=1, crit2die=2)
+    for k in ('symp_probs', 'severe_probs', 'crit_probs', 'death_probs'):
+        pars['prognoses'][k][:] = 1.0
+    w = infectiousness_table(pars, 10, n_samples=1000)
+    # infectious lags 1..5 (length 5: switch 0.3 -> early while d - 1 < 1.5)
+    np.testing.assert_allclose(w[9], [0, hi, hi, lo, lo, lo, 0, 0, 0, 0], rtol=1e-12)
+
+
+def _ref(theta, csr, bins, rt, w, n_days, f):
+    import torch
+    return relax_ref.expected_infections(torch.as_tensor(theta, dtype=torch.float64), csr, bins, rt, w, n_days, f)
+
+
+def test_relax_reference_two_agents_by_hand():
+    csr = [(np.array([0, 1, 2]), np.array([1, 0]))]
+    bins = np.array([3, 7])
+    rt = np.array([1.2, 0.7])
+    rng = np.random.default_rng(0)
+    w = rng.uniform(0.2, 1.5, (10, 5))
+    w[:, 0] = 0
+    f = 0.1
+    sig = 0.5 + 0.1 * np.arange(10)
+    beta, bl = 0.05, 2.0
+    theta = np.concatenate([[beta, bl], sig])
+    E = _ref(theta, csr, bins, rt, w, 3, f).numpy()
+    # day 0: nothing is infectious yet
+    assert E[0] == 0.0
+    # day 1: tr_j = rt_j f w(1); x_i = beta bl sigma_i tr_other; inf_i = (1-f) x_i
+    tr1 = rt * f * w[bins, 1]
+    x1 = beta * bl * sig[bins] * tr1[::-1]
+    inf1 = (1 - f) * x1
+    assert E[1] == pytest.approx(inf1.sum(), rel=1e-14)
+    # day 2: tr_j = rt_j (D_j(1) w(1) + D_j(0) w(2)); s_i(2) = (1-f)(1-x1_i)
+    tr2 = rt * (inf1 * w[bins, 1] + f * w[bins, 2])
+    x2 = beta * bl * sig[bins] * tr2[::-1]
+    assert E[2] == pytest.approx(((1 - f) * (1 - x1) * x2).sum(), rel=1e-14)
+
+
+def _random_graph(n, n_layers, deg, seed):
+    rng = np.random.default_rng(seed)
+    csr = []
+    for _ in range(n_layers):
+        m = n * deg // 2
+        p1, p2 = rng.integers(0, n, m), rng.integers(0, n, m)
+        src = np.concatenate([p1, p2])
+        tgt = np.concatenate([p2, p1])
+        keep = src != tgt
+        src, tgt = src[keep], tgt[keep]
+        order = np.lexsort((src, tgt))
+        indptr = np.zeros(n + 1, dtype=np.int64)
+        indptr[1:] = np.cumsum(np.bincount(tgt, minlength=n))
+        csr.append((indptr, src[order].astype(np.int32)))
+    bins = rng.integers(0, 10, n)
+    rt = rng.uniform(0.2, 3.0, n)
+    w = np.zeros((10, 12))
+    w[:, 1:] = rng.uniform(0.0, 1.6, (10, 11))
+    return csr, bins, rt, w
+
+
+def test_relax_reference_beta_zero_and_saturation():
+    csr, bins, rt, w = _random_graph(30, 2, 6, 1)
+    theta = np.concatenate([[0.0, 1.0, 2.0], np.ones(10)])
+    assert np.all(_ref(theta, csr, bins, rt, w, 6, 0.2).numpy() == 0)
+    # saturated edges: every infectious neighbour caps at XMAX, so L_i = deg_i * log1p(-XMAX) on day 1
+    theta = np.concatenate([[1e6, 1.0, 1.0], np.ones(10)])
+    E = _ref(theta, csr, bins, rt, w, 2, 0.2).numpy()
+    deg = sum(np.diff(indptr) for indptr, _ in csr)
+    want = (0.8 * (1 - np.exp(deg * np.log1p(-XMAX)))).sum()
+    assert E[1] == pytest.approx(want, rel=1e-12)
+
+
+def test_relax_reference_gradient_matches_central_differences():
+    import torch
+    csr, bins, rt, w = _random_graph(60, 3, 8, 2)
+    theta = np.concatenate([[0.03, 1.5, 0.7, 0.4], np.linspace(0.5, 1.5, 10)])
+    c = np.random.default_rng(3).uniform(0.5, 1.5, 10)
+    th = torch.tensor(theta, requires_grad=True)
+    J = (torch.as_tensor(c) * relax_ref.expected_infections(th, csr, bins, rt, w, 10, 0.05)).sum()
+    J.backward()
+    g = th.grad.numpy()
+    h = 1e-6
+    fd = np.zeros_like(theta)
+    for k in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd[k] = (c @ _ref(tp, csr, bins, rt, w, 10, 0.05).numpy() - c @ _ref(tm, csr, bins, rt, w, 10, 0.05).numpy()) \
+            / (2 * h * theta[k])
+    assert np.linalg.norm(g - fd) <= 1e-7 * np.linalg.norm(fd)
+
+
+# ------------------------------------------------------------------ GPU
+
+def _sim(n, n_days, seed=1, rand_seed=None, **pars):
+    import covasim_amd as cv
+    from tests.helpers import make_population
+    pop = make_population(n=n, seed=seed, pop_type='hybrid')
+    return cv.Sim(pop_size=n, pop_type='hybrid', n_days=n_days, rand_seed=seed if rand_seed is None else rand_seed,
+                  popdict=pop, use_waning=False, **pars).initialize()
+
+
+def _cpu_inputs(relax):
+    return relax.csr_host(), relax.bins_np, relax._rt.cpu().numpy().astype(np.float64), relax.w, relax.n_days, \
+        relax.seed_frac
+
+
+def _rel(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+@pytest.mark.gpu
+def test_relax_forward_and_adjoint_match_cpu_reference(gpu):
+    import torch
+    from covasim_amd.relax import Relaxation
+    sim = _sim(1500, 25)
+    r64 = Relaxation(sim, precision=64, w_samples=20000)
+    r32 = Relaxation(sim, precision=32, w_samples=20000)
+    theta = r64.theta0() * np.linspace(0.8, 1.2, r64.n_params)
+    theta[0] = 0.05  # a faster epidemic than the default, so every day carries weight
+    csr, bins, rt, w, T, f = _cpu_inputs(r64)
+    c = np.random.default_rng(4).uniform(0.5, 1.5, T)
+    th = torch.tensor(theta, requires_grad=True)
+    e_ref = relax_ref.expected_infections(th, csr, bins, rt, w, T, f)
+    (torch.as_tensor(c) * e_ref).sum().backward()
+    g_ref = th.grad.numpy()
+    e_ref = e_ref.detach().numpy()
+    assert e_ref[1:].min() > 0 and e_ref.sum() > 10
+    for relax, tol_e, tol_g in [(r64, 1e-10, 1e-9), (r32, 1e-4, 1e-4)]:
+        t = torch.tensor(theta, requires_grad=True)
+        e = relax.expected(t)
+        np.testing.assert_allclose(e.detach().numpy()[1:], e_ref[1:], rtol=tol_e)
+        (torch.as_tensor(c) * e).sum().backward()
+        assert _rel(t.grad.numpy(), g_ref) <= tol_g, (relax.precision, t.grad.numpy(), g_ref)
+
+
+@pytest.mark.gpu
+def test_relax_checkpoint_interval_does_not_change_the_gradient(gpu):
+    from covasim_amd.relax import Relaxation
+    sim = _sim(1200, 23)
+    grads = []
+    for ck in (1, 7, 23):
+        r = Relaxation(sim, precision=32, checkpoint_every=ck, w_samples=20000)
+        theta = r.theta0()
+        theta[0] = 0.05
+        r.forward_np(theta)
+        grads.append(r.backward_np(theta, np.linspace(1, 2, r.n_days)))
+    assert np.array_equal(grads[0], grads[1]) and np.array_equal(grads[0], grads[2])
+
+
+@pytest.mark.gpu
+def test_relax_1m_adjoint_matches_fp64_central_differences(gpu):
+    """1M-agent hybrid population, 60 days: float32 adjoint vs float64 central differences."""
+    from covasim_amd.relax import Relaxation
+    sim = _sim(1_000_000, 60, seed=5)
+    r32 = Relaxation(sim, precision=32)
+    r64 = Relaxation(sim, precision=64)
+    theta = r32.theta0()
+    c = np.random.default_rng(6).uniform(0.5, 1.5, r32.n_days) / 1e4
+    e = r32.forward_np(theta)
+    assert e.sum() > 1e4  # the relaxed epidemic grows over the 60 days
+    g = r32.backward_np(theta, c)
+    h = 1e-4
+    fd = np.zeros_like(theta)
+    for k in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd[k] = (c @ r64.forward_np(tp) - c @ r64.forward_np(tm)) / (2 * h * theta[k])
+    err = _rel(g, fd)
+    print(f'1M relaxation: |g32 - fd64| / |fd64| = {err:.2e}', flush=True)
+    assert err <= 1e-4, (g, fd)
+
+
+@pytest.mark.gpu
+def test_relax_calibration_recovers_the_incidence_curve(gpu):
+    from covasim_amd.relax import Relaxation, calibrate
+    n, T = 20000, 40
+    observed = []
+    for seed in (1, 2, 3, 4):  # the stochastic sim at known parameters, same population
+        sim = _sim(n, T, seed=7, rand_seed=seed)
+        sim.run()
+        observed.append(sim.results['new_infections'].values)
+    observed = np.mean(observed, axis=0)
+    relax = Relaxation(_sim(n, T, seed=7), precision=32)
+    truth = relax.theta0()
+    start = truth * np.random.default_rng(8).uniform(0.6, 1.5, relax.n_params)
+    fitted, hist = calibrate(relax, observed, start, iters=80, lr=0.05)
+    assert hist[-1] < 0.2 * hist[0], hist[::10]
