@@ -53,6 +53,28 @@ WORKLOADS = {
 }
 
 
+def measured_traffic(workload, tuning):
+    """HBM bytes per edge-pass launch from the committed counter summary (profiles/), or None.
+
+    The summary holds FETCH_SIZE and WRITE_SIZE means over the edge-pass dispatches of this workload,
+    already scaled by what the same counters read for a device copy of known size.
+    """
+    path = os.path.join(ROOT, 'profiles', f'edge_pass_traffic_{workload}.txt')
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if ':' in line and not line.startswith('#'):
+            k, v = line.split(':', 1)
+            vals[k.strip()] = v.strip()
+    if vals.get('tuning', 'default') != (json.dumps(tuning, sort_keys=True) if tuning else 'default'):
+        return None
+    try:
+        return float(vals['fetch_bytes_per_launch']) + float(vals['write_bytes_per_launch'])
+    except (KeyError, ValueError):
+        return None
+
+
 def build_population(w, seed):
     from covasim_amd import parameters as cvpar
     from covasim_amd import population as cvpop
@@ -246,6 +268,10 @@ def main():
     ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
     ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass tile depth (2 or 4 chunks per thread)')
     ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
+    ap.add_argument('--edge-wave-tiles', type=int, default=None,
+                    help='1: every wave streams its own round-robin share of the edges; 0: workgroup tiles')
+    ap.add_argument('--eager-only', action='store_true',
+                    help='counter-collection mode: skip the graph replays; value is then the eager loop rate')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='MiB of one device-to-device copy after the run (a known-size kernel for counter scaling)')
     args = ap.parse_args()
@@ -263,6 +289,9 @@ def main():
     if args.edge_chunks is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_CHUNKS, args.edge_chunks), 'tuning')
         tuning['edge_chunks'] = args.edge_chunks
+    if args.edge_wave_tiles is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_WAVE_TILES, args.edge_wave_tiles), 'tuning')
+        tuning['edge_wave_tiles'] = args.edge_wave_tiles
     if args.edge_lds is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, args.edge_lds), 'tuning')
         tuning['edge_lds'] = args.edge_lds
@@ -299,7 +328,9 @@ def main():
         _lib.reset_workspace(sim._w)
 
     P = people.struct()
-    with torch.cuda.stream(stream):
+
+    def graph_run():
+        """Capture all days as one graph, replay it for warmup, then time one replay from a fresh state."""
         reset()
         torch.cuda.synchronize()
         gexec = C.c_void_p()
@@ -318,12 +349,17 @@ def main():
         wall = time.perf_counter() - t0
         print(f'timed {steps} days in {wall * 1e3:.1f} ms', file=sys.stderr, flush=True)
         res = sim._w['results'].cpu().numpy().view(np.uint32)
-        cum_inf = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
+        cum = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
         _lib.check(L.cvk_graph_destroy(gexec), 'graph destroy')
+        return wall, res.copy(), cum
 
+    with torch.cuda.stream(stream):
+        if not args.eager_only:
+            wall, res, cum_inf = graph_run()
         # Edge-pass duration: the same days again, eager, with HIP events on this stream around each edge pass.
         reset()
         torch.cuda.synchronize()
+        t_eager = time.perf_counter()
         edge_ms = []
         st = C.c_void_p(stream.cuda_stream)
         pr, wk = sim._params, sim._work
@@ -349,9 +385,15 @@ def main():
             _lib.check(L.cvk_count_pass(P, pr, wk, t, st), 'count')
             edge_ms.append((e0, e1))
         torch.cuda.synchronize()
+        eager_wall = time.perf_counter() - t_eager
         edge_ms = [a.elapsed_time(b) for a, b in edge_ms]
         res2 = sim._w['results'].cpu().numpy().view(np.uint32)
-        eager_matches_graph = bool(np.array_equal(res[:steps], res2[:steps]))
+        if args.eager_only:  # counter-collection mode: no graph run to compare with
+            wall, res = eager_wall, res2
+            cum_inf = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
+            eager_matches_graph = None
+        else:
+            eager_matches_graph = bool(np.array_equal(res[:steps], res2[:steps]))
 
     n = w['pop_size']
     avg_edge_s = float(np.mean(edge_ms)) / 1e3
@@ -364,9 +406,12 @@ def main():
         'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': w['desc'], 'agents': n, 'days': steps, 'edges': int(n_edges),
                    'dynamic_edges': int(dyn_edges), 'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
-                   'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {})},
+                   'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {}),
+                   **({'mode': 'eager launches (counter collection); value is not the graph-replay rate'}
+                      if args.eager_only else {})},
         'roofline': {'bound': 'hbm', 'kernel': 'edge_pass_kernel', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': measured_traffic(args.workload, tuning),
                      'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes,
                      'launch_us_p10_p50_p90_max': [round(float(x) * 1e3, 1) for x in
                                                    np.percentile(edge_ms, [10, 50, 90, 100])]},

@@ -33,6 +33,7 @@ DUR_ORDER = ['exp2inf', 'inf2sym', 'sym2sev', 'sev2crit', 'asym2rec', 'mild2rec'
 
 TUNE_EDGE_LDS = 1
 TUNE_EDGE_CHUNKS = 2
+TUNE_EDGE_WAVE_TILES = 3
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15
@@ -56,7 +57,7 @@ RV_NAMES = ['new_infections_by_variant', 'new_symptomatic_by_variant', 'new_seve
 R_COLS = R_BYVAR + len(RV_NAMES) * MAX_VARIANTS
 RF_COLS = 4
 CTL_WORDS = 64
-CTL_TOUCHED, CTL_LOG_COUNT, CTL_LOG_OVERFLOW, CTL_SEL_SHORT = 0, 2, 4, 10
+CTL_LOG_COUNT, CTL_LOG_OVERFLOW, CTL_SEL_SHORT = 2, 4, 10
 
 
 class Dur(C.Structure):
@@ -106,14 +107,14 @@ class WorkS(C.Structure):
     _fields_ = [('winner', C.c_void_p), ('touched', C.c_void_p), ('bitmap', C.c_void_p), ('ctl', C.c_void_p),
                 ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
                 ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('susmap', C.c_void_p),
-                ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p),
+                ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p), ('bound', C.c_void_p),
                 ('npts', C.c_int32), ('pad', C.c_int32)]
 
 
 MAX_REDUCE_BLOCKS = 65536
 PARTIAL_COLS = 64
 WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'susmap',
-                   'rec', 'partials', 'partials_f']
+                   'rec', 'partials', 'partials_f', 'bound']
 
 
 def make_workspace(n, npts, log_capacity, device):
@@ -122,7 +123,7 @@ def make_workspace(n, npts, log_capacity, device):
     words = ((n + 127) // 128) * 4
     w = dict(
         winner=torch.full((n,), -1, dtype=torch.int64, device=device),
-        touched=torch.zeros(n, dtype=torch.int32, device=device),
+        touched=torch.zeros((n + 31) // 32, dtype=torch.int32, device=device),
         bitmap=torch.zeros(words, dtype=torch.int32, device=device),
         susmap=torch.zeros(words, dtype=torch.int32, device=device),
         rec=torch.zeros(n, dtype=torch.int64, device=device),
@@ -133,6 +134,7 @@ def make_workspace(n, npts, log_capacity, device):
         select_hist=torch.zeros(256, dtype=torch.int32, device=device),
         partials=torch.zeros((MAX_REDUCE_BLOCKS, PARTIAL_COLS), dtype=torch.int32, device=device),
         partials_f=torch.zeros((MAX_REDUCE_BLOCKS, RF_COLS), dtype=torch.float64, device=device),
+        bound=torch.zeros(n, dtype=torch.int16, device=device),
     )
     s = WorkS()
     for k in WORK_PTR_FIELDS:
@@ -207,7 +209,7 @@ def lib():
     L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cvk_set_tuning.argtypes = [i32, i32]
     L.cvk_workspace_sizes.argtypes = [i64, i32] + [C.POINTER(C.c_int64)] * 5
-    L.cvk_workspace_sizes2.argtypes = [i64] + [C.POINTER(C.c_int64)] * 3
+    L.cvk_workspace_sizes2.argtypes = [i64] + [C.POINTER(C.c_int64)] * 4
     L.cvk_viral_load.argtypes = [i32, vp, vp, vp, i64, f32, f32, f32, vp, vp]
     L.cvk_trans_sus.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp, vp, f32, f32, f32, vp, i64, vp, vp, vp]
     L.cvk_infections.argtypes = [f32, vp, vp, vp, i64, vp, vp, i64, vp, u64, i32, i32, i32, i32, vp, vp, vp, i64,

@@ -45,10 +45,11 @@ struct BatchCommon {
   double* partials_f;    // [CVK_RF_COLS][part_stride]
   int64_t part_stride;
   uint64_t* rec;
+  uint16_t* bound;
   uint32_t* bitmap;
   uint32_t* susmap;
   uint64_t* winner;
-  int32_t* touched;
+  uint32_t* touched;     // bitmap over storage indices: targets holding a key
   uint32_t* ctl;         // global counters (CVK_CTL_* indices)
 };
 
@@ -92,12 +93,12 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
   S.bitmap = (uint64_t*)(C.bitmap + B.bm_off);
   S.susmap = (uint64_t*)(C.susmap + B.bm_off);
   S.rec = C.rec + B.a_off;
+  S.bound = C.bound + B.a_off;
   S.partials = C.partials;
   S.partials_f = C.partials_f;
   S.part_stride = C.part_stride;
   S.part_col = blockIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.transitions) {
-    C.ctl[CVK_CTL_TOUCHED] = 0;
     C.ctl[CVK_CTL_EDGE_TILES] = 0;
   }
   agent_block<APT>(C.P, C.params[s], S, a.t, a.prologue, a.transitions);
@@ -143,7 +144,7 @@ __device__ inline void batch_load_tile(const BatchEdgeArgs& a, int64_t tile, Edg
   const int32_t seg = a.tile_seg[tile];
   T.seg = seg;
   T.e0 = (tile - a.seg_tile_start[seg]) * (int64_t)(BLOCK * 4 * CHUNKS);
-  load_tile_edges<BLOCK, CHUNKS>(a.segs[seg], T);
+  load_tile_edges<BLOCK, CHUNKS>(a.segs[seg], T, threadIdx.x);
 }
 
 template <int BLOCK, int CHUNKS>
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
     __syncthreads();
     batch_load_tile<BLOCK, CHUNKS>(a, s_tiles[par], nxt);
     par ^= 1;
-    if (in_lds) process_tile<true, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur);
-    else process_tile<false, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur);
+    if (in_lds) process_tile<true, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
+    else process_tile<false, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
     cur = nxt;
   }
 }
@@ -202,80 +203,86 @@ __global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon 
   for (int j = threadIdx.x; j <= C.n_sims; j += INFECT_BLOCK) s_off[j] = ia.a_off[j];
   __syncthreads();
   const int32_t t = ia.t;
-  const uint32_t n_touched = C.ctl[CVK_CTL_TOUCHED];
-  const uint32_t stride = gridDim.x * INFECT_BLOCK;
-  const uint32_t n_rounds = (n_touched + stride - 1) / stride;
-  for (uint32_t r = 0; r < n_rounds; ++r) {
-    const uint32_t j = r * stride + blockIdx.x * INFECT_BLOCK + threadIdx.x;
-    InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
-    int32_t source = -1, s = 0;
-    int64_t il = 0;
-    if (j < n_touched) {
-      const int64_t g = C.touched[j];
-      int32_t lo = 0, hi = C.n_sims - 1;  // the sim with s_off[s] <= g < s_off[s + 1]
-      while (lo < hi) {
-        const int32_t mid = (lo + hi + 1) >> 1;
-        if (s_off[mid] <= g) lo = mid;
-        else hi = mid - 1;
-      }
-      s = lo;
-      il = g - s_off[s];
-      const cvk_params& prm = C.params[s];
-      const uint32_t* row = C.results + ((int64_t)s * C.npts + t) * CVK_R_COLS;
-      const bool hosp_max = ia.allow_beds && prm.n_beds_hosp >= 0 && (int64_t)row[CVK_R_MID_SEVERE] > prm.n_beds_hosp;
-      const bool icu_max = ia.allow_beds && prm.n_beds_icu >= 0 && (int64_t)row[CVK_R_MID_CRITICAL] > prm.n_beds_icu;
-      o = infect_target(C.P, prm, C.winner, g, (uint32_t)il, C.n_total, t, hosp_max, icu_max);
-      if (o.inf && (int32_t)o.l < C.n_layers) {
-        const SegView& L = ia.segs[s * C.n_layers + (int32_t)o.l];
-        source = key_source(o.key, L.p1, L.p2);
-      }
-    }
-    // infection log, people.py:509-511 (indices within the sim, sim in bits 12+ of the last word)
-    const uint64_t m = __ballot(o.inf);
-    if (m) {
-      unsigned long long base = 0;
-      if (lane_id() == 0) base = atomicAdd((unsigned long long*)&C.ctl[CVK_CTL_LOG_COUNT], (unsigned long long)__popcll(m));
-      base = __shfl(base, 0, 64);
-      if (o.inf) {
-        const unsigned long long pos = base + lanes_below(m);
-        if ((int64_t)pos < ia.log_capacity) {
-          int4 rec;
-          rec.x = source;
-          rec.y = (int32_t)il;
-          rec.z = t;
-          rec.w = (int32_t)(o.l | (o.v << 8) | ((uint32_t)s << 12));
-          ((int4*)ia.log)[pos] = rec;
-        } else {
-          C.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
+  const int64_t n_words = (C.n_total + 31) / 32;
+  const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
+  const int64_t n_sweeps = (n_words + stride - 1) / stride;
+  for (int64_t sw = 0; sw < n_sweeps; ++sw) {
+    const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
+    uint32_t bits = w < n_words ? C.touched[w] : 0u;
+    if (bits) C.touched[w] = 0u;
+    while (__ballot(bits != 0)) {  // wave-uniform: one set bit per lane per round
+      InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
+      int32_t source = -1, s = 0;
+      int64_t il = 0;
+      const int32_t bit = take_bit(bits);
+      if (bit >= 0 && w * 32 + bit < C.n_total) {
+        const int64_t g = w * 32 + bit;
+        int32_t lo = 0, hi = C.n_sims - 1;  // the sim with s_off[s] <= g < s_off[s + 1]
+        while (lo < hi) {
+          const int32_t mid = (lo + hi + 1) >> 1;
+          if (s_off[mid] <= g) lo = mid;
+          else hi = mid - 1;
+        }
+        s = lo;
+        il = g - s_off[s];
+        const cvk_params& prm = C.params[s];
+        const uint32_t* row = C.results + ((int64_t)s * C.npts + t) * CVK_R_COLS;
+        const bool hosp_max = ia.allow_beds && prm.n_beds_hosp >= 0 && (int64_t)row[CVK_R_MID_SEVERE] > prm.n_beds_hosp;
+        const bool icu_max = ia.allow_beds && prm.n_beds_icu >= 0 && (int64_t)row[CVK_R_MID_CRITICAL] > prm.n_beds_icu;
+        o = infect_target(C.P, prm, C.winner, g, (uint32_t)il, C.n_total, t, hosp_max, icu_max);
+        if (o.inf && (int32_t)o.l < C.n_layers) {
+          const SegView& L = ia.segs[s * C.n_layers + (int32_t)o.l];
+          source = key_source(o.key, L.p1, L.p2);
         }
       }
-    }
-    // flows: lanes peeled by sim (targets of one sim are mostly adjacent), one atomic per column per sim
-    bool pending = ia.count_flows && o.inf;
-    while (true) {
-      const uint64_t pm = __ballot(pending);
-      if (!pm) break;
-      const int lead = __ffsll((unsigned long long)pm) - 1;
-      const int32_t sl = __shfl(s, lead, 64);
-      const bool mine = pending && s == sl;
-      uint32_t cnt[N_INFECT_COLS];
-      cnt[0] = (uint32_t)__popcll(__ballot(mine));
-      cnt[1] = (uint32_t)__popcll(__ballot(mine && o.reinf));
-      for (int vv = 0; vv < MAXV; ++vv) {
-        if (vv < ia.n_variants) {
-          cnt[2 + vv] = (uint32_t)__popcll(__ballot(mine && o.v == (uint32_t)vv));
-          cnt[2 + MAXV + vv] = (uint32_t)__popcll(__ballot(mine && o.symp && o.v == (uint32_t)vv));
-          cnt[2 + 2 * MAXV + vv] = (uint32_t)__popcll(__ballot(mine && o.sev && o.v == (uint32_t)vv));
-        } else {
-          cnt[2 + vv] = cnt[2 + MAXV + vv] = cnt[2 + 2 * MAXV + vv] = 0;
+      // infection log, people.py:509-511 (indices within the sim, sim in bits 12+ of the last word)
+      const uint64_t m = __ballot(o.inf);
+      if (m) {
+        unsigned long long base = 0;
+        if (lane_id() == 0)
+          base = atomicAdd((unsigned long long*)&C.ctl[CVK_CTL_LOG_COUNT], (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (o.inf) {
+          const unsigned long long pos = base + lanes_below(m);
+          if ((int64_t)pos < ia.log_capacity) {
+            int4 rec;
+            rec.x = source;
+            rec.y = (int32_t)il;
+            rec.z = t;
+            rec.w = (int32_t)(o.l | (o.v << 8) | ((uint32_t)s << 12));
+            ((int4*)ia.log)[pos] = rec;
+          } else {
+            C.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
+          }
         }
       }
-      if ((int)lane_id() == lead) {
-        uint32_t* row = C.results + ((int64_t)sl * C.npts + t) * CVK_R_COLS;
-        for (int k = 0; k < N_INFECT_COLS; ++k)
-          if (cnt[k]) atomicAdd(&row[infect_col(k)], cnt[k]);
+      // flows: lanes peeled by sim (targets of one sim are adjacent), one atomic per column per sim
+      bool pending = ia.count_flows && o.inf;
+      while (true) {
+        const uint64_t pm = __ballot(pending);
+        if (!pm) break;
+        const int lead = __ffsll((unsigned long long)pm) - 1;
+        const int32_t sl = __shfl(s, lead, 64);
+        const bool mine = pending && s == sl;
+        uint32_t cnt[N_INFECT_COLS];
+        cnt[0] = (uint32_t)__popcll(__ballot(mine));
+        cnt[1] = (uint32_t)__popcll(__ballot(mine && o.reinf));
+        for (int vv = 0; vv < MAXV; ++vv) {
+          if (vv < ia.n_variants) {
+            cnt[2 + vv] = (uint32_t)__popcll(__ballot(mine && o.v == (uint32_t)vv));
+            cnt[2 + MAXV + vv] = (uint32_t)__popcll(__ballot(mine && o.symp && o.v == (uint32_t)vv));
+            cnt[2 + 2 * MAXV + vv] = (uint32_t)__popcll(__ballot(mine && o.sev && o.v == (uint32_t)vv));
+          } else {
+            cnt[2 + vv] = cnt[2 + MAXV + vv] = cnt[2 + 2 * MAXV + vv] = 0;
+          }
+        }
+        if ((int)lane_id() == lead) {
+          uint32_t* row = C.results + ((int64_t)sl * C.npts + t) * CVK_R_COLS;
+          for (int k = 0; k < N_INFECT_COLS; ++k)
+            if (cnt[k]) atomicAdd(&row[infect_col(k)], cnt[k]);
+        }
+        pending = pending && !mine;
       }
-      pending = pending && !mine;
     }
   }
 }
@@ -398,7 +405,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_choose_mark_kernel(BatchCom
     if (key > thr) continue;
     const uint64_t wk = make_key(0, variant, layer_code, 0, (uint64_t)li);
     const uint64_t old = atomicMin((unsigned long long*)&C.winner[g], (unsigned long long)wk);
-    if (old == KEY_NONE) C.touched[atomicAdd(&C.ctl[CVK_CTL_TOUCHED], 1u)] = (int32_t)g;
+    if (old == KEY_NONE) atomicOr(&C.touched[g >> 5], 1u << (g & 31));
   }
 }
 
@@ -465,10 +472,11 @@ struct Batch {
   int32_t* d_rs_chunk_seg = nullptr;
   int64_t* d_rs_chunk_first = nullptr;
   uint64_t* d_rec = nullptr;
+  uint16_t* d_bound = nullptr;
   uint32_t* d_bitmap = nullptr;
   uint32_t* d_susmap = nullptr;
   uint64_t* d_winner = nullptr;
-  int32_t* d_touched = nullptr;
+  uint32_t* d_touched = nullptr;
   uint32_t* d_ctl = nullptr;
   uint32_t* d_results = nullptr;
   double* d_results_f = nullptr;
@@ -485,7 +493,7 @@ struct Batch {
 
   ~Batch() {
     void* ptrs[] = {d_sims, d_params, d_segs, d_views, d_a_off, d_seg_tile_start, d_tile_seg, d_block_sim,
-                    d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bitmap, d_susmap, d_winner, d_touched, d_ctl,
+                    d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bound, d_bitmap, d_susmap, d_winner, d_touched, d_ctl,
                     d_results, d_results_f, d_partials, d_partials_f, d_log, d_hist, d_prefix, d_remain,
                     d_all, d_short, d_k};
     for (void* p : ptrs)
@@ -508,6 +516,7 @@ struct Batch {
     c.partials_f = d_partials_f;
     c.part_stride = part_stride;
     c.rec = d_rec;
+    c.bound = d_bound;
     c.bitmap = d_bitmap;
     c.susmap = d_susmap;
     c.winner = d_winner;
@@ -877,10 +886,11 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
   rc = rc ? rc : dev_alloc(&b->d_rs_chunk_first, std::max<int64_t>(1, b->n_rs_chunks));
   rc = rc ? rc : upload(b->d_rs_chunk_first, rs_first.data(), b->n_rs_chunks);
   rc = rc ? rc : dev_alloc(&b->d_rec, b->N);
+  rc = rc ? rc : dev_alloc(&b->d_bound, b->N);
   rc = rc ? rc : dev_alloc(&b->d_bitmap, words);
   rc = rc ? rc : dev_alloc(&b->d_susmap, words);
   rc = rc ? rc : dev_alloc(&b->d_winner, b->N);
-  rc = rc ? rc : dev_alloc(&b->d_touched, b->N);
+  rc = rc ? rc : dev_alloc(&b->d_touched, (b->N + 31) / 32);
   rc = rc ? rc : dev_alloc(&b->d_ctl, CVK_CTL_WORDS);
   rc = rc ? rc : dev_alloc(&b->d_results, (int64_t)b->S * npts * CVK_R_COLS);
   rc = rc ? rc : dev_alloc(&b->d_results_f, (int64_t)b->S * npts * CVK_RF_COLS);
@@ -901,13 +911,13 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
     const cvk_params& p = b->params[s];
     const BatchSim& bs = b->sims[s];
     V.rec = b->d_rec + bs.a_off;
+    V.bound = b->d_bound + bs.a_off;
     V.winner = b->d_winner + bs.a_off;
     V.sus_imm = P.sus_imm + bs.a_off;
     V.sus_stride = b->N;
     V.bitmap = b->d_bitmap + bs.bm_off;
     V.susmap = b->d_susmap + bs.bm_off;
     V.touched = b->d_touched;
-    V.touched_count = b->d_ctl + CVK_CTL_TOUCHED;
     V.a_off = bs.a_off;
     V.seed = p.seed;
     V.n_agents = (uint32_t)bs.n;
@@ -950,6 +960,7 @@ extern "C" int cvk_batch_reset(void* handle, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   const dim3 grid((unsigned)std::max(1, 4 * b.n_cu)), block(256);
   hipLaunchKernelGGL(fill_u64_kernel, grid, block, 0, st, b.d_winner, b.N, KEY_NONE);
+  hipLaunchKernelGGL(fill_u32_kernel, grid, block, 0, st, b.d_touched, (b.N + 31) / 32, 0u);
   hipLaunchKernelGGL(fill_u32_kernel, grid, block, 0, st, b.d_ctl, (int64_t)CVK_CTL_WORDS, 0u);
   hipLaunchKernelGGL(fill_u32_kernel, grid, block, 0, st, b.d_results, (int64_t)b.S * b.npts * CVK_R_COLS, 0u);
   hipLaunchKernelGGL(fill_u64_kernel, grid, block, 0, st, (uint64_t*)b.d_results_f, (int64_t)b.S * b.npts * CVK_RF_COLS, 0ull);

@@ -76,12 +76,12 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   S.bitmap = (uint64_t*)W.bitmap;
   S.susmap = (uint64_t*)W.susmap;
   S.rec = W.rec;
+  S.bound = W.bound;
   S.partials = W.partials;
   S.partials_f = W.partials_f;
   S.part_stride = CVK_MAX_REDUCE_BLOCKS;
   S.part_col = blockIdx.x;
   if (blockIdx.x == 0 && threadIdx.x == 0 && a.transitions) {
-    W.ctl[CVK_CTL_TOUCHED] = 0;       // the new day's target list
     W.ctl[CVK_CTL_EDGE_TILES] = 0;    // the edge pass's tile counter
   }
   agent_block<AGENTS_PER_THREAD>(P, prm, S, a.t, a.prologue, a.transitions);
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void choose_mark_kernel(cvk_people P, cvk_work
     if (key > thr) continue;
     const uint64_t wk = make_key(0, variant, layer_code, 0, (uint64_t)i);
     const uint64_t old = atomicMin((unsigned long long*)&W.winner[i], (unsigned long long)wk);
-    if (old == KEY_NONE) W.touched[atomicAdd(&W.ctl[CVK_CTL_TOUCHED], 1u)] = (int32_t)i;
+    if (old == KEY_NONE) atomicOr(&W.touched[i >> 5], 1u << (i & 31));
   }
 }
 
@@ -196,7 +196,8 @@ constexpr int edge_tile(int chunks) { return EDGE_BLOCK * 4 * chunks; }
 struct EdgeArgs {
   SimView sim;
   SegView seg[CVK_MAX_LAYERS];
-  int64_t tile_start[CVK_MAX_LAYERS + 1];
+  int64_t tile_start[CVK_MAX_LAYERS + 1];       // block tiles (EDGE_BLOCK threads) per layer
+  int64_t wave_tile_start[CVK_MAX_LAYERS + 1];  // wave tiles (one wave) per layer
   int32_t n_layers;
   int32_t bitmap_words;
   int32_t t;
@@ -215,10 +216,24 @@ __device__ inline void load_tile(const EdgeArgs& a, int64_t tile, EdgeTile<CHUNK
   while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
   T.seg = l;
   T.e0 = (tile - a.tile_start[l]) * edge_tile(CHUNKS);
-  load_tile_edges<EDGE_BLOCK, CHUNKS>(a.seg[l], T);
+  load_tile_edges<EDGE_BLOCK, CHUNKS>(a.seg[l], T, threadIdx.x);
 }
 
-template <bool LDS, int CHUNKS>
+// Wave tiles: 64 lanes x 4 x CHUNKS edges handled by one wave on its own.
+template <int CHUNKS>
+__device__ inline void load_wave_tile(const EdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
+  T.tile = tile;
+  if (tile >= a.wave_tile_start[a.n_layers]) return;
+  int l = 0;
+  while (l + 1 < a.n_layers && tile >= a.wave_tile_start[l + 1]) ++l;
+  T.seg = l;
+  T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * CHUNKS);
+  load_tile_edges<64, CHUNKS>(a.seg[l], T, lane_id());
+}
+
+// WAVE = false: workgroup tiles handed out by an atomic counter (dynamic balance, one barrier per tile).
+// WAVE = true: wave tiles assigned round-robin over every wave of the grid (no barrier, no atomic).
+template <bool LDS, int CHUNKS, bool WAVE>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ int64_t s_tiles[2];
@@ -233,6 +248,18 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
   __syncthreads();
+  if (WAVE) {  // the next wave tile's endpoints are in flight while this one is processed
+    const int64_t n_wt = a.wave_tile_start[a.n_layers];
+    const int64_t stride = (int64_t)gridDim.x * (EDGE_BLOCK / 64);
+    EdgeTile<CHUNKS> cur, nxt;
+    load_wave_tile(a, (int64_t)blockIdx.x * (EDGE_BLOCK / 64) + (threadIdx.x >> 6), cur);
+    while (cur.tile < n_wt) {
+      load_wave_tile(a, cur.tile + stride, nxt);
+      process_tile<LDS, 64, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur, lane_id());
+      cur = nxt;
+    }
+    return;
+  }
   const int64_t n_tiles = a.tile_start[a.n_layers];
   // tiles are handed out dynamically; the next tile's endpoints are in flight while this one is processed
   EdgeTile<CHUNKS> cur, nxt;
@@ -243,7 +270,7 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     __syncthreads();
     load_tile(a, s_tiles[par], nxt);
     par ^= 1;
-    process_tile<LDS, EDGE_BLOCK, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur);
+    process_tile<LDS, EDGE_BLOCK, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
     cur = nxt;
   }
 }
@@ -266,56 +293,63 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
   if (threadIdx.x < N_INFECT_COLS) s_cnt[threadIdx.x] = 0;
   __syncthreads();
   const int32_t t = ia.t;
-  const uint32_t n_touched = W.ctl[CVK_CTL_TOUCHED];
   const uint32_t* row = day_row(W, t);
   const bool hosp_max = ia.allow_beds && prm.n_beds_hosp >= 0 && (int64_t)row[CVK_R_MID_SEVERE] > prm.n_beds_hosp;
   const bool icu_max = ia.allow_beds && prm.n_beds_icu >= 0 && (int64_t)row[CVK_R_MID_CRITICAL] > prm.n_beds_icu;
-  const uint32_t stride = gridDim.x * INFECT_BLOCK;
-  const uint32_t n_rounds = (n_touched + stride - 1) / stride;
-  for (uint32_t r = 0; r < n_rounds; ++r) {
-    const uint32_t j = r * stride + blockIdx.x * INFECT_BLOCK + threadIdx.x;
-    InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
-    int32_t source = -1, i = 0;
-    if (j < n_touched) {
-      i = W.touched[j];
-      o = infect_target(P, prm, W.winner, i, (uint32_t)i, P.n, t, hosp_max, icu_max);
-      if (o.inf && (int32_t)o.l < ia.n_layers) source = key_source(o.key, ia.p1[o.l], ia.p2[o.l]);
-    }
-    // infection log, people.py:509-511: one wave-aggregated append per round
-    const uint64_t m = __ballot(o.inf);
-    if (m) {
-      unsigned long long base = 0;
-      if (lane_id() == 0) base = atomicAdd((unsigned long long*)&W.ctl[CVK_CTL_LOG_COUNT], (unsigned long long)__popcll(m));
-      base = __shfl(base, 0, 64);
-      if (o.inf) {
-        const unsigned long long pos = base + lanes_below(m);
-        if ((int64_t)pos < W.log_capacity) {
-          int4 rec;
-          rec.x = source;
-          rec.y = i;
-          rec.z = t;
-          rec.w = (int32_t)(o.l | (o.v << 8));
-          ((int4*)W.log)[pos] = rec;
-        } else {
-          W.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
+  // one touched-bitmap word (32 agents) per thread per sweep; each round takes one set bit per lane
+  const int64_t n_words = (P.n + 31) / 32;
+  const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
+  const int64_t n_sweeps = (n_words + stride - 1) / stride;
+  for (int64_t sw = 0; sw < n_sweeps; ++sw) {
+    const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
+    uint32_t bits = w < n_words ? W.touched[w] : 0u;
+    if (bits) W.touched[w] = 0u;
+    while (__ballot(bits != 0)) {  // wave-uniform
+      InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
+      int32_t source = -1, i = 0;
+      const int32_t b = take_bit(bits);
+      if (b >= 0 && w * 32 + b < P.n) {
+        i = (int32_t)(w * 32 + b);
+        o = infect_target(P, prm, W.winner, i, (uint32_t)i, P.n, t, hosp_max, icu_max);
+        if (o.inf && (int32_t)o.l < ia.n_layers) source = key_source(o.key, ia.p1[o.l], ia.p2[o.l]);
+      }
+      // infection log, people.py:509-511: one wave-aggregated append per round
+      const uint64_t m = __ballot(o.inf);
+      if (m) {
+        unsigned long long base = 0;
+        if (lane_id() == 0)
+          base = atomicAdd((unsigned long long*)&W.ctl[CVK_CTL_LOG_COUNT], (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, 64);
+        if (o.inf) {
+          const unsigned long long pos = base + lanes_below(m);
+          if ((int64_t)pos < W.log_capacity) {
+            int4 rec;
+            rec.x = source;
+            rec.y = i;
+            rec.z = t;
+            rec.w = (int32_t)(o.l | (o.v << 8));
+            ((int4*)W.log)[pos] = rec;
+          } else {
+            W.ctl[CVK_CTL_LOG_OVERFLOW] = 1;
+          }
         }
       }
-    }
-    if (ia.count_flows && m) {
-      const uint32_t n_inf = (uint32_t)__popcll(m);
-      const uint32_t n_reinf = (uint32_t)__popcll(__ballot(o.inf && o.reinf));
-      if (lane_id() == 0) {
-        atomicAdd(&s_cnt[0], n_inf);
-        if (n_reinf) atomicAdd(&s_cnt[1], n_reinf);
-      }
-      for (int vv = 0; vv < prm.n_variants; ++vv) {
-        const uint32_t a0 = (uint32_t)__popcll(__ballot(o.inf && o.v == (uint32_t)vv));
-        const uint32_t a1 = (uint32_t)__popcll(__ballot(o.inf && o.symp && o.v == (uint32_t)vv));
-        const uint32_t a2 = (uint32_t)__popcll(__ballot(o.inf && o.sev && o.v == (uint32_t)vv));
+      if (ia.count_flows && m) {
+        const uint32_t n_inf = (uint32_t)__popcll(m);
+        const uint32_t n_reinf = (uint32_t)__popcll(__ballot(o.inf && o.reinf));
         if (lane_id() == 0) {
-          if (a0) atomicAdd(&s_cnt[2 + vv], a0);
-          if (a1) atomicAdd(&s_cnt[2 + MAXV + vv], a1);
-          if (a2) atomicAdd(&s_cnt[2 + 2 * MAXV + vv], a2);
+          atomicAdd(&s_cnt[0], n_inf);
+          if (n_reinf) atomicAdd(&s_cnt[1], n_reinf);
+        }
+        for (int vv = 0; vv < prm.n_variants; ++vv) {
+          const uint32_t a0 = (uint32_t)__popcll(__ballot(o.inf && o.v == (uint32_t)vv));
+          const uint32_t a1 = (uint32_t)__popcll(__ballot(o.inf && o.symp && o.v == (uint32_t)vv));
+          const uint32_t a2 = (uint32_t)__popcll(__ballot(o.inf && o.sev && o.v == (uint32_t)vv));
+          if (lane_id() == 0) {
+            if (a0) atomicAdd(&s_cnt[2 + vv], a0);
+            if (a1) atomicAdd(&s_cnt[2 + MAXV + vv], a1);
+            if (a2) atomicAdd(&s_cnt[2 + 2 * MAXV + vv], a2);
+          }
         }
       }
     }
@@ -330,6 +364,7 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
 // ===========================================================================
 static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
 static int g_edge_chunks = 2;  // tunable: 4-edge chunks per thread per edge tile (2 or 4)
+static int g_edge_wave = 0;    // tunable: 1 = wave tiles assigned round-robin, 0 = workgroup tiles from a counter
 
 struct DeviceCaps {
   int device = -1;
@@ -369,7 +404,8 @@ static bool people_ok(const cvk_people* P) {
 
 static bool work_ok(const cvk_work* W) {
   return W && W->winner && W->touched && W->bitmap && W->ctl && W->results && W->results_f && W->select_hist &&
-         W->susmap && W->rec && W->partials && W->partials_f && W->npts > 0 && (W->log_capacity == 0 || W->log);
+         W->susmap && W->rec && W->partials && W->partials_f && W->bound && W->npts > 0 &&
+         (W->log_capacity == 0 || W->log);
 }
 
 static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* W, int32_t t) {
@@ -448,6 +484,9 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       if (value != 2 && value != 4) return fail(CVK_E_INVALID, "cvk_set_tuning: edge chunks must be 2 or 4");
       g_edge_chunks = value;
       return CVK_OK;
+    case CVK_TUNE_EDGE_WAVE_TILES:
+      g_edge_wave = value ? 1 : 0;
+      return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
@@ -485,7 +524,7 @@ extern "C" int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winn
                                    int64_t* bitmap_bytes, int64_t* results_bytes, int64_t* results_f_bytes) {
   if (n_agents <= 0 || npts <= 0) return fail(CVK_E_INVALID, "cvk_workspace_sizes: sizes must be positive");
   if (winner_bytes) *winner_bytes = n_agents * 8;
-  if (touched_bytes) *touched_bytes = n_agents * 4;
+  if (touched_bytes) *touched_bytes = ((n_agents + 31) / 32) * 4;
   if (bitmap_bytes) *bitmap_bytes = bitmap_words_for(n_agents) * 4;
   if (results_bytes) *results_bytes = (int64_t)npts * CVK_R_COLS * 4;
   if (results_f_bytes) *results_f_bytes = (int64_t)npts * CVK_RF_COLS * 8;
@@ -493,11 +532,12 @@ extern "C" int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winn
 }
 
 extern "C" int cvk_workspace_sizes2(int64_t n_agents, int64_t* rec_bytes, int64_t* partials_bytes,
-                                    int64_t* partials_f_bytes) {
+                                    int64_t* partials_f_bytes, int64_t* bound_bytes) {
   if (n_agents <= 0) return fail(CVK_E_INVALID, "cvk_workspace_sizes2: n_agents must be positive");
   if (rec_bytes) *rec_bytes = n_agents * 8;
   if (partials_bytes) *partials_bytes = (int64_t)CVK_MAX_REDUCE_BLOCKS * CVK_PARTIAL_COLS * 4;
   if (partials_f_bytes) *partials_f_bytes = (int64_t)CVK_MAX_REDUCE_BLOCKS * CVK_RF_COLS * 8;
+  if (bound_bytes) *bound_bytes = n_agents * 2;
   return CVK_OK;
 }
 
@@ -585,10 +625,12 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* fns[4] = {(const void*)edge_pass_kernel<false, 2>, (const void*)edge_pass_kernel<true, 2>,
-                          (const void*)edge_pass_kernel<false, 4>, (const void*)edge_pass_kernel<true, 4>};
+    const void* fns[8] = {(const void*)edge_pass_kernel<false, 2, false>, (const void*)edge_pass_kernel<true, 2, false>,
+                          (const void*)edge_pass_kernel<false, 4, false>, (const void*)edge_pass_kernel<true, 4, false>,
+                          (const void*)edge_pass_kernel<false, 2, true>, (const void*)edge_pass_kernel<true, 2, true>,
+                          (const void*)edge_pass_kernel<false, 4, true>, (const void*)edge_pass_kernel<true, 4, true>};
     int64_t b_min = -1;
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 8; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -613,13 +655,13 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   a.n_layers = params->n_layers;
   SimView& S = a.sim;
   S.rec = work->rec;
+  S.bound = work->bound;
   S.winner = work->winner;
   S.sus_imm = people->sus_imm;
   S.sus_stride = people->n;
   S.bitmap = work->bitmap;
   S.susmap = work->susmap;
   S.touched = work->touched;
-  S.touched_count = work->ctl + CVK_CTL_TOUCHED;
   S.a_off = 0;
   S.seed = params->seed;
   S.n_agents = (uint32_t)people->n;
@@ -640,7 +682,7 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   int rc0 = edge_dynamic_lds(&budget);
   if (rc0) return rc0;
   const bool use_lds = lds_bytes <= budget && g_edge_lds != 0;
-  int64_t tiles = 0;
+  int64_t tiles = 0, wave_tiles = 0;
   for (int l = 0; l < params->n_layers; ++l) {
     const cvk_layer& L = layers[l];
     if (L.n_edges < 0 || (L.n_edges > 0 && (!L.p1 || !L.p2 || !L.beta)))
@@ -658,19 +700,28 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     G.beta_ones = L.beta_ones;
     G.l = l;
     a.tile_start[l] = tiles;
+    a.wave_tile_start[l] = wave_tiles;
     tiles += (L.n_edges + edge_tile(g_edge_chunks) - 1) / edge_tile(g_edge_chunks);
+    wave_tiles += (L.n_edges + 256 * g_edge_chunks - 1) / (256 * g_edge_chunks);
   }
-  a.tile_start[params->n_layers] = tiles;
-  for (int l = params->n_layers + 1; l <= CVK_MAX_LAYERS; ++l) a.tile_start[l] = tiles;
+  for (int l = params->n_layers; l <= CVK_MAX_LAYERS; ++l) {
+    a.tile_start[l] = tiles;
+    a.wave_tile_start[l] = wave_tiles;
+  }
   if (tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
   const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1));
   const size_t shmem = (size_t)(use_lds ? lds_bytes : queue_bytes);
-  if (g_edge_chunks == 4) {
-    if (use_lds) hipLaunchKernelGGL((edge_pass_kernel<true, 4>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
-    else hipLaunchKernelGGL((edge_pass_kernel<false, 4>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
-  } else {
-    if (use_lds) hipLaunchKernelGGL((edge_pass_kernel<true, 2>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
-    else hipLaunchKernelGGL((edge_pass_kernel<false, 2>), dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
+  const dim3 g(grid), b(EDGE_BLOCK);
+  const int variant = (g_edge_wave ? 4 : 0) | (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
+  switch (variant) {
+    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2, false>), g, b, shmem, st, a); break;
+    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2, false>), g, b, shmem, st, a); break;
+    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4, false>), g, b, shmem, st, a); break;
+    case 3: hipLaunchKernelGGL((edge_pass_kernel<true, 4, false>), g, b, shmem, st, a); break;
+    case 4: hipLaunchKernelGGL((edge_pass_kernel<false, 2, true>), g, b, shmem, st, a); break;
+    case 5: hipLaunchKernelGGL((edge_pass_kernel<true, 2, true>), g, b, shmem, st, a); break;
+    case 6: hipLaunchKernelGGL((edge_pass_kernel<false, 4, true>), g, b, shmem, st, a); break;
+    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4, true>), g, b, shmem, st, a); break;
   }
   return check_launch("cvk_edge_pass");
 }

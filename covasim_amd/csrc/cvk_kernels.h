@@ -94,6 +94,15 @@ constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV, N_AGENT_COLS = N_ST
 constexpr int N_AGENT_FCOLS = 3;
 static_assert(N_AGENT_COLS <= CVK_PARTIAL_COLS, "partial row too small");
 
+// Upper-bound codes of the edge-pass prefilter: the float's bits above bit 18 (8 exponent and 5
+// mantissa bits), rounded up, so that bound_decode(bound_code(x)) >= x for every x >= 0.
+constexpr uint32_t BOUND_CODE_MAX = 0x1FDFu;  // the largest code that decodes to a finite float
+__device__ inline uint32_t bound_code(float x) {
+  const uint32_t c = (__float_as_uint(x) + 0x3FFFFu) >> 18;
+  return c < BOUND_CODE_MAX ? c : BOUND_CODE_MAX;
+}
+__device__ inline float bound_decode(uint32_t code) { return __uint_as_float((code & 0x1FFFu) << 18); }
+
 // Per-agent inputs, all loaded before any is used (one memory round trip).
 struct AgentIn {
   uint32_t f, vs;
@@ -133,6 +142,7 @@ struct AgentSpan {
   uint64_t* bitmap;    // the sim's infectious bitmap (64 agents per word, local indexing)
   uint64_t* susmap;    // the sim's susceptible bitmap
   uint64_t* rec;       // the sim's edge records (local indexing)
+  uint16_t* bound;     // the sim's upper-bound codes (local indexing)
   uint32_t* partials;  // column-major partial rows; this block writes entry `part_col` of each column
   double* partials_f;
   int64_t part_stride;
@@ -156,6 +166,11 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
   const int64_t n_words = (S.n_local + 63) / 64;
   const bool closing = prologue || !transitions;  // stocks + update_nab of the previous (or this) day
   const int32_t tc = transitions ? t - 1 : t;
+  float quar_max = 0.0f, iso_max = 0.0f;  // over layers: the bound covers every layer's factor
+  for (int l = 0; l < prm.n_layers; ++l) {
+    quar_max = fmaxf(quar_max, prm.quar_factor[l]);
+    iso_max = fmaxf(iso_max, prm.iso_factor[l]);
+  }
 
   int64_t idx[APT];
   AgentIn in[APT];
@@ -261,6 +276,19 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         sus_bit = (f & CVK_SUSCEPTIBLE) && x.rs != 0.0f;
         const float val = (f & CVK_INFECTIOUS) ? x.rt : x.rs;
         S.rec[li] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
+        // prefilter bound (edge_candidate): the source's factors for any layer, or the target's
+        uint32_t bc = 0;
+        if (inf_bit) {
+          float ub = x.rt;
+          if (f & CVK_QUARANTINED) ub = ub * quar_max;
+          if (!(f & CVK_SYMPTOMATIC)) ub = ub * prm.asymp_factor;
+          if (f & CVK_ISOLATED) ub = ub * iso_max;
+          ub = ub * ((f & CVK_EARLY) ? prm.vl_hi : prm.vl_lo);
+          bc = (inf_var(f) << 13) | bound_code(ub);
+        } else if (sus_bit) {
+          bc = bound_code((f & CVK_QUARANTINED) ? x.rs * quar_max : x.rs);
+        }
+        S.bound[li] = (uint16_t)bc;
         if (f & CVK_SEVERE) ev |= 1u << 7;
         if (f & CVK_CRITICAL) ev |= 1u << 8;
       }
@@ -380,13 +408,13 @@ __device__ inline uint64_t selection_key(uint64_t seed, int32_t t, uint32_t stre
 // One sim as the edge pass sees it (pointers offset to the sim's slices; local indexing).
 struct SimView {
   const uint64_t* rec;
+  const uint16_t* bound;    // per-agent upper-bound codes (prefilter), see AgentSpan
   uint64_t* winner;
   const float* sus_imm;     // row v at sus_imm + v * sus_stride
   int64_t sus_stride;
   const uint32_t* bitmap;
   const uint32_t* susmap;
-  int32_t* touched;         // storage indices (a_off + local) of targets with a first key
-  uint32_t* touched_count;
+  uint32_t* touched;        // bitmap over storage indices (a_off + local): targets holding a key
   int64_t a_off;
   uint64_t seed;
   uint32_t n_agents;
@@ -416,8 +444,32 @@ constexpr int EDGE_QCAP = 64 * 8;  // per-wave candidate queue: 4 edges x 2 dire
 // A candidate slot (infectious source, susceptible target) of edge e: the float32 products of
 // compute_trans_sus (utils.py:88-89) and compute_infections (utils.py:117), the float64 uniform
 // test (utils.py:123), and the first-occurrence dedup as an atomicMin.
+__device__ inline double edge_uniform(const SimView& S, const SegView& L, int32_t t, uint32_t v, int dir, int64_t e) {
+  if (L.unif) return L.unif[((int64_t)v * 2 + dir) * L.n + e];
+  const uint64_t ue = (uint64_t)e;
+  const u32x4 w = philox_seeded(S.seed, (uint32_t)ue, (uint32_t)t, STREAM_TX | (v << 8) | (uint32_t)L.l,
+                                (uint32_t)(ue >> 32));
+  return dir ? u01(w.z, w.w) : u01(w.x, w.y);
+}
+
+// Exact-rejection prefilter: p <= p_ub = beta_v * beta_layer * bound(src) * bound(tgt) (each bound at or
+// above the factor it covers; the 1 + 2^-10 margin absorbs float rounding of both products), so
+// u >= p_ub implies u >= p and the candidate is rejected without gathering the two 8-byte records.
+constexpr float BOUND_MARGIN = 1.0f + 1.0f / 1024.0f;
+
 __device__ inline void edge_candidate(const SimView& S, const SegView& L, int32_t t, int dir, int64_t e, int32_t src,
                                       int32_t tgt) {
+  uint32_t v_pre = 0xFFFFFFFFu;
+  double u_pre = 0.0;
+  if (L.beta_ones) {  // per-edge weights are not in the bound
+    const uint32_t qs = S.bound[src], qt = S.bound[tgt];
+    const uint32_t vq = qs >> 13;
+    if (vq >= (uint32_t)S.n_variants) return;  // not infectious with a transmitting factor
+    const float p_ub = (((S.beta_v[vq] * L.beta_layer) * bound_decode(qs)) * bound_decode(qt)) * BOUND_MARGIN;
+    u_pre = edge_uniform(S, L, t, vq, dir, e);
+    if (!(u_pre < (double)p_ub)) return;
+    v_pre = vq;
+  }
   const uint64_t rsrc = S.rec[src];
   const uint64_t rtgt = S.rec[tgt];
   const uint32_t fs = (uint32_t)(rsrc >> 32);
@@ -437,19 +489,14 @@ __device__ inline void edge_candidate(const SimView& S, const SegView& L, int32_
   const float eb = L.beta_ones ? 1.0f : L.beta[e];
   const float p = ((S.beta_v[v] * eb) * rt) * rs;                // utils.py:117
   if (p == 0.0f) return;                                         // utils.py:118
-  double u;
-  if (L.unif) {
-    u = L.unif[((int64_t)v * 2 + dir) * L.n + e];
-  } else {
-    const uint64_t ue = (uint64_t)e;
-    const u32x4 w = philox_seeded(S.seed, (uint32_t)ue, (uint32_t)t, STREAM_TX | (v << 8) | (uint32_t)L.l,
-                                  (uint32_t)(ue >> 32));
-    u = dir ? u01(w.z, w.w) : u01(w.x, w.y);
-  }
+  const double u = v == v_pre ? u_pre : edge_uniform(S, L, t, v, dir, e);
   if (!(u < (double)p)) return;                                  // utils.py:123
   const uint64_t key = make_key(1, v, (uint32_t)L.l, (uint32_t)dir, (uint64_t)e);
   const uint64_t old = atomicMin((unsigned long long*)&S.winner[tgt], (unsigned long long)key);
-  if (old == KEY_NONE) S.touched[atomicAdd(S.touched_count, 1u)] = (int32_t)(S.a_off + tgt);
+  if (old == KEY_NONE) {  // first key of this target today: mark it for the infect pass
+    const int64_t g = S.a_off + tgt;
+    atomicOr(&S.touched[g >> 5], 1u << (g & 31));
+  }
 }
 
 // One tile's endpoints for this thread, in vector registers (CHUNKS 4-edge chunks; every index into
@@ -473,14 +520,16 @@ __device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
 }
 
 // Issues the 16-byte endpoint loads of tile T (non-temporal: streamed once). T.e0/seg set by the caller.
+// BLOCK: threads sharing the tile (the workgroup, or one wave for wave tiles); tid: this thread's
+// index among them.
 template <int BLOCK, int CHUNKS>
-__device__ inline void load_tile_edges(const SegView& L, EdgeTile<CHUNKS>& T) {
+__device__ inline void load_tile_edges(const SegView& L, EdgeTile<CHUNKS>& T, uint32_t tid) {
   constexpr int TILE = BLOCK * 4 * CHUNKS;
   const int64_t rem = L.n - T.e0;
   T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
 #pragma unroll
   for (int c = 0; c < CHUNKS; ++c) {
-    const int64_t e = T.e0 + ((int64_t)c * BLOCK + threadIdx.x) * 4;
+    const int64_t e = T.e0 + ((int64_t)c * BLOCK + tid) * 4;
     T.x[c] = load4(L.p1, e, L.n);
     T.y[c] = load4(L.p2, e, L.n);
   }
@@ -490,10 +539,11 @@ __device__ inline void load_tile_edges(const SegView& L, EdgeTile<CHUNKS>& T) {
 // per lane per round (endpoints fetched from the owning lane's registers).
 template <bool LDS, int BLOCK>
 __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                     uint32_t* s_q, int64_t e0, int32_t n_valid, int c, const v4i xs, const v4i ys) {
+                                     uint32_t* s_q, int64_t e0, int32_t n_valid, int c, const v4i xs, const v4i ys,
+                                     uint32_t tid) {
   const uint32_t lane = lane_id();
-  const uint32_t wave_base = threadIdx.x & ~63u;
-  const int32_t local0 = (c * BLOCK + (int32_t)threadIdx.x) * 4;  // tile-local index of edge q = 0
+  const uint32_t wave_base = tid & ~63u;
+  const int32_t local0 = (c * BLOCK + (int32_t)tid) * 4;  // tile-local index of edge q = 0
   const uint32_t* bits = LDS ? s_bits : S.bitmap;
   uint32_t cand = 0;  // bit q*2 + dir: infectious source and susceptible target
 #pragma unroll
@@ -543,9 +593,10 @@ __device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t
 
 template <bool LDS, int BLOCK, int CHUNKS>
 __device__ inline void process_tile(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                    uint32_t* s_q, const EdgeTile<CHUNKS>& T) {
+                                    uint32_t* s_q, const EdgeTile<CHUNKS>& T, uint32_t tid) {
 #pragma unroll
-  for (int c = 0; c < CHUNKS; ++c) process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T.e0, T.n_valid, c, T.x[c], T.y[c]);
+  for (int c = 0; c < CHUNKS; ++c)
+    process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T.e0, T.n_valid, c, T.x[c], T.y[c], tid);
 }
 
 // ===========================================================================
@@ -559,31 +610,48 @@ struct InfectOut {
 
 // g: storage index; il: index within the sim (counter field of the draws); stride: row stride of
 // the [variant][agent] arrays. Resets the target's winner key. The caller resolves the source.
+// Every per-agent input is loaded before the first store (the stores may alias the inputs for the
+// compiler, which would otherwise serialise one memory round trip per field).
 __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params& prm, uint64_t* winner, int64_t g,
                                           uint32_t il, int64_t stride, int32_t t, bool hosp_max, bool icu_max) {
   InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
   const uint64_t key = winner[g];
-  winner[g] = KEY_NONE;
+  const uint32_t f0 = P.flags[g];
   o.key = key;
-  uint32_t f = P.flags[g];
-  if (key == KEY_NONE || !(f & CVK_SUSCEPTIBLE)) return o;  // people.py:470 keeps susceptibles only
-  const uint64_t seed = prm.seed;
+  if (key == KEY_NONE || !(f0 & CVK_SUSCEPTIBLE)) {  // people.py:470 keeps susceptibles only
+    winner[g] = KEY_NONE;
+    return o;
+  }
   const uint32_t v = key_variant(key);
+  // inputs (one round trip)
+  const uint32_t vs0 = P.vstate[g];
+  const float peak0 = P.peak_nab[g];
+  const int32_t nb0 = P.n_breakthroughs[g];
+  const float rt0 = P.rel_trans[g];
+  const int32_t ninf0 = P.n_infections[g];
+  const float drec0 = P.date_recovered[g];
+  const float symp_p = P.symp_prob[g];
+  const float sev_p = P.severe_prob[g];
+  const float crit_p = P.crit_prob[g];
+  const float death_p = P.death_prob[g];
+  const float symp_imm = P.symp_imm[(int64_t)v * stride + g];
+  const float sev_imm = P.sev_imm[(int64_t)v * stride + g];
+  const float nab0 = prm.use_waning ? P.nab[g] : 0.0f;
+  winner[g] = KEY_NONE;
+  const uint64_t seed = prm.seed;
   o.v = v;
   o.l = key_layer(key);
   o.inf = true;
   // breakthrough, people.py:486-491
-  if (P.peak_nab[g] != 0.0f) {
-    const int32_t nb = P.n_breakthroughs[g];
-    if (nb == 0) P.rel_trans[g] = P.rel_trans[g] * prm.trans_redux;
-    P.n_breakthroughs[g] = nb + 1;
+  if (peak0 != 0.0f) {
+    if (nb0 == 0) P.rel_trans[g] = rt0 * prm.trans_redux;
+    P.n_breakthroughs[g] = nb0 + 1;
   }
   // states, people.py:494-506
-  f = (f & ~(CVK_SUSCEPTIBLE | CVK_NAIVE | CVK_RECOVERED | CVK_DIAGNOSED)) | CVK_EXPOSED;
-  P.flags[g] = f;
-  P.vstate[g] = (P.vstate[g] & ~0xFu) | v | (1u << (8 + v));
-  P.n_infections[g] += 1;
-  o.reinf = !isnan(P.date_recovered[g]);
+  P.flags[g] = (f0 & ~(CVK_SUSCEPTIBLE | CVK_NAIVE | CVK_RECOVERED | CVK_DIAGNOSED)) | CVK_EXPOSED;
+  P.vstate[g] = (vs0 & ~0xFu) | v | (1u << (8 + v));
+  P.n_infections[g] = ninf0 + 1;
+  o.reinf = !isnan(drec0);
   // outcome tree, people.py:513-579
   const float d_e2i = (float)draw_dur(prm.dur[CVK_DUR_EXP2INF], seed, il, t, SLOT_EXP2INF);
   P.dur_exp2inf[g] = d_e2i;
@@ -598,7 +666,7 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
   float date_rec = fnan;
   double u_symp, u_sev, u_crit, u_death;
   draw_pair(seed, il, t, SLOT_BERN_SYMP_SEV, u_symp, u_sev);
-  const float sp = (prm.rel_symp_prob[v] * P.symp_prob[g]) * (1.0f - P.symp_imm[(int64_t)v * stride + g]);
+  const float sp = (prm.rel_symp_prob[v] * symp_p) * (1.0f - symp_imm);
   o.symp = u_symp < (double)sp;
   int imm_class = 0;  // 0 asymptomatic, 1 mild, 2 severe (rel_imm_symp)
   if (!o.symp) {
@@ -610,7 +678,7 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
     P.dur_inf2sym[g] = d_i2s;
     const float date_sym = date_inf + d_i2s;
     P.date_symptomatic[g] = date_sym;
-    const float sevp = (prm.rel_severe_prob[v] * P.severe_prob[g]) * (1.0f - P.sev_imm[(int64_t)v * stride + g]);
+    const float sevp = (prm.rel_severe_prob[v] * sev_p) * (1.0f - sev_imm);
     o.sev = u_sev < (double)sevp;
     if (!o.sev) {
       imm_class = 1;
@@ -624,7 +692,7 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
       const float date_sev = date_sym + d_s2s;
       P.date_severe[g] = date_sev;
       draw_pair(seed, il, t, SLOT_BERN_CRIT_DEATH, u_crit, u_death);
-      const float critp = (prm.rel_crit_prob[v] * P.crit_prob[g]) * (hosp_max ? prm.no_hosp_factor : 1.0f);
+      const float critp = (prm.rel_crit_prob[v] * crit_p) * (hosp_max ? prm.no_hosp_factor : 1.0f);
       if (!(u_crit < (double)critp)) {
         const double dd = draw_dur(prm.dur[CVK_DUR_SEV2REC], seed, il, t, SLOT_DUR_C);
         date_rec = (float)((double)date_sev + dd);
@@ -634,7 +702,7 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
         P.dur_sev2crit[g] = d_s2c;
         const float date_crit = date_sev + d_s2c;
         P.date_critical[g] = date_crit;
-        const float deathp = (prm.rel_death_prob[v] * P.death_prob[g]) * (icu_max ? prm.no_icu_factor : 1.0f);
+        const float deathp = (prm.rel_death_prob[v] * death_p) * (icu_max ? prm.no_icu_factor : 1.0f);
         const float dsum = ((d_e2i + d_i2s) + d_s2s) + d_s2c;
         if (!(u_death < (double)deathp)) {
           const double dd = draw_dur(prm.dur[CVK_DUR_CRIT2REC], seed, il, t, SLOT_DUR_D);
@@ -651,8 +719,8 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
   P.date_recovered[g] = date_rec;
   // update_peak_nab, immunity.py:138-202
   if (prm.use_waning) {
-    if (P.nab[g] > 0.0f) {
-      P.peak_nab[g] = P.peak_nab[g] * prm.nab_boost;
+    if (nab0 > 0.0f) {
+      P.peak_nab[g] = peak0 * prm.nab_boost;
     } else {
       const double init_nab = draw_dur(prm.nab_init, seed, il, t, SLOT_NAB);
       P.peak_nab[g] = (float)((pow(2.0, init_nab) * prm.rel_imm_symp[imm_class]) * prm.nab_norm);
@@ -667,6 +735,14 @@ __device__ inline int32_t key_source(uint64_t key, const int32_t* p1, const int3
   if (key_class(key) != 1) return -1;
   const int64_t e = (int64_t)key_index(key);
   return key_dir(key) ? p2[e] : p1[e];
+}
+
+// Next set bit of a touched-bitmap word for this lane (bits is consumed); -1 when none is left.
+__device__ inline int32_t take_bit(uint32_t& bits) {
+  if (!bits) return -1;
+  const int32_t b = __ffs(bits) - 1;
+  bits &= bits - 1;
+  return b;
 }
 
 constexpr int INFECT_BLOCK = 256;

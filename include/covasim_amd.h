@@ -172,9 +172,9 @@ typedef struct {
 /* Device workspace (sizes from cvk_workspace_sizes). */
 typedef struct {
   uint64_t* winner;      /* [n] min transmission key per target; all-ones = none */
-  int32_t* touched;      /* [n] targets with a winner key this day */
+  uint32_t* touched;     /* [(n + 31) / 32] bit i: agent i holds a key this day (cleared by the infect pass) */
   uint32_t* bitmap;      /* [bitmap_words] bit i: agent i infectious with rel_trans != 0 */
-  uint32_t* ctl;         /* [64] control words: touched count, choose state, tickets, flags */
+  uint32_t* ctl;         /* [64] control words: log count, choose state, tile counter, flags */
   uint32_t* results;     /* [npts][CVK_R_COLS] */
   double* results_f;     /* [npts][CVK_RF_COLS] */
   int32_t* log;          /* [log_capacity][4] infection records: source, target, day, layer | variant<<8 */
@@ -185,6 +185,9 @@ typedef struct {
                             else rel_sus), high word the flags */
   uint32_t* partials;    /* [CVK_PARTIAL_COLS][CVK_MAX_REDUCE_BLOCKS] per-block counts (column-major) */
   double* partials_f;    /* [CVK_RF_COLS][CVK_MAX_REDUCE_BLOCKS] per-block sums (column-major) */
+  uint16_t* bound;       /* [n] per-agent upper bound of the edge-pass factor: infectious agents
+                            variant << 13 | code of rel_trans x own factors, susceptible agents the code
+                            of rel_sus x quarantine factor, others 0; code = float bits >> 18 rounded up */
   int32_t npts;
   int32_t pad;
 } cvk_work;
@@ -193,7 +196,7 @@ typedef struct {
 #define CVK_PARTIAL_COLS 64
 
 /* control word indices of cvk_work.ctl */
-#define CVK_CTL_TOUCHED 0
+#define CVK_CTL_UNUSED0 0          /* reserved */
 #define CVK_CTL_LOG_COUNT 2       /* 64-bit counter in words 2 (lo) and 3 (hi) */
 #define CVK_CTL_LOG_OVERFLOW 4
 #define CVK_CTL_SEL_PREFIX 6      /* cvk_choose radix-select state: 64-bit prefix in words 6, 7 */
@@ -214,14 +217,18 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 #define CVK_TUNE_EDGE_LDS 1
 /* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks each edge-pass thread loads per tile, 2 (default) or 4. */
 #define CVK_TUNE_EDGE_CHUNKS 2
+/* CVK_TUNE_EDGE_WAVE_TILES: 1 = each wave streams its own round-robin share of the edges (no
+ * per-tile barrier or atomic), 0 (default) = workgroup tiles from an atomic counter. */
+#define CVK_TUNE_EDGE_WAVE_TILES 3
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);
 /* Sizes in bytes of the workspace arrays for n agents and npts days. */
 int cvk_workspace_sizes(int64_t n_agents, int32_t npts, int64_t* winner_bytes, int64_t* touched_bytes,
                         int64_t* bitmap_bytes, int64_t* results_bytes, int64_t* results_f_bytes);
-/* Bytes of the remaining workspace arrays: rec (8 per agent), partials and partials_f. */
-int cvk_workspace_sizes2(int64_t n_agents, int64_t* rec_bytes, int64_t* partials_bytes, int64_t* partials_f_bytes);
+/* Bytes of the remaining workspace arrays: rec (8 per agent), partials, partials_f and bound (2 per agent). */
+int cvk_workspace_sizes2(int64_t n_agents, int64_t* rec_bytes, int64_t* partials_bytes, int64_t* partials_f_bytes,
+                         int64_t* bound_bytes);
 
 /* ---- 1:1 mirrors of the reference's numba kernels (parity boundary) ---- */
 
@@ -255,7 +262,7 @@ int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, const float
 /* Seed infections at t = 0 (sim.py:505-532): choose k distinct agents (eligible = all, or
  * susceptible only when susceptible_only != 0) with Philox selection keys on stream
  * `stream_id`, and register them as infections of class `layer_code` / `variant` in
- * work->winner / work->touched. */
+ * work->winner / work->touched (bitmap). */
 int cvk_choose(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
                uint32_t stream_id, int64_t k, int32_t susceptible_only, int32_t layer_code,
                int32_t variant, void* stream);

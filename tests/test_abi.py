@@ -54,7 +54,7 @@ def test_host_side_argument_errors():
     w = _lib.WorkS()
     sizes = [C.c_int64() for _ in range(5)]
     assert L.cvk_workspace_sizes(1000, 61, *[C.byref(s) for s in sizes]) == 0
-    assert [s.value for s in sizes] == [8000, 4000, 128, 61 * _lib.R_COLS * 4, 61 * _lib.RF_COLS * 8]
+    assert [s.value for s in sizes] == [8000, 128, 128, 61 * _lib.R_COLS * 4, 61 * _lib.RF_COLS * 8]
     assert L.cvk_workspace_sizes(0, 61, *[C.byref(s) for s in sizes]) == _lib.E_INVALID
     assert L.cvk_graph_launch(None, None) == _lib.E_INVALID
     with pytest.raises(ValueError):

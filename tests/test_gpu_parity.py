@@ -150,7 +150,14 @@ CONFIGS = {
                             pars=dict(use_waning=True, beta=0.02,
                                       variants=[dict(rel_beta=1.7, days=[10], n_imports=30)])),
     'weighted_edges_l2_bitmap': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
-                                     pars=dict(use_waning=True, beta=0.04), edge_lds=0),
+                                     pars=dict(use_waning=True, beta=0.04), tuning=dict(edge_lds=0)),
+    'hybrid_dynamic_wave_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                      pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                      tuning=dict(edge_wave_tiles=1)),
+    'variants_waning_deep_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                       pars=dict(use_waning=True, beta=0.02,
+                                                 variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
+                                       tuning=dict(edge_chunks=4, edge_wave_tiles=1)),
     'beds_limited': dict(pop=dict(n=3000, pop_type='random'),
                          pars=dict(use_waning=False, beta=0.08, n_beds_hosp=2, n_beds_icu=1,
                                    rel_severe_prob=20.0, rel_crit_prob=5.0)),
@@ -165,11 +172,15 @@ def test_full_sim_matches_oracle(gpu, name):
     pars = dict(cfg['pars'])
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
-    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, cfg.get('edge_lds', 1)), 'tuning')
+    keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 2),
+                edge_wave_tiles=(_lib.TUNE_EDGE_WAVE_TILES, 0))
+    for k, v in cfg.get('tuning', {}).items():
+        _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
         sim, osim, ores = run_pair(pop, n_days=45, seed=11, variants=variants, **pars)
     finally:
-        _lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, 1)
+        for key, default in keys.values():
+            _lib.lib().cvk_set_tuning(key, default)
     assert sim.results['cum_infections'].values[-1] > 50  # the epidemic took off
     assert_results_equal(sim, ores, nv=1 + len(variants))
     assert_people_equal(sim, osim)
