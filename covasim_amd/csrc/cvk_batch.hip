@@ -79,7 +79,8 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
       const uint64_t seed = C.params[s].seed;
       const uint32_t n = (uint32_t)C.sims[s].n;
       for (int64_t e = e0 + threadIdx.x; e < e1; e += AGENT_BLOCK)
-        resample_edge(seed, a.t, (uint32_t)L.l, n, e, (int32_t*)L.p1, (int32_t*)L.p2, (float*)L.beta);
+        resample_edge(seed, a.t, (uint32_t)L.l, n, e, (int32_t*)L.p1, (int32_t*)L.p2,
+                      a.resample == 2 ? (float*)L.beta : nullptr);
     }
     return;
   }
@@ -199,24 +200,26 @@ struct BatchInfectArgs {
 };
 
 __global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon C, BatchInfectArgs ia) {
-  __shared__ int64_t s_off[CVK_BATCH_MAX_SIMS + 1];
-  for (int j = threadIdx.x; j <= C.n_sims; j += INFECT_BLOCK) s_off[j] = ia.a_off[j];
+  __shared__ int32_t s_off[CVK_BATCH_MAX_SIMS + 1];  // agent offsets (total agents < 2^31)
+  for (int j = threadIdx.x; j <= C.n_sims; j += INFECT_BLOCK) s_off[j] = (int32_t)ia.a_off[j];
   __syncthreads();
   const int32_t t = ia.t;
+  __shared__ int32_t s_list[INFECT_BLOCK / 64][INFECT_WAVE_LIST];
+  int32_t* list = s_list[threadIdx.x >> 6];
   const int64_t n_words = (C.n_total + 31) / 32;
   const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
   const int64_t n_sweeps = (n_words + stride - 1) / stride;
   for (int64_t sw = 0; sw < n_sweeps; ++sw) {
     const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
-    uint32_t bits = w < n_words ? C.touched[w] : 0u;
-    if (bits) C.touched[w] = 0u;
-    while (__ballot(bits != 0)) {  // wave-uniform: one set bit per lane per round
+    const uint32_t total = compact_touched(C.touched, w, n_words, C.n_total, list);
+    for (uint32_t base = 0; base < total; base += 64) {  // wave-uniform: one target per lane per round
       InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
       int32_t source = -1, s = 0;
       int64_t il = 0;
-      const int32_t bit = take_bit(bits);
-      if (bit >= 0 && w * 32 + bit < C.n_total) {
-        const int64_t g = w * 32 + bit;
+      const uint32_t jj = base + lane_id();
+      const int32_t gi = jj < total ? list[jj] : -1;
+      if (gi >= 0) {
+        const int64_t g = gi;
         int32_t lo = 0, hi = C.n_sims - 1;  // the sim with s_off[s] <= g < s_off[s + 1]
         while (lo < hi) {
           const int32_t mid = (lo + hi + 1) >> 1;
@@ -284,6 +287,9 @@ __global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon 
         pending = pending && !mine;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the list is rewritten by the next sweep
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -747,7 +753,7 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       rc = launch_infect(b, 0, 0, st);
       if (rc) return rc;
     }
-    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, 1, st);
+    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : 1, st);  // weights set on the first resample
     if (rc) return rc;
     if (imp) {
       for (int32_t j = 0; j < imp->n_events; ++j) {

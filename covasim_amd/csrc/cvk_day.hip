@@ -296,20 +296,23 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
   const uint32_t* row = day_row(W, t);
   const bool hosp_max = ia.allow_beds && prm.n_beds_hosp >= 0 && (int64_t)row[CVK_R_MID_SEVERE] > prm.n_beds_hosp;
   const bool icu_max = ia.allow_beds && prm.n_beds_icu >= 0 && (int64_t)row[CVK_R_MID_CRITICAL] > prm.n_beds_icu;
-  // one touched-bitmap word (32 agents) per thread per sweep; each round takes one set bit per lane
+  // one touched-bitmap word (32 agents) per thread per sweep, compacted per wave into an LDS list;
+  // each round gives every lane one target
+  __shared__ int32_t s_list[INFECT_BLOCK / 64][INFECT_WAVE_LIST];
+  int32_t* list = s_list[threadIdx.x >> 6];
   const int64_t n_words = (P.n + 31) / 32;
   const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
   const int64_t n_sweeps = (n_words + stride - 1) / stride;
   for (int64_t sw = 0; sw < n_sweeps; ++sw) {
     const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
-    uint32_t bits = w < n_words ? W.touched[w] : 0u;
-    if (bits) W.touched[w] = 0u;
-    while (__ballot(bits != 0)) {  // wave-uniform
+    const uint32_t total = compact_touched(W.touched, w, n_words, P.n, list);
+    for (uint32_t base = 0; base < total; base += 64) {  // wave-uniform
       InfectOut o = {KEY_NONE, false, false, false, false, 0u, 0u};
       int32_t source = -1, i = 0;
-      const int32_t b = take_bit(bits);
-      if (b >= 0 && w * 32 + b < P.n) {
-        i = (int32_t)(w * 32 + b);
+      const uint32_t j = base + lane_id();
+      const int32_t g = j < total ? list[j] : -1;
+      if (g >= 0) {
+        i = g;
         o = infect_target(P, prm, W.winner, i, (uint32_t)i, P.n, t, hosp_max, icu_max);
         if (o.inf && (int32_t)o.l < ia.n_layers) source = key_source(o.key, ia.p1[o.l], ia.p2[o.l]);
       }
@@ -353,6 +356,9 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
         }
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // the list is rewritten by the next sweep
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
   if (ia.count_flows && threadIdx.x < N_INFECT_COLS && s_cnt[threadIdx.x])
@@ -434,6 +440,7 @@ static int day_reduce_launch(cvk_work* work, const DayReduce& r, hipStream_t st)
   return check_launch("day_reduce");
 }
 
+// resample: 0 none, 1 endpoints only (weights already all ones), 2 endpoints and weights.
 static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                             cvk_work* work, int32_t t, int prologue, int transitions, int resample, hipStream_t st,
                             DayReduce* reduce_out) {
@@ -452,7 +459,7 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
       ResampleLayer& R = a.rs[a.n_resample++];
       R.p1 = L.p1;
       R.p2 = L.p2;
-      R.beta = L.beta;
+      R.beta = resample == 2 ? L.beta : nullptr;
       R.n = L.n_edges;
       R.layer = (uint32_t)l;
       max_dyn = std::max(max_dyn, L.n_edges);
@@ -771,7 +778,8 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
     }
     // day t begins with the stocks of day t-1 (already closed if t == t0)
     DayReduce r;
-    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, 1, st, &r);
+    // the first resample of the run also sets every weight to 1; later ones leave them as they are
+    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : 1, st, &r);
     if (rc) return rc;
     if (imports) {
       for (int j = 0; j < imports->n; ++j) {

@@ -386,14 +386,15 @@ __device__ inline void write_agent_rows(uint32_t* results, double* results_f, co
   __syncthreads();
 }
 
-// Dynamic-layer resample of one edge: iid uniform endpoints, beta = 1 (base.py:1849-1876).
+// Dynamic-layer resample of one edge: iid uniform endpoints, beta = 1 (base.py:1849-1876). beta may be
+// null when the layer's weights are already all ones (every resample after the first of a run).
 __device__ inline void resample_edge(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e,
                                      int32_t* p1, int32_t* p2, float* beta) {
   const uint64_t ue = (uint64_t)e;
   const u32x4 w = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(ue >> 32));
   p1[e] = (int32_t)below(w.x, w.y, n_agents);
   p2[e] = (int32_t)below(w.z, w.w, n_agents);
-  beta[e] = 1.0f;
+  if (beta) beta[e] = 1.0f;
 }
 
 // Selection key of cvk_choose: philox word << 32 | local agent index (distinct per agent).
@@ -737,15 +738,34 @@ __device__ inline int32_t key_source(uint64_t key, const int32_t* p1, const int3
   return key_dir(key) ? p2[e] : p1[e];
 }
 
-// Next set bit of a touched-bitmap word for this lane (bits is consumed); -1 when none is left.
-__device__ inline int32_t take_bit(uint32_t& bits) {
-  if (!bits) return -1;
-  const int32_t b = __ffs(bits) - 1;
-  bits &= bits - 1;
-  return b;
-}
-
 constexpr int INFECT_BLOCK = 256;
+constexpr int INFECT_WAVE_LIST = 64 * 32;  // every bit of one wave's 64 words
+
+// Compacts the set bits of this lane's touched word w (cleared in memory) into the wave's LDS list
+// `list` (storage indices, ascending lane order); returns the wave's total. All lanes call it.
+__device__ inline uint32_t compact_touched(uint32_t* touched, int64_t w, int64_t n_words, int64_t n, int32_t* list) {
+  uint32_t bits = w < n_words ? touched[w] : 0u;
+  if (bits) touched[w] = 0u;
+  const uint32_t cnt = (uint32_t)__popc(bits);
+  uint32_t incl = cnt;  // inclusive scan over the wave
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if ((int)lane_id() >= off) incl += y;
+  }
+  uint32_t pos = incl - cnt;
+  while (bits) {
+    const int32_t b = __ffs(bits) - 1;
+    bits &= bits - 1;
+    const int64_t g = w * 32 + b;
+    list[pos++] = g < n ? (int32_t)g : -1;
+  }
+  const uint32_t total = __shfl(incl, 63, 64);
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return total;
+}
 constexpr int N_INFECT_COLS = 2 + 3 * MAXV;
 
 // Column of the day row for infect-pass counter k.
