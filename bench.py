@@ -57,7 +57,8 @@ def measured_traffic(workload, tuning):
     """HBM bytes per edge-pass launch from the committed counter summary (profiles/), or None.
 
     The summary holds FETCH_SIZE and WRITE_SIZE means over the edge-pass dispatches of this workload,
-    already scaled by what the same counters read for a device copy of known size.
+    already scaled by what the same counters read for a device copy of known size; it counts only
+    if it was taken with the same tuning and library version (cvk_version) as this run.
     """
     path = os.path.join(ROOT, 'profiles', f'edge_pass_traffic_{workload}.txt')
     if not os.path.exists(path):
@@ -68,6 +69,9 @@ def measured_traffic(workload, tuning):
             k, v = line.split(':', 1)
             vals[k.strip()] = v.strip()
     if vals.get('tuning', 'default') != (json.dumps(tuning, sort_keys=True) if tuning else 'default'):
+        return None
+    from covasim_amd import _lib
+    if vals.get('library_version') != str(_lib.lib().cvk_version()):  # counted on another kernel build
         return None
     try:
         return float(vals['fetch_bytes_per_launch']) + float(vals['write_bytes_per_launch'])
@@ -266,10 +270,10 @@ def main():
     ap.add_argument('--cpu-budget', type=float, default=15.0)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
-    ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass tile depth (2 or 4 chunks per thread)')
+    ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass wave tile depth (2 or 4 chunks of 4 edges per lane)')
     ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
-    ap.add_argument('--edge-wave-tiles', type=int, default=None,
-                    help='1: every wave streams its own round-robin share of the edges; 0: workgroup tiles')
+    ap.add_argument('--generate-dynamic', type=int, default=None,
+                    help='1: the day loop computes dynamic-layer endpoints in place; 0: writes and reads them daily')
     ap.add_argument('--eager-only', action='store_true',
                     help='counter-collection mode: skip the graph replays; value is then the eager loop rate')
     ap.add_argument('--copy-probe', type=int, default=0,
@@ -286,12 +290,12 @@ def main():
 
     w = WORKLOADS[args.workload]
     tuning = {}
+    if args.generate_dynamic is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, args.generate_dynamic), 'tuning')
+        tuning['generate_dynamic'] = args.generate_dynamic
     if args.edge_chunks is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_CHUNKS, args.edge_chunks), 'tuning')
         tuning['edge_chunks'] = args.edge_chunks
-    if args.edge_wave_tiles is not None:
-        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_WAVE_TILES, args.edge_wave_tiles), 'tuning')
-        tuning['edge_wave_tiles'] = args.edge_wave_tiles
     if args.edge_lds is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, args.edge_lds), 'tuning')
         tuning['edge_lds'] = args.edge_lds

@@ -88,19 +88,35 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
 }
 
 // Writes the day rows from the summed partial rows of day_begin (block-wide).
+// The infect pass of the fused day loop leaves its per-block flow counts in the last
+// INFECT_PART_BLOCKS entries of the partial columns (the agent pass uses the first ones), so no
+// result row is the target of one atomic per block; the next reduction adds them to their day's row.
+constexpr int INFECT_PART_BLOCKS = 1024;
+constexpr int INFECT_PART_BASE = CVK_MAX_REDUCE_BLOCKS - INFECT_PART_BLOCKS;
+
 struct DayReduce {
-  uint32_t n_blocks;  // 0: nothing to reduce
+  uint32_t n_blocks;  // agent-pass partial rows to reduce (0: none)
   int32_t t;
   int32_t prologue;
   int32_t transitions;
+  uint32_t infect_blocks;  // infect-pass partial rows of day infect_t to add (0: none)
+  int32_t infect_t;
 };
 
 __device__ void day_reduce(const cvk_work& W, const DayReduce& r) {
   __shared__ uint32_t s_tot[N_AGENT_COLS];
   __shared__ double s_totf[N_AGENT_FCOLS];
-  sum_partials<N_AGENT_COLS, N_AGENT_FCOLS>(W.partials, W.partials_f, CVK_MAX_REDUCE_BLOCKS, 0, r.n_blocks, s_tot,
-                                            s_totf);
-  write_agent_rows(W.results, W.results_f, s_tot, s_totf, r.t, r.prologue, r.transitions);
+  if (r.n_blocks) {
+    sum_partials<N_AGENT_COLS, N_AGENT_FCOLS>(W.partials, W.partials_f, CVK_MAX_REDUCE_BLOCKS, 0, r.n_blocks, s_tot,
+                                              s_totf);
+    write_agent_rows(W.results, W.results_f, s_tot, s_totf, r.t, r.prologue, r.transitions);
+  }
+  if (r.infect_blocks) {
+    sum_partials<N_INFECT_COLS, 0>(W.partials, W.partials_f, CVK_MAX_REDUCE_BLOCKS, INFECT_PART_BASE, r.infect_blocks,
+                                   s_tot, s_totf);
+    uint32_t* row = day_row(W, r.infect_t);
+    if (threadIdx.x < N_INFECT_COLS) row[infect_col(threadIdx.x)] += s_tot[threadIdx.x];
+  }
 }
 
 __global__ __launch_bounds__(1024) void day_reduce_kernel(cvk_work W, DayReduce r) { day_reduce(W, r); }
@@ -190,89 +206,68 @@ __global__ __launch_bounds__(256) void choose_mark_kernel(cvk_people P, cvk_work
 // edge_pass
 // ===========================================================================
 constexpr int EDGE_BLOCK = 1024;
-// 4-edge chunks per thread per tile: 2 (8192-edge tiles) or 4 (16384), chosen by cvk_set_tuning
-constexpr int edge_tile(int chunks) { return EDGE_BLOCK * 4 * chunks; }
+constexpr int EDGE_WAVES = EDGE_BLOCK / 64;
 
 struct EdgeArgs {
   SimView sim;
   SegView seg[CVK_MAX_LAYERS];
-  int64_t tile_start[CVK_MAX_LAYERS + 1];       // block tiles (EDGE_BLOCK threads) per layer
-  int64_t wave_tile_start[CVK_MAX_LAYERS + 1];  // wave tiles (one wave) per layer
+  int64_t wave_tile_start[CVK_MAX_LAYERS + 1];  // wave tiles (64 x 4 x K edges) per layer, prefix
   int32_t n_layers;
   int32_t bitmap_words;
   int32_t t;
   int32_t pad;
-  uint32_t* tile_counter;  // zeroed by the agent pass of the day
-  cvk_work work;           // for the fused day reduction
-  DayReduce reduce;        // day_begin partial rows to reduce in block 0 (n_blocks == 0: none)
+  cvk_work work;     // for the fused day reduction
+  DayReduce reduce;  // day_begin partial rows to reduce in block 0 (n_blocks == 0: none)
 };
 
-// Locates tile `tile` (layer by tile_start) and issues its endpoint loads.
-template <int CHUNKS>
-__device__ inline void load_tile(const EdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
-  T.tile = tile;
-  if (tile >= a.tile_start[a.n_layers]) return;
-  int l = 0;
-  while (l + 1 < a.n_layers && tile >= a.tile_start[l + 1]) ++l;
-  T.seg = l;
-  T.e0 = (tile - a.tile_start[l]) * edge_tile(CHUNKS);
-  load_tile_edges<EDGE_BLOCK, CHUNKS>(a.seg[l], T, threadIdx.x);
-}
-
-// Wave tiles: 64 lanes x 4 x CHUNKS edges handled by one wave on its own.
-template <int CHUNKS>
-__device__ inline void load_wave_tile(const EdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
-  T.tile = tile;
-  if (tile >= a.wave_tile_start[a.n_layers]) return;
-  int l = 0;
-  while (l + 1 < a.n_layers && tile >= a.wave_tile_start[l + 1]) ++l;
-  T.seg = l;
-  T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * CHUNKS);
-  load_tile_edges<64, CHUNKS>(a.seg[l], T, lane_id());
-}
-
-// WAVE = false: workgroup tiles handed out by an atomic counter (dynamic balance, one barrier per tile).
-// WAVE = true: wave tiles assigned round-robin over every wave of the grid (no barrier, no atomic).
-template <bool LDS, int CHUNKS, bool WAVE>
+// Workgroup b owns wave tiles [T*b/G, T*(b+1)/G) of the day (T tiles over all layers, G workgroups);
+// its waves take the next tile from an LDS counter, so no wave waits for another and the global
+// memory system sees no tile atomics. A wave's exact-test queue is drained when its tiles change layer
+// and at the end.
+template <bool LDS, int K>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
-  __shared__ int64_t s_tiles[2];
+  __shared__ uint32_t s_next;
   const int nbw = LDS ? a.bitmap_words : 0;
-  uint32_t* s_bits = s_mem;
-  uint32_t* s_q = s_mem + nbw + (threadIdx.x >> 6) * EDGE_QCAP;  // this wave's candidate queue
-  if (blockIdx.x == 0 && a.reduce.n_blocks) day_reduce(a.work, a.reduce);  // the agent pass's counts
-  if (threadIdx.x == 0) s_tiles[0] = atomicAdd(a.tile_counter, 1u);
+  const WaveQ Q = wave_queues(s_mem + nbw + (threadIdx.x >> 6) * EDGE_WAVE_WORDS);
+  // block 0 sums the previous passes' partial rows and, when other workgroups exist, takes no tiles
+  const bool reducing = a.reduce.n_blocks || a.reduce.infect_blocks;
+  if (blockIdx.x == 0 && reducing) day_reduce(a.work, a.reduce);
+  if (threadIdx.x == 0) s_next = 0;
   if (LDS) {
     const uint4* src = (const uint4*)a.sim.bitmap;
-    uint4* dst = (uint4*)s_bits;
+    uint4* dst = (uint4*)s_mem;
     for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
   __syncthreads();
-  if (WAVE) {  // the next wave tile's endpoints are in flight while this one is processed
-    const int64_t n_wt = a.wave_tile_start[a.n_layers];
-    const int64_t stride = (int64_t)gridDim.x * (EDGE_BLOCK / 64);
-    EdgeTile<CHUNKS> cur, nxt;
-    load_wave_tile(a, (int64_t)blockIdx.x * (EDGE_BLOCK / 64) + (threadIdx.x >> 6), cur);
-    while (cur.tile < n_wt) {
-      load_wave_tile(a, cur.tile + stride, nxt);
-      process_tile<LDS, 64, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur, lane_id());
-      cur = nxt;
+  const uint32_t* bits = LDS ? s_mem : a.sim.bitmap;
+  const int64_t n_wt = a.wave_tile_start[a.n_layers];
+  const uint32_t skip = reducing && gridDim.x > 1 ? 1u : 0u;
+  const int64_t workers = (int64_t)(gridDim.x - skip);
+  const int64_t wb = (int64_t)blockIdx.x - skip;
+  const int64_t first = wb < 0 ? 0 : n_wt * wb / workers, last = wb < 0 ? 0 : n_wt * (wb + 1) / workers;
+  uint32_t n_x = 0;
+  int32_t q_seg = 0;
+  for (;;) {
+    uint32_t k = 0;
+    if (lane_id() == 0) k = atomicAdd(&s_next, 1u);
+    k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
+    const int64_t tile = first + k;
+    if (tile >= last) break;
+    int l = 0;
+    while (l + 1 < a.n_layers && tile >= a.wave_tile_start[l + 1]) ++l;
+    WaveTile<K> T;
+    T.seg = l;
+    T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * K);
+    load_wave_edges<K>(a.sim, a.seg[l], a.t, T);
+    if (l != q_seg) {
+      if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
+      n_x = 0;
+      q_seg = l;
     }
-    return;
+    wave_tile_pass<LDS, K>(a.sim, a.seg[l], a.t, bits, Q, T, n_x);
   }
-  const int64_t n_tiles = a.tile_start[a.n_layers];
-  // tiles are handed out dynamically; the next tile's endpoints are in flight while this one is processed
-  EdgeTile<CHUNKS> cur, nxt;
-  load_tile(a, s_tiles[0], cur);
-  int par = 1;
-  while (cur.tile < n_tiles) {
-    if (threadIdx.x == 0) s_tiles[par] = atomicAdd(a.tile_counter, 1u);
-    __syncthreads();
-    load_tile(a, s_tiles[par], nxt);
-    par ^= 1;
-    process_tile<LDS, EDGE_BLOCK, CHUNKS>(a.sim, a.seg[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
-    cur = nxt;
-  }
+  if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
 }
 
 // ===========================================================================
@@ -285,6 +280,8 @@ struct InfectArgs {
   int32_t count_flows;
   int32_t allow_beds;
   int32_t n_layers;
+  uint32_t gen_mask;  // bit l: layer l's endpoints of day t come from the resample stream
+  int32_t partial;    // 1: flow counts go to the infect partial rows (reduced later), 0: atomics on the row
 };
 
 __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
@@ -314,7 +311,8 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
       if (g >= 0) {
         i = g;
         o = infect_target(P, prm, W.winner, i, (uint32_t)i, P.n, t, hosp_max, icu_max);
-        if (o.inf && (int32_t)o.l < ia.n_layers) source = key_source(o.key, ia.p1[o.l], ia.p2[o.l]);
+        if (o.inf && (int32_t)o.l < ia.n_layers)
+          source = key_source(o.key, ia.p1[o.l], ia.p2[o.l], (ia.gen_mask >> o.l) & 1u, prm.seed, t, (uint32_t)P.n);
       }
       // infection log, people.py:509-511: one wave-aggregated append per round
       const uint64_t m = __ballot(o.inf);
@@ -361,16 +359,20 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   __syncthreads();
-  if (ia.count_flows && threadIdx.x < N_INFECT_COLS && s_cnt[threadIdx.x])
-    atomicAdd(&day_row(W, t)[infect_col(threadIdx.x)], s_cnt[threadIdx.x]);
+  if (ia.count_flows && threadIdx.x < N_INFECT_COLS) {
+    if (ia.partial)
+      W.partials[(int64_t)threadIdx.x * CVK_MAX_REDUCE_BLOCKS + INFECT_PART_BASE + blockIdx.x] = s_cnt[threadIdx.x];
+    else if (s_cnt[threadIdx.x])
+      atomicAdd(&day_row(W, t)[infect_col(threadIdx.x)], s_cnt[threadIdx.x]);
+  }
 }
 
 // ===========================================================================
 // host side
 // ===========================================================================
 static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
-static int g_edge_chunks = 2;  // tunable: 4-edge chunks per thread per edge tile (2 or 4)
-static int g_edge_wave = 0;    // tunable: 1 = wave tiles assigned round-robin, 0 = workgroup tiles from a counter
+static int g_edge_chunks = 4;  // tunable: 4-edge chunks per lane per wave tile (2 or 4)
+static int g_gen_dynamic = 0;  // tunable: the fused day loop computes dynamic layers' endpoints in place
 
 struct DeviceCaps {
   int device = -1;
@@ -418,7 +420,7 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
   if (!people_ok(P)) return fail(CVK_E_INVALID, "people struct has a null pointer or n <= 0");
   if (!prm || prm->n_agents != P->n) return fail(CVK_E_INVALID, "params.n_agents does not match people.n");
   if (P->n >= (1ll << 31) - 1) return fail(CVK_E_INVALID, "more than 2^31-2 agents");
-  if ((int64_t)grid_for(P->n, AGENT_BLOCK * AGENTS_PER_THREAD) > CVK_MAX_REDUCE_BLOCKS)
+  if ((int64_t)grid_for(P->n, AGENT_BLOCK * AGENTS_PER_THREAD) > INFECT_PART_BASE)
     return fail(CVK_E_CAPACITY, "too many agents for the reduction workspace (CVK_MAX_REDUCE_BLOCKS)");
   if (prm->n_variants < 1 || prm->n_variants > CVK_MAX_VARIANTS)
     return fail(CVK_E_INVALID, "n_variants must be in [1, 8]");
@@ -435,7 +437,7 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
 static uint32_t agent_blocks_for(int64_t n) { return (uint32_t)grid_for(n, AGENT_BLOCK * AGENTS_PER_THREAD); }
 
 static int day_reduce_launch(cvk_work* work, const DayReduce& r, hipStream_t st) {
-  if (!r.n_blocks) return CVK_OK;
+  if (!r.n_blocks && !r.infect_blocks) return CVK_OK;
   hipLaunchKernelGGL(day_reduce_kernel, dim3(1), dim3(1024), 0, st, *work, r);
   return check_launch("day_reduce");
 }
@@ -473,6 +475,8 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
   reduce_out->t = t;
   reduce_out->prologue = prologue;
   reduce_out->transitions = transitions;
+  reduce_out->infect_blocks = 0;
+  reduce_out->infect_t = 0;
   return check_launch("day_begin");
 }
 
@@ -491,14 +495,14 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       if (value != 2 && value != 4) return fail(CVK_E_INVALID, "cvk_set_tuning: edge chunks must be 2 or 4");
       g_edge_chunks = value;
       return CVK_OK;
-    case CVK_TUNE_EDGE_WAVE_TILES:
-      g_edge_wave = value ? 1 : 0;
+    case CVK_TUNE_GENERATE_DYNAMIC:
+      g_gen_dynamic = value ? 1 : 0;
       return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
 }
-extern "C" int cvk_version(void) { return 2; }
+extern "C" int cvk_version(void) { return 3; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
 extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
@@ -632,12 +636,10 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* fns[8] = {(const void*)edge_pass_kernel<false, 2, false>, (const void*)edge_pass_kernel<true, 2, false>,
-                          (const void*)edge_pass_kernel<false, 4, false>, (const void*)edge_pass_kernel<true, 4, false>,
-                          (const void*)edge_pass_kernel<false, 2, true>, (const void*)edge_pass_kernel<true, 2, true>,
-                          (const void*)edge_pass_kernel<false, 4, true>, (const void*)edge_pass_kernel<true, 4, true>};
+    const void* fns[4] = {(const void*)edge_pass_kernel<false, 2>, (const void*)edge_pass_kernel<true, 2>,
+                          (const void*)edge_pass_kernel<false, 4>, (const void*)edge_pass_kernel<true, 4>};
     int64_t b_min = -1;
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 4; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -653,8 +655,11 @@ static int edge_dynamic_lds(int64_t* budget) {
   return CVK_OK;
 }
 
+// generate: dynamic layers' endpoints of day t are computed from the resample stream (fused day loop,
+// which writes them to the layer only when it closes its last day).
 static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
-                          int32_t t, const double* const* uniforms, hipStream_t st, const DayReduce* reduce) {
+                          int32_t t, const double* const* uniforms, hipStream_t st, const DayReduce* reduce,
+                          bool generate) {
   EdgeArgs a;
   memset(&a, 0, sizeof(a));
   a.work = *work;
@@ -680,16 +685,16 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   for (int v = 0; v < MAXV; ++v) S.beta_v[v] = params->beta_v[v];
   a.bitmap_words = (int32_t)bitmap_words_for(people->n);
   a.t = t;
-  a.tile_counter = work->ctl + CVK_CTL_EDGE_TILES;
 
   DeviceCaps& c = caps();
-  const int64_t queue_bytes = (int64_t)(EDGE_BLOCK / 64) * EDGE_QCAP * 4;
+  const int64_t queue_bytes = (int64_t)EDGE_WAVES * EDGE_WAVE_WORDS * 4;
   const int64_t lds_bytes = (int64_t)a.bitmap_words * 4 + queue_bytes;
   int64_t budget = 0;
   int rc0 = edge_dynamic_lds(&budget);
   if (rc0) return rc0;
   const bool use_lds = lds_bytes <= budget && g_edge_lds != 0;
-  int64_t tiles = 0, wave_tiles = 0;
+  const int64_t wave_tile = 64 * 4 * (int64_t)g_edge_chunks;
+  int64_t wave_tiles = 0;
   for (int l = 0; l < params->n_layers; ++l) {
     const cvk_layer& L = layers[l];
     if (L.n_edges < 0 || (L.n_edges > 0 && (!L.p1 || !L.p2 || !L.beta)))
@@ -704,31 +709,24 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     G.beta_layer = params->beta_layer[l];
     G.iso_factor = params->iso_factor[l];
     G.quar_factor = params->quar_factor[l];
-    G.beta_ones = L.beta_ones;
+    G.gen = generate && L.dynamic ? 1 : 0;
+    G.beta_ones = L.beta_ones || G.gen;  // a resampled layer's weights are all 1
     G.l = l;
-    a.tile_start[l] = tiles;
     a.wave_tile_start[l] = wave_tiles;
-    tiles += (L.n_edges + edge_tile(g_edge_chunks) - 1) / edge_tile(g_edge_chunks);
-    wave_tiles += (L.n_edges + 256 * g_edge_chunks - 1) / (256 * g_edge_chunks);
+    wave_tiles += (L.n_edges + wave_tile - 1) / wave_tile;
   }
-  for (int l = params->n_layers; l <= CVK_MAX_LAYERS; ++l) {
-    a.tile_start[l] = tiles;
-    a.wave_tile_start[l] = wave_tiles;
-  }
-  if (tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
-  const int grid = (int)std::min<int64_t>(tiles, (int64_t)std::max(c.n_cu, 1));
+  for (int l = params->n_layers; l <= CVK_MAX_LAYERS; ++l) a.wave_tile_start[l] = wave_tiles;
+  if (wave_tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
+  // one workgroup per CU (the LDS bitmap takes most of a CU's LDS), at least one wave tile each
+  const int grid = (int)std::min<int64_t>((wave_tiles + EDGE_WAVES - 1) / EDGE_WAVES, (int64_t)std::max(c.n_cu, 1));
   const size_t shmem = (size_t)(use_lds ? lds_bytes : queue_bytes);
   const dim3 g(grid), b(EDGE_BLOCK);
-  const int variant = (g_edge_wave ? 4 : 0) | (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
+  const int variant = (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
   switch (variant) {
-    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2, false>), g, b, shmem, st, a); break;
-    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2, false>), g, b, shmem, st, a); break;
-    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4, false>), g, b, shmem, st, a); break;
-    case 3: hipLaunchKernelGGL((edge_pass_kernel<true, 4, false>), g, b, shmem, st, a); break;
-    case 4: hipLaunchKernelGGL((edge_pass_kernel<false, 2, true>), g, b, shmem, st, a); break;
-    case 5: hipLaunchKernelGGL((edge_pass_kernel<true, 2, true>), g, b, shmem, st, a); break;
-    case 6: hipLaunchKernelGGL((edge_pass_kernel<false, 4, true>), g, b, shmem, st, a); break;
-    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4, true>), g, b, shmem, st, a); break;
+    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2>), g, b, shmem, st, a); break;
+    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2>), g, b, shmem, st, a); break;
+    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4>), g, b, shmem, st, a); break;
+    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4>), g, b, shmem, st, a); break;
   }
   return check_launch("cvk_edge_pass");
 }
@@ -738,22 +736,28 @@ extern "C" int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, 
   int rc = validate(people, params, work, t);
   if (rc) return rc;
   if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "cvk_edge_pass: layers is null");
-  return edge_pass_impl(people, layers, params, work, t, uniforms, (hipStream_t)stream, nullptr);
+  return edge_pass_impl(people, layers, params, work, t, uniforms, (hipStream_t)stream, nullptr, false);
 }
 
+// partial_blocks (fused day loop): non-null = the flow counts go to the infect partial rows, whose
+// count of workgroups is stored there for the next reduction; null = atomics on the day row.
 static int infect_impl(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
-                       int32_t t, int32_t count_flows, int32_t allow_beds, hipStream_t st) {
+                       int32_t t, int32_t count_flows, int32_t allow_beds, hipStream_t st, bool generate = false,
+                       uint32_t* partial_blocks = nullptr) {
   InfectArgs ia;
   memset(&ia, 0, sizeof(ia));
   for (int l = 0; l < params->n_layers; ++l) {
     ia.p1[l] = layers[l].p1;
     ia.p2[l] = layers[l].p2;
+    if (generate && layers[l].dynamic) ia.gen_mask |= 1u << l;
   }
   ia.t = t;
   ia.n_layers = params->n_layers;
   ia.count_flows = count_flows;
   ia.allow_beds = allow_beds;
-  const int grid = std::max(caps().n_cu, 1);
+  const int grid = std::min(std::max(caps().n_cu, 1), INFECT_PART_BLOCKS);
+  ia.partial = partial_blocks && count_flows ? 1 : 0;
+  if (partial_blocks) *partial_blocks = ia.partial ? (uint32_t)grid : 0u;
   hipLaunchKernelGGL(infect_pass_kernel, dim3(grid), dim3(INFECT_BLOCK), 0, st, *people, *params, *work, ia);
   return check_launch("cvk_infect_pass");
 }
@@ -769,6 +773,10 @@ extern "C" int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers
 static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
                          int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, hipStream_t st) {
   int rc;
+  // With generated dynamic layers, no day's resample is written until the last day is closed.
+  bool gen = false;
+  for (int l = 0; l < params->n_layers; ++l) gen = gen || (g_gen_dynamic && layers[l].dynamic && layers[l].n_edges > 0);
+  uint32_t infect_blocks = 0;  // infect partial rows not yet added to their day's row
   for (int32_t t = t0; t < t1; ++t) {
     if (t == 0 && pop_infected > 0) {  // init_infections, sim.py:505-532 (flows not counted)
       rc = choose_impl(people, params, work, 0, STREAM_CHOOSE, pop_infected, 0, CVK_LAYER_SEED, 0, st);
@@ -779,8 +787,10 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
     // day t begins with the stocks of day t-1 (already closed if t == t0)
     DayReduce r;
     // the first resample of the run also sets every weight to 1; later ones leave them as they are
-    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : 1, st, &r);
+    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, gen ? 0 : (t == t0 ? 2 : 1), st, &r);
     if (rc) return rc;
+    r.infect_blocks = infect_blocks;  // the previous day's flow counts
+    r.infect_t = t - 1;
     if (imports) {
       for (int j = 0; j < imports->n; ++j) {
         if (imports->days[j] != t) continue;
@@ -790,15 +800,17 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
         if (rc) return rc;
       }
     }
-    rc = edge_pass_impl(people, layers, params, work, t, nullptr, st, &r);  // block 0 reduces day_begin's rows
+    rc = edge_pass_impl(people, layers, params, work, t, nullptr, st, &r, gen);  // block 0 reduces day_begin's rows
     if (rc) return rc;
-    rc = infect_impl(people, layers, params, work, t, 1, 1, st);
+    rc = infect_impl(people, layers, params, work, t, 1, 1, st, gen, &infect_blocks);
     if (rc) return rc;
   }
-  // close the last day: stocks and NAb update
+  // close the last day: stocks and NAb update, and (generated layers) the last day's contacts, weights 1
   DayReduce r;
-  rc = day_begin_launch(people, layers, params, work, t1 - 1, 0, 0, 0, st, &r);
+  rc = day_begin_launch(people, layers, params, work, t1 - 1, 0, 0, gen ? 2 : 0, st, &r);
   if (rc) return rc;
+  r.infect_blocks = infect_blocks;
+  r.infect_t = t1 - 1;
   return day_reduce_launch(work, r, st);
 }
 

@@ -386,14 +386,24 @@ __device__ inline void write_agent_rows(uint32_t* results, double* results_f, co
   __syncthreads();
 }
 
-// Dynamic-layer resample of one edge: iid uniform endpoints, beta = 1 (base.py:1849-1876). beta may be
-// null when the layer's weights are already all ones (every resample after the first of a run).
-__device__ inline void resample_edge(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e,
-                                     int32_t* p1, int32_t* p2, float* beta) {
+// Dynamic-layer resample of edge e on day t: iid uniform endpoints (base.py:1849-1876), a pure function
+// of (seed, t, layer, e), so a consumer may compute them instead of reading them back.
+struct EdgeEnds {
+  int32_t p1, p2;
+};
+__device__ inline EdgeEnds dynamic_endpoints(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e) {
   const uint64_t ue = (uint64_t)e;
   const u32x4 w = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(ue >> 32));
-  p1[e] = (int32_t)below(w.x, w.y, n_agents);
-  p2[e] = (int32_t)below(w.z, w.w, n_agents);
+  return EdgeEnds{(int32_t)below(w.x, w.y, n_agents), (int32_t)below(w.z, w.w, n_agents)};
+}
+
+// Writes the resample of edge e, beta = 1. beta may be null when the layer's weights are already all
+// ones (every resample after the first of a run).
+__device__ inline void resample_edge(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e,
+                                     int32_t* p1, int32_t* p2, float* beta) {
+  const EdgeEnds d = dynamic_endpoints(seed, t, layer, n_agents, e);
+  p1[e] = d.p1;
+  p2[e] = d.p2;
   if (beta) beta[e] = 1.0f;
 }
 
@@ -435,7 +445,8 @@ struct SegView {
   float beta_layer, iso_factor, quar_factor;
   int32_t beta_ones;   // every per-edge beta is 1.0f: no gather needed
   int32_t l;           // layer index (key and counter field)
-  int32_t pad;
+  int32_t gen;         // dynamic layer of a fused day loop: endpoints come from the day's resample
+                       // stream (dynamic_endpoints), p1/p2 are not read
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
@@ -600,6 +611,289 @@ __device__ inline void process_tile(const SimView& S, const SegView& L, int32_t 
     process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T.e0, T.n_valid, c, T.x[c], T.y[c], tid);
 }
 
+// ---------------------------------------------------------------------------
+// Wave-tile edge pass: a wave owns K x 256 consecutive edges of one layer and meets memory latency
+// three times per tile at most (endpoints, bound codes, and, when the bitmap is in L2, the bits),
+// because the per-candidate work in between touches no memory:
+//   1. infectious bits of both endpoints; an edge is a candidate in at most one direction (the
+//      infectious endpoint is the source; an infectious agent is never susceptible);
+//   2. the 2-byte bound codes of both endpoints of every mixed edge, issued together;
+//   3. candidates with a susceptible target go onto the wave's LDS stack with a rounded-up p_ub;
+//      64 at a time they draw their uniform and test u < p_ub (no memory access);
+//   4. the few that pass are queued (LDS) for the exact float32 test of the reference, which
+//      gathers the two 8-byte records; the queue is drained 32 entries at a time, so that memory
+//      round trip is shared by many tiles.
+// The set of (target, key) pairs offered to atomicMin is exactly the reference's, in another order.
+// ---------------------------------------------------------------------------
+constexpr int EDGE_CQ = 320;  // candidate stack per wave: < 64 carried + 256 from one chunk
+constexpr int EDGE_XQ = 32;   // deferred exact tests per wave
+constexpr int EDGE_WAVE_WORDS = EDGE_CQ + 5 * EDGE_XQ;  // LDS words per wave
+
+// p_ub kept in 18 bits (float bits 30..13, rounded up, so the decoded value is never below it);
+// PCODE_INF decodes to +inf (always take the exact test).
+constexpr uint32_t PCODE_INF = 0x7F800000u >> 13;
+__device__ inline uint32_t pcode_up(float p) {
+  const uint32_t c = (__float_as_uint(p) + 0x1FFFu) >> 13;
+  return c < PCODE_INF ? c : PCODE_INF;
+}
+__device__ inline float pcode_decode(uint32_t c) { return __uint_as_float(c << 13); }
+
+// The largest float <= u (u >= 0): u < p iff rz(u) < p for every float p, so the deferred test
+// compares floats and stays exact.
+__device__ inline float float_rz(double u) {
+  const float f = (float)u;
+  return (double)f > u ? __uint_as_float(__float_as_uint(f) - 1u) : f;
+}
+
+struct WaveQ {
+  uint32_t* cand;  // [EDGE_CQ] lane | q << 6 | c << 8 | dir << 10 | v << 11 | pcode << 14
+  uint32_t* xs;    // [EDGE_XQ] source
+  uint32_t* xt;    // target
+  uint32_t* xe;    // edge index, low 32 bits
+  uint32_t* xm;    // dir | v << 1 | (edge >> 32) << 4
+  uint32_t* xu;    // float_rz(u)
+};
+__device__ inline WaveQ wave_queues(uint32_t* base) {
+  WaveQ Q;
+  Q.cand = base;
+  Q.xs = base + EDGE_CQ;
+  Q.xt = Q.xs + EDGE_XQ;
+  Q.xe = Q.xt + EDGE_XQ;
+  Q.xm = Q.xe + EDGE_XQ;
+  Q.xu = Q.xm + EDGE_XQ;
+  return Q;
+}
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// beta_v[v] for a lane-varying v, selected among registers (an indexed read of the kernel argument
+// would be a memory round trip per use).
+__device__ inline float beta_of(const SimView& S, uint32_t v) {
+  float b = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(S.beta_v[0])));
+#pragma unroll
+  for (int j = 1; j < MAXV; ++j) {
+    const float bj = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(S.beta_v[j])));
+    b = v == (uint32_t)j ? bj : b;
+  }
+  return b;
+}
+
+template <int K>
+struct WaveTile {
+  int64_t e0;       // first edge of the tile in its layer
+  int32_t seg;      // segment (sim x layer) of the tile
+  int32_t n_valid;  // edges of the tile that exist
+  v4i x[K], y[K];   // lane's edges (c * 64 + lane) * 4 + q of the tile
+};
+
+// The tile's endpoints: loaded, or for a generated dynamic layer computed from the day's resample stream.
+template <int K>
+__device__ inline void load_wave_edges(const SimView& S, const SegView& L, int32_t t, WaveTile<K>& T) {
+  constexpr int TILE = 64 * 4 * K;
+  const int64_t rem = L.n - T.e0;
+  T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
+  const uint32_t lane = lane_id();
+  if (L.gen) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t e = T.e0 + ((int64_t)c * 64 + lane) * 4 + q;
+        EdgeEnds d = {0, 0};
+        if (e < L.n) d = dynamic_endpoints(S.seed, t, (uint32_t)L.l, S.n_agents, e);
+        T.x[c][q] = d.p1;
+        T.y[c][q] = d.p2;
+      }
+    }
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    const int64_t e = T.e0 + ((int64_t)c * 64 + lane) * 4;
+    T.x[c] = load4(L.p1, e, L.n);
+    T.y[c] = load4(L.p2, e, L.n);
+  }
+}
+
+// The reference's float32 test for one candidate slot (utils.py:88-89, 116-123) and the
+// first-occurrence dedup. uf = float_rz(u) of the slot's draw for variant vq.
+__device__ inline void exact_test(const SimView& S, const SegView& L, int32_t t, int32_t src, int32_t tgt, int64_t e,
+                                  uint32_t dir, uint32_t vq, float uf) {
+  const uint64_t rsrc = S.rec[src];
+  const uint64_t rtgt = S.rec[tgt];
+  const uint32_t fs = (uint32_t)(rsrc >> 32);
+  const uint32_t ft = (uint32_t)(rtgt >> 32);
+  const uint32_t v = inf_var(fs);
+  if (!(fs & CVK_INFECTIOUS) || v >= (uint32_t)S.n_variants || !(ft & CVK_SUSCEPTIBLE)) return;
+  float rt = __uint_as_float((uint32_t)rsrc);                   // rel_trans * inf (== 1)
+  if (fs & CVK_QUARANTINED) rt = rt * L.quar_factor;             // * f_quar
+  if (!(fs & CVK_SYMPTOMATIC)) rt = rt * S.asymp_factor;         // * f_asymp
+  if (fs & CVK_ISOLATED) rt = rt * L.iso_factor;                 // * f_iso
+  rt = rt * L.beta_layer;                                        // * beta_layer
+  rt = rt * ((fs & CVK_EARLY) ? S.vl_hi : S.vl_lo);              // * viral_load
+  if (rt == 0.0f) return;                                        // utils.py:116
+  float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
+  if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
+  if (S.use_sus_imm) rs = rs * (1.0f - S.sus_imm[(int64_t)v * S.sus_stride + tgt]);
+  const float eb = L.beta_ones ? 1.0f : L.beta[e];
+  const float p = ((beta_of(S, v) * eb) * rt) * rs;              // utils.py:117
+  if (p == 0.0f) return;                                         // utils.py:118
+  const bool hit = v == vq ? uf < p : edge_uniform(S, L, t, v, (int)dir, e) < (double)p;  // utils.py:123
+  if (!hit) return;
+  const uint64_t key = make_key(1, v, (uint32_t)L.l, dir, (uint64_t)e);
+  const uint64_t old = atomicMin((unsigned long long*)&S.winner[tgt], (unsigned long long)key);
+  if (old == KEY_NONE) {  // first key of this target today: mark it for the infect pass
+    const int64_t g = S.a_off + tgt;
+    atomicOr(&S.touched[g >> 5], 1u << (g & 31));
+  }
+}
+
+// Exact tests of the n_x (<= 64) queued slots, one per lane; every slot is from segment L.
+__device__ inline void flush_exact(const SimView& S, const SegView& L, int32_t t, const WaveQ& Q, uint32_t n_x) {
+  wave_sync_lds();
+  const uint32_t lane = lane_id();
+  if (lane < n_x) {
+    const uint32_t m = Q.xm[lane];
+    const int64_t e = (int64_t)Q.xe[lane] | ((int64_t)(m >> 4) << 32);
+    exact_test(S, L, t, (int32_t)Q.xs[lane], (int32_t)Q.xt[lane], e, m & 1u, (m >> 1) & 7u,
+               __uint_as_float(Q.xu[lane]));
+  }
+  wave_sync_lds();
+}
+
+// One round: the top min(64, n_c) stacked candidates draw their uniform; those below p_ub are queued
+// for the exact test (or tested at once when more than the queue holds pass together).
+template <int K>
+__device__ inline void candidate_round(const SimView& S, const SegView& L, int32_t t, const WaveQ& Q,
+                                       const WaveTile<K>& T, uint32_t& n_c, uint32_t& n_x) {
+  wave_sync_lds();
+  const uint32_t lane = lane_id();
+  const uint32_t take = n_c < 64u ? n_c : 64u;
+  const uint32_t base = n_c - take;
+  const bool act = lane < take;
+  const uint32_t ent = act ? Q.cand[base + lane] : 0u;
+  n_c = base;
+  wave_sync_lds();  // the slots just read may be overwritten by the next pushes
+  const uint32_t ol = ent & 63u, q = (ent >> 6) & 3u, c = (ent >> 8) & 3u, dir = (ent >> 10) & 1u;
+  const uint32_t vq = (ent >> 11) & 7u;
+  const int64_t e = T.e0 + ((int64_t)c * 64 + ol) * 4 + q;
+  double u = 1.0;
+  if (act) u = edge_uniform(S, L, t, vq, (int)dir, e);
+  const bool pass = act && u < (double)pcode_decode(ent >> 14);
+  const uint64_t m = __ballot(pass);
+  if (!m) return;  // wave-uniform
+  // endpoints of the passing slots, from the lanes that loaded them
+  int32_t xv = 0, yv = 0;
+#pragma unroll
+  for (int cc = 0; cc < K; ++cc) {
+    if (!__ballot(pass && c == (uint32_t)cc)) continue;
+#pragma unroll
+    for (int qq = 0; qq < 4; ++qq) {
+      const int32_t a = __shfl(T.x[cc][qq], (int)ol, 64);
+      const int32_t b = __shfl(T.y[cc][qq], (int)ol, 64);
+      if (c == (uint32_t)cc && q == (uint32_t)qq) {
+        xv = a;
+        yv = b;
+      }
+    }
+  }
+  const int32_t src = dir ? yv : xv, tgt = dir ? xv : yv;
+  const uint32_t k = (uint32_t)__popcll(m);
+  if (k > (uint32_t)EDGE_XQ) {
+    if (pass) exact_test(S, L, t, src, tgt, e, dir, vq, float_rz(u));
+    return;
+  }
+  if (n_x + k > (uint32_t)EDGE_XQ) {
+    flush_exact(S, L, t, Q, n_x);
+    n_x = 0;
+  }
+  if (pass) {
+    const uint32_t pos = n_x + lanes_below(m);
+    Q.xs[pos] = (uint32_t)src;
+    Q.xt[pos] = (uint32_t)tgt;
+    Q.xe[pos] = (uint32_t)e;
+    Q.xm[pos] = dir | (vq << 1) | ((uint32_t)((uint64_t)e >> 32) << 4);
+    Q.xu[pos] = __float_as_uint(float_rz(u));
+  }
+  n_x += k;
+}
+
+// One wave tile. bits: the infectious bitmap (LDS or global). n_x: the wave's exact-queue fill, whose
+// slots all belong to segment L.
+template <bool LDS, int K>
+__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* bits,
+                                      const WaveQ& Q, const WaveTile<K>& T, uint32_t& n_x) {
+  const uint32_t lane = lane_id();
+  uint32_t mixed = 0, from_x = 0;  // bit c*4+q: exactly one endpoint infectious; it is x (p1)
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
+      // endpoints outside [0, n) are ignored (the reference leaves them undefined)
+      const bool ok = (c * 64 + (int)lane) * 4 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents;
+      const uint32_t xs = ok ? xi : 0u, ys = ok ? yi : 0u;
+      const bool ix = ok && ((bits[xs >> 5] >> (xs & 31)) & 1u);
+      const bool iy = ok && ((bits[ys >> 5] >> (ys & 31)) & 1u);
+      if (ix != iy) mixed |= 1u << (c * 4 + q);
+      if (ix) from_x |= 1u << (c * 4 + q);
+    }
+  }
+  // bound codes of both endpoints of every mixed edge, all in flight together
+  uint32_t bs[K * 4], bt[K * 4];
+#pragma unroll
+  for (int j = 0; j < K * 4; ++j) {
+    bs[j] = 0;
+    bt[j] = 0;
+    if ((mixed >> j) & 1u) {
+      const uint32_t xi = (uint32_t)T.x[j >> 2][j & 3], yi = (uint32_t)T.y[j >> 2][j & 3];
+      const bool sx = (from_x >> j) & 1u;
+      bs[j] = S.bound[sx ? xi : yi];
+      bt[j] = S.bound[sx ? yi : xi];
+    }
+  }
+  uint32_t n_c = 0;
+#pragma unroll
+  for (int c = 0; c < K; ++c) {
+    uint32_t ent[4];
+    bool has[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = c * 4 + q;
+      has[q] = false;
+      ent[q] = 0;
+      const uint32_t vq = bs[j] >> 13;
+      // a non-infectious target's code is its susceptibility bound, or 0 if it cannot be infected
+      if (((mixed >> j) & 1u) && vq < (uint32_t)S.n_variants && bt[j] != 0u) {
+        uint32_t pc = PCODE_INF;  // weighted layers: per-edge beta is not in the bound
+        if (L.beta_ones) {
+          const float bv = beta_of(S, vq);
+          const float p_ub = (((bv * L.beta_layer) * bound_decode(bs[j])) * bound_decode(bt[j])) * BOUND_MARGIN;
+          // p_ub == 0 with nonzero factors can only be underflow: leave such slots to the exact test
+          pc = p_ub >= 0x1p-100f ? pcode_up(p_ub) : ((bv == 0.0f || L.beta_layer == 0.0f) ? 0u : PCODE_INF);
+        }
+        if (pc) {
+          has[q] = true;
+          const uint32_t dir = ((from_x >> j) & 1u) ? 0u : 1u;
+          ent[q] = lane | ((uint32_t)q << 6) | ((uint32_t)c << 8) | (dir << 10) | (vq << 11) | (pc << 14);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const uint64_t m = __ballot(has[q]);
+      if (has[q]) Q.cand[n_c + lanes_below(m)] = ent[q];
+      n_c += (uint32_t)__popcll(m);
+    }
+    while (n_c >= 64u) candidate_round<K>(S, L, t, Q, T, n_c, n_x);
+  }
+  if (n_c) candidate_round<K>(S, L, t, Q, T, n_c, n_x);
+}
+
 // ===========================================================================
 // People.infect for one target (people.py:470-579 + immunity.py:138-202)
 // ===========================================================================
@@ -732,9 +1026,15 @@ __device__ inline InfectOut infect_target(const cvk_people& P, const cvk_params&
 }
 
 // Source of an infection key (-1 for seeds and imports); p1/p2 are the sim's arrays of the key's layer.
-__device__ inline int32_t key_source(uint64_t key, const int32_t* p1, const int32_t* p2) {
+// gen: the key's layer is a generated dynamic layer of day t (endpoints from the resample stream).
+__device__ inline int32_t key_source(uint64_t key, const int32_t* p1, const int32_t* p2, bool gen = false,
+                                     uint64_t seed = 0, int32_t t = 0, uint32_t n_agents = 0) {
   if (key_class(key) != 1) return -1;
   const int64_t e = (int64_t)key_index(key);
+  if (gen) {
+    const EdgeEnds d = dynamic_endpoints(seed, t, key_layer(key), n_agents, e);
+    return key_dir(key) ? d.p2 : d.p1;
+  }
   return key_dir(key) ? p2[e] : p1[e];
 }
 

@@ -203,7 +203,7 @@ typedef struct {
 #define CVK_CTL_SEL_REMAIN 8
 #define CVK_CTL_SEL_ALL 9
 #define CVK_CTL_SEL_SHORT 10      /* set when fewer agents were eligible than requested */
-#define CVK_CTL_EDGE_TILES 11     /* dynamic tile counter of the edge pass (zeroed by the agent pass) */
+#define CVK_CTL_EDGE_TILES 11     /* dynamic tile counter of the batched edge pass (zeroed by the agent pass) */
 #define CVK_CTL_WORDS 64
 
 /* ---- library ---- */
@@ -215,11 +215,13 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 /* No reference counterpart: launch tunables (process-wide). CVK_TUNE_EDGE_LDS = 0 keeps the edge
  * pass's infectious bitmap in global memory (L2) even when it fits in LDS; 1 (default) stages it. */
 #define CVK_TUNE_EDGE_LDS 1
-/* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks each edge-pass thread loads per tile, 2 (default) or 4. */
+/* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks per lane of an edge-pass wave tile, 2 or 4 (default). */
 #define CVK_TUNE_EDGE_CHUNKS 2
-/* CVK_TUNE_EDGE_WAVE_TILES: 1 = each wave streams its own round-robin share of the edges (no
- * per-tile barrier or atomic), 0 (default) = workgroup tiles from an atomic counter. */
-#define CVK_TUNE_EDGE_WAVE_TILES 3
+/* CVK_TUNE_GENERATE_DYNAMIC: 1 = inside cvk_run_days / graphs, the edge and infect passes compute
+ * each day's dynamic-layer endpoints from the resample stream, and the layer arrays are written
+ * once, with the last day's contacts, when the run closes its last day; 0 (default) = every day's
+ * resample is written to the layer and read back. The results are identical. */
+#define CVK_TUNE_GENERATE_DYNAMIC 4
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

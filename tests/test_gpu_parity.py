@@ -137,6 +137,12 @@ def assert_people_equal(sim, osim):
         assert np.array_equal(gp.get(key), getattr(op, key)), key
     for key in ['sus_imm', 'symp_imm', 'sev_imm']:
         assert np.array_equal(bits(gp.get(key)), bits(getattr(op, key))), key
+    # contact layers after the run (a dynamic layer holds the last day's resample, weights 1)
+    for lk, (p1, p2, beta) in osim.layers.items():
+        lay = gp.contacts[lk]
+        assert np.array_equal(lay.tensor('p1').cpu().numpy(), p1), lk
+        assert np.array_equal(lay.tensor('p2').cpu().numpy(), p2), lk
+        assert np.array_equal(bits(lay.tensor('beta').cpu().numpy()), bits(np.asarray(beta, np.float32))), lk
 
 
 CONFIGS = {
@@ -151,13 +157,18 @@ CONFIGS = {
                                       variants=[dict(rel_beta=1.7, days=[10], n_imports=30)])),
     'weighted_edges_l2_bitmap': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
                                      pars=dict(use_waning=True, beta=0.04), tuning=dict(edge_lds=0)),
-    'hybrid_dynamic_wave_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
-                                      pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
-                                      tuning=dict(edge_wave_tiles=1)),
-    'variants_waning_deep_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+    'hybrid_dynamic_shallow_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                         pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                         tuning=dict(edge_chunks=2)),
+    'variants_waning_shallow_l2': dict(pop=dict(n=4000, pop_type='hybrid'),
                                        pars=dict(use_waning=True, beta=0.02,
                                                  variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
-                                       tuning=dict(edge_chunks=4, edge_wave_tiles=1)),
+                                       tuning=dict(edge_chunks=2, edge_lds=0)),
+    'hybrid_dynamic_generated_contacts': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                              pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                              tuning=dict(generate_dynamic=1)),
+    # p_ub >= 1 for most slots: whole rounds pass the prefilter (exact tests in place and queue drains)
+    'dense_exact_tests': dict(pop=dict(n=3000, pop_type='random'), pars=dict(use_waning=False, beta=0.9)),
     'beds_limited': dict(pop=dict(n=3000, pop_type='random'),
                          pars=dict(use_waning=False, beta=0.08, n_beds_hosp=2, n_beds_icu=1,
                                    rel_severe_prob=20.0, rel_crit_prob=5.0)),
@@ -172,8 +183,8 @@ def test_full_sim_matches_oracle(gpu, name):
     pars = dict(cfg['pars'])
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
-    keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 2),
-                edge_wave_tiles=(_lib.TUNE_EDGE_WAVE_TILES, 0))
+    keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 4),
+                generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
