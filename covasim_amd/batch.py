@@ -28,7 +28,7 @@ from . import _lib
 __all__ = ['BatchSim']
 
 # launch shapes the autotuner tries: (edge_block, edge_chunks, edge_blocks_per_cu, agents_per_thread)
-EDGE_SHAPES = [(1024, 2, 1), (1024, 1, 1), (512, 2, 1), (512, 1, 1), (1024, 1, 2), (512, 2, 2), (512, 1, 2)]
+EDGE_SHAPES = [(1024, 4, 1), (1024, 2, 1), (512, 4, 1), (512, 2, 1), (512, 4, 2), (512, 2, 2)]
 AGENT_SHAPES = [1, 2, 4]
 
 

@@ -99,9 +99,6 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
   S.partials_f = C.partials_f;
   S.part_stride = C.part_stride;
   S.part_col = blockIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.transitions) {
-    C.ctl[CVK_CTL_EDGE_TILES] = 0;
-  }
   agent_block<APT>(C.P, C.params[s], S, a.t, a.prologue, a.transitions);
 }
 
@@ -130,61 +127,67 @@ __global__ __launch_bounds__(1024) void batch_reduce_kernel(BatchCommon C, Batch
 struct BatchEdgeArgs {
   const SimView* views;        // [S]
   const SegView* segs;         // [S * L]
-  const int64_t* seg_tile_start;  // [S * L + 1]
-  const int32_t* tile_seg;     // [n_tiles]
+  const int64_t* seg_tile_start;  // [S * L + 1] first wave tile of each segment
+  const int32_t* tile_seg;     // [n_tiles] segment of each wave tile
   int64_t n_tiles;
   int32_t lds_words;           // bitmap words the LDS stage holds (larger sims read L2)
   int32_t t;
   BatchReduce reduce;
 };
 
-template <int BLOCK, int CHUNKS>
-__device__ inline void batch_load_tile(const BatchEdgeArgs& a, int64_t tile, EdgeTile<CHUNKS>& T) {
-  T.tile = tile;
-  if (tile >= a.n_tiles) return;
-  const int32_t seg = a.tile_seg[tile];
-  T.seg = seg;
-  T.e0 = (tile - a.seg_tile_start[seg]) * (int64_t)(BLOCK * 4 * CHUNKS);
-  load_tile_edges<BLOCK, CHUNKS>(a.segs[seg], T, threadIdx.x);
-}
-
-template <int BLOCK, int CHUNKS>
+// Workgroup b owns the wave tiles [T*b/G, T*(b+1)/G) of the day, ordered by (sim, layer). It walks
+// them one sim at a time: the sim's infectious bitmap is staged in LDS (when it fits), then the
+// block's waves take the sim's tiles from an LDS counter, as in the single-sim pass. A wave's exact
+// queue is drained when its tiles change segment and when the sim is done.
+template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchEdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
-  __shared__ int64_t s_tiles[2];
-  uint32_t* s_bits = s_mem;
-  uint32_t* s_q = s_mem + a.lds_words + (threadIdx.x >> 6) * EDGE_QCAP;
+  __shared__ uint32_t s_next;
+  const WaveQ Q = wave_queues(s_mem + a.lds_words + (threadIdx.x >> 6) * EDGE_WAVE_WORDS);
   if (a.reduce.active)  // the agent pass's counts, one sim per block
     for (int32_t s = blockIdx.x; s < C.n_sims; s += gridDim.x) batch_reduce_sim(C, a.reduce, s);
-  uint32_t* tile_counter = C.ctl + CVK_CTL_EDGE_TILES;
-  if (threadIdx.x == 0) s_tiles[0] = atomicAdd(tile_counter, 1u);
-  __syncthreads();
-  EdgeTile<CHUNKS> cur, nxt;
-  batch_load_tile<BLOCK, CHUNKS>(a, s_tiles[0], cur);
-  int32_t loaded = -1;
-  bool in_lds = false;
-  int par = 1;
-  while (cur.tile < a.n_tiles) {
-    if (threadIdx.x == 0) s_tiles[par] = atomicAdd(tile_counter, 1u);
-    const int32_t s = cur.seg / C.n_layers;
-    const SimView& S = a.views[s];
-    if (s != loaded) {  // block-uniform: stage this sim's infectious bitmap
-      const int32_t nw = (int32_t)(((int64_t)S.n_agents + 127) / 128 * 4);
-      in_lds = nw <= a.lds_words;
-      __syncthreads();  // every wave is done with the previous sim's bitmap
-      if (in_lds) {
-        const uint4* src = (const uint4*)S.bitmap;
-        uint4* dst = (uint4*)s_bits;
-        for (int j = threadIdx.x; j < (nw >> 2); j += BLOCK) dst[j] = src[j];
-      }
-      loaded = s;
+  const int64_t first = a.n_tiles * blockIdx.x / gridDim.x, last = a.n_tiles * (blockIdx.x + 1) / gridDim.x;
+  int64_t tile0 = first;
+  while (tile0 < last) {  // block-uniform: one sim of the range per iteration
+    const int32_t s = a.tile_seg[tile0] / C.n_layers;
+    const int64_t sim_end = a.seg_tile_start[(int64_t)(s + 1) * C.n_layers];
+    const int64_t sub_last = sim_end < last ? sim_end : last;
+    const SimView S = uniform_copy(a.views[s]);
+    const int32_t nw = (int32_t)(((int64_t)S.n_agents + 127) / 128 * 4);
+    const bool in_lds = nw <= a.lds_words;
+    __syncthreads();  // every wave is done with the previous sim's bitmap and counter
+    if (in_lds) {
+      const uint4* src = (const uint4*)S.bitmap;
+      uint4* dst = (uint4*)s_mem;
+      for (int j = threadIdx.x; j < (nw >> 2); j += BLOCK) dst[j] = src[j];
     }
+    if (threadIdx.x == 0) s_next = 0;
     __syncthreads();
-    batch_load_tile<BLOCK, CHUNKS>(a, s_tiles[par], nxt);
-    par ^= 1;
-    if (in_lds) process_tile<true, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
-    else process_tile<false, BLOCK, CHUNKS>(S, a.segs[cur.seg], a.t, s_bits, s_q, cur, threadIdx.x);
-    cur = nxt;
+    uint32_t n_x = 0;
+    int32_t q_seg = -1;
+    SegView L = uniform_copy(a.segs[a.tile_seg[tile0]]);  // segment of the wave's tiles and queued tests
+    for (;;) {
+      uint32_t k = 0;
+      if (lane_id() == 0) k = atomicAdd(&s_next, 1u);
+      k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
+      const int64_t tile = tile0 + k;
+      if (tile >= sub_last) break;
+      const int32_t seg = a.tile_seg[tile];
+      WaveTile<K> T;
+      T.seg = seg;
+      T.e0 = (tile - a.seg_tile_start[seg]) * (int64_t)(64 * 4 * K);
+      if (seg != q_seg) {
+        if (n_x) flush_exact(S, L, a.t, Q, n_x);
+        n_x = 0;
+        q_seg = seg;
+        L = uniform_copy(a.segs[seg]);
+      }
+      load_wave_edges<K>(S, L, a.t, T);
+      if (in_lds) wave_tile_pass<true, K>(S, L, a.t, s_mem, Q, T, n_x);
+      else wave_tile_pass<false, K>(S, L, a.t, S.bitmap, Q, T, n_x);
+    }
+    if (n_x) flush_exact(S, L, a.t, Q, n_x);
+    tile0 = sub_last;
   }
 }
 
@@ -459,7 +462,7 @@ struct Batch {
   std::vector<SegView> segs;           // [S * L]
   std::vector<SimView> views;          // [S]
   std::vector<BatchSim> sims;          // [S]
-  cvk_batch_config cfg{1024, 2, 1, 2};
+  cvk_batch_config cfg{1024, 4, 1, 2};
   int64_t n_tiles = 0, n_rs_chunks = 0;
   uint32_t agent_blocks = 0;
   int64_t part_stride = 0;
@@ -535,7 +538,7 @@ struct Batch {
 static int64_t words_for(int64_t n) { return ((n + 127) / 128) * 4; }
 
 static bool config_ok(const cvk_batch_config& c) {
-  return (c.edge_block == 512 || c.edge_block == 1024) && (c.edge_chunks == 1 || c.edge_chunks == 2) &&
+  return (c.edge_block == 512 || c.edge_block == 1024) && (c.edge_chunks == 2 || c.edge_chunks == 4) &&
          (c.edge_blocks_per_cu == 1 || c.edge_blocks_per_cu == 2) &&
          (c.agents_per_thread == 1 || c.agents_per_thread == 2 || c.agents_per_thread == 4);
 }
@@ -554,7 +557,7 @@ static int apply_config(Batch& b, const cvk_batch_config& c) {
     blocks += nb;
   }
   if ((int64_t)blocks > b.part_stride) return fail(CVK_E_CAPACITY, "cvk_batch_set_config: partial rows too small");
-  const int64_t tile = (int64_t)c.edge_block * 4 * c.edge_chunks;
+  const int64_t tile = 64 * 4 * (int64_t)c.edge_chunks;  // wave tiles
   std::vector<int64_t> seg_tile_start((size_t)b.S * b.L + 1);
   std::vector<int32_t> tile_seg;
   int64_t tiles = 0;
@@ -608,10 +611,10 @@ static int edge_lds_budget(int max_lds, int64_t* budget) {
 
 static int edge_budget_for(const Batch& b, int64_t* budget) {
   const cvk_batch_config& c = b.cfg;
-  if (c.edge_block == 1024 && c.edge_chunks == 2) return edge_lds_budget<1024, 2>(b.max_lds, budget);
-  if (c.edge_block == 1024) return edge_lds_budget<1024, 1>(b.max_lds, budget);
-  if (c.edge_chunks == 2) return edge_lds_budget<512, 2>(b.max_lds, budget);
-  return edge_lds_budget<512, 1>(b.max_lds, budget);
+  if (c.edge_block == 1024 && c.edge_chunks == 4) return edge_lds_budget<1024, 4>(b.max_lds, budget);
+  if (c.edge_block == 1024) return edge_lds_budget<1024, 2>(b.max_lds, budget);
+  if (c.edge_chunks == 4) return edge_lds_budget<512, 4>(b.max_lds, budget);
+  return edge_lds_budget<512, 2>(b.max_lds, budget);
 }
 
 static int launch_day_begin(Batch& b, int32_t t, int prologue, int transitions, int resample, hipStream_t st) {
@@ -643,7 +646,7 @@ static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t 
   int rc = edge_budget_for(b, &budget);
   if (rc) return rc;
   const cvk_batch_config& c = b.cfg;
-  const int64_t queue_bytes = (int64_t)(c.edge_block / 64) * EDGE_QCAP * 4;
+  const int64_t queue_bytes = (int64_t)(c.edge_block / 64) * EDGE_WAVE_WORDS * 4;
   // two workgroups per CU split the CU's LDS
   const int64_t per_block = std::min<int64_t>(budget, (int64_t)b.max_lds / c.edge_blocks_per_cu - 1024);
   int64_t lds_words = ((per_block - queue_bytes) / 4) & ~3ll;
@@ -661,14 +664,14 @@ static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t 
   const size_t shmem = (size_t)(lds_words * 4 + queue_bytes);
   const dim3 grid((unsigned)std::max(1, b.n_cu * c.edge_blocks_per_cu));
   const BatchCommon C = b.common();
-  if (c.edge_block == 1024 && c.edge_chunks == 2)
-    hipLaunchKernelGGL((batch_edge_kernel<1024, 2>), grid, dim3(1024), shmem, st, C, a);
+  if (c.edge_block == 1024 && c.edge_chunks == 4)
+    hipLaunchKernelGGL((batch_edge_kernel<1024, 4>), grid, dim3(1024), shmem, st, C, a);
   else if (c.edge_block == 1024)
-    hipLaunchKernelGGL((batch_edge_kernel<1024, 1>), grid, dim3(1024), shmem, st, C, a);
-  else if (c.edge_chunks == 2)
-    hipLaunchKernelGGL((batch_edge_kernel<512, 2>), grid, dim3(512), shmem, st, C, a);
+    hipLaunchKernelGGL((batch_edge_kernel<1024, 2>), grid, dim3(1024), shmem, st, C, a);
+  else if (c.edge_chunks == 4)
+    hipLaunchKernelGGL((batch_edge_kernel<512, 4>), grid, dim3(512), shmem, st, C, a);
   else
-    hipLaunchKernelGGL((batch_edge_kernel<512, 1>), grid, dim3(512), shmem, st, C, a);
+    hipLaunchKernelGGL((batch_edge_kernel<512, 2>), grid, dim3(512), shmem, st, C, a);
   return check_launch("cvk_batch edge_pass");
 }
 

@@ -81,9 +81,6 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   S.partials_f = W.partials_f;
   S.part_stride = CVK_MAX_REDUCE_BLOCKS;
   S.part_col = blockIdx.x;
-  if (blockIdx.x == 0 && threadIdx.x == 0 && a.transitions) {
-    W.ctl[CVK_CTL_EDGE_TILES] = 0;    // the edge pass's tile counter
-  }
   agent_block<AGENTS_PER_THREAD>(P, prm, S, a.t, a.prologue, a.transitions);
 }
 

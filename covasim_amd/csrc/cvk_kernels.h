@@ -276,7 +276,7 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         sus_bit = (f & CVK_SUSCEPTIBLE) && x.rs != 0.0f;
         const float val = (f & CVK_INFECTIOUS) ? x.rt : x.rs;
         S.rec[li] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
-        // prefilter bound (edge_candidate): the source's factors for any layer, or the target's
+        // prefilter bound (wave_tile_pass): the source's factors for any layer, or the target's
         uint32_t bc = 0;
         if (inf_bit) {
           float ub = x.rt;
@@ -451,13 +451,32 @@ struct SegView {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 
-constexpr int EDGE_QCAP = 64 * 8;  // per-wave candidate queue: 4 edges x 2 directions x 64 lanes
+// Pointers that came out of device memory (the batched views) have no known address space, so the
+// compiler emits flat accesses, which count on the LDS wait counter too: every LDS wait would also wait
+// for outstanding memory loads. These casts name the global address space.
+#define CVK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline CVK_GLOBAL T* gmem(T* p) {
+  return (CVK_GLOBAL T*)p;
+}
 
-// A candidate slot (infectious source, susceptible target) of edge e: the float32 products of
-// compute_trans_sus (utils.py:88-89) and compute_infections (utils.py:117), the float64 uniform
-// test (utils.py:123), and the first-occurrence dedup as an atomicMin.
+// A copy of a wave-uniform struct held in scalar registers (each word through readfirstlane), so that
+// its fields are neither vector registers nor reloaded after the kernel's stores.
+template <class T>
+__device__ inline T uniform_copy(const T& in) {
+  static_assert(sizeof(T) % 4 == 0, "uniform_copy needs whole words");
+  T out;
+  const uint32_t* src = (const uint32_t*)&in;
+  uint32_t* dst = (uint32_t*)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+// The float64 uniform of a candidate slot (utils.py:123): supplied, or drawn from the slot's
+// counter stream (edge, day, variant, layer; the two directions use the two halves of one draw).
 __device__ inline double edge_uniform(const SimView& S, const SegView& L, int32_t t, uint32_t v, int dir, int64_t e) {
-  if (L.unif) return L.unif[((int64_t)v * 2 + dir) * L.n + e];
+  if (L.unif) return gmem(L.unif)[((int64_t)v * 2 + dir) * L.n + e];
   const uint64_t ue = (uint64_t)e;
   const u32x4 w = philox_seeded(S.seed, (uint32_t)ue, (uint32_t)t, STREAM_TX | (v << 8) | (uint32_t)L.l,
                                 (uint32_t)(ue >> 32));
@@ -469,146 +488,14 @@ __device__ inline double edge_uniform(const SimView& S, const SegView& L, int32_
 // u >= p_ub implies u >= p and the candidate is rejected without gathering the two 8-byte records.
 constexpr float BOUND_MARGIN = 1.0f + 1.0f / 1024.0f;
 
-__device__ inline void edge_candidate(const SimView& S, const SegView& L, int32_t t, int dir, int64_t e, int32_t src,
-                                      int32_t tgt) {
-  uint32_t v_pre = 0xFFFFFFFFu;
-  double u_pre = 0.0;
-  if (L.beta_ones) {  // per-edge weights are not in the bound
-    const uint32_t qs = S.bound[src], qt = S.bound[tgt];
-    const uint32_t vq = qs >> 13;
-    if (vq >= (uint32_t)S.n_variants) return;  // not infectious with a transmitting factor
-    const float p_ub = (((S.beta_v[vq] * L.beta_layer) * bound_decode(qs)) * bound_decode(qt)) * BOUND_MARGIN;
-    u_pre = edge_uniform(S, L, t, vq, dir, e);
-    if (!(u_pre < (double)p_ub)) return;
-    v_pre = vq;
-  }
-  const uint64_t rsrc = S.rec[src];
-  const uint64_t rtgt = S.rec[tgt];
-  const uint32_t fs = (uint32_t)(rsrc >> 32);
-  const uint32_t ft = (uint32_t)(rtgt >> 32);
-  const uint32_t v = inf_var(fs);
-  if (!(fs & CVK_INFECTIOUS) || v >= (uint32_t)S.n_variants || !(ft & CVK_SUSCEPTIBLE)) return;
-  float rt = __uint_as_float((uint32_t)rsrc);                   // rel_trans * inf (== 1)
-  if (fs & CVK_QUARANTINED) rt = rt * L.quar_factor;             // * f_quar
-  if (!(fs & CVK_SYMPTOMATIC)) rt = rt * S.asymp_factor;         // * f_asymp
-  if (fs & CVK_ISOLATED) rt = rt * L.iso_factor;                 // * f_iso
-  rt = rt * L.beta_layer;                                        // * beta_layer
-  rt = rt * ((fs & CVK_EARLY) ? S.vl_hi : S.vl_lo);              // * viral_load
-  if (rt == 0.0f) return;                                        // utils.py:116
-  float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
-  if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
-  if (S.use_sus_imm) rs = rs * (1.0f - S.sus_imm[(int64_t)v * S.sus_stride + tgt]);
-  const float eb = L.beta_ones ? 1.0f : L.beta[e];
-  const float p = ((S.beta_v[v] * eb) * rt) * rs;                // utils.py:117
-  if (p == 0.0f) return;                                         // utils.py:118
-  const double u = v == v_pre ? u_pre : edge_uniform(S, L, t, v, dir, e);
-  if (!(u < (double)p)) return;                                  // utils.py:123
-  const uint64_t key = make_key(1, v, (uint32_t)L.l, (uint32_t)dir, (uint64_t)e);
-  const uint64_t old = atomicMin((unsigned long long*)&S.winner[tgt], (unsigned long long)key);
-  if (old == KEY_NONE) {  // first key of this target today: mark it for the infect pass
-    const int64_t g = S.a_off + tgt;
-    atomicOr(&S.touched[g >> 5], 1u << (g & 31));
-  }
-}
-
-// One tile's endpoints for this thread, in vector registers (CHUNKS 4-edge chunks; every index into
-// x/y is a compile-time constant after unrolling, so the arrays stay in registers).
-template <int CHUNKS>
-struct EdgeTile {
-  int64_t tile;     // tile number (>= the tile count: no tile)
-  int64_t e0;       // first edge of the tile in its layer
-  int32_t seg;      // segment (sim x layer) of the tile
-  int32_t n_valid;  // edges of the tile that exist (the last tile of a layer is partial)
-  v4i x[CHUNKS], y[CHUNKS];
-};
-
-__device__ inline v4i load4(const int32_t* p, int64_t e, int64_t n) {
-  if (e + 3 < n) return __builtin_nontemporal_load((const v4i*)(p + e));
+__device__ inline v4i load4(const int32_t* p0, int64_t e, int64_t n) {
+  const CVK_GLOBAL int32_t* p = gmem(p0);
+  if (e + 3 < n) return __builtin_nontemporal_load((const CVK_GLOBAL v4i*)(p + e));
   v4i r = {0, 0, 0, 0};
   if (e < n) r[0] = p[e];
   if (e + 1 < n) r[1] = p[e + 1];
   if (e + 2 < n) r[2] = p[e + 2];
   return r;
-}
-
-// Issues the 16-byte endpoint loads of tile T (non-temporal: streamed once). T.e0/seg set by the caller.
-// BLOCK: threads sharing the tile (the workgroup, or one wave for wave tiles); tid: this thread's
-// index among them.
-template <int BLOCK, int CHUNKS>
-__device__ inline void load_tile_edges(const SegView& L, EdgeTile<CHUNKS>& T, uint32_t tid) {
-  constexpr int TILE = BLOCK * 4 * CHUNKS;
-  const int64_t rem = L.n - T.e0;
-  T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
-#pragma unroll
-  for (int c = 0; c < CHUNKS; ++c) {
-    const int64_t e = T.e0 + ((int64_t)c * BLOCK + tid) * 4;
-    T.x[c] = load4(L.p1, e, L.n);
-    T.y[c] = load4(L.p2, e, L.n);
-  }
-}
-
-// Candidate bits from the bitmaps, wave-local compaction into the LDS queue, then one queued slot
-// per lane per round (endpoints fetched from the owning lane's registers).
-template <bool LDS, int BLOCK>
-__device__ inline void process_chunk(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                     uint32_t* s_q, int64_t e0, int32_t n_valid, int c, const v4i xs, const v4i ys,
-                                     uint32_t tid) {
-  const uint32_t lane = lane_id();
-  const uint32_t wave_base = tid & ~63u;
-  const int32_t local0 = (c * BLOCK + (int32_t)tid) * 4;  // tile-local index of edge q = 0
-  const uint32_t* bits = LDS ? s_bits : S.bitmap;
-  uint32_t cand = 0;  // bit q*2 + dir: infectious source and susceptible target
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t xi = (uint32_t)xs[q], yi = (uint32_t)ys[q];
-    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-    if (local0 + q < n_valid && xi < S.n_agents && yi < S.n_agents) {
-      const bool ix = (bits[xi >> 5] >> (xi & 31)) & 1u;
-      const bool iy = (bits[yi >> 5] >> (yi & 31)) & 1u;
-      if (ix && ((S.susmap[yi >> 5] >> (yi & 31)) & 1u)) cand |= 1u << (2 * q);
-      if (iy && ((S.susmap[xi >> 5] >> (xi & 31)) & 1u)) cand |= 2u << (2 * q);
-    }
-  }
-  uint32_t total = 0;
-#pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const bool has = (cand >> b) & 1u;
-    const uint64_t m = __ballot(has);
-    if (has) s_q[total + lanes_below(m)] = (lane << 3) | (uint32_t)b;
-    total += (uint32_t)__popcll(m);
-  }
-  if (total == 0) return;  // wave-uniform
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  for (uint32_t base = 0; base < total; base += 64) {
-    const uint32_t j = base + lane;
-    const uint32_t code = j < total ? s_q[j] : 0u;
-    const int src_lane = (int)(code >> 3);
-    const int q = (int)((code >> 1) & 3u);
-    const int dir = (int)(code & 1u);
-    const int x0 = __shfl(xs[0], src_lane, 64), x1 = __shfl(xs[1], src_lane, 64);
-    const int x2 = __shfl(xs[2], src_lane, 64), x3 = __shfl(xs[3], src_lane, 64);
-    const int y0 = __shfl(ys[0], src_lane, 64), y1 = __shfl(ys[1], src_lane, 64);
-    const int y2 = __shfl(ys[2], src_lane, 64), y3 = __shfl(ys[3], src_lane, 64);
-    if (j < total) {
-      const int32_t xv = q == 0 ? x0 : (q == 1 ? x1 : (q == 2 ? x2 : x3));
-      const int32_t yv = q == 0 ? y0 : (q == 1 ? y1 : (q == 2 ? y2 : y3));
-      const int64_t e = e0 + ((int64_t)c * BLOCK + wave_base + (uint32_t)src_lane) * 4 + q;
-      edge_candidate(S, L, t, dir, e, dir ? yv : xv, dir ? xv : yv);
-    }
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <bool LDS, int BLOCK, int CHUNKS>
-__device__ inline void process_tile(const SimView& S, const SegView& L, int32_t t, const uint32_t* s_bits,
-                                    uint32_t* s_q, const EdgeTile<CHUNKS>& T, uint32_t tid) {
-#pragma unroll
-  for (int c = 0; c < CHUNKS; ++c)
-    process_chunk<LDS, BLOCK>(S, L, t, s_bits, s_q, T.e0, T.n_valid, c, T.x[c], T.y[c], tid);
 }
 
 // ---------------------------------------------------------------------------
@@ -723,8 +610,8 @@ __device__ inline void load_wave_edges(const SimView& S, const SegView& L, int32
 // first-occurrence dedup. uf = float_rz(u) of the slot's draw for variant vq.
 __device__ inline void exact_test(const SimView& S, const SegView& L, int32_t t, int32_t src, int32_t tgt, int64_t e,
                                   uint32_t dir, uint32_t vq, float uf) {
-  const uint64_t rsrc = S.rec[src];
-  const uint64_t rtgt = S.rec[tgt];
+  const uint64_t rsrc = gmem(S.rec)[src];
+  const uint64_t rtgt = gmem(S.rec)[tgt];
   const uint32_t fs = (uint32_t)(rsrc >> 32);
   const uint32_t ft = (uint32_t)(rtgt >> 32);
   const uint32_t v = inf_var(fs);
@@ -738,17 +625,17 @@ __device__ inline void exact_test(const SimView& S, const SegView& L, int32_t t,
   if (rt == 0.0f) return;                                        // utils.py:116
   float rs = __uint_as_float((uint32_t)rtgt);                   // rel_sus * sus (== 1)
   if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
-  if (S.use_sus_imm) rs = rs * (1.0f - S.sus_imm[(int64_t)v * S.sus_stride + tgt]);
-  const float eb = L.beta_ones ? 1.0f : L.beta[e];
+  if (S.use_sus_imm) rs = rs * (1.0f - gmem(S.sus_imm)[(int64_t)v * S.sus_stride + tgt]);
+  const float eb = L.beta_ones ? 1.0f : gmem(L.beta)[e];
   const float p = ((beta_of(S, v) * eb) * rt) * rs;              // utils.py:117
   if (p == 0.0f) return;                                         // utils.py:118
   const bool hit = v == vq ? uf < p : edge_uniform(S, L, t, v, (int)dir, e) < (double)p;  // utils.py:123
   if (!hit) return;
   const uint64_t key = make_key(1, v, (uint32_t)L.l, dir, (uint64_t)e);
-  const uint64_t old = atomicMin((unsigned long long*)&S.winner[tgt], (unsigned long long)key);
+  const uint64_t old = __hip_atomic_fetch_min(gmem(S.winner) + tgt, key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   if (old == KEY_NONE) {  // first key of this target today: mark it for the infect pass
     const int64_t g = S.a_off + tgt;
-    atomicOr(&S.touched[g >> 5], 1u << (g & 31));
+    __hip_atomic_fetch_or(gmem(S.touched) + (g >> 5), 1u << (g & 31), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 
@@ -825,9 +712,10 @@ __device__ inline void candidate_round(const SimView& S, const SegView& L, int32
 // One wave tile. bits: the infectious bitmap (LDS or global). n_x: the wave's exact-queue fill, whose
 // slots all belong to segment L.
 template <bool LDS, int K>
-__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* bits,
+__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* bits0,
                                       const WaveQ& Q, const WaveTile<K>& T, uint32_t& n_x) {
   const uint32_t lane = lane_id();
+  const CVK_GLOBAL uint16_t* bound = gmem(S.bound);
   uint32_t mixed = 0, from_x = 0;  // bit c*4+q: exactly one endpoint infectious; it is x (p1)
 #pragma unroll
   for (int c = 0; c < K; ++c) {
@@ -837,8 +725,16 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
       // endpoints outside [0, n) are ignored (the reference leaves them undefined)
       const bool ok = (c * 64 + (int)lane) * 4 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents;
       const uint32_t xs = ok ? xi : 0u, ys = ok ? yi : 0u;
-      const bool ix = ok && ((bits[xs >> 5] >> (xs & 31)) & 1u);
-      const bool iy = ok && ((bits[ys >> 5] >> (ys & 31)) & 1u);
+      uint32_t wx, wy;
+      if (LDS) {
+        wx = bits0[xs >> 5];
+        wy = bits0[ys >> 5];
+      } else {
+        wx = gmem(bits0)[xs >> 5];
+        wy = gmem(bits0)[ys >> 5];
+      }
+      const bool ix = ok && ((wx >> (xs & 31)) & 1u);
+      const bool iy = ok && ((wy >> (ys & 31)) & 1u);
       if (ix != iy) mixed |= 1u << (c * 4 + q);
       if (ix) from_x |= 1u << (c * 4 + q);
     }
@@ -852,8 +748,8 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
     if ((mixed >> j) & 1u) {
       const uint32_t xi = (uint32_t)T.x[j >> 2][j & 3], yi = (uint32_t)T.y[j >> 2][j & 3];
       const bool sx = (from_x >> j) & 1u;
-      bs[j] = S.bound[sx ? xi : yi];
-      bt[j] = S.bound[sx ? yi : xi];
+      bs[j] = bound[sx ? xi : yi];
+      bt[j] = bound[sx ? yi : xi];
     }
   }
   uint32_t n_c = 0;

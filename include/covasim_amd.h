@@ -203,7 +203,6 @@ typedef struct {
 #define CVK_CTL_SEL_REMAIN 8
 #define CVK_CTL_SEL_ALL 9
 #define CVK_CTL_SEL_SHORT 10      /* set when fewer agents were eligible than requested */
-#define CVK_CTL_EDGE_TILES 11     /* dynamic tile counter of the batched edge pass (zeroed by the agent pass) */
 #define CVK_CTL_WORDS 64
 
 /* ---- library ---- */
@@ -344,7 +343,7 @@ typedef struct {
 /* Launch shape of the batch kernels (chosen by an autotuner; see cvk_batch_set_config). */
 typedef struct {
   int32_t edge_block;       /* 512 or 1024 threads per edge-pass workgroup */
-  int32_t edge_chunks;      /* 1 or 2 four-edge chunks per thread per tile */
+  int32_t edge_chunks;      /* 2 or 4 four-edge chunks per lane of an edge-pass wave tile */
   int32_t edge_blocks_per_cu;  /* 1 or 2 */
   int32_t agents_per_thread;   /* 1, 2 or 4 in the agent pass */
 } cvk_batch_config;
