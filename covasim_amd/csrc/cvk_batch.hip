@@ -78,8 +78,9 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
       const int64_t e1 = std::min<int64_t>(e0 + RS_CHUNK, L.n);
       const uint64_t seed = C.params[s].seed;
       const uint32_t n = (uint32_t)C.sims[s].n;
-      for (int64_t e = e0 + threadIdx.x; e < e1; e += AGENT_BLOCK)
-        resample_edge(seed, a.t, (uint32_t)L.l, n, e, (int32_t*)L.p1, (int32_t*)L.p2,
+      const uint32_t thr = resample_thr(n);
+      for (int64_t q = e0 / 2 + threadIdx.x; q < (e1 + 1) / 2; q += AGENT_BLOCK)  // chunks start at even edges
+        resample_pair(seed, a.t, (uint32_t)L.l, n, thr, q, L.n, (int32_t*)L.p1, (int32_t*)L.p2,
                       a.resample == 2 ? (float*)L.beta : nullptr);
     }
     return;
@@ -851,6 +852,7 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
     }
     if (need > 0 && (!d->p1[l] || !d->p2[l] || !d->beta[l])) return fail(CVK_E_INVALID, "cvk_batch_create: null layer array");
     if (((uintptr_t)d->p1[l] | (uintptr_t)d->p2[l]) & 15) return fail(CVK_E_ALIGN, "cvk_batch_create: p1/p2 must be 16-byte aligned");
+    if ((uintptr_t)d->beta[l] & 7) return fail(CVK_E_ALIGN, "cvk_batch_create: beta must be 8-byte aligned");
   }
   std::vector<int32_t> rs_seg;
   std::vector<int64_t> rs_first;

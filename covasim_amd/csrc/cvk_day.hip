@@ -62,9 +62,11 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   if (blockIdx.x >= a.agent_blocks) {
     const int64_t stride = (int64_t)(gridDim.x - a.agent_blocks) * AGENT_BLOCK;
     const int64_t start = (int64_t)(blockIdx.x - a.agent_blocks) * AGENT_BLOCK + threadIdx.x;
-    for (int r = 0; r < a.n_resample; ++r) {
+    const uint32_t thr = resample_thr((uint32_t)P.n);
+    for (int r = 0; r < a.n_resample; ++r) {  // one thread per pair of edges
       const ResampleLayer& R = a.rs[r];
-      for (int64_t e = start; e < R.n; e += stride) resample_edge(prm.seed, a.t, R.layer, (uint32_t)P.n, e, R.p1, R.p2, R.beta);
+      for (int64_t q = start; q < (R.n + 1) / 2; q += stride)
+        resample_pair(prm.seed, a.t, R.layer, (uint32_t)P.n, thr, q, R.n, R.p1, R.p2, R.beta);
     }
     return;
   }
@@ -455,6 +457,8 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
       const cvk_layer& L = layers[l];
       if (!L.dynamic || L.n_edges <= 0) continue;
       if (!L.p1 || !L.p2 || !L.beta) return fail(CVK_E_INVALID, "dynamic layer has a null pointer");
+      if (((uintptr_t)L.p1 | (uintptr_t)L.p2 | (uintptr_t)L.beta) & 7)
+        return fail(CVK_E_ALIGN, "dynamic layer: p1/p2/beta must be 8-byte aligned");
       ResampleLayer& R = a.rs[a.n_resample++];
       R.p1 = L.p1;
       R.p2 = L.p2;
@@ -577,6 +581,8 @@ extern "C" int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, c
   if (params->n_agents <= 0 || params->n_agents >= (1ll << 31))
     return fail(CVK_E_INVALID, "cvk_resample_layer: bad n_agents");
   if (layer->n_edges == 0) return CVK_OK;
+  if (((uintptr_t)layer->p1 | (uintptr_t)layer->p2 | (uintptr_t)layer->beta) & 7)
+    return fail(CVK_E_ALIGN, "cvk_resample_layer: p1/p2/beta must be 8-byte aligned");
   // only resample blocks: agent_blocks = 0
   DayBeginArgs a;
   memset(&a, 0, sizeof(a));

@@ -28,6 +28,8 @@ constexpr uint32_t STREAM_INFECT = 0x20000000u;
 constexpr uint32_t STREAM_CHOOSE = 0x30000000u;
 constexpr uint32_t STREAM_IMPORT = 0x40000000u;    // | variant
 constexpr uint32_t STREAM_RESAMPLE = 0x50000000u;  // | layer
+constexpr uint32_t STREAM_RESAMPLE_RETRY = 0x60000000u;  // | layer
+constexpr uint32_t RESAMPLE_MAX_RETRY = 255;
 
 // Per-infection draw slots (counter word c3 of STREAM_INFECT).
 constexpr uint32_t SLOT_EXP2INF = 0;
@@ -73,11 +75,6 @@ __host__ __device__ inline double u01(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) * (1.0 / 9007199254740992.0);
 }
 
-// Uniform integer in [0, n), n < 2^32: floor(((a<<32)|b) * n / 2^64).
-__host__ __device__ inline uint32_t below(uint32_t a, uint32_t b, uint32_t n) {
-  const uint64_t lo = ((uint64_t)b * n) >> 32;
-  return (uint32_t)(((uint64_t)a * n + lo) >> 32);
-}
 
 __device__ inline double box_muller(double u0, double u1) {
   return sqrt(-2.0 * log1p(-u0)) * cos((2.0 * 3.141592653589793) * u1);

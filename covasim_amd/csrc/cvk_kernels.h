@@ -19,6 +19,29 @@ constexpr int AGENT_BLOCK = 256;
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Pointers that came out of device memory (the batched views) have no known address space, so the
+// compiler emits flat accesses, which count on the LDS wait counter too: every LDS wait would also wait
+// for outstanding memory loads. These casts name the global address space.
+#define CVK_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ inline CVK_GLOBAL T* gmem(T* p) {
+  return (CVK_GLOBAL T*)p;
+}
+
+// A copy of a wave-uniform struct held in scalar registers (each word through readfirstlane), so that
+// its fields are neither vector registers nor reloaded after the kernel's stores.
+template <class T>
+__device__ inline T uniform_copy(const T& in) {
+  static_assert(sizeof(T) % 4 == 0, "uniform_copy needs whole words");
+  T out;
+  const uint32_t* src = (const uint32_t*)&in;
+  uint32_t* dst = (uint32_t*)&out;
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
+  return out;
+}
+
+
 __device__ inline uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
@@ -386,25 +409,69 @@ __device__ inline void write_agent_rows(uint32_t* results, double* results_f, co
   __syncthreads();
 }
 
-// Dynamic-layer resample of edge e on day t: iid uniform endpoints (base.py:1849-1876), a pure function
-// of (seed, t, layer, e), so a consumer may compute them instead of reading them back.
+// Dynamic-layer resample on day t: iid uniform endpoints (base.py:1849-1876), a pure function of
+// (seed, t, layer, e), so a consumer may compute them instead of reading them back. Edges 2q and 2q+1
+// share one draw (words 0, 1 and 2, 3 are their p1, p2). Each endpoint is an unbiased bounded
+// integer by multiply-and-reject: w * n is accepted when its low word is at least
+// (2^32 - n) mod n (resample_thr), else the same-position word of the edge's retry draws is used
+// (at most RESAMPLE_MAX_RETRY; the last is accepted as is, with probability below 10^-600).
+__device__ inline uint32_t resample_thr(uint32_t n) { return (0u - n) % n; }
+
+__device__ inline int32_t resample_endpoint(uint64_t seed, int32_t t, uint32_t layer, uint32_t n, uint32_t thr,
+                                            int64_t e, int j, uint32_t w) {
+  uint64_t m = (uint64_t)w * n;
+  if ((uint32_t)m < thr) {  // rare: < n / 2^32 of the words
+    const uint64_t ue = (uint64_t)e;
+    for (uint32_t k = 0; k < RESAMPLE_MAX_RETRY; ++k) {
+      const u32x4 r = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE_RETRY | layer,
+                                    ((uint32_t)(ue >> 32) << 8) | k);
+      m = (uint64_t)(j ? r.y : r.x) * n;
+      if ((uint32_t)m >= thr) break;
+    }
+  }
+  return (int32_t)(m >> 32);
+}
+
 struct EdgeEnds {
   int32_t p1, p2;
 };
-__device__ inline EdgeEnds dynamic_endpoints(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e) {
-  const uint64_t ue = (uint64_t)e;
-  const u32x4 w = philox_seeded(seed, (uint32_t)ue, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(ue >> 32));
-  return EdgeEnds{(int32_t)below(w.x, w.y, n_agents), (int32_t)below(w.z, w.w, n_agents)};
+
+// The draw of pair q (edges 2q, 2q+1).
+__device__ inline u32x4 resample_pair_draw(uint64_t seed, int32_t t, uint32_t layer, int64_t q) {
+  const uint64_t uq = (uint64_t)q;
+  return philox_seeded(seed, (uint32_t)uq, (uint32_t)t, STREAM_RESAMPLE | layer, (uint32_t)(uq >> 32));
 }
 
-// Writes the resample of edge e, beta = 1. beta may be null when the layer's weights are already all
-// ones (every resample after the first of a run).
-__device__ inline void resample_edge(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e,
-                                     int32_t* p1, int32_t* p2, float* beta) {
-  const EdgeEnds d = dynamic_endpoints(seed, t, layer, n_agents, e);
-  p1[e] = d.p1;
-  p2[e] = d.p2;
-  if (beta) beta[e] = 1.0f;
+__device__ inline EdgeEnds dynamic_endpoints(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, int64_t e) {
+  const u32x4 w = resample_pair_draw(seed, t, layer, e >> 1);
+  const uint32_t thr = resample_thr(n_agents);
+  const bool odd = e & 1;
+  return EdgeEnds{resample_endpoint(seed, t, layer, n_agents, thr, e, 0, odd ? w.z : w.x),
+                  resample_endpoint(seed, t, layer, n_agents, thr, e, 1, odd ? w.w : w.y)};
+}
+
+// Writes the resample of pair q (edges 2q and, when it exists, 2q+1 of an n_edges layer), beta = 1.
+// beta may be null when the layer's weights are already all ones (every resample after the first of
+// a run). thr = resample_thr(n_agents).
+__device__ inline void resample_pair(uint64_t seed, int32_t t, uint32_t layer, uint32_t n_agents, uint32_t thr,
+                                     int64_t q, int64_t n_edges, int32_t* p1, int32_t* p2, float* beta) {
+  const u32x4 w = resample_pair_draw(seed, t, layer, q);
+  const int64_t e = 2 * q;
+  const int32_t a1 = resample_endpoint(seed, t, layer, n_agents, thr, e, 0, w.x);
+  const int32_t a2 = resample_endpoint(seed, t, layer, n_agents, thr, e, 1, w.y);
+  if (e + 1 < n_edges) {
+    const int32_t b1 = resample_endpoint(seed, t, layer, n_agents, thr, e + 1, 0, w.z);
+    const int32_t b2 = resample_endpoint(seed, t, layer, n_agents, thr, e + 1, 1, w.w);
+    typedef int32_t i32x2 __attribute__((ext_vector_type(2)));
+    typedef float f32x2 __attribute__((ext_vector_type(2)));
+    *(CVK_GLOBAL i32x2*)gmem(p1 + e) = i32x2{a1, b1};
+    *(CVK_GLOBAL i32x2*)gmem(p2 + e) = i32x2{a2, b2};
+    if (beta) *(CVK_GLOBAL f32x2*)gmem(beta + e) = f32x2{1.0f, 1.0f};
+  } else {
+    gmem(p1)[e] = a1;
+    gmem(p2)[e] = a2;
+    if (beta) gmem(beta)[e] = 1.0f;
+  }
 }
 
 // Selection key of cvk_choose: philox word << 32 | local agent index (distinct per agent).
@@ -450,28 +517,6 @@ struct SegView {
 };
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-
-// Pointers that came out of device memory (the batched views) have no known address space, so the
-// compiler emits flat accesses, which count on the LDS wait counter too: every LDS wait would also wait
-// for outstanding memory loads. These casts name the global address space.
-#define CVK_GLOBAL __attribute__((address_space(1)))
-template <class T>
-__device__ inline CVK_GLOBAL T* gmem(T* p) {
-  return (CVK_GLOBAL T*)p;
-}
-
-// A copy of a wave-uniform struct held in scalar registers (each word through readfirstlane), so that
-// its fields are neither vector registers nor reloaded after the kernel's stores.
-template <class T>
-__device__ inline T uniform_copy(const T& in) {
-  static_assert(sizeof(T) % 4 == 0, "uniform_copy needs whole words");
-  T out;
-  const uint32_t* src = (const uint32_t*)&in;
-  uint32_t* dst = (uint32_t*)&out;
-#pragma unroll
-  for (int i = 0; i < (int)(sizeof(T) / 4); ++i) dst[i] = __builtin_amdgcn_readfirstlane(src[i]);
-  return out;
-}
 
 // The float64 uniform of a candidate slot (utils.py:123): supplied, or drawn from the slot's
 // counter stream (edge, day, variant, layer; the two directions use the two halves of one draw).
@@ -584,16 +629,20 @@ __device__ inline void load_wave_edges(const SimView& S, const SegView& L, int32
   const int64_t rem = L.n - T.e0;
   T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
   const uint32_t lane = lane_id();
-  if (L.gen) {
+  if (L.gen) {  // edges 4i .. 4i+3 of a lane are two resample pairs
+    const uint32_t thr = resample_thr(S.n_agents);
 #pragma unroll
     for (int c = 0; c < K; ++c) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t e = T.e0 + ((int64_t)c * 64 + lane) * 4 + q;
-        EdgeEnds d = {0, 0};
-        if (e < L.n) d = dynamic_endpoints(S.seed, t, (uint32_t)L.l, S.n_agents, e);
-        T.x[c][q] = d.p1;
-        T.y[c][q] = d.p2;
+      for (int h = 0; h < 2; ++h) {
+        const int64_t e = T.e0 + ((int64_t)c * 64 + lane) * 4 + 2 * h;
+        u32x4 w = {0u, 0u, 0u, 0u};
+        if (e < L.n) w = resample_pair_draw(S.seed, t, (uint32_t)L.l, e >> 1);
+        const uint32_t l = (uint32_t)L.l;
+        T.x[c][2 * h] = e < L.n ? resample_endpoint(S.seed, t, l, S.n_agents, thr, e, 0, w.x) : 0;
+        T.y[c][2 * h] = e < L.n ? resample_endpoint(S.seed, t, l, S.n_agents, thr, e, 1, w.y) : 0;
+        T.x[c][2 * h + 1] = e + 1 < L.n ? resample_endpoint(S.seed, t, l, S.n_agents, thr, e + 1, 0, w.z) : 0;
+        T.y[c][2 * h + 1] = e + 1 < L.n ? resample_endpoint(S.seed, t, l, S.n_agents, thr, e + 1, 1, w.w) : 0;
       }
     }
     return;

@@ -105,9 +105,18 @@ static double u01(uint32_t a, uint32_t b) {
   return ((double)(a >> 5) * 67108864.0 + (double)(b >> 6)) / 9007199254740992.0;
 }
 
-static uint32_t below(uint32_t a, uint32_t b, uint32_t n) {
-  const uint64_t lo = ((uint64_t)b * n) >> 32;
-  return (uint32_t)(((uint64_t)a * n + lo) >> 32);
+/* Resampled endpoint j (0: p1, 1: p2) of edge e with first word w0: Lemire's multiply-and-reject,
+ * rejected words replaced from the retry stream (oracle/philox.py, resample_edges). */
+#define STREAM_RESAMPLE_RETRY 0x60000000u
+static uint32_t resample_endpoint(uint64_t seed, int32_t t, uint32_t l, int64_t e, int j, uint32_t w0, uint32_t n) {
+  const uint32_t thr = (uint32_t)((0x100000000ull - n) % n);
+  uint64_t m = (uint64_t)w0 * n;
+  for (uint32_t k = 0; k < 255 && (uint32_t)m < thr; ++k) {
+    uint32_t c[4] = {(uint32_t)e, (uint32_t)t, STREAM_RESAMPLE_RETRY | l, (uint32_t)(((uint64_t)e >> 32) << 8) | k};
+    philox(c, seed);
+    m = (uint64_t)c[j] * n;
+  }
+  return (uint32_t)(m >> 32);
 }
 
 static void slot_pair(uint64_t seed, uint32_t agent, int32_t t, uint32_t slot, double* u0, double* u1) {
@@ -378,12 +387,15 @@ int port_step(const port_params* P, port_people* S, port_layer* layers, int32_t 
     if (!P->dynamic[l]) continue;
     port_layer* L = &layers[l];
 #pragma omp parallel for schedule(static)
-    for (int64_t e = 0; e < L->n_edges; ++e) {
-      uint32_t c[4] = {(uint32_t)e, (uint32_t)t, STREAM_RESAMPLE | (uint32_t)l, (uint32_t)((uint64_t)e >> 32)};
+    for (int64_t q = 0; q < (L->n_edges + 1) / 2; ++q) {  /* one draw per pair of edges */
+      uint32_t c[4] = {(uint32_t)q, (uint32_t)t, STREAM_RESAMPLE | (uint32_t)l, (uint32_t)((uint64_t)q >> 32)};
       philox(c, P->seed);
-      L->p1[e] = (int32_t)below(c[0], c[1], (uint32_t)n);
-      L->p2[e] = (int32_t)below(c[2], c[3], (uint32_t)n);
-      L->beta[e] = 1.0f;
+      for (int h = 0; h < 2 && 2 * q + h < L->n_edges; ++h) {
+        const int64_t e = 2 * q + h;
+        L->p1[e] = (int32_t)resample_endpoint(P->seed, t, (uint32_t)l, e, 0, c[2 * h], (uint32_t)n);
+        L->p2[e] = (int32_t)resample_endpoint(P->seed, t, (uint32_t)l, e, 1, c[2 * h + 1], (uint32_t)n);
+        L->beta[e] = 1.0f;
+      }
     }
   }
   int64_t n_sev = 0, n_crit = 0;

@@ -20,8 +20,15 @@ transmission, edge e       (e_lo, t, TX | v<<8 | l, e_hi)   (w0,w1)->dir 0, (w2,
 infection outcome slot s   (agent, t, INFECT, s)           (w0,w1)->u0, (w2,w3)->u1
 seed-infection choice      (agent, 0, CHOOSE, 0)           w0 = selection key
 variant import choice      (agent, t, IMPORT | v, 0)       w0 = selection key
-dynamic-layer resample     (e_lo, t, RESAMPLE | l, e_hi)   (w0,w1)->p1, (w2,w3)->p2
+dynamic-layer resample     (q_lo, t, RESAMPLE | l, q_hi)   edge 2q: w0->p1, w1->p2;
+  (pair q = e >> 1)                                         edge 2q+1: w2->p1, w3->p2
+resample retry k of edge e (e_lo, t, RETRY | l, e_hi<<8|k)  w0->p1, w1->p2
 =========================  ==============================  =====================
+
+Resampled endpoints are unbiased bounded integers (Lemire's multiply-and-reject):
+m = w * n, accepted when (m mod 2^32) >= (2^32 - n) mod n, value m >> 32; a rejected
+endpoint takes the same-position word of retry k = 0, 1, ... (at most 255 retries; the
+256th word is accepted as is, which happens with probability below 10^-600).
 """
 
 import numpy as np
@@ -37,6 +44,8 @@ STREAM_INFECT = 0x20000000
 STREAM_CHOOSE = 0x30000000
 STREAM_IMPORT = 0x40000000
 STREAM_RESAMPLE = 0x50000000
+STREAM_RESAMPLE_RETRY = 0x60000000
+RESAMPLE_MAX_RETRY = 255
 
 # Fixed per-infection draw slots (see People.infect restatement in covasim_oracle.py)
 SLOT_EXP2INF = 0
@@ -132,14 +141,39 @@ def choose_k(seed, t, stream, eligible, k):
     return np.sort(eligible[sel])
 
 
+def bounded(w, n):
+    """(value, accepted) of Lemire's multiply-and-reject for 32-bit words w and 0 < n < 2^32."""
+    n = np.uint64(n)
+    m = np.asarray(w, dtype=np.uint64) * n
+    thr = (np.uint64(1 << 32) - n) % n
+    return (m >> np.uint64(32)).astype(np.int64), (m & MASK32) >= thr
+
+
 def resample_edges(seed, t, l, n_edges, n_agents):
     """Dynamic-layer resample: iid uniform endpoints for every edge (Layer.update with frac=1)."""
     k0, k1 = seed_key(seed)
-    e = np.arange(n_edges, dtype=np.uint64)
-    w = philox4x32(e & MASK32, t, STREAM_RESAMPLE | l, e >> np.uint64(32), k0, k1)
-    p1 = below(w[0], w[1], n_agents).astype(np.int32)
-    p2 = below(w[2], w[3], n_agents).astype(np.int32)
-    return p1, p2
+    q = np.arange((n_edges + 1) // 2, dtype=np.uint64)
+    w = philox4x32(q & MASK32, t, STREAM_RESAMPLE | l, q >> np.uint64(32), k0, k1)
+    words1 = np.empty(2 * len(q), dtype=np.uint64)
+    words2 = np.empty(2 * len(q), dtype=np.uint64)
+    words1[0::2], words2[0::2] = w[0], w[1]
+    words1[1::2], words2[1::2] = w[2], w[3]
+    out = []
+    for j, words in enumerate((words1[:n_edges], words2[:n_edges])):
+        val, ok = bounded(words, n_agents)
+        for k in range(RESAMPLE_MAX_RETRY):
+            bad = np.flatnonzero(~ok)
+            if len(bad) == 0:
+                break
+            e = bad.astype(np.uint64)
+            wr = philox4x32(e & MASK32, t, STREAM_RESAMPLE_RETRY | l, ((e >> np.uint64(32)) << np.uint64(8)) | np.uint64(k),
+                            k0, k1)
+            v2, ok2 = bounded(wr[j], n_agents)
+            val[bad] = v2
+            last = k == RESAMPLE_MAX_RETRY - 1
+            ok[bad] = ok2 | last
+        out.append(val.astype(np.int32))
+    return out[0], out[1]
 
 
 def box_muller(u0, u1):

@@ -28,6 +28,29 @@ def random_dates(n, rng):
     return start, rec, dead
 
 
+@pytest.mark.parametrize('n_agents', [4000, 1_500_000_000])
+def test_resample_layer_matches_oracle(gpu, n_agents):
+    """Layer.update via cvk_resample_layer, odd edge count; n = 1.5e9 rejects ~30 % of first words (retry stream)."""
+    import ctypes as C
+    import torch
+    from covasim_amd import _lib
+    from oracle import philox as px
+    n_edges = 100_001
+    p1 = torch.full((n_edges,), -1, dtype=torch.int32, device='cuda')
+    p2 = torch.full((n_edges,), -1, dtype=torch.int32, device='cuda')
+    beta = torch.zeros(n_edges, dtype=torch.float32, device='cuda')
+    lay = _lib.LayerS(p1.data_ptr(), p2.data_ptr(), beta.data_ptr(), n_edges, 1, 1)
+    prm = _lib.Params()
+    prm.n_agents = n_agents
+    prm.seed = 77
+    _lib.check(_lib.lib().cvk_resample_layer(C.byref(lay), 3, C.byref(prm), 12, _lib.stream_ptr()), 'resample')
+    torch.cuda.synchronize()
+    q1, q2 = px.resample_edges(77, 12, 3, n_edges, n_agents)
+    assert np.array_equal(p1.cpu().numpy(), q1)
+    assert np.array_equal(p2.cpu().numpy(), q2)
+    assert bool((beta == 1.0).all())
+
+
 def test_viral_load_bit_exact(gpu):
     import covasim_amd as cv
     rng = np.random.default_rng(3)

@@ -110,6 +110,48 @@ def test_bernoulli_rates_and_independent_halves():
     assert not rng.bernoulli(np.zeros(100), np.arange(100), 7, 2, 1).any()
 
 
+def test_bounded_accept_rule_is_exactly_uniform():
+    """Lemire's rule: every value in [0, n) keeps the same number of accepted 32-bit words."""
+    two32 = 1 << 32
+    for n in (1, 3, 7, 1000, 999_983, 10_000_000, 1_500_000_000, 2**31 - 1):
+        thr = (two32 - n) % n
+        vs = range(n) if n <= 1000 else [0, 1, n // 3, n // 2, n - 2, n - 1]
+        # accepted w with value v: v * 2^32 + thr <= w * n < (v + 1) * 2^32
+        counts = {-(-((v + 1) * two32) // n) - -(-(v * two32 + thr) // n) for v in vs}
+        assert counts == {(two32 - thr) // n}, n
+    # the vectorised restatement applies the same rule
+    w = np.array([0, 1, 2**31, two32 - 1, 123456789], dtype=np.uint64)
+    for n in (7, 1_500_000_000):
+        val, ok = px.bounded(w, n)
+        thr = (two32 - n) % n
+        assert list(val) == [int(x) * n >> 32 for x in w]
+        assert list(ok) == [(int(x) * n) % two32 >= thr for x in w]
+
+
+def test_resample_retry_stream():
+    """n = 1.5e9 rejects ~30 % of first words: retries come from their own stream and stay uniform."""
+    n_agents, n_edges = 1_500_000_000, 200_000
+    p1, p2 = px.resample_edges(4, 9, 3, n_edges, n_agents)
+    assert p1.min() >= 0 and p1.max() < n_agents and p2.min() >= 0 and p2.max() < n_agents
+    for p in (p1, p2):
+        assert _chi2_pvalue(np.bincount(p // 15_000_000, minlength=100).astype(float),
+                            np.full(100, n_edges / 100)) > 1e-4
+    # the first word of edge e is word (e % 2) * 2 + j of pair e >> 1; rejected ones were replaced
+    k0, k1 = px.seed_key(4)
+    q = np.arange(n_edges // 2, dtype=np.uint64)
+    w = px.philox4x32(q & px.MASK32, 9, px.STREAM_RESAMPLE | 3, q >> np.uint64(32), k0, k1)
+    first = np.empty(n_edges, dtype=np.uint64)
+    first[0::2], first[1::2] = w[0], w[2]
+    v0, ok0 = px.bounded(first, n_agents)
+    assert 0.25 < 1 - ok0.mean() < 0.35
+    assert np.array_equal(p1[ok0], v0[ok0])
+    assert not np.array_equal(p1[~ok0], v0[~ok0])
+    e = np.flatnonzero(~ok0)[:50].astype(np.uint64)
+    wr = px.philox4x32(e & px.MASK32, 9, px.STREAM_RESAMPLE_RETRY | 3, (e >> np.uint64(32)) << np.uint64(8), k0, k1)
+    vr, okr = px.bounded(wr[0], n_agents)
+    assert np.array_equal(p1[e.astype(np.int64)][okr], vr[okr])
+
+
 def test_resample_endpoints_uniform():
     """Layer.update with frac = 1 (base.py:1849-1876): every edge gets iid uniform endpoints, self-loops allowed."""
     n_agents, n_edges = 1000, 300_000
