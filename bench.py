@@ -272,6 +272,8 @@ def main():
     ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
     ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass wave tile depth (2 or 4 chunks of 4 edges per lane)')
     ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
+    ap.add_argument('--overlap-resample', type=int, default=None,
+                    help='1: the next day\'s resample runs on a side branch beside the infect pass; 0: in the agent pass')
     ap.add_argument('--generate-dynamic', type=int, default=None,
                     help='1: the day loop computes dynamic-layer endpoints in place; 0: writes and reads them daily')
     ap.add_argument('--eager-only', action='store_true',
@@ -290,6 +292,9 @@ def main():
 
     w = WORKLOADS[args.workload]
     tuning = {}
+    if args.overlap_resample is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_OVERLAP_RESAMPLE, args.overlap_resample), 'tuning')
+        tuning['overlap_resample'] = args.overlap_resample
     if args.generate_dynamic is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, args.generate_dynamic), 'tuning')
         tuning['generate_dynamic'] = args.generate_dynamic

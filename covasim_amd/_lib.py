@@ -34,6 +34,7 @@ DUR_ORDER = ['exp2inf', 'inf2sym', 'sym2sev', 'sev2crit', 'asym2rec', 'mild2rec'
 TUNE_EDGE_LDS = 1
 TUNE_EDGE_CHUNKS = 2
 TUNE_GENERATE_DYNAMIC = 4
+TUNE_OVERLAP_RESAMPLE = 5
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15

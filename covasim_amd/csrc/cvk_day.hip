@@ -281,10 +281,25 @@ struct InfectArgs {
   int32_t n_layers;
   uint32_t gen_mask;  // bit l: layer l's endpoints of day t come from the resample stream
   int32_t partial;    // 1: flow counts go to the infect partial rows (reduced later), 0: atomics on the row
+  uint32_t infect_blocks;  // workgroups [0, infect_blocks) infect; the rest resample day rs_t
+  int32_t rs_t;
+  int32_t n_resample;
+  ResampleLayer rs[CVK_MAX_LAYERS];
 };
 
 __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
                                                                    InfectArgs ia) {
+  if (blockIdx.x >= ia.infect_blocks) {  // the next day's resample, one thread per pair of edges
+    const int64_t stride = (int64_t)(gridDim.x - ia.infect_blocks) * INFECT_BLOCK;
+    const int64_t start = (int64_t)(blockIdx.x - ia.infect_blocks) * INFECT_BLOCK + threadIdx.x;
+    const uint32_t thr = resample_thr((uint32_t)P.n);
+    for (int r = 0; r < ia.n_resample; ++r) {
+      const ResampleLayer& R = ia.rs[r];
+      for (int64_t q = start; q < (R.n + 1) / 2; q += stride)
+        resample_pair(prm.seed, ia.rs_t, R.layer, (uint32_t)P.n, thr, q, R.n, R.p1, R.p2, R.beta);
+    }
+    return;
+  }
   __shared__ uint32_t s_cnt[N_INFECT_COLS];
   if (threadIdx.x < N_INFECT_COLS) s_cnt[threadIdx.x] = 0;
   __syncthreads();
@@ -297,7 +312,7 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
   __shared__ int32_t s_list[INFECT_BLOCK / 64][INFECT_WAVE_LIST];
   int32_t* list = s_list[threadIdx.x >> 6];
   const int64_t n_words = (P.n + 31) / 32;
-  const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
+  const int64_t stride = (int64_t)ia.infect_blocks * INFECT_BLOCK;
   const int64_t n_sweeps = (n_words + stride - 1) / stride;
   for (int64_t sw = 0; sw < n_sweeps; ++sw) {
     const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
@@ -372,6 +387,7 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
 static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
 static int g_edge_chunks = 4;  // tunable: 4-edge chunks per lane per wave tile (2 or 4)
 static int g_gen_dynamic = 0;  // tunable: the fused day loop computes dynamic layers' endpoints in place
+static int g_overlap_resample = 1;  // tunable: the fused day loop resamples on a side branch
 
 struct DeviceCaps {
   int device = -1;
@@ -441,17 +457,12 @@ static int day_reduce_launch(cvk_work* work, const DayReduce& r, hipStream_t st)
   return check_launch("day_reduce");
 }
 
+// The dynamic layers' resample arguments of a day_begin launch; returns the blocks they need.
 // resample: 0 none, 1 endpoints only (weights already all ones), 2 endpoints and weights.
-static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
-                            cvk_work* work, int32_t t, int prologue, int transitions, int resample, hipStream_t st,
-                            DayReduce* reduce_out) {
-  DayBeginArgs a;
-  memset(&a, 0, sizeof(a));
-  a.t = t;
-  a.prologue = prologue;
-  a.transitions = transitions;
-  a.agent_blocks = agent_blocks_for(people->n);
+static int fill_resample(DayBeginArgs& a, const cvk_layer* layers, const cvk_params* params, int resample,
+                         uint32_t* rs_blocks) {
   int64_t max_dyn = 0;
+  *rs_blocks = 0;
   if (resample) {
     for (int l = 0; l < params->n_layers; ++l) {
       const cvk_layer& L = layers[l];
@@ -468,8 +479,24 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
       max_dyn = std::max(max_dyn, L.n_edges);
     }
   }
-  const uint32_t rs_blocks =
-      a.n_resample ? (uint32_t)std::min<int64_t>(grid_for(max_dyn, AGENT_BLOCK), 4 * std::max(caps().n_cu, 1)) : 0u;
+  *rs_blocks = a.n_resample ? (uint32_t)std::min<int64_t>(grid_for((max_dyn + 1) / 2, AGENT_BLOCK),
+                                                          4 * std::max(caps().n_cu, 1))
+                            : 0u;
+  return CVK_OK;
+}
+
+static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
+                            cvk_work* work, int32_t t, int prologue, int transitions, int resample, hipStream_t st,
+                            DayReduce* reduce_out) {
+  DayBeginArgs a;
+  memset(&a, 0, sizeof(a));
+  a.t = t;
+  a.prologue = prologue;
+  a.transitions = transitions;
+  a.agent_blocks = agent_blocks_for(people->n);
+  uint32_t rs_blocks = 0;
+  int rc = fill_resample(a, layers, params, resample, &rs_blocks);
+  if (rc) return rc;
   hipLaunchKernelGGL(day_begin_kernel, dim3(a.agent_blocks + rs_blocks), dim3(AGENT_BLOCK), 0, st, *people, *params,
                      *work, a);
   reduce_out->n_blocks = a.agent_blocks;
@@ -498,6 +525,9 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       return CVK_OK;
     case CVK_TUNE_GENERATE_DYNAMIC:
       g_gen_dynamic = value ? 1 : 0;
+      return CVK_OK;
+    case CVK_TUNE_OVERLAP_RESAMPLE:
+      g_overlap_resample = value ? 1 : 0;
       return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
@@ -744,9 +774,11 @@ extern "C" int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, 
 
 // partial_blocks (fused day loop): non-null = the flow counts go to the infect partial rows, whose
 // count of workgroups is stored there for the next reduction; null = atomics on the day row.
+// resample_t >= 0: the launch also resamples the dynamic layers for day resample_t (extra workgroups);
+// the caller sets generate so that no source lookup reads a layer being rewritten.
 static int infect_impl(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
                        int32_t t, int32_t count_flows, int32_t allow_beds, hipStream_t st, bool generate = false,
-                       uint32_t* partial_blocks = nullptr) {
+                       uint32_t* partial_blocks = nullptr, int32_t resample_t = -1) {
   InfectArgs ia;
   memset(&ia, 0, sizeof(ia));
   for (int l = 0; l < params->n_layers; ++l) {
@@ -761,7 +793,19 @@ static int infect_impl(const cvk_people* people, const cvk_layer* layers, const 
   const int grid = std::min(std::max(caps().n_cu, 1), INFECT_PART_BLOCKS);
   ia.partial = partial_blocks && count_flows ? 1 : 0;
   if (partial_blocks) *partial_blocks = ia.partial ? (uint32_t)grid : 0u;
-  hipLaunchKernelGGL(infect_pass_kernel, dim3(grid), dim3(INFECT_BLOCK), 0, st, *people, *params, *work, ia);
+  ia.infect_blocks = (uint32_t)grid;
+  uint32_t rs_blocks = 0;
+  if (resample_t >= 0) {
+    DayBeginArgs ra;
+    memset(&ra, 0, sizeof(ra));
+    int rc = fill_resample(ra, layers, params, 1, &rs_blocks);
+    if (rc) return rc;
+    ia.rs_t = resample_t;
+    ia.n_resample = ra.n_resample;
+    for (int r = 0; r < ra.n_resample; ++r) ia.rs[r] = ra.rs[r];
+  }
+  hipLaunchKernelGGL(infect_pass_kernel, dim3(grid + rs_blocks), dim3(INFECT_BLOCK), 0, st, *people, *params, *work,
+                     ia);
   return check_launch("cvk_infect_pass");
 }
 
@@ -773,6 +817,12 @@ extern "C" int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers
   return infect_impl(people, layers, params, work, t, count_flows, count_flows, (hipStream_t)stream);
 }
 
+static bool has_dynamic(const cvk_layer* layers, const cvk_params* params) {
+  for (int l = 0; l < params->n_layers; ++l)
+    if (layers[l].dynamic && layers[l].n_edges > 0) return true;
+  return false;
+}
+
 static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
                          int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, hipStream_t st) {
   int rc;
@@ -780,6 +830,9 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
   bool gen = false;
   for (int l = 0; l < params->n_layers; ++l) gen = gen || (g_gen_dynamic && layers[l].dynamic && layers[l].n_edges > 0);
   uint32_t infect_blocks = 0;  // infect partial rows not yet added to their day's row
+  // Day t+1's resample runs as extra workgroups of day t's infect pass (store-bound work beside
+  // latency-bound work); the infect pass then takes dynamic-layer sources from the resample stream.
+  const bool overlap = !gen && g_overlap_resample && t1 - t0 > 1 && has_dynamic(layers, params);
   for (int32_t t = t0; t < t1; ++t) {
     if (t == 0 && pop_infected > 0) {  // init_infections, sim.py:505-532 (flows not counted)
       rc = choose_impl(people, params, work, 0, STREAM_CHOOSE, pop_infected, 0, CVK_LAYER_SEED, 0, st);
@@ -790,7 +843,8 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
     // day t begins with the stocks of day t-1 (already closed if t == t0)
     DayReduce r;
     // the first resample of the run also sets every weight to 1; later ones leave them as they are
-    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, gen ? 0 : (t == t0 ? 2 : 1), st, &r);
+    const int rs_mode = gen ? 0 : (t == t0 ? 2 : (overlap ? 0 : 1));
+    rc = day_begin_launch(people, layers, params, work, t, t > t0 ? 1 : 0, 1, rs_mode, st, &r);
     if (rc) return rc;
     r.infect_blocks = infect_blocks;  // the previous day's flow counts
     r.infect_t = t - 1;
@@ -805,7 +859,9 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
     }
     rc = edge_pass_impl(people, layers, params, work, t, nullptr, st, &r, gen);  // block 0 reduces day_begin's rows
     if (rc) return rc;
-    rc = infect_impl(people, layers, params, work, t, 1, 1, st, gen, &infect_blocks);
+    // tomorrow's resample (once today's edges have been read) with today's infections
+    rc = infect_impl(people, layers, params, work, t, 1, 1, st, gen || overlap, &infect_blocks,
+                     overlap && t + 1 < t1 ? t + 1 : -1);
     if (rc) return rc;
   }
   // close the last day: stocks and NAb update, and (generated layers) the last day's contacts, weights 1

@@ -221,6 +221,10 @@ int cvk_abi_layout(int64_t* out, int32_t n);
  * once, with the last day's contacts, when the run closes its last day; 0 (default) = every day's
  * resample is written to the layer and read back. The results are identical. */
 #define CVK_TUNE_GENERATE_DYNAMIC 4
+/* CVK_TUNE_OVERLAP_RESAMPLE: 1 (default) = inside cvk_run_days / graphs, day t+1's resample of the
+ * dynamic layers runs as extra workgroups of day t's infect pass, and that pass takes dynamic-layer
+ * sources from the resample stream; 0 = each day's resample runs inside that day's agent pass. */
+#define CVK_TUNE_OVERLAP_RESAMPLE 5
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

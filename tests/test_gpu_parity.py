@@ -187,6 +187,9 @@ CONFIGS = {
                                        pars=dict(use_waning=True, beta=0.02,
                                                  variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
                                        tuning=dict(edge_chunks=2, edge_lds=0)),
+    'hybrid_dynamic_resample_in_agent_pass': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                                  pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1)),
+                                                  tuning=dict(overlap_resample=0)),
     'hybrid_dynamic_generated_contacts': dict(pop=dict(n=4000, pop_type='hybrid'),
                                               pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
                                               tuning=dict(generate_dynamic=1)),
@@ -207,7 +210,8 @@ def test_full_sim_matches_oracle(gpu, name):
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
     keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 4),
-                generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0))
+                generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0),
+                overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
