@@ -144,7 +144,8 @@ template <int BLOCK, int K>
 __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchEdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
-  const WaveQ Q = wave_queues(s_mem + a.lds_words + (threadIdx.x >> 6) * EDGE_WAVE_WORDS);
+  __shared__ float s_beta[MAXV];
+  const WaveQ Q = wave_queues(s_mem + a.lds_words + (threadIdx.x >> 6) * EDGE_WAVE_WORDS, s_beta);
   if (a.reduce.active)  // the agent pass's counts, one sim per block
     for (int32_t s = blockIdx.x; s < C.n_sims; s += gridDim.x) batch_reduce_sim(C, a.reduce, s);
   const int64_t first = a.n_tiles * blockIdx.x / gridDim.x, last = a.n_tiles * (blockIdx.x + 1) / gridDim.x;
@@ -163,6 +164,7 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
       for (int j = threadIdx.x; j < (nw >> 2); j += BLOCK) dst[j] = src[j];
     }
     if (threadIdx.x == 0) s_next = 0;
+    if (threadIdx.x < MAXV) s_beta[threadIdx.x] = gmem(a.views)[s].beta_v[threadIdx.x];
     __syncthreads();
     uint32_t n_x = 0;
     int32_t q_seg = -1;

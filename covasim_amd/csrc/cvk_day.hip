@@ -223,12 +223,16 @@ struct EdgeArgs {
 // its waves take the next tile from an LDS counter, so no wave waits for another and the global
 // memory system sees no tile atomics. A wave's exact-test queue is drained when its tiles change layer
 // and at the end.
-template <bool LDS, int K>
+// GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other instantiations do
+// not carry that code.
+template <bool LDS, int K, bool GEN>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
+  __shared__ float s_beta[MAXV];
   const int nbw = LDS ? a.bitmap_words : 0;
-  const WaveQ Q = wave_queues(s_mem + nbw + (threadIdx.x >> 6) * EDGE_WAVE_WORDS);
+  const WaveQ Q = wave_queues(s_mem + nbw + (threadIdx.x >> 6) * EDGE_WAVE_WORDS, s_beta);
+  if (threadIdx.x < MAXV) s_beta[threadIdx.x] = a.sim.beta_v[threadIdx.x];
   // block 0 sums the previous passes' partial rows and, when other workgroups exist, takes no tiles
   const bool reducing = a.reduce.n_blocks || a.reduce.infect_blocks;
   if (blockIdx.x == 0 && reducing) day_reduce(a.work, a.reduce);
@@ -258,7 +262,7 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     WaveTile<K> T;
     T.seg = l;
     T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * K);
-    load_wave_edges<K>(a.sim, a.seg[l], a.t, T);
+    load_wave_edges<K, GEN>(a.sim, a.seg[l], a.t, T);
     if (l != q_seg) {
       if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
       n_x = 0;
@@ -533,7 +537,7 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
 }
-extern "C" int cvk_version(void) { return 3; }
+extern "C" int cvk_version(void) { return 4; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
 extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
@@ -669,10 +673,12 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* fns[4] = {(const void*)edge_pass_kernel<false, 2>, (const void*)edge_pass_kernel<true, 2>,
-                          (const void*)edge_pass_kernel<false, 4>, (const void*)edge_pass_kernel<true, 4>};
+    const void* fns[8] = {(const void*)edge_pass_kernel<false, 2, false>, (const void*)edge_pass_kernel<true, 2, false>,
+                          (const void*)edge_pass_kernel<false, 4, false>, (const void*)edge_pass_kernel<true, 4, false>,
+                          (const void*)edge_pass_kernel<false, 2, true>,  (const void*)edge_pass_kernel<true, 2, true>,
+                          (const void*)edge_pass_kernel<false, 4, true>,  (const void*)edge_pass_kernel<true, 4, true>};
     int64_t b_min = -1;
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < 8; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -754,12 +760,18 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   const int grid = (int)std::min<int64_t>((wave_tiles + EDGE_WAVES - 1) / EDGE_WAVES, (int64_t)std::max(c.n_cu, 1));
   const size_t shmem = (size_t)(use_lds ? lds_bytes : queue_bytes);
   const dim3 g(grid), b(EDGE_BLOCK);
-  const int variant = (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
+  bool any_gen = false;
+  for (int l = 0; l < params->n_layers; ++l) any_gen = any_gen || a.seg[l].gen;
+  const int variant = (any_gen ? 4 : 0) | (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
   switch (variant) {
-    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2>), g, b, shmem, st, a); break;
-    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2>), g, b, shmem, st, a); break;
-    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4>), g, b, shmem, st, a); break;
-    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4>), g, b, shmem, st, a); break;
+    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2, false>), g, b, shmem, st, a); break;
+    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2, false>), g, b, shmem, st, a); break;
+    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4, false>), g, b, shmem, st, a); break;
+    case 3: hipLaunchKernelGGL((edge_pass_kernel<true, 4, false>), g, b, shmem, st, a); break;
+    case 4: hipLaunchKernelGGL((edge_pass_kernel<false, 2, true>), g, b, shmem, st, a); break;
+    case 5: hipLaunchKernelGGL((edge_pass_kernel<true, 2, true>), g, b, shmem, st, a); break;
+    case 6: hipLaunchKernelGGL((edge_pass_kernel<false, 4, true>), g, b, shmem, st, a); break;
+    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4, true>), g, b, shmem, st, a); break;
   }
   return check_launch("cvk_edge_pass");
 }

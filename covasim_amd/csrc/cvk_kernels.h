@@ -578,6 +578,8 @@ __device__ inline float float_rz(double u) {
 }
 
 struct WaveQ {
+  const float* beta;  // the sim's beta_v[MAXV] in LDS (a lane-varying index into the kernel argument would be
+                      // a memory round trip per use, and a register select spills scalar registers)
   uint32_t* cand;  // [EDGE_CQ] lane | q << 6 | c << 8 | dir << 10 | v << 11 | pcode << 14
   uint32_t* xs;    // [EDGE_XQ] source
   uint32_t* xt;    // target
@@ -585,8 +587,9 @@ struct WaveQ {
   uint32_t* xm;    // dir | v << 1 | (edge >> 32) << 4
   uint32_t* xu;    // float_rz(u)
 };
-__device__ inline WaveQ wave_queues(uint32_t* base) {
+__device__ inline WaveQ wave_queues(uint32_t* base, const float* s_beta) {
   WaveQ Q;
+  Q.beta = s_beta;
   Q.cand = base;
   Q.xs = base + EDGE_CQ;
   Q.xt = Q.xs + EDGE_XQ;
@@ -602,18 +605,6 @@ __device__ inline void wave_sync_lds() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// beta_v[v] for a lane-varying v, selected among registers (an indexed read of the kernel argument
-// would be a memory round trip per use).
-__device__ inline float beta_of(const SimView& S, uint32_t v) {
-  float b = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(S.beta_v[0])));
-#pragma unroll
-  for (int j = 1; j < MAXV; ++j) {
-    const float bj = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(S.beta_v[j])));
-    b = v == (uint32_t)j ? bj : b;
-  }
-  return b;
-}
-
 template <int K>
 struct WaveTile {
   int64_t e0;       // first edge of the tile in its layer
@@ -622,14 +613,15 @@ struct WaveTile {
   v4i x[K], y[K];   // lane's edges (c * 64 + lane) * 4 + q of the tile
 };
 
-// The tile's endpoints: loaded, or for a generated dynamic layer computed from the day's resample stream.
-template <int K>
+// The tile's endpoints: loaded, or for a generated dynamic layer (GEN instantiations only) computed
+// from the day's resample stream.
+template <int K, bool GEN = false>
 __device__ inline void load_wave_edges(const SimView& S, const SegView& L, int32_t t, WaveTile<K>& T) {
   constexpr int TILE = 64 * 4 * K;
   const int64_t rem = L.n - T.e0;
   T.n_valid = (int32_t)(rem < TILE ? rem : TILE);
   const uint32_t lane = lane_id();
-  if (L.gen) {  // edges 4i .. 4i+3 of a lane are two resample pairs
+  if (GEN && L.gen) {  // edges 4i .. 4i+3 of a lane are two resample pairs
     const uint32_t thr = resample_thr(S.n_agents);
 #pragma unroll
     for (int c = 0; c < K; ++c) {
@@ -658,7 +650,7 @@ __device__ inline void load_wave_edges(const SimView& S, const SegView& L, int32
 // The reference's float32 test for one candidate slot (utils.py:88-89, 116-123) and the
 // first-occurrence dedup. uf = float_rz(u) of the slot's draw for variant vq.
 __device__ inline void exact_test(const SimView& S, const SegView& L, int32_t t, int32_t src, int32_t tgt, int64_t e,
-                                  uint32_t dir, uint32_t vq, float uf) {
+                                  uint32_t dir, uint32_t vq, float uf, const float* s_beta) {
   const uint64_t rsrc = gmem(S.rec)[src];
   const uint64_t rtgt = gmem(S.rec)[tgt];
   const uint32_t fs = (uint32_t)(rsrc >> 32);
@@ -676,7 +668,7 @@ __device__ inline void exact_test(const SimView& S, const SegView& L, int32_t t,
   if (ft & CVK_QUARANTINED) rs = rs * L.quar_factor;
   if (S.use_sus_imm) rs = rs * (1.0f - gmem(S.sus_imm)[(int64_t)v * S.sus_stride + tgt]);
   const float eb = L.beta_ones ? 1.0f : gmem(L.beta)[e];
-  const float p = ((beta_of(S, v) * eb) * rt) * rs;              // utils.py:117
+  const float p = ((s_beta[v] * eb) * rt) * rs;                  // utils.py:117
   if (p == 0.0f) return;                                         // utils.py:118
   const bool hit = v == vq ? uf < p : edge_uniform(S, L, t, v, (int)dir, e) < (double)p;  // utils.py:123
   if (!hit) return;
@@ -696,7 +688,7 @@ __device__ inline void flush_exact(const SimView& S, const SegView& L, int32_t t
     const uint32_t m = Q.xm[lane];
     const int64_t e = (int64_t)Q.xe[lane] | ((int64_t)(m >> 4) << 32);
     exact_test(S, L, t, (int32_t)Q.xs[lane], (int32_t)Q.xt[lane], e, m & 1u, (m >> 1) & 7u,
-               __uint_as_float(Q.xu[lane]));
+               __uint_as_float(Q.xu[lane]), Q.beta);
   }
   wave_sync_lds();
 }
@@ -740,7 +732,7 @@ __device__ inline void candidate_round(const SimView& S, const SegView& L, int32
   const int32_t src = dir ? yv : xv, tgt = dir ? xv : yv;
   const uint32_t k = (uint32_t)__popcll(m);
   if (k > (uint32_t)EDGE_XQ) {
-    if (pass) exact_test(S, L, t, src, tgt, e, dir, vq, float_rz(u));
+    if (pass) exact_test(S, L, t, src, tgt, e, dir, vq, float_rz(u), Q.beta);
     return;
   }
   if (n_x + k > (uint32_t)EDGE_XQ) {
@@ -816,7 +808,7 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
       if (((mixed >> j) & 1u) && vq < (uint32_t)S.n_variants && bt[j] != 0u) {
         uint32_t pc = PCODE_INF;  // weighted layers: per-edge beta is not in the bound
         if (L.beta_ones) {
-          const float bv = beta_of(S, vq);
+          const float bv = Q.beta[vq];
           const float p_ub = (((bv * L.beta_layer) * bound_decode(bs[j])) * bound_decode(bt[j])) * BOUND_MARGIN;
           // p_ub == 0 with nonzero factors can only be underflow: leave such slots to the exact test
           pc = p_ub >= 0x1p-100f ? pcode_up(p_ub) : ((bv == 0.0f || L.beta_layer == 0.0f) ? 0u : PCODE_INF);
