@@ -35,6 +35,7 @@ TUNE_EDGE_LDS = 1
 TUNE_EDGE_CHUNKS = 2
 TUNE_GENERATE_DYNAMIC = 4
 TUNE_OVERLAP_RESAMPLE = 5
+TUNE_EDGE_COARSE = 6
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15
@@ -107,14 +108,14 @@ class LayerS(C.Structure):
 class WorkS(C.Structure):
     _fields_ = [('winner', C.c_void_p), ('touched', C.c_void_p), ('bitmap', C.c_void_p), ('ctl', C.c_void_p),
                 ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
-                ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('susmap', C.c_void_p),
+                ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('coarse', C.c_void_p),
                 ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p), ('bound', C.c_void_p),
                 ('npts', C.c_int32), ('pad', C.c_int32)]
 
 
 MAX_REDUCE_BLOCKS = 65536
 PARTIAL_COLS = 64
-WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'susmap',
+WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'coarse',
                    'rec', 'partials', 'partials_f', 'bound']
 
 
@@ -126,7 +127,7 @@ def make_workspace(n, npts, log_capacity, device):
         winner=torch.full((n,), -1, dtype=torch.int64, device=device),
         touched=torch.zeros((n + 31) // 32, dtype=torch.int32, device=device),
         bitmap=torch.zeros(words, dtype=torch.int32, device=device),
-        susmap=torch.zeros(words, dtype=torch.int32, device=device),
+        coarse=torch.zeros(words, dtype=torch.int32, device=device),
         rec=torch.zeros(n, dtype=torch.int64, device=device),
         ctl=torch.zeros(CTL_WORDS, dtype=torch.int32, device=device),
         results=torch.zeros((npts, R_COLS), dtype=torch.int32, device=device),
@@ -148,7 +149,7 @@ def make_workspace(n, npts, log_capacity, device):
 def reset_workspace(w):
     """Clear a workspace for a fresh run (device-side fills, no host copies)."""
     w['winner'].fill_(-1)
-    for k in ('touched', 'bitmap', 'susmap', 'rec', 'ctl', 'results', 'results_f', 'select_hist'):
+    for k in ('touched', 'bitmap', 'coarse', 'rec', 'ctl', 'results', 'results_f', 'select_hist'):
         w[k].zero_()
 
 

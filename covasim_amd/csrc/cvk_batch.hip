@@ -47,7 +47,6 @@ struct BatchCommon {
   uint64_t* rec;
   uint16_t* bound;
   uint32_t* bitmap;
-  uint32_t* susmap;
   uint64_t* winner;
   uint32_t* touched;     // bitmap over storage indices: targets holding a key
   uint32_t* ctl;         // global counters (CVK_CTL_* indices)
@@ -93,7 +92,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
   S.n_local = B.n;
   S.n_stride = C.n_total;
   S.bitmap = (uint64_t*)(C.bitmap + B.bm_off);
-  S.susmap = (uint64_t*)(C.susmap + B.bm_off);
+  S.coarse = nullptr;  // the batched edge pass stages sims' whole bitmaps
   S.rec = C.rec + B.a_off;
   S.bound = C.bound + B.a_off;
   S.partials = C.partials;
@@ -186,8 +185,8 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
         L = uniform_copy(a.segs[seg]);
       }
       load_wave_edges<K>(S, L, a.t, T);
-      if (in_lds) wave_tile_pass<true, K>(S, L, a.t, s_mem, Q, T, n_x);
-      else wave_tile_pass<false, K>(S, L, a.t, S.bitmap, Q, T, n_x);
+      if (in_lds) wave_tile_pass<BITS_LDS, K>(S, L, a.t, s_mem, Q, T, n_x);
+      else wave_tile_pass<BITS_L2, K>(S, L, a.t, nullptr, Q, T, n_x);
     }
     if (n_x) flush_exact(S, L, a.t, Q, n_x);
     tile0 = sub_last;
@@ -486,7 +485,6 @@ struct Batch {
   uint64_t* d_rec = nullptr;
   uint16_t* d_bound = nullptr;
   uint32_t* d_bitmap = nullptr;
-  uint32_t* d_susmap = nullptr;
   uint64_t* d_winner = nullptr;
   uint32_t* d_touched = nullptr;
   uint32_t* d_ctl = nullptr;
@@ -505,7 +503,7 @@ struct Batch {
 
   ~Batch() {
     void* ptrs[] = {d_sims, d_params, d_segs, d_views, d_a_off, d_seg_tile_start, d_tile_seg, d_block_sim,
-                    d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bound, d_bitmap, d_susmap, d_winner, d_touched, d_ctl,
+                    d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bound, d_bitmap, d_winner, d_touched, d_ctl,
                     d_results, d_results_f, d_partials, d_partials_f, d_log, d_hist, d_prefix, d_remain,
                     d_all, d_short, d_k};
     for (void* p : ptrs)
@@ -530,7 +528,6 @@ struct Batch {
     c.rec = d_rec;
     c.bound = d_bound;
     c.bitmap = d_bitmap;
-    c.susmap = d_susmap;
     c.winner = d_winner;
     c.touched = d_touched;
     c.ctl = d_ctl;
@@ -901,7 +898,6 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
   rc = rc ? rc : dev_alloc(&b->d_rec, b->N);
   rc = rc ? rc : dev_alloc(&b->d_bound, b->N);
   rc = rc ? rc : dev_alloc(&b->d_bitmap, words);
-  rc = rc ? rc : dev_alloc(&b->d_susmap, words);
   rc = rc ? rc : dev_alloc(&b->d_winner, b->N);
   rc = rc ? rc : dev_alloc(&b->d_touched, (b->N + 31) / 32);
   rc = rc ? rc : dev_alloc(&b->d_ctl, CVK_CTL_WORDS);
@@ -929,7 +925,7 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
     V.sus_imm = P.sus_imm + bs.a_off;
     V.sus_stride = b->N;
     V.bitmap = b->d_bitmap + bs.bm_off;
-    V.susmap = b->d_susmap + bs.bm_off;
+    V.coarse = nullptr;
     V.touched = b->d_touched;
     V.a_off = bs.a_off;
     V.seed = p.seed;

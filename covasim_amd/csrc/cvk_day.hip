@@ -56,6 +56,7 @@ struct DayBeginArgs {
 };
 
 constexpr int AGENTS_PER_THREAD = 2;
+static_assert(AGENT_BLOCK * AGENTS_PER_THREAD == 512, "one coarse-map word per agent-pass workgroup");
 
 __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cvk_params prm, cvk_work W,
                                                                 DayBeginArgs a) {
@@ -76,7 +77,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   S.n_local = P.n;
   S.n_stride = P.n;
   S.bitmap = (uint64_t*)W.bitmap;
-  S.susmap = (uint64_t*)W.susmap;
+  S.coarse = W.coarse;  // one word per block: AGENT_BLOCK * AGENTS_PER_THREAD == 512
   S.rec = W.rec;
   S.bound = W.bound;
   S.partials = W.partials;
@@ -212,7 +213,7 @@ struct EdgeArgs {
   SegView seg[CVK_MAX_LAYERS];
   int64_t wave_tile_start[CVK_MAX_LAYERS + 1];  // wave tiles (64 x 4 x K edges) per layer, prefix
   int32_t n_layers;
-  int32_t bitmap_words;
+  int32_t lds_words;  // words of the LDS map (fine bitmap or coarse map, a multiple of 4)
   int32_t t;
   int32_t pad;
   cvk_work work;     // for the fused day reduction
@@ -223,27 +224,28 @@ struct EdgeArgs {
 // its waves take the next tile from an LDS counter, so no wave waits for another and the global
 // memory system sees no tile atomics. A wave's exact-test queue is drained when its tiles change layer
 // and at the end.
-// GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other instantiations do
-// not carry that code.
-template <bool LDS, int K, bool GEN>
+// MODE: BITS_LDS / BITS_COARSE / BITS_L2 (the LDS map is a.lds_words words of the fine bitmap or of
+// the coarse map). GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other
+// instantiations do not carry that code.
+template <int MODE, int K, bool GEN>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
   __shared__ float s_beta[MAXV];
-  const int nbw = LDS ? a.bitmap_words : 0;
+  const int nbw = MODE == BITS_L2 ? 0 : a.lds_words;
   const WaveQ Q = wave_queues(s_mem + nbw + (threadIdx.x >> 6) * EDGE_WAVE_WORDS, s_beta);
   if (threadIdx.x < MAXV) s_beta[threadIdx.x] = a.sim.beta_v[threadIdx.x];
   // block 0 sums the previous passes' partial rows and, when other workgroups exist, takes no tiles
   const bool reducing = a.reduce.n_blocks || a.reduce.infect_blocks;
   if (blockIdx.x == 0 && reducing) day_reduce(a.work, a.reduce);
   if (threadIdx.x == 0) s_next = 0;
-  if (LDS) {
-    const uint4* src = (const uint4*)a.sim.bitmap;
+  if (MODE != BITS_L2) {
+    const uint4* src = (const uint4*)(MODE == BITS_LDS ? a.sim.bitmap : a.sim.coarse);
     uint4* dst = (uint4*)s_mem;
     for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
   __syncthreads();
-  const uint32_t* bits = LDS ? s_mem : a.sim.bitmap;
+  const uint32_t* bits = s_mem;
   const int64_t n_wt = a.wave_tile_start[a.n_layers];
   const uint32_t skip = reducing && gridDim.x > 1 ? 1u : 0u;
   const int64_t workers = (int64_t)(gridDim.x - skip);
@@ -268,10 +270,28 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
       n_x = 0;
       q_seg = l;
     }
-    wave_tile_pass<LDS, K>(a.sim, a.seg[l], a.t, bits, Q, T, n_x);
+    wave_tile_pass<MODE, K>(a.sim, a.seg[l], a.t, bits, Q, T, n_x);
   }
   if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
 }
+
+// Launchers of every instantiation, indexed [mode][K == 4][GEN].
+template <int MODE, int K, bool GEN>
+static void edge_launch(dim3 g, dim3 b, size_t shmem, hipStream_t st, const EdgeArgs& a) {
+  hipLaunchKernelGGL((edge_pass_kernel<MODE, K, GEN>), g, b, shmem, st, a);
+}
+typedef void (*EdgeLaunchFn)(dim3, dim3, size_t, hipStream_t, const EdgeArgs&);
+#define CVK_EDGE_ROW(M)                                                                                  \
+  {{edge_launch<M, 2, false>, edge_launch<M, 2, true>}, {edge_launch<M, 4, false>, edge_launch<M, 4, true>}}
+static const EdgeLaunchFn kEdgeLaunch[3][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                                  CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_ROW
+#define CVK_EDGE_ROW(M)                                                                          \
+  {{(const void*)edge_pass_kernel<M, 2, false>, (const void*)edge_pass_kernel<M, 2, true>},     \
+   {(const void*)edge_pass_kernel<M, 4, false>, (const void*)edge_pass_kernel<M, 4, true>}}
+static const void* const kEdgeFns[3][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                              CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_ROW
 
 // ===========================================================================
 // infect_pass
@@ -391,7 +411,8 @@ __global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P,
 static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
 static int g_edge_chunks = 4;  // tunable: 4-edge chunks per lane per wave tile (2 or 4)
 static int g_gen_dynamic = 0;  // tunable: the fused day loop computes dynamic layers' endpoints in place
-static int g_overlap_resample = 1;  // tunable: the fused day loop resamples on a side branch
+static int g_overlap_resample = 1;  // tunable: the fused day loop resamples inside the previous infect pass
+static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
 
 struct DeviceCaps {
   int device = -1;
@@ -419,6 +440,7 @@ static int grid_for(int64_t n, int block) {
 }
 
 static int64_t bitmap_words_for(int64_t n) { return ((n + 127) / 128) * 4; }  // multiple of 4 words
+static int64_t coarse_words_for(int64_t n) { return ((n + 2047) / 2048) * 4; }  // 512 agents a word, multiple of 4
 
 static bool people_ok(const cvk_people* P) {
   return P && P->n > 0 && P->flags && P->vstate && P->date_exposed && P->date_infectious && P->date_symptomatic &&
@@ -431,7 +453,7 @@ static bool people_ok(const cvk_people* P) {
 
 static bool work_ok(const cvk_work* W) {
   return W && W->winner && W->touched && W->bitmap && W->ctl && W->results && W->results_f && W->select_hist &&
-         W->susmap && W->rec && W->partials && W->partials_f && W->bound && W->npts > 0 &&
+         W->coarse && W->rec && W->partials && W->partials_f && W->bound && W->npts > 0 &&
          (W->log_capacity == 0 || W->log);
 }
 
@@ -447,8 +469,8 @@ static int validate(const cvk_people* P, const cvk_params* prm, const cvk_work* 
   if (prm->use_waning && (!prm->nab_kin || prm->nab_kin_len <= 0))
     return fail(CVK_E_INVALID, "use_waning requires a nab_kin table");
   if (!work_ok(W)) return fail(CVK_E_INVALID, "work struct has a null pointer");
-  if (((uintptr_t)W->bitmap | (uintptr_t)W->susmap | (uintptr_t)W->rec) & 15)
-    return fail(CVK_E_ALIGN, "bitmap, susmap and rec must be 16-byte aligned");
+  if (((uintptr_t)W->bitmap | (uintptr_t)W->coarse | (uintptr_t)W->rec) & 15)
+    return fail(CVK_E_ALIGN, "bitmap, coarse and rec must be 16-byte aligned");
   if (t < 0 || t >= W->npts) return fail(CVK_E_INVALID, "day index outside the results table");
   return CVK_OK;
 }
@@ -532,6 +554,9 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       return CVK_OK;
     case CVK_TUNE_OVERLAP_RESAMPLE:
       g_overlap_resample = value ? 1 : 0;
+      return CVK_OK;
+    case CVK_TUNE_EDGE_COARSE:
+      g_edge_coarse = value ? 1 : 0;
       return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
@@ -673,12 +698,9 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* fns[8] = {(const void*)edge_pass_kernel<false, 2, false>, (const void*)edge_pass_kernel<true, 2, false>,
-                          (const void*)edge_pass_kernel<false, 4, false>, (const void*)edge_pass_kernel<true, 4, false>,
-                          (const void*)edge_pass_kernel<false, 2, true>,  (const void*)edge_pass_kernel<true, 2, true>,
-                          (const void*)edge_pass_kernel<false, 4, true>,  (const void*)edge_pass_kernel<true, 4, true>};
+    const void* const* fns = &kEdgeFns[0][0][0];
     int64_t b_min = -1;
-    for (int j = 0; j < 8; ++j) {
+    for (int j = 0; j < 12; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -711,7 +733,6 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   S.sus_imm = people->sus_imm;
   S.sus_stride = people->n;
   S.bitmap = work->bitmap;
-  S.susmap = work->susmap;
   S.touched = work->touched;
   S.a_off = 0;
   S.seed = params->seed;
@@ -722,16 +743,19 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   S.vl_hi = params->vl_hi;
   S.vl_lo = params->vl_lo;
   for (int v = 0; v < MAXV; ++v) S.beta_v[v] = params->beta_v[v];
-  a.bitmap_words = (int32_t)bitmap_words_for(people->n);
+  S.coarse = work->coarse;
   a.t = t;
 
   DeviceCaps& c = caps();
   const int64_t queue_bytes = (int64_t)EDGE_WAVES * EDGE_WAVE_WORDS * 4;
-  const int64_t lds_bytes = (int64_t)a.bitmap_words * 4 + queue_bytes;
   int64_t budget = 0;
   int rc0 = edge_dynamic_lds(&budget);
   if (rc0) return rc0;
-  const bool use_lds = lds_bytes <= budget && g_edge_lds != 0;
+  const int64_t fine_words = bitmap_words_for(people->n), coarse_words = coarse_words_for(people->n);
+  int mode = BITS_L2;
+  if (g_edge_lds && fine_words * 4 + queue_bytes <= budget) mode = BITS_LDS;
+  else if (g_edge_coarse && work->coarse && coarse_words * 4 + queue_bytes <= budget) mode = BITS_COARSE;
+  a.lds_words = (int32_t)(mode == BITS_LDS ? fine_words : mode == BITS_COARSE ? coarse_words : 0);
   const int64_t wave_tile = 64 * 4 * (int64_t)g_edge_chunks;
   int64_t wave_tiles = 0;
   for (int l = 0; l < params->n_layers; ++l) {
@@ -758,21 +782,10 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   if (wave_tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
   // one workgroup per CU (the LDS bitmap takes most of a CU's LDS), at least one wave tile each
   const int grid = (int)std::min<int64_t>((wave_tiles + EDGE_WAVES - 1) / EDGE_WAVES, (int64_t)std::max(c.n_cu, 1));
-  const size_t shmem = (size_t)(use_lds ? lds_bytes : queue_bytes);
-  const dim3 g(grid), b(EDGE_BLOCK);
+  const size_t shmem = (size_t)((int64_t)a.lds_words * 4 + queue_bytes);
   bool any_gen = false;
   for (int l = 0; l < params->n_layers; ++l) any_gen = any_gen || a.seg[l].gen;
-  const int variant = (any_gen ? 4 : 0) | (g_edge_chunks == 4 ? 2 : 0) | (use_lds ? 1 : 0);
-  switch (variant) {
-    case 0: hipLaunchKernelGGL((edge_pass_kernel<false, 2, false>), g, b, shmem, st, a); break;
-    case 1: hipLaunchKernelGGL((edge_pass_kernel<true, 2, false>), g, b, shmem, st, a); break;
-    case 2: hipLaunchKernelGGL((edge_pass_kernel<false, 4, false>), g, b, shmem, st, a); break;
-    case 3: hipLaunchKernelGGL((edge_pass_kernel<true, 4, false>), g, b, shmem, st, a); break;
-    case 4: hipLaunchKernelGGL((edge_pass_kernel<false, 2, true>), g, b, shmem, st, a); break;
-    case 5: hipLaunchKernelGGL((edge_pass_kernel<true, 2, true>), g, b, shmem, st, a); break;
-    case 6: hipLaunchKernelGGL((edge_pass_kernel<false, 4, true>), g, b, shmem, st, a); break;
-    default: hipLaunchKernelGGL((edge_pass_kernel<true, 4, true>), g, b, shmem, st, a); break;
-  }
+  kEdgeLaunch[mode][g_edge_chunks == 4 ? 1 : 0][any_gen ? 1 : 0](dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
   return check_launch("cvk_edge_pass");
 }
 

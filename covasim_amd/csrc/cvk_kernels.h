@@ -163,7 +163,8 @@ struct AgentSpan {
   int64_t n_local;     // agents in the sim
   int64_t n_stride;    // row stride of the [variant][agent] arrays (all agents in storage)
   uint64_t* bitmap;    // the sim's infectious bitmap (64 agents per word, local indexing)
-  uint64_t* susmap;    // the sim's susceptible bitmap
+  uint32_t* coarse;    // one bit per 16 agents (OR of bitmap bits), one word per 512 agents; null: not
+                       // written (needs AGENT_BLOCK * APT == 512 and li0 a multiple of 512)
   uint64_t* rec;       // the sim's edge records (local indexing)
   uint16_t* bound;     // the sim's upper-bound codes (local indexing)
   uint32_t* partials;  // column-major partial rows; this block writes entry `part_col` of each column
@@ -189,6 +190,7 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
   const int64_t n_words = (S.n_local + 63) / 64;
   const bool closing = prologue || !transitions;  // stocks + update_nab of the previous (or this) day
   const int32_t tc = transitions ? t - 1 : t;
+  uint32_t coarse_word = 0;  // this wave's bits of the block's coarse word (wave-uniform)
   float quar_max = 0.0f, iso_max = 0.0f;  // over layers: the bound covers every layer's factor
   for (int l = 0; l < prm.n_layers; ++l) {
     quar_max = fmaxf(quar_max, prm.quar_factor[l]);
@@ -340,22 +342,24 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         if (v < nv) c[DC_BASE + 9 + v] += (uint32_t)__popcll(__ballot(ev_var == (uint32_t)v));
       // one 64-bit ballot per wave covers 64 consecutive agents
       const uint64_t ib = __ballot(inf_bit);
-      const uint64_t sb = __ballot(sus_bit);
       const int64_t word = li >> 6;
-      if (lane_id() == 0 && word < n_words) {
-        S.bitmap[word] = ib;
-        S.susmap[word] = sb;
-      }
+      if (lane_id() == 0 && word < n_words) S.bitmap[word] = ib;
+      // coarse bits of this ballot's four 16-agent groups, at their place in the block's 512 agents
+      const uint32_t nib = (uint32_t)((ib & 0xFFFFull) != 0) | ((uint32_t)(((ib >> 16) & 0xFFFFull) != 0) << 1) |
+                           ((uint32_t)(((ib >> 32) & 0xFFFFull) != 0) << 2) | ((uint32_t)((ib >> 48) != 0) << 3);
+      coarse_word |= nib << (uint32_t)((((li - S.li0) & ~63ll) >> 4) & 31);
     }
   }
 
   // block partial row: per-wave counts through LDS (no atomics), float sums in wave order
   __shared__ uint32_t s_wc[AGENT_BLOCK / 64][N_AGENT_COLS];
   __shared__ double s_wd[AGENT_BLOCK / 64][N_AGENT_FCOLS];
+  __shared__ uint32_t s_coarse[AGENT_BLOCK / 64];
   const int w = threadIdx.x >> 6;
   if (lane_id() == 0) {
 #pragma unroll
     for (int k = 0; k < N_AGENT_COLS; ++k) s_wc[w][k] = c[k];
+    s_coarse[w] = coarse_word;
   }
   if (waning) {
 #pragma unroll
@@ -376,6 +380,11 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
     if (waning)
       for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wd[j][k];
     S.partials_f[(size_t)k * S.part_stride + S.part_col] = tot;
+  } else if (threadIdx.x == 128 && S.coarse && transitions) {
+    uint32_t cw = 0;
+#pragma unroll
+    for (int j = 0; j < AGENT_BLOCK / 64; ++j) cw |= s_coarse[j];
+    S.coarse[S.li0 >> 9] = cw;
   }
 }
 
@@ -491,7 +500,7 @@ struct SimView {
   const float* sus_imm;     // row v at sus_imm + v * sus_stride
   int64_t sus_stride;
   const uint32_t* bitmap;
-  const uint32_t* susmap;
+  const uint32_t* coarse;   // one bit per 16 agents (single-sim day loop; see AgentSpan), or null
   uint32_t* touched;        // bitmap over storage indices (a_off + local): targets holding a key
   int64_t a_off;
   uint64_t seed;
@@ -750,35 +759,53 @@ __device__ inline void candidate_round(const SimView& S, const SegView& L, int32
   n_x += k;
 }
 
-// One wave tile. bits: the infectious bitmap (LDS or global). n_x: the wave's exact-queue fill, whose
-// slots all belong to segment L.
-template <bool LDS, int K>
-__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* bits0,
+// Where the edge pass finds the infectious bits: the fine bitmap staged in LDS; a coarse map in LDS
+// (one bit per 16 agents) with fine words read from L2 only for endpoints whose group has an
+// infectious agent; or the fine bitmap in L2 only.
+constexpr int BITS_LDS = 0, BITS_COARSE = 1, BITS_L2 = 2;
+
+// One wave tile. lds_bits: the LDS map of MODE (fine or coarse; unused for BITS_L2). n_x: the wave's
+// exact-queue fill, whose slots all belong to segment L.
+template <int MODE, int K>
+__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* lds_bits,
                                       const WaveQ& Q, const WaveTile<K>& T, uint32_t& n_x) {
   const uint32_t lane = lane_id();
   const CVK_GLOBAL uint16_t* bound = gmem(S.bound);
+  const CVK_GLOBAL uint32_t* fine = gmem(S.bitmap);
+  // the bitmap words of both endpoints of every edge, all requests in flight before the first use
+  uint32_t okm = 0;  // bit c*4+q: the edge exists and both endpoints are in [0, n)
+  uint32_t wx[K * 4], wy[K * 4];
+#pragma unroll
+  for (int j = 0; j < K * 4; ++j) {
+    const int c = j >> 2, q = j & 3;
+    const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
+    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
+    const bool ok = (c * 64 + (int)lane) * 4 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents;
+    if (ok) okm |= 1u << j;
+    const uint32_t xs = ok ? xi : 0u, ys = ok ? yi : 0u;
+    if (MODE == BITS_LDS) {
+      wx[j] = lds_bits[xs >> 5];
+      wy[j] = lds_bits[ys >> 5];
+    } else if (MODE == BITS_COARSE) {
+      wx[j] = 0u;
+      wy[j] = 0u;
+      if ((lds_bits[xs >> 9] >> ((xs >> 4) & 31)) & 1u) wx[j] = fine[xs >> 5];
+      if ((lds_bits[ys >> 9] >> ((ys >> 4) & 31)) & 1u) wy[j] = fine[ys >> 5];
+    } else {
+      wx[j] = fine[xs >> 5];
+      wy[j] = fine[ys >> 5];
+    }
+  }
   uint32_t mixed = 0, from_x = 0;  // bit c*4+q: exactly one endpoint infectious; it is x (p1)
 #pragma unroll
-  for (int c = 0; c < K; ++c) {
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
-      // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-      const bool ok = (c * 64 + (int)lane) * 4 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents;
-      const uint32_t xs = ok ? xi : 0u, ys = ok ? yi : 0u;
-      uint32_t wx, wy;
-      if (LDS) {
-        wx = bits0[xs >> 5];
-        wy = bits0[ys >> 5];
-      } else {
-        wx = gmem(bits0)[xs >> 5];
-        wy = gmem(bits0)[ys >> 5];
-      }
-      const bool ix = ok && ((wx >> (xs & 31)) & 1u);
-      const bool iy = ok && ((wy >> (ys & 31)) & 1u);
-      if (ix != iy) mixed |= 1u << (c * 4 + q);
-      if (ix) from_x |= 1u << (c * 4 + q);
-    }
+  for (int j = 0; j < K * 4; ++j) {
+    const int c = j >> 2, q = j & 3;
+    const bool ok = (okm >> j) & 1u;
+    const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
+    const bool ix = ok && ((wx[j] >> (xi & 31)) & 1u);
+    const bool iy = ok && ((wy[j] >> (yi & 31)) & 1u);
+    if (ix != iy) mixed |= 1u << j;
+    if (ix) from_x |= 1u << j;
   }
   // bound codes of both endpoints of every mixed edge, all in flight together
   uint32_t bs[K * 4], bt[K * 4];

@@ -180,7 +180,9 @@ typedef struct {
   int32_t* log;          /* [log_capacity][4] infection records: source, target, day, layer | variant<<8 */
   int64_t log_capacity;
   uint32_t* select_hist; /* [256] histogram scratch of cvk_choose */
-  uint32_t* susmap;      /* [bitmap_words] bit i: agent i susceptible with rel_sus != 0 */
+  uint32_t* coarse;      /* [>= 4 * ceil(N / 2048)] bit j of word w: some agent of [512w + 16j, 512w + 16j + 16)
+                            is infectious (OR of bitmap bits); the edge pass stages it in LDS when the
+                            bitmap does not fit there (the caller may allocate bitmap_words) */
   uint64_t* rec;         /* [n] per-agent edge-pass record: low word float (rel_trans if infectious,
                             else rel_sus), high word the flags */
   uint32_t* partials;    /* [CVK_PARTIAL_COLS][CVK_MAX_REDUCE_BLOCKS] per-block counts (column-major) */
@@ -225,6 +227,10 @@ int cvk_abi_layout(int64_t* out, int32_t n);
  * dynamic layers runs as extra workgroups of day t's infect pass, and that pass takes dynamic-layer
  * sources from the resample stream; 0 = each day's resample runs inside that day's agent pass. */
 #define CVK_TUNE_OVERLAP_RESAMPLE 5
+/* CVK_TUNE_EDGE_COARSE: 1 (default) = when the infectious bitmap does not fit in LDS, the edge pass
+ * stages the coarse map there and reads bitmap words from L2 only for endpoints whose 16-agent group
+ * has an infectious agent; 0 = every bitmap word is read from L2. */
+#define CVK_TUNE_EDGE_COARSE 6
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

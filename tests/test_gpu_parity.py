@@ -178,6 +178,7 @@ CONFIGS = {
     'variants_waning': dict(pop=dict(n=4000, pop_type='hybrid'),
                             pars=dict(use_waning=True, beta=0.02,
                                       variants=[dict(rel_beta=1.7, days=[10], n_imports=30)])),
+    # bitmap not staged: the coarse map in LDS filters the L2 reads of bitmap words
     'weighted_edges_l2_bitmap': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
                                      pars=dict(use_waning=True, beta=0.04), tuning=dict(edge_lds=0)),
     'hybrid_dynamic_shallow_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
@@ -186,7 +187,7 @@ CONFIGS = {
     'variants_waning_shallow_l2': dict(pop=dict(n=4000, pop_type='hybrid'),
                                        pars=dict(use_waning=True, beta=0.02,
                                                  variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
-                                       tuning=dict(edge_chunks=2, edge_lds=0)),
+                                       tuning=dict(edge_chunks=2, edge_lds=0, edge_coarse=0)),
     'hybrid_dynamic_resample_in_agent_pass': dict(pop=dict(n=4000, pop_type='hybrid'),
                                                   pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1)),
                                                   tuning=dict(overlap_resample=0)),
@@ -211,7 +212,7 @@ def test_full_sim_matches_oracle(gpu, name):
     variants = pars.pop('variants', [])
     keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 4),
                 generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0),
-                overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1))
+                overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
