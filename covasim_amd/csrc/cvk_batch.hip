@@ -65,23 +65,30 @@ struct BatchDayArgs {
   const SegView* segs;        // [S * L]
 };
 
+// The resample work items [k0, n) with stride `step` (2048-edge chunks of dynamic segments) for day t.
+__device__ inline void batch_resample_chunks(const BatchCommon& C, const SegView* segs, const int32_t* chunk_seg,
+                                             const int64_t* chunk_first, int32_t k0, int32_t n, int32_t step,
+                                             int32_t t, bool weights) {
+  for (int32_t k = k0; k < n; k += step) {
+    const int32_t seg = chunk_seg[k];
+    const int32_t s = seg / C.n_layers;
+    const SegView& L = segs[seg];
+    const int64_t e0 = chunk_first[k];
+    const int64_t e1 = std::min<int64_t>(e0 + RS_CHUNK, L.n);
+    const uint64_t seed = C.params[s].seed;
+    const uint32_t na = (uint32_t)C.sims[s].n;
+    const uint32_t thr = resample_thr(na);
+    for (int64_t q = e0 / 2 + threadIdx.x; q < (e1 + 1) / 2; q += AGENT_BLOCK)  // chunks start at even edges
+      resample_pair(seed, t, (uint32_t)L.l, na, thr, q, L.n, (int32_t*)L.p1, (int32_t*)L.p2,
+                    weights ? (float*)L.beta : nullptr);
+  }
+}
+
 template <int APT>
 __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommon C, BatchDayArgs a) {
-  if (blockIdx.x >= a.agent_blocks) {
-    // resample: one 2048-edge chunk of one dynamic segment per block iteration
-    for (int32_t k = (int32_t)(blockIdx.x - a.agent_blocks); k < a.n_rs_chunks; k += (int32_t)(gridDim.x - a.agent_blocks)) {
-      const int32_t seg = a.rs_chunk_seg[k];
-      const int32_t s = seg / C.n_layers;
-      const SegView& L = a.segs[seg];
-      const int64_t e0 = a.rs_chunk_first[k];
-      const int64_t e1 = std::min<int64_t>(e0 + RS_CHUNK, L.n);
-      const uint64_t seed = C.params[s].seed;
-      const uint32_t n = (uint32_t)C.sims[s].n;
-      const uint32_t thr = resample_thr(n);
-      for (int64_t q = e0 / 2 + threadIdx.x; q < (e1 + 1) / 2; q += AGENT_BLOCK)  // chunks start at even edges
-        resample_pair(seed, a.t, (uint32_t)L.l, n, thr, q, L.n, (int32_t*)L.p1, (int32_t*)L.p2,
-                      a.resample == 2 ? (float*)L.beta : nullptr);
-    }
+  if (blockIdx.x >= a.agent_blocks) {  // resample: one 2048-edge chunk of one dynamic segment per iteration
+    batch_resample_chunks(C, a.segs, a.rs_chunk_seg, a.rs_chunk_first, (int32_t)(blockIdx.x - a.agent_blocks),
+                          a.n_rs_chunks, (int32_t)(gridDim.x - a.agent_blocks), a.t, a.resample == 2);
     return;
   }
   const int32_t s = a.block_sim[blockIdx.x];
@@ -202,9 +209,21 @@ struct BatchInfectArgs {
   int32_t* log;
   int64_t log_capacity;
   int32_t t, count_flows, allow_beds, n_variants;
+  uint32_t infect_blocks;      // workgroups [0, infect_blocks) infect; the rest resample day rs_t
+  uint32_t gen_mask;           // bit l: layer l's sources come from the resample stream of day t
+  int32_t rs_t;
+  int32_t n_rs_chunks;
+  const int32_t* rs_chunk_seg;
+  const int64_t* rs_chunk_first;
 };
 
-__global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon C, BatchInfectArgs ia) {
+// Occupancy hint 3 waves/SIMD, as for the single-sim infect pass: resample workgroups share the kernel.
+__global__ __launch_bounds__(INFECT_BLOCK, 3) void batch_infect_kernel(BatchCommon C, BatchInfectArgs ia) {
+  if (blockIdx.x >= ia.infect_blocks) {  // the next day's resample
+    batch_resample_chunks(C, ia.segs, ia.rs_chunk_seg, ia.rs_chunk_first, (int32_t)(blockIdx.x - ia.infect_blocks),
+                          ia.n_rs_chunks, (int32_t)(gridDim.x - ia.infect_blocks), ia.rs_t, false);
+    return;
+  }
   __shared__ int32_t s_off[CVK_BATCH_MAX_SIMS + 1];  // agent offsets (total agents < 2^31)
   for (int j = threadIdx.x; j <= C.n_sims; j += INFECT_BLOCK) s_off[j] = (int32_t)ia.a_off[j];
   __syncthreads();
@@ -212,7 +231,7 @@ __global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon 
   __shared__ int32_t s_list[INFECT_BLOCK / 64][INFECT_WAVE_LIST];
   int32_t* list = s_list[threadIdx.x >> 6];
   const int64_t n_words = (C.n_total + 31) / 32;
-  const int64_t stride = (int64_t)gridDim.x * INFECT_BLOCK;
+  const int64_t stride = (int64_t)ia.infect_blocks * INFECT_BLOCK;
   const int64_t n_sweeps = (n_words + stride - 1) / stride;
   for (int64_t sw = 0; sw < n_sweeps; ++sw) {
     const int64_t w = sw * stride + (int64_t)blockIdx.x * INFECT_BLOCK + threadIdx.x;
@@ -240,7 +259,8 @@ __global__ __launch_bounds__(INFECT_BLOCK) void batch_infect_kernel(BatchCommon 
         o = infect_target(C.P, prm, C.winner, g, (uint32_t)il, C.n_total, t, hosp_max, icu_max);
         if (o.inf && (int32_t)o.l < C.n_layers) {
           const SegView& L = ia.segs[s * C.n_layers + (int32_t)o.l];
-          source = key_source(o.key, L.p1, L.p2);
+          source = key_source(o.key, L.p1, L.p2, (ia.gen_mask >> o.l) & 1u, prm.seed, t,
+                              (uint32_t)(s_off[s + 1] - s_off[s]));
         }
       }
       // infection log, people.py:509-511 (indices within the sim, sim in bits 12+ of the last word)
@@ -675,7 +695,9 @@ static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t 
   return check_launch("cvk_batch edge_pass");
 }
 
-static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st) {
+// resample_t >= 0: the launch also resamples the dynamic layers for day resample_t (extra workgroups),
+// and the sources of dynamic-layer infections come from the resample stream.
+static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st, int32_t resample_t = -1) {
   BatchInfectArgs ia;
   memset(&ia, 0, sizeof(ia));
   ia.a_off = b.d_a_off;
@@ -686,7 +708,19 @@ static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st) {
   ia.count_flows = count_flows;
   ia.allow_beds = count_flows;
   ia.n_variants = b.V;
-  hipLaunchKernelGGL(batch_infect_kernel, dim3(std::max(1, b.n_cu)), dim3(INFECT_BLOCK), 0, st, b.common(), ia);
+  ia.infect_blocks = (uint32_t)std::max(1, b.n_cu);
+  uint32_t rs_blocks = 0;
+  if (resample_t >= 0 && b.n_rs_chunks > 0) {
+    ia.rs_t = resample_t;
+    ia.n_rs_chunks = (int32_t)b.n_rs_chunks;
+    ia.rs_chunk_seg = b.d_rs_chunk_seg;
+    ia.rs_chunk_first = b.d_rs_chunk_first;
+    for (int l = 0; l < b.L; ++l)
+      if (b.dynamic[l]) ia.gen_mask |= 1u << l;
+    rs_blocks = (uint32_t)std::min<int64_t>(b.n_rs_chunks, 8 * b.n_cu);
+  }
+  hipLaunchKernelGGL(batch_infect_kernel, dim3(ia.infect_blocks + rs_blocks), dim3(INFECT_BLOCK), 0, st, b.common(),
+                     ia);
   return check_launch("cvk_batch infect_pass");
 }
 
@@ -749,6 +783,8 @@ static int upload_counts(Batch& b, const int64_t* pop_infected, const cvk_batch_
 
 static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk_batch_imports* imp, hipStream_t st) {
   int rc;
+  // as in the single-sim loop: day t+1's resample runs as extra workgroups of day t's infect pass
+  const bool overlap = overlap_resample_enabled() && b.n_rs_chunks > 0 && t1 - t0 > 1;
   for (int32_t t = t0; t < t1; ++t) {
     if (t == 0 && seeds) {
       rc = launch_choose(b, 0, 0, STREAM_CHOOSE, 0, CVK_LAYER_SEED, 0, st);
@@ -756,7 +792,7 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       rc = launch_infect(b, 0, 0, st);
       if (rc) return rc;
     }
-    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : 1, st);  // weights set on the first resample
+    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : (overlap ? 0 : 1), st);  // weights: first resample
     if (rc) return rc;
     if (imp) {
       for (int32_t j = 0; j < imp->n_events; ++j) {
@@ -769,7 +805,7 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
     BatchReduce red{1, t, t > t0 ? 1 : 0, 1};
     rc = launch_edge(b, t, red, st);
     if (rc) return rc;
-    rc = launch_infect(b, t, 1, st);
+    rc = launch_infect(b, t, 1, st, overlap && t + 1 < t1 ? t + 1 : -1);
     if (rc) return rc;
   }
   rc = launch_day_begin(b, t1 - 1, 0, 0, 0, st);

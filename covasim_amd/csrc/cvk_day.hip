@@ -311,8 +311,10 @@ struct InfectArgs {
   ResampleLayer rs[CVK_MAX_LAYERS];
 };
 
-__global__ __launch_bounds__(INFECT_BLOCK) void infect_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
-                                                                   InfectArgs ia) {
+// Occupancy hint 3 waves/SIMD (168 VGPRs, a few spills in the latency-bound outcome tree): the store-bound
+// resample workgroups sharing this kernel need the occupancy more than the infect lanes need registers.
+__global__ __launch_bounds__(INFECT_BLOCK, 3) void infect_pass_kernel(cvk_people P, cvk_params prm, cvk_work W,
+                                                                      InfectArgs ia) {
   if (blockIdx.x >= ia.infect_blocks) {  // the next day's resample, one thread per pair of edges
     const int64_t stride = (int64_t)(gridDim.x - ia.infect_blocks) * INFECT_BLOCK;
     const int64_t start = (int64_t)(blockIdx.x - ia.infect_blocks) * INFECT_BLOCK + threadIdx.x;
@@ -413,6 +415,7 @@ static int g_edge_chunks = 4;  // tunable: 4-edge chunks per lane per wave tile 
 static int g_gen_dynamic = 0;  // tunable: the fused day loop computes dynamic layers' endpoints in place
 static int g_overlap_resample = 1;  // tunable: the fused day loop resamples inside the previous infect pass
 static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
+int overlap_resample_enabled() { return g_overlap_resample; }
 
 struct DeviceCaps {
   int device = -1;

@@ -11,6 +11,9 @@ namespace cvk {
 // Thread-local message of the last failure (returned by cvk_last_error()).
 char* last_error_buffer();
 
+// CVK_TUNE_OVERLAP_RESAMPLE as set by cvk_set_tuning (read by the batched day loop too).
+int overlap_resample_enabled();
+
 inline int fail(int code, const char* msg) {
   snprintf(last_error_buffer(), 512, "%s", msg);
   return code;
