@@ -272,6 +272,8 @@ def main():
     ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
     ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass wave tile depth (2 or 4 chunks of 4 edges per lane)')
     ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
+    ap.add_argument('--edge-prefetch', type=int, default=None,
+                    help='1: issue the next wave tile\'s endpoint loads before this tile\'s candidates')
     ap.add_argument('--edge-coarse', type=int, default=None,
                     help='1: with the bitmap in L2, filter its reads by a coarse map in LDS; 0: no filter')
     ap.add_argument('--overlap-resample', type=int, default=None,
@@ -303,6 +305,9 @@ def main():
     if args.edge_chunks is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_CHUNKS, args.edge_chunks), 'tuning')
         tuning['edge_chunks'] = args.edge_chunks
+    if args.edge_prefetch is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_PREFETCH, args.edge_prefetch), 'tuning')
+        tuning['edge_prefetch'] = args.edge_prefetch
     if args.edge_coarse is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_COARSE, args.edge_coarse), 'tuning')
         tuning['edge_coarse'] = args.edge_coarse

@@ -36,6 +36,7 @@ TUNE_EDGE_CHUNKS = 2
 TUNE_GENERATE_DYNAMIC = 4
 TUNE_OVERLAP_RESAMPLE = 5
 TUNE_EDGE_COARSE = 6
+TUNE_EDGE_PREFETCH = 7
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15

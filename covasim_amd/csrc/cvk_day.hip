@@ -226,8 +226,9 @@ struct EdgeArgs {
 // and at the end.
 // MODE: BITS_LDS / BITS_COARSE / BITS_L2 (the LDS map is a.lds_words words of the fine bitmap or of
 // the coarse map). GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other
-// instantiations do not carry that code.
-template <int MODE, int K, bool GEN>
+// instantiations do not carry that code. PF: the next tile's endpoint loads are issued before this
+// tile's candidates are processed.
+template <int MODE, int K, bool GEN, bool PF>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
@@ -253,44 +254,63 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   const int64_t first = wb < 0 ? 0 : n_wt * wb / workers, last = wb < 0 ? 0 : n_wt * (wb + 1) / workers;
   uint32_t n_x = 0;
   int32_t q_seg = 0;
-  for (;;) {
+  // the wave's next tile from the block's counter: its layer and first edge (seg -1: none left)
+  auto grab = [&](WaveTile<K>& T) {
     uint32_t k = 0;
     if (lane_id() == 0) k = atomicAdd(&s_next, 1u);
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
     const int64_t tile = first + k;
-    if (tile >= last) break;
+    T.seg = -1;
+    if (tile >= last) return;
     int l = 0;
     while (l + 1 < a.n_layers && tile >= a.wave_tile_start[l + 1]) ++l;
-    WaveTile<K> T;
     T.seg = l;
     T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * K);
-    load_wave_edges<K, GEN>(a.sim, a.seg[l], a.t, T);
+  };
+  WaveTile<K> cur, nxt;
+  grab(cur);
+  if (cur.seg >= 0) load_wave_edges<K, GEN>(a.sim, a.seg[cur.seg], a.t, cur);
+  while (cur.seg >= 0) {
+    const int l = cur.seg;
     if (l != q_seg) {
       if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
       n_x = 0;
       q_seg = l;
     }
-    wave_tile_pass<MODE, K>(a.sim, a.seg[l], a.t, bits, Q, T, n_x);
+    TileBounds<K> B;
+    tile_bits_and_bounds<MODE, K>(a.sim, bits, cur, B);
+    // the next tile's endpoints are issued after this tile's bound loads (loads complete in order, so
+    // waiting for the bounds does not wait for them) and arrive while the candidates are processed
+    grab(nxt);
+    if (PF && nxt.seg >= 0) load_wave_edges<K, GEN>(a.sim, a.seg[nxt.seg], a.t, nxt);
+    tile_candidates<K>(a.sim, a.seg[l], a.t, Q, cur, B, n_x);
+    if (!PF && nxt.seg >= 0) load_wave_edges<K, GEN>(a.sim, a.seg[nxt.seg], a.t, nxt);
+    cur = nxt;
   }
   if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
 }
 
-// Launchers of every instantiation, indexed [mode][K == 4][GEN].
-template <int MODE, int K, bool GEN>
+// Launchers of every instantiation, indexed [mode][K == 4][GEN][PF].
+template <int MODE, int K, bool GEN, bool PF>
 static void edge_launch(dim3 g, dim3 b, size_t shmem, hipStream_t st, const EdgeArgs& a) {
-  hipLaunchKernelGGL((edge_pass_kernel<MODE, K, GEN>), g, b, shmem, st, a);
+  hipLaunchKernelGGL((edge_pass_kernel<MODE, K, GEN, PF>), g, b, shmem, st, a);
 }
 typedef void (*EdgeLaunchFn)(dim3, dim3, size_t, hipStream_t, const EdgeArgs&);
-#define CVK_EDGE_ROW(M)                                                                                  \
-  {{edge_launch<M, 2, false>, edge_launch<M, 2, true>}, {edge_launch<M, 4, false>, edge_launch<M, 4, true>}}
-static const EdgeLaunchFn kEdgeLaunch[3][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
-                                                  CVK_EDGE_ROW(BITS_L2)};
+#define CVK_EDGE_K(M, K) \
+  {{edge_launch<M, K, false, false>, edge_launch<M, K, false, true>}, \
+   {edge_launch<M, K, true, false>, edge_launch<M, K, true, true>}}
+#define CVK_EDGE_ROW(M) {CVK_EDGE_K(M, 2), CVK_EDGE_K(M, 4)}
+static const EdgeLaunchFn kEdgeLaunch[3][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                                     CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_K
 #undef CVK_EDGE_ROW
-#define CVK_EDGE_ROW(M)                                                                          \
-  {{(const void*)edge_pass_kernel<M, 2, false>, (const void*)edge_pass_kernel<M, 2, true>},     \
-   {(const void*)edge_pass_kernel<M, 4, false>, (const void*)edge_pass_kernel<M, 4, true>}}
-static const void* const kEdgeFns[3][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
-                                              CVK_EDGE_ROW(BITS_L2)};
+#define CVK_EDGE_K(M, K)                                                                                        \
+  {{(const void*)edge_pass_kernel<M, K, false, false>, (const void*)edge_pass_kernel<M, K, false, true>},     \
+   {(const void*)edge_pass_kernel<M, K, true, false>, (const void*)edge_pass_kernel<M, K, true, true>}}
+#define CVK_EDGE_ROW(M) {CVK_EDGE_K(M, 2), CVK_EDGE_K(M, 4)}
+static const void* const kEdgeFns[3][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                                 CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_K
 #undef CVK_EDGE_ROW
 
 // ===========================================================================
@@ -411,10 +431,11 @@ __global__ __launch_bounds__(INFECT_BLOCK, 3) void infect_pass_kernel(cvk_people
 // host side
 // ===========================================================================
 static int g_edge_lds = 1;     // tunable: stage the infectious bitmap in LDS when it fits
-static int g_edge_chunks = 4;  // tunable: 4-edge chunks per lane per wave tile (2 or 4)
+static int g_edge_chunks = 2;  // tunable: 4-edge chunks per lane per wave tile (2 or 4)
 static int g_gen_dynamic = 0;  // tunable: the fused day loop computes dynamic layers' endpoints in place
 static int g_overlap_resample = 1;  // tunable: the fused day loop resamples inside the previous infect pass
 static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
+static int g_edge_prefetch = 0;     // tunable: issue the next wave tile's endpoint loads before the candidates
 int overlap_resample_enabled() { return g_overlap_resample; }
 
 struct DeviceCaps {
@@ -561,11 +582,14 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
     case CVK_TUNE_EDGE_COARSE:
       g_edge_coarse = value ? 1 : 0;
       return CVK_OK;
+    case CVK_TUNE_EDGE_PREFETCH:
+      g_edge_prefetch = value ? 1 : 0;
+      return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
 }
-extern "C" int cvk_version(void) { return 4; }
+extern "C" int cvk_version(void) { return 5; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
 extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
@@ -701,9 +725,9 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* const* fns = &kEdgeFns[0][0][0];
+    const void* const* fns = &kEdgeFns[0][0][0][0];
     int64_t b_min = -1;
-    for (int j = 0; j < 12; ++j) {
+    for (int j = 0; j < 24; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -788,7 +812,8 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   const size_t shmem = (size_t)((int64_t)a.lds_words * 4 + queue_bytes);
   bool any_gen = false;
   for (int l = 0; l < params->n_layers; ++l) any_gen = any_gen || a.seg[l].gen;
-  kEdgeLaunch[mode][g_edge_chunks == 4 ? 1 : 0][any_gen ? 1 : 0](dim3(grid), dim3(EDGE_BLOCK), shmem, st, a);
+  kEdgeLaunch[mode][g_edge_chunks == 4 ? 1 : 0][any_gen ? 1 : 0][g_edge_prefetch ? 1 : 0](dim3(grid), dim3(EDGE_BLOCK),
+                                                                                       shmem, st, a);
   return check_launch("cvk_edge_pass");
 }
 

@@ -764,25 +764,34 @@ __device__ inline void candidate_round(const SimView& S, const SegView& L, int32
 // infectious agent; or the fine bitmap in L2 only.
 constexpr int BITS_LDS = 0, BITS_COARSE = 1, BITS_L2 = 2;
 
-// One wave tile. lds_bits: the LDS map of MODE (fine or coarse; unused for BITS_L2). n_x: the wave's
-// exact-queue fill, whose slots all belong to segment L.
+// A tile between its two phases: which edges are mixed, and their endpoints' bound codes (the loads
+// may still be in flight).
+template <int K>
+struct TileBounds {
+  uint32_t mixed, from_x;  // bit c*4+q: exactly one endpoint infectious; it is x (p1)
+  uint32_t bs[K * 4], bt[K * 4];
+};
+
+// Phase 1 of a wave tile: infectious bits of every endpoint, then the bound loads of the mixed edges
+// (issued, not waited for). lds_bits: the LDS map of MODE (fine or coarse; unused for BITS_L2).
 template <int MODE, int K>
-__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* lds_bits,
-                                      const WaveQ& Q, const WaveTile<K>& T, uint32_t& n_x) {
-  const uint32_t lane = lane_id();
+__device__ inline void tile_bits_and_bounds(const SimView& S, const uint32_t* lds_bits, const WaveTile<K>& T,
+                                            TileBounds<K>& B) {
   const CVK_GLOBAL uint16_t* bound = gmem(S.bound);
   const CVK_GLOBAL uint32_t* fine = gmem(S.bitmap);
-  // the bitmap words of both endpoints of every edge, all requests in flight before the first use
-  uint32_t okm = 0;  // bit c*4+q: the edge exists and both endpoints are in [0, n)
+  // The bitmap words of both endpoints of every edge, all requests in flight before the first use.
+  // Branch-free: edges past the end of the layer were loaded as (0, 0), which is never mixed (an agent
+  // cannot be infectious and not infectious), and endpoints outside [0, n) (the reference leaves them
+  // undefined) are masked out after a clamped lookup.
+  const uint32_t nm1 = S.n_agents - 1u;
+  uint32_t okm = 0;  // bit c*4+q: both endpoints in [0, n)
   uint32_t wx[K * 4], wy[K * 4];
 #pragma unroll
   for (int j = 0; j < K * 4; ++j) {
     const int c = j >> 2, q = j & 3;
     const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
-    // endpoints outside [0, n) are ignored (the reference leaves them undefined)
-    const bool ok = (c * 64 + (int)lane) * 4 + q < T.n_valid && xi < S.n_agents && yi < S.n_agents;
-    if (ok) okm |= 1u << j;
-    const uint32_t xs = ok ? xi : 0u, ys = ok ? yi : 0u;
+    okm |= (uint32_t)(max(xi, yi) <= nm1) << j;
+    const uint32_t xs = min(xi, nm1), ys = min(yi, nm1);
     if (MODE == BITS_LDS) {
       wx[j] = lds_bits[xs >> 5];
       wy[j] = lds_bits[ys >> 5];
@@ -796,30 +805,42 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
       wy[j] = fine[ys >> 5];
     }
   }
-  uint32_t mixed = 0, from_x = 0;  // bit c*4+q: exactly one endpoint infectious; it is x (p1)
+  uint32_t mixed = 0, from_x = 0;
 #pragma unroll
   for (int j = 0; j < K * 4; ++j) {
     const int c = j >> 2, q = j & 3;
-    const bool ok = (okm >> j) & 1u;
+    const uint32_t ok = (okm >> j) & 1u;
     const uint32_t xi = (uint32_t)T.x[c][q], yi = (uint32_t)T.y[c][q];
-    const bool ix = ok && ((wx[j] >> (xi & 31)) & 1u);
-    const bool iy = ok && ((wy[j] >> (yi & 31)) & 1u);
-    if (ix != iy) mixed |= 1u << j;
-    if (ix) from_x |= 1u << j;
+    const uint32_t ix = (wx[j] >> (xi & 31)) & ok;
+    const uint32_t iy = (wy[j] >> (yi & 31)) & ok;
+    mixed |= ((ix ^ iy) & 1u) << j;
+    from_x |= (ix & 1u) << j;
   }
+  B.mixed = mixed;
+  B.from_x = from_x;
   // bound codes of both endpoints of every mixed edge, all in flight together
-  uint32_t bs[K * 4], bt[K * 4];
 #pragma unroll
   for (int j = 0; j < K * 4; ++j) {
-    bs[j] = 0;
-    bt[j] = 0;
+    B.bs[j] = 0;
+    B.bt[j] = 0;
     if ((mixed >> j) & 1u) {
       const uint32_t xi = (uint32_t)T.x[j >> 2][j & 3], yi = (uint32_t)T.y[j >> 2][j & 3];
       const bool sx = (from_x >> j) & 1u;
-      bs[j] = bound[sx ? xi : yi];
-      bt[j] = bound[sx ? yi : xi];
+      B.bs[j] = bound[sx ? xi : yi];
+      B.bt[j] = bound[sx ? yi : xi];
     }
   }
+}
+
+// Phase 2 of a wave tile: candidates onto the LDS stack, rounds of draws and prefilter tests, exact
+// tests queued. n_x: the wave's exact-queue fill, whose slots all belong to segment L.
+template <int K>
+__device__ inline void tile_candidates(const SimView& S, const SegView& L, int32_t t, const WaveQ& Q,
+                                       const WaveTile<K>& T, const TileBounds<K>& B, uint32_t& n_x) {
+  const uint32_t lane = lane_id();
+  const uint32_t mixed = B.mixed, from_x = B.from_x;
+  const uint32_t* bs = B.bs;
+  const uint32_t* bt = B.bt;
   uint32_t n_c = 0;
 #pragma unroll
   for (int c = 0; c < K; ++c) {
@@ -856,6 +877,15 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
     while (n_c >= 64u) candidate_round<K>(S, L, t, Q, T, n_c, n_x);
   }
   if (n_c) candidate_round<K>(S, L, t, Q, T, n_c, n_x);
+}
+
+// One wave tile, both phases.
+template <int MODE, int K>
+__device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_t t, const uint32_t* lds_bits,
+                                      const WaveQ& Q, const WaveTile<K>& T, uint32_t& n_x) {
+  TileBounds<K> B;
+  tile_bits_and_bounds<MODE, K>(S, lds_bits, T, B);
+  tile_candidates<K>(S, L, t, Q, T, B, n_x);
 }
 
 // ===========================================================================

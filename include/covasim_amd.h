@@ -216,7 +216,7 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 /* No reference counterpart: launch tunables (process-wide). CVK_TUNE_EDGE_LDS = 0 keeps the edge
  * pass's infectious bitmap in global memory (L2) even when it fits in LDS; 1 (default) stages it. */
 #define CVK_TUNE_EDGE_LDS 1
-/* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks per lane of an edge-pass wave tile, 2 or 4 (default). */
+/* CVK_TUNE_EDGE_CHUNKS: 4-edge chunks per lane of an edge-pass wave tile, 2 (default) or 4. */
 #define CVK_TUNE_EDGE_CHUNKS 2
 /* CVK_TUNE_GENERATE_DYNAMIC: 1 = inside cvk_run_days / graphs, the edge and infect passes compute
  * each day's dynamic-layer endpoints from the resample stream, and the layer arrays are written
@@ -231,6 +231,9 @@ int cvk_abi_layout(int64_t* out, int32_t n);
  * stages the coarse map there and reads bitmap words from L2 only for endpoints whose 16-agent group
  * has an infectious agent; 0 = every bitmap word is read from L2. */
 #define CVK_TUNE_EDGE_COARSE 6
+/* CVK_TUNE_EDGE_PREFETCH: 1 = each edge-pass wave issues its next tile's endpoint loads after this
+ * tile's bound loads, before its candidates are processed; 0 (default) = after. */
+#define CVK_TUNE_EDGE_PREFETCH 7
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

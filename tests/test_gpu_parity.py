@@ -181,13 +181,16 @@ CONFIGS = {
     # bitmap not staged: the coarse map in LDS filters the L2 reads of bitmap words
     'weighted_edges_l2_bitmap': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
                                      pars=dict(use_waning=True, beta=0.04), tuning=dict(edge_lds=0)),
-    'hybrid_dynamic_shallow_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
-                                         pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
-                                         tuning=dict(edge_chunks=2)),
+    'hybrid_dynamic_deep_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                      pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                      tuning=dict(edge_chunks=4)),
     'variants_waning_shallow_l2': dict(pop=dict(n=4000, pop_type='hybrid'),
                                        pars=dict(use_waning=True, beta=0.02,
                                                  variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
                                        tuning=dict(edge_chunks=2, edge_lds=0, edge_coarse=0)),
+    'hybrid_dynamic_prefetch_tiles': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                          pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1)),
+                                          tuning=dict(edge_chunks=2, edge_prefetch=1)),
     'hybrid_dynamic_resample_in_agent_pass': dict(pop=dict(n=4000, pop_type='hybrid'),
                                                   pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1)),
                                                   tuning=dict(overlap_resample=0)),
@@ -210,9 +213,10 @@ def test_full_sim_matches_oracle(gpu, name):
     pars = dict(cfg['pars'])
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
-    keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 4),
+    keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 2),
                 generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0),
-                overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1))
+                overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1),
+                edge_prefetch=(_lib.TUNE_EDGE_PREFETCH, 0))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
