@@ -174,7 +174,7 @@ typedef struct {
   uint64_t* winner;      /* [n] min transmission key per target; all-ones = none */
   uint32_t* touched;     /* [(n + 31) / 32] bit i: agent i holds a key this day (cleared by the infect pass) */
   uint32_t* bitmap;      /* [bitmap_words] bit i: agent i infectious with rel_trans != 0 */
-  uint32_t* ctl;         /* [64] control words: log count, choose state, tile counter, flags */
+  uint32_t* ctl;         /* [64] control words: log count, choose state, flags */
   uint32_t* results;     /* [npts][CVK_R_COLS] */
   double* results_f;     /* [npts][CVK_RF_COLS] */
   int32_t* log;          /* [log_capacity][4] infection records: source, target, day, layer | variant<<8 */
@@ -286,7 +286,7 @@ int cvk_agent_pass(const cvk_people* people, const cvk_params* params, cvk_work*
 int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
                        void* stream);
 /* uniforms: NULL (Philox) or per layer a device float64 array [n_variants][2][n_edges].
- * Must follow cvk_agent_pass of the same day (it reads that pass's bitmaps, records and tile counter). */
+ * Must follow cvk_agent_pass of the same day (it reads that pass's bitmaps, records and bound codes). */
 int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                   cvk_work* work, int32_t t, const double* const* uniforms, void* stream);
 int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
