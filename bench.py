@@ -47,6 +47,10 @@ WORKLOADS = {
                     desc='differentiable relaxation on the hybrid 1M population (static layers), 60 days: '
                          'float32 forward + hand-written adjoint with checkpoints every 10 days, gradient wrt '
                          '4 layer betas, beta and 10 age-bin rel_sus; calibration to a stochastic incidence curve'),
+    'popgen10m': dict(pop_size=10_000_000, pop_type='hybrid', households='categorical', popgen=True,
+                      pars=dict(),
+                      desc='population build of the hybrid 10M population on the device: ages, age-correlated '
+                           'households 1..7, s/w/c random layers, prognosis arrays (cvk_pop_*)'),
     'random20k': dict(pop_size=20_000, pop_type='random', n_days=60, households='poisson',
                       pars=dict(beta=0.016, use_waning=True, pop_infected=20),
                       desc='random 20k agents, one layer a with Poisson(20) contacts, 60 days, beta 0.016'),
@@ -260,6 +264,54 @@ def run_relax(w, args):
     print(json.dumps(out), flush=True)
 
 
+def run_popgen(w, args):
+    """One step = one build of the whole population on the device (ages, households, layers,
+    per-agent arrays), host synchronisations for the layer sizes included."""
+    import torch
+    from covasim_amd import parameters as cvpar
+    from covasim_amd import popgen as pg
+    n = w['pop_size']
+    pars = cvpar.make_pars(pop_size=n, pop_type=w['pop_type'], rand_seed=args.seed)
+    warm_pars = cvpar.make_pars(pop_size=10_000, pop_type=w['pop_type'], rand_seed=args.seed)
+    pg.make_population(warm_pars, households=w['households'])
+    steps = args.steps or 3
+    warmup = 1 if args.warmup is None else args.warmup
+    for _ in range(warmup):
+        pop = pg.make_population(pars, households=w['households'])
+        del pop
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        pop = pg.make_population(pars, households=w['households'])
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    n_edges = sum(int(v['p1'].numel()) for v in pop['contacts'].values())
+    out_bytes = 12 * n_edges + 29 * n  # p1, p2, beta per edge; age, sex and six float32 arrays per agent
+    achieved = out_bytes / wall / 1e9
+    line = {
+        'metric': 'population build throughput (agents/s)', 'value': n / wall, 'unit': 'agents/s', 'n_gpus': 1,
+        'steps': steps, 'warmup': warmup, 'ms_per_step': wall * 1e3, 'higher_is_better': True, 'scaling': 'weak',
+        'vs_baseline': None, 'dtype': 'f64 draws, int32 edges', 'data': 'synthetic',
+        'config': {'workload': w['desc'], 'agents': n, 'edges': n_edges,
+                   'layers': {k: int(v['p1'].numel()) for k, v in pop['contacts'].items()}},
+        'roofline': {'bound': 'hbm', 'kernel': 'whole build (all cvk_pop_* launches and host synchronisations)',
+                     'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+                     'traffic': None, 'output_bytes': out_bytes},
+    }
+    del pop
+    if not args.no_cpu_baseline:
+        from oracle.cport import popgen as op
+        sample = max(10_000, int(n // 10))
+        spars = cvpar.make_pars(pop_size=sample, pop_type=w['pop_type'], rand_seed=args.seed)
+        t1 = time.perf_counter()
+        op.make_population(pg.pop_tables(spars, households=w['households']), pg.layer_plan(spars), sample)
+        el = time.perf_counter() - t1
+        line['cpu_baseline'] = dict(value=sample / el, unit='agents/s', cores=1, kind='port',
+                                    sample=f'oracle C restatement (gcc -O3) building the same kind of population '
+                                           f'with {sample} agents, {el:.1f} s on 1 host thread')
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -316,6 +368,8 @@ def main():
         tuning['edge_lds'] = args.edge_lds
     if 'n_sims' in w:
         return run_batch(w, args)
+    if w.get('popgen'):
+        return run_popgen(w, args)
     if w.get('relax'):
         return run_relax(w, args)
     steps = args.steps or w['n_days']

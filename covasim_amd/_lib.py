@@ -10,6 +10,8 @@ import os
 
 MAX_LAYERS = 8
 MAX_VARIANTS = 8
+POP_MAX_AGE_BINS, POP_MAX_HH_SIZES, POP_MAX_PROG_BINS = 32, 64, 16
+POP_COUNT_POISSON, POP_COUNT_NEGBIN = 0, 1
 N_DURS = 9
 
 OK, E_INVALID, E_CAPACITY, E_HIP, E_ALIGN = 0, -1, -2, -3, -4
@@ -189,6 +191,23 @@ class RelaxDesc(C.Structure):
                 ('age_bin', C.c_void_p), ('rel_trans', C.c_void_p), ('w_host', C.c_void_p)]
 
 
+class PopGen(C.Structure):
+    _fields_ = [('seed', C.c_uint64), ('n_age_bins', C.c_int32), ('n_hh_sizes', C.c_int32),
+                ('hh_size_min', C.c_int32), ('correlate', C.c_int32), ('n_prog_bins', C.c_int32), ('pad', C.c_int32),
+                ('age_cdf', C.c_double * POP_MAX_AGE_BINS), ('age_lo', C.c_double * POP_MAX_AGE_BINS),
+                ('age_width', C.c_double * POP_MAX_AGE_BINS), ('hh_cdf', C.c_double * POP_MAX_HH_SIZES),
+                ('age_noise', C.c_double), ('prog_cutoff', C.c_double * POP_MAX_PROG_BINS),
+                ('trans_or', C.c_double * POP_MAX_PROG_BINS), ('symp_prob', C.c_float * POP_MAX_PROG_BINS),
+                ('severe_prob', C.c_float * POP_MAX_PROG_BINS), ('crit_prob', C.c_float * POP_MAX_PROG_BINS),
+                ('death_prob', C.c_float * POP_MAX_PROG_BINS), ('rel_sus', C.c_float * POP_MAX_PROG_BINS),
+                ('nb_p0', C.c_double), ('nb_q', C.c_double), ('nb_r', C.c_double), ('nb_step', C.c_double)]
+
+
+class PopCounts(C.Structure):
+    _fields_ = [('kind', C.c_int32), ('pad', C.c_int32), ('p0', C.c_double), ('lam', C.c_double), ('r', C.c_double),
+                ('q', C.c_double)]
+
+
 _LIB = None
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'libcovasim_amd.so')
 
@@ -242,6 +261,19 @@ def lib():
     L.cvk_relax_destroy.argtypes = [vp]
     L.cvk_relax_forward.argtypes = [vp, vp, vp, vp]
     L.cvk_relax_backward.argtypes = [vp, vp, vp, vp, vp]
+    PG, PC = C.POINTER(PopGen), C.POINTER(PopCounts)
+    L.cvk_pop_work_bytes.argtypes = [i64, i64]
+    L.cvk_pop_work_bytes.restype = i64
+    L.cvk_pop_ages.argtypes = [PG, i64, vp, vp, vp]
+    L.cvk_pop_households_plan.argtypes = [PG, i64, vp, i64, vp, i64, vp, vp]
+    L.cvk_pop_households_fill.argtypes = [PG, i64, vp, i64, vp, i64, vp, vp, vp, vp, i64, vp]
+    L.cvk_pop_members.argtypes = [i64, vp, C.c_double, C.c_double, vp, vp, vp, i64, vp]
+    L.cvk_pop_contacts_plan.argtypes = [PG, i32, i64, PC, vp, vp, i64, vp]
+    L.cvk_pop_contacts_fill.argtypes = [PG, i32, i64, vp, vp, vp, vp, vp, i64, vp]
+    L.cvk_pop_people.argtypes = [PG, i64, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
+    for name in ['cvk_pop_ages', 'cvk_pop_households_plan', 'cvk_pop_households_fill', 'cvk_pop_members',
+                 'cvk_pop_contacts_plan', 'cvk_pop_contacts_fill', 'cvk_pop_people']:
+        getattr(L, name).restype = C.c_int
     for name in ['cvk_relax_create', 'cvk_relax_destroy', 'cvk_relax_forward', 'cvk_relax_backward']:
         getattr(L, name).restype = C.c_int
     for name in ['cvk_batch_create', 'cvk_batch_destroy', 'cvk_batch_set_config', 'cvk_batch_get_config',
@@ -266,7 +298,7 @@ def _check_layout(L):
             C.sizeof(ImportsS), Params.dur.offset, Params.nab_init.offset, Params.rel_imm_symp.offset,
             Params.immunity.offset, Params.nab_kin.offset, PeopleS.sev_imm.offset, WorkS.log_capacity.offset,
             WorkS.npts.offset, R_COLS, RF_COLS, C.sizeof(BatchDesc), BatchDesc.people.offset, C.sizeof(BatchImports),
-            C.sizeof(BatchConfig), C.sizeof(RelaxDesc)]
+            C.sizeof(BatchConfig), C.sizeof(RelaxDesc), C.sizeof(PopGen), PopGen.nb_p0.offset, C.sizeof(PopCounts)]
     if got != want:
         raise RuntimeError(f'covasim_amd: ctypes struct layout mismatch with the library: {got} != {want}')
 

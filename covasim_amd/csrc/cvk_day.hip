@@ -602,7 +602,8 @@ extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
       (int64_t)offsetof(cvk_work, log_capacity), (int64_t)offsetof(cvk_work, npts),
       (int64_t)CVK_R_COLS, (int64_t)CVK_RF_COLS, (int64_t)sizeof(cvk_batch_desc),
       (int64_t)offsetof(cvk_batch_desc, people), (int64_t)sizeof(cvk_batch_imports), (int64_t)sizeof(cvk_batch_config),
-      (int64_t)sizeof(cvk_relax_desc)};
+      (int64_t)sizeof(cvk_relax_desc), (int64_t)sizeof(cvk_popgen), (int64_t)offsetof(cvk_popgen, nb_p0),
+      (int64_t)sizeof(cvk_pop_counts)};
   const int32_t m = (int32_t)(sizeof(vals) / sizeof(vals[0]));
   if (!out || n < m) return fail(CVK_E_INVALID, "cvk_abi_layout: output too small");
   for (int32_t j = 0; j < m; ++j) out[j] = vals[j];

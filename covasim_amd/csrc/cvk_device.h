@@ -30,6 +30,15 @@ constexpr uint32_t STREAM_IMPORT = 0x40000000u;    // | variant
 constexpr uint32_t STREAM_RESAMPLE = 0x50000000u;  // | layer
 constexpr uint32_t STREAM_RESAMPLE_RETRY = 0x60000000u;  // | layer
 constexpr uint32_t RESAMPLE_MAX_RETRY = 255;
+// Population generation (cvk_popgen.hip); the low byte of COUNT / EDGE / EDGE_RETRY is the layer slot.
+constexpr uint32_t STREAM_POP_AGE = 0x70000000u;
+constexpr uint32_t STREAM_POP_AGE_NOISE = 0x70000100u;
+constexpr uint32_t STREAM_POP_PERSON = 0x70000200u;
+constexpr uint32_t STREAM_POP_HH_SIZE = 0x70000300u;
+constexpr uint32_t STREAM_POP_HH_PERM = 0x70000400u;
+constexpr uint32_t STREAM_POP_COUNT = 0x70000500u;
+constexpr uint32_t STREAM_POP_EDGE = 0x70000600u;
+constexpr uint32_t STREAM_POP_EDGE_RETRY = 0x70000700u;
 
 // Per-infection draw slots (counter word c3 of STREAM_INFECT).
 constexpr uint32_t SLOT_EXP2INF = 0;

@@ -154,6 +154,9 @@ class People:
         if key not in self._d or key in ('flags', 'vstate'):
             raise KeyError(f'Cannot set "{key}" on the device state')
         t = self._d[key]
+        if isinstance(value, torch.Tensor):
+            t.copy_(value.to(device=t.device, dtype=t.dtype).expand(t.shape))
+            return
         arr = np.asarray(value, dtype=np.float32 if t.dtype == torch.float32 else np.int32)
         if arr.shape != tuple(t.shape):
             arr = np.broadcast_to(arr, tuple(t.shape))

@@ -55,7 +55,7 @@ class Sim:
     """A single simulation on one GPU (sim.py:49-80)."""
 
     def __init__(self, pars=None, label=None, people=None, popdict=None, households='poisson', device=None,
-                 log_capacity=None, **kwargs):
+                 log_capacity=None, popgen='host', **kwargs):
         merged = dict(pars or {})
         merged.update(kwargs)
         variants = merged.pop('variants', [])
@@ -65,6 +65,9 @@ class Sim:
         self.people = people
         self.popdict = popdict
         self.households = households
+        if popgen not in ('host', 'device'):
+            raise ValueError(f'popgen must be "host" or "device", not "{popgen}"')
+        self.popgen = popgen  # 'device': build the population with the cvk_pop_* kernels
         self.device = device
         self.log_capacity = log_capacity
         self.initialized = False
@@ -147,6 +150,11 @@ class Sim:
             self.people.pars = self.pars
             return
         popdict = self.popdict
+        if popdict is None and self.popgen == 'device':
+            from . import popgen as cvpg
+            popdict = cvpg.make_population(self.pars, households=self.households, device=self.device)
+            popdict['age'] = popdict['age'].cpu().numpy()
+            popdict['sex'] = popdict['sex'].cpu().numpy()
         if popdict is None:
             popdict = cvpop.make_randpop(self.pars, rng, households=self.households)
         arrays = {k: popdict[k] for k in ('symp_prob', 'severe_prob', 'crit_prob', 'death_prob', 'rel_sus',

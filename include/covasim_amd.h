@@ -21,6 +21,8 @@
  *   cvk_infect_pass    covasim/people.py:435-586 People.infect (+ immunity.py:138-202 update_peak_nab)
  *   cvk_count_pass     covasim/sim.py:651-674   stock counts (base.py:1133-1145) + immunity.py:205-213 update_nab
  *   cvk_run_days       covasim/sim.py:558-685   Sim.step, repeated (the day loop of Sim.run, sim.py:733-755)
+ *   cvk_pop_*          covasim/population.py:143-364 make_randpop / make_*_contacts,
+ *                      people.py:139-160 set_prognoses (population generation, before the day loop)
  */
 #ifndef COVASIM_AMD_H
 #define COVASIM_AMD_H
@@ -382,6 +384,112 @@ int cvk_batch_copy_results(void* handle, uint32_t* results_host, double* results
 /* Synchronous copy of up to capacity infection records (source, target, day, layer | variant << 8
  * | sim << 12; source and target are indices within the sim); *count_out = records written. */
 int cvk_batch_copy_log(void* handle, int32_t* log_host, int64_t capacity, int64_t* count_out);
+
+/* ---- population generation ----
+ * Replaces covasim/population.py:143-226 make_randpop (ages, sexes), :241-284 make_random_contacts,
+ * :287-329 make_microstructured_contacts, :332-364 make_hybrid_contacts and the per-agent draws of
+ * people.py:139-160 set_prognoses + utils.py:409-426 n_neg_binomial (rel_trans). The population is
+ * a pure function of (seed, tables, sizes): every draw comes from Philox4x32-10 with the counter
+ * (c0, c1) = an agent, household, member or edge index, c2 = a CVK_POP_* stream id (| layer slot),
+ * so the result does not depend on launch shape and a CPU restatement reproduces it bit for bit.
+ *
+ * Draw rules (u01(a, b) = ((a >> 5) * 2^26 + (b >> 6)) / 2^53; words w = (x, y, z, w)):
+ *  age of agent i    stream AGE (c0 = i): bin = #{k : age_cdf[k] < u01(x, y)} (capped at the last
+ *                    row), age = (float)(age_lo[bin] + age_width[bin] * u01(z, w)) (double math).
+ *  sort key of i     stream AGE_NOISE (c0 = i): s = (x + y + z + w) * 2^-32 (Irwin-Hall of 4, exact),
+ *                    key = clamp(floor(age_d + (s - 2) * age_noise) + 64, 0, 255), age_d the double age.
+ *  household h size  stream HH_SIZE (c0, c1 = h): entry = #{k : hh_cdf[k] < u01(x, y)} (capped),
+ *                    size = hh_size_min + entry. Households h = 0, 1, ... are taken until they cover
+ *                    n agents; the last one is cut to fit.
+ *  household order   correlate = 0: household h holds agents [start_h, start_h + size_h) as drawn.
+ *                    correlate = 1: agents are ordered by (key, agent index); household h takes the
+ *                    next size_h agents of that order and is then moved to position pi(h), pi a
+ *                    4-round Feistel permutation of [0, H) (round r: F = philox(R, r, HH_PERM, 0).x
+ *                    & mask over 2b-bit words, b = max(1, ceil(bits(H - 1) / 2)), cycle-walked below
+ *                    H); final agent indices follow the permuted household order.
+ *  household edges   every pair (a, b), a < b, of each household, in (p1, p2) order.
+ *  layer members     agents with lo <= age < hi, ascending (all agents when mapping is not used).
+ *  contact counts    member m, stream COUNT | slot (c0 = m): k = the smallest count whose CDF reaches
+ *                    u01(x, y) (Poisson: p0 = e^-lam, p_{k+1} = p_k * lam / (k+1); negative binomial:
+ *                    p_{k+1} = p_k * ((k + r) / (k + 1)) * q); edges of m = round-half-even(k / 2).
+ *  contact partners  edge e, stream EDGE | slot (c0, c1 = e >> 2): word e & 3 of the draw, mapped to
+ *                    [0, n_members) by Lemire's multiply (rejecting low words below 2^32 mod n; retry
+ *                    k uses word x of stream EDGE_RETRY | slot with c0, c1 = e, c3 = k, at most 255).
+ *                    p1 = member of the edge (its own edges are consecutive), p2 = mapping[partner].
+ *  sex, rel_trans    stream PERSON (c0 = i): sex = x & 1; rel_trans = (float)(trans_or[bin] *
+ *                    (nb * nb_step)), nb the negative binomial count of u01(y, z) as above
+ *                    (p0 = nb_p0, r = nb_r, q = nb_q); bin = #{k : prog_cutoff[k] <= age} - 1.
+ */
+#define CVK_POP_MAX_AGE_BINS 32
+#define CVK_POP_MAX_HH_SIZES 64
+#define CVK_POP_MAX_PROG_BINS 16
+#define CVK_POP_COUNT_POISSON 0
+#define CVK_POP_COUNT_NEGBIN 1
+
+typedef struct {
+  uint64_t seed;
+  int32_t n_age_bins;          /* rows of the age table, 1..CVK_POP_MAX_AGE_BINS */
+  int32_t n_hh_sizes;          /* entries of hh_cdf, 1..CVK_POP_MAX_HH_SIZES */
+  int32_t hh_size_min;         /* size of hh_cdf entry 0 (1 for categorical sizes, 0 for Poisson) */
+  int32_t correlate;           /* 1: households from the noisy-age order, shuffled; 0: consecutive agents */
+  int32_t n_prog_bins;         /* 1..CVK_POP_MAX_PROG_BINS */
+  int32_t pad;
+  double age_cdf[CVK_POP_MAX_AGE_BINS];
+  double age_lo[CVK_POP_MAX_AGE_BINS];
+  double age_width[CVK_POP_MAX_AGE_BINS];
+  double hh_cdf[CVK_POP_MAX_HH_SIZES];
+  double age_noise;            /* 15 * sqrt(3) gives a sort-key noise of standard deviation 15 years */
+  double prog_cutoff[CVK_POP_MAX_PROG_BINS];
+  double trans_or[CVK_POP_MAX_PROG_BINS];
+  float symp_prob[CVK_POP_MAX_PROG_BINS], severe_prob[CVK_POP_MAX_PROG_BINS];
+  float crit_prob[CVK_POP_MAX_PROG_BINS], death_prob[CVK_POP_MAX_PROG_BINS];
+  float rel_sus[CVK_POP_MAX_PROG_BINS];
+  double nb_p0, nb_q, nb_r, nb_step;  /* rel_trans draw: P(0), 1 - p, dispersion, value of one count */
+} cvk_popgen;
+
+/* Contact-count distribution of a random layer. */
+typedef struct {
+  int32_t kind;                /* CVK_POP_COUNT_POISSON or CVK_POP_COUNT_NEGBIN */
+  int32_t pad;
+  double p0;                   /* P(count = 0): e^-lam, or (r / (r + mean))^r */
+  double lam;                  /* Poisson mean */
+  double r, q;                 /* negative binomial dispersion and 1 - p */
+} cvk_pop_counts;
+
+/* Bytes of the workspace the population calls take (`work`, 256-byte aligned device memory):
+ * enough for n agents and h_cap household slots. */
+int64_t cvk_pop_work_bytes(int64_t n, int64_t h_cap);
+/* Ages (float32[n]) and, when g->correlate, sort keys (uint8[n]; may be NULL otherwise). */
+int cvk_pop_ages(const cvk_popgen* g, int64_t n, float* age, uint8_t* key, void* stream);
+/* Plans the households of n agents from household slots [0, h_cap): sizes, order and edge
+ * offsets, kept in `work`. Writes totals[0] = H (households used, or -1 when h_cap slots do not
+ * cover n agents: call again with a larger h_cap) and totals[1] = household edges. */
+int cvk_pop_households_plan(const cvk_popgen* g, int64_t n, const uint8_t* key, int64_t h_cap, void* work,
+                            int64_t work_bytes, int64_t* totals, void* stream);
+/* After a plan with the same arguments: age_out[n] = age_in in household order (age_in itself
+ * when !g->correlate), household edges p1/p2 (n_edges = totals[1]) and beta = 1 (beta may be NULL). */
+int cvk_pop_households_fill(const cvk_popgen* g, int64_t n, const float* age_in, int64_t h_cap, void* work,
+                            int64_t work_bytes, float* age_out, int32_t* p1, int32_t* p2, float* beta,
+                            int64_t n_edges, void* stream);
+/* Members of an age band: mapping[] = agents with lo <= age < hi, ascending; *total = count.
+ * mapping must hold n entries. This call, cvk_pop_contacts_plan and cvk_pop_people reuse the
+ * workspace: fill the households before calling them. */
+int cvk_pop_members(int64_t n, const float* age, double lo, double hi, int32_t* mapping, int64_t* total,
+                    void* work, int64_t work_bytes, void* stream);
+/* Edge offsets of a random layer over n_members members: offsets[n_members + 1], the last entry
+ * being the layer's edge count. `slot` is the layer's index in the population (0..255). */
+int cvk_pop_contacts_plan(const cvk_popgen* g, int32_t slot, int64_t n_members, const cvk_pop_counts* counts,
+                          int64_t* offsets, void* work, int64_t work_bytes, void* stream);
+/* Fills a planned random layer: p1/p2 (mapping[member], or the member itself when mapping is
+ * NULL), beta = 1 (beta may be NULL). p2 and beta must be 16-byte aligned. */
+int cvk_pop_contacts_fill(const cvk_popgen* g, int32_t slot, int64_t n_members, const int32_t* mapping,
+                          const int64_t* offsets, int32_t* p1, int32_t* p2, float* beta, int64_t n_edges,
+                          void* stream);
+/* Per-agent draws and prognoses of n agents with final ages `age`: sex (uint8), symp_prob,
+ * severe_prob, crit_prob, death_prob, rel_sus, rel_trans (float32). */
+int cvk_pop_people(const cvk_popgen* g, int64_t n, const float* age, uint8_t* sex, float* symp_prob,
+                   float* severe_prob, float* crit_prob, float* death_prob, float* rel_sus, float* rel_trans,
+                   void* work, int64_t work_bytes, void* stream);
 
 /* ---- differentiable relaxation ----
  * No reference counterpart: an expected-infection relaxation of the transmission step

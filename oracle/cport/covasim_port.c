@@ -533,3 +533,6 @@ int port_threads(void) {
   return 1;
 #endif
 }
+
+/* Exported for popgen_port.c (the population generator's restatement, same library). */
+void port_philox(uint32_t c[4], uint64_t seed) { philox(c, seed); }

@@ -19,7 +19,9 @@ def declared_functions():
 def test_header_declares_entry_points():
     names = declared_functions()
     for must in ['cvk_viral_load', 'cvk_trans_sus', 'cvk_infections', 'cvk_agent_pass', 'cvk_edge_pass',
-                 'cvk_infect_pass', 'cvk_count_pass', 'cvk_run_days', 'cvk_choose', 'cvk_resample_layer']:
+                 'cvk_infect_pass', 'cvk_count_pass', 'cvk_run_days', 'cvk_choose', 'cvk_resample_layer',
+                 'cvk_pop_ages', 'cvk_pop_households_plan', 'cvk_pop_households_fill', 'cvk_pop_members',
+                 'cvk_pop_contacts_plan', 'cvk_pop_contacts_fill', 'cvk_pop_people']:
         assert must in names
 
 
@@ -35,8 +37,9 @@ def test_struct_layout_report():
     L = _lib.lib()
     out = (C.c_int64 * 32)()
     m = L.cvk_abi_layout(out, 32)
-    assert m == 21
+    assert m == 24
     assert out[0] == C.sizeof(_lib.Dur) == 40
+    assert out[21] == C.sizeof(_lib.PopGen) and out[23] == C.sizeof(_lib.PopCounts) == 40
     assert L.cvk_abi_layout(out, 3) < 0  # too-small output is an error, not an overflow
 
 
@@ -59,3 +62,31 @@ def test_host_side_argument_errors():
     assert L.cvk_graph_launch(None, None) == _lib.E_INVALID
     with pytest.raises(ValueError):
         _lib.check(L.cvk_agent_pass(None, None, None, 0, None), 'agent pass')
+
+
+def test_population_argument_errors():
+    """The cvk_pop_* entry points reject bad tables, sizes and workspaces before any device work."""
+    from covasim_amd import _lib
+    L = _lib.lib()
+    g = _lib.PopGen()
+    assert L.cvk_pop_ages(C.byref(g), 10, None, None, None) == _lib.E_INVALID  # zero-row tables
+    g.n_age_bins, g.n_hh_sizes, g.n_prog_bins, g.hh_size_min, g.correlate = 19, 7, 10, 1, 1
+    assert L.cvk_pop_ages(C.byref(g), 10, None, None, None) == _lib.E_INVALID  # null outputs
+    assert L.cvk_pop_ages(C.byref(g), -1, None, None, None) == _lib.E_INVALID
+    assert L.cvk_pop_work_bytes(1000, 500) > 0 and L.cvk_pop_work_bytes(-1, 5) == 0
+    totals = C.c_void_p(256)  # never dereferenced: the call fails on its workspace first
+    key = C.c_void_p(512)
+    assert L.cvk_pop_households_plan(C.byref(g), 1000, key, 500, C.c_void_p(4096 + 8), 1 << 20, totals,
+                                     None) == _lib.E_ALIGN
+    assert L.cvk_pop_households_plan(C.byref(g), 1000, key, 500, C.c_void_p(4096), 64, totals,
+                                     None) == _lib.E_CAPACITY
+    assert L.cvk_pop_households_plan(C.byref(g), 1000, None, 500, C.c_void_p(4096), 1 << 20, totals,
+                                     None) == _lib.E_INVALID  # correlate needs sort keys
+    c = _lib.PopCounts(kind=_lib.POP_COUNT_POISSON, p0=0.0, lam=800.0)
+    assert L.cvk_pop_contacts_plan(C.byref(g), 0, 100, C.byref(c), C.c_void_p(256), C.c_void_p(4096), 1 << 20,
+                                   None) == _lib.E_INVALID
+    assert b'p0' in L.cvk_last_error()
+    assert L.cvk_pop_contacts_fill(C.byref(g), 300, 100, None, C.c_void_p(256), None, None, None, 10,
+                                   None) == _lib.E_INVALID  # slot above 255
+    g.nb_p0 = 0.0
+    assert L.cvk_pop_people(C.byref(g), 10, C.c_void_p(256), *([C.c_void_p(256)] * 7), None, 0, None) == _lib.E_INVALID
