@@ -183,6 +183,39 @@ def run_batch(w, args):
         for e in execs:
             L.cvk_graph_destroy(e)
     single_inf = np.array([int(sim._w['results'][:, _lib.R_COL['new_infections']].sum().item()) for sim in sims])
+    # results of all sims: device finalisation (sim.py:764-1001) and reduction over sims (run.py:220-303)
+    batch.device_results()
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    out_rows, out_vrows = batch.device_results()
+    torch.cuda.synchronize()
+    fin_ms = (time.perf_counter() - t3) * 1e3
+    from covasim_amd import results as cvres
+    t4 = time.perf_counter()
+    red = [cvres.reduce_tables(out_rows), cvres.reduce_tables(out_vrows)]
+    torch.cuda.synchronize()
+    red_ms = (time.perf_counter() - t4) * 1e3
+    del red
+    results_line = {'finalize_ms': fin_ms, 'reduce_ms': red_ms, 'rows': int(out_rows.shape[1]),
+                    'variant_rows': int(out_vrows.shape[1])}
+    if not args.no_cpu_baseline:
+        from oracle import results_ref as rr
+        t5 = time.perf_counter()
+        res_h, resf_h = batch.raw_results()
+        dates = {k: batch._d[k].cpu().numpy() for k in ('date_infectious', 'date_recovered', 'date_dead')}
+        host_rows = []
+        for si, sim in enumerate(sims):
+            a, b = int(batch.agent_offsets[si]), int(batch.agent_offsets[si + 1])
+            dur = sim.pars['dur']
+            R = rr.finalize(res_h[si], resf_h[si], _lib.R_COL, _lib.R_BYVAR, _lib.MAX_VARIANTS, batch.n_variants,
+                            float(b - a), sim.pars['pop_infected'], sim.pars['use_waning'], sim.import_schedule(),
+                            dates['date_infectious'][a:b], dates['date_recovered'][a:b], dates['date_dead'][a:b],
+                            dur['exp2inf']['par1'] + dur['asym2rec']['par1'])
+            host_rows.append(np.stack([np.asarray(R[k], dtype=float) for k in cvres.F_ROW_NAMES]))
+        rr.reduce(np.stack(host_rows))
+        results_line['cpu_finalize_reduce_ms'] = (time.perf_counter() - t5) * 1e3
+        results_line['cpu_sample'] = ('numpy restatement on 1 host thread, including the copy of the result tables '
+                                      'and the three date arrays to the host')
     value = n_agents * steps / wall_batch
     single = n_agents * steps / wall_single
     out = {
@@ -196,6 +229,7 @@ def run_batch(w, args):
                     'one_at_a_time_wall_s': wall_single, 'speedup': wall_single / wall_batch,
                     'identical_infection_totals': bool(np.array_equal(batch_inf, single_inf)),
                     'autotuned_shape': list(tuned), 'autotune_trials_ms': batch.tuning['trials_ms']},
+        'results': results_line,
     }
     print(json.dumps(out), flush=True)
 

@@ -191,6 +191,13 @@ class RelaxDesc(C.Structure):
                 ('age_bin', C.c_void_p), ('rel_trans', C.c_void_p), ('w_host', C.c_void_p)]
 
 
+class FinalizeDesc(C.Structure):
+    _fields_ = [('n_sims', C.c_int32), ('npts', C.c_int32), ('n_variants', C.c_int32), ('use_waning', C.c_int32),
+                ('r_eff_smoothing', C.c_int32), ('doubling_window', C.c_int32), ('max_doubling_time', C.c_double),
+                ('pop_infected_host', C.c_void_p), ('initial_period_host', C.c_void_p),
+                ('agent_offsets_host', C.c_void_p)]
+
+
 class PopGen(C.Structure):
     _fields_ = [('seed', C.c_uint64), ('n_age_bins', C.c_int32), ('n_hh_sizes', C.c_int32),
                 ('hh_size_min', C.c_int32), ('correlate', C.c_int32), ('n_prog_bins', C.c_int32), ('pad', C.c_int32),
@@ -261,6 +268,15 @@ def lib():
     L.cvk_relax_destroy.argtypes = [vp]
     L.cvk_relax_forward.argtypes = [vp, vp, vp, vp]
     L.cvk_relax_backward.argtypes = [vp, vp, vp, vp, vp]
+    FD = C.POINTER(FinalizeDesc)
+    L.cvk_batch_results.argtypes = [vp, C.POINTER(C.c_void_p), C.POINTER(C.c_void_p)]
+    L.cvk_batch_results.restype = C.c_int
+    L.cvk_finalize_work_bytes.argtypes = [FD]
+    L.cvk_finalize_work_bytes.restype = i64
+    L.cvk_finalize.argtypes = [FD, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
+    L.cvk_finalize.restype = C.c_int
+    L.cvk_reduce.argtypes = [i32, i64, vp, i32, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
+    L.cvk_reduce.restype = C.c_int
     PG, PC = C.POINTER(PopGen), C.POINTER(PopCounts)
     L.cvk_pop_work_bytes.argtypes = [i64, i64]
     L.cvk_pop_work_bytes.restype = i64

@@ -24,6 +24,7 @@ import time
 import numpy as np
 
 from . import _lib
+from . import results as cvres
 
 __all__ = ['BatchSim']
 
@@ -229,12 +230,42 @@ class BatchSim:
                    'cvk_batch_copy_results')
         return res, resf
 
+    def device_results(self):
+        """Finalised rows of every sim on the device (cvk_finalize over the batch's tables):
+        [S][F_ROWS][npts] and [S][FV_ROWS][n_variants][npts] float64 tensors."""
+        L = _lib.lib()
+        rp, rfp = C.c_void_p(), C.c_void_p()
+        _lib.check(L.cvk_batch_results(self._handle, C.byref(rp), C.byref(rfp)), 'cvk_batch_results')
+        imports = np.stack([sim.import_schedule() for sim in self.sims])
+        d = self._d
+        return cvres.finalize_tables(rp.value, rfp.value, self.npts, self.n_variants, self.sims[0].pars['use_waning'],
+                                     [sim.pars['pop_infected'] for sim in self.sims],
+                                     [cvres.initial_period(sim.pars, self.npts) for sim in self.sims],
+                                     self.agent_offsets, imports, d['date_infectious'], d['date_recovered'],
+                                     d['date_dead'])
+
     def finalize(self):
-        res, resf = self.raw_results()
+        out, out_v = self.device_results()
+        out, out_v = out.cpu().numpy(), out_v.cpu().numpy()
+        res, _ = self.raw_results()
         for si, sim in enumerate(self.sims):
             sim.results_ready = False
-            sim.finalize(raw=(res[si].astype(np.float64), resf[si]))
+            sim.finalize(rows=(out[si], out_v[si], res[si]))
             sim.complete = True
+
+    def reduce(self, quantiles=None, use_mean=False, bounds=None):
+        """MultiSim.reduce (run.py:220-303) over the batch, on the device: for every result row, the
+        median and quantiles (default 0.1 / 0.9) or the mean -/+ bounds (default 2) standard
+        deviations across sims. Returns results like a Sim's, each Result carrying low and high."""
+        out, out_v = self.device_results()
+        parts = [cvres.reduce_tables(t, quantiles=quantiles, use_mean=use_mean, bounds=bounds) for t in (out, out_v)]
+        (c, lo, hi), (cv, lov, hiv) = [[x.cpu().numpy() for x in p] for p in parts]
+        R = cvres.to_results(c, cv)
+        for i, name in enumerate(cvres.F_ROW_NAMES):
+            R[name].low, R[name].high = lo[i], hi[i]
+        for i, name in enumerate(cvres.FV_ROW_NAMES):
+            R['variant'][name].low, R['variant'][name].high = lov[i], hiv[i]
+        return R
 
     def infection_log(self):
         """Records (sim, source, target, date, layer, variant); source/target are indices within the sim."""

@@ -1087,6 +1087,14 @@ extern "C" int cvk_batch_copy_results(void* handle, uint32_t* results_host, doub
   return CVK_OK;
 }
 
+extern "C" int cvk_batch_results(void* handle, const uint32_t** results_out, const double** results_f_out) {
+  if (!handle) return fail(CVK_E_INVALID, "cvk_batch_results: null handle");
+  Batch& b = *(Batch*)handle;
+  if (results_out) *results_out = b.d_results;
+  if (results_f_out) *results_f_out = b.d_results_f;
+  return CVK_OK;
+}
+
 extern "C" int cvk_batch_copy_log(void* handle, int32_t* log_host, int64_t capacity, int64_t* count_out) {
   if (!handle || !count_out) return fail(CVK_E_INVALID, "cvk_batch_copy_log: null argument");
   Batch& b = *(Batch*)handle;

@@ -14,6 +14,7 @@ import time
 import numpy as np
 
 from . import _lib
+from . import results as cvres
 from . import immunity as cvimm
 from . import parameters as cvpar
 from . import population as cvpop
@@ -361,64 +362,44 @@ class Sim:
         resf = self._w['results_f'].cpu().numpy()
         return res, resf
 
-    def finalize(self, raw=None):
-        """Per-day results, cumulative flows and derived states (sim.py:764-850).
+    def finalize(self, rows=None):
+        """Per-day results, cumulative flows, derived states, doubling time and r_eff
+        (sim.py:764-1001), computed on the device by cvk_finalize.
 
-        raw: optional (counts [npts][cols], float sums [npts][4]) from another runner (BatchSim).
+        rows: optional (rows [F_ROWS][npts], variant rows [FV_ROWS][nv][npts], raw counts
+        [npts][cols]) already finalised for this sim by a batch (BatchSim).
         """
         if self.results_ready:
             raise AlreadyRunError('Simulation has already been finalized')
-        res, resf = raw if raw is not None else self.raw_results()
-        npts, nv = self.npts, int(self.pars['n_variants'])
-        R = {}
-        col = _lib.R_COL
-        for key in RESULT_FLOWS:
-            name = f'new_{key}'
-            R[name] = Result(name, values=res[:, col[name]].copy() if name in col else np.zeros(npts))
-        for key in RESULT_STOCKS:
-            R[f'n_{key}'] = Result(f'n_{key}', values=res[:, col[f'n_{key}']].copy())
-        for key in RESULT_FLOWS:
-            R[f'cum_{key}'] = Result(f'cum_{key}', values=np.cumsum(R[f'new_{key}'].values))
-        R['cum_infections'].values += self.pars['pop_infected']
-        imp = np.zeros(npts)
-        days, vars_, nimp = self._imp_arrays
-        for j in range(len(days)):
-            imp[days[j]] += nimp[j]
-        R['n_imports'] = Result('n_imports', values=imp)
-        n_alive_agents = res[:, col['n_alive']]
-        with np.errstate(divide='ignore', invalid='ignore'):  # 0/0 -> nan when everyone died, as sim.py:672
-            R['pop_nabs'] = Result('pop_nabs', scale=False, values=resf[:, 0] / n_alive_agents)
-        R['pop_protection'] = Result('pop_protection', scale=False, values=resf[:, 1] / (nv * self.people.n))
-        R['pop_symp_protection'] = Result('pop_symp_protection', scale=False, values=resf[:, 2] / (nv * self.people.n))
-        V = {}
-        for k, name in enumerate(_lib.RV_NAMES):
-            base = _lib.R_BYVAR + k * _lib.MAX_VARIANTS
-            V[name] = Result(name, values=res[:, base:base + nv].T.copy())
-        for key in RESULT_FLOWS_BY_VARIANT:
-            V[f'cum_{key}'] = Result(f'cum_{key}', values=np.cumsum(V[f'new_{key}'].values, axis=1))
-        V['cum_infections_by_variant'].values[0] += self.pars['pop_infected']
-        R['variant'] = V
-        self.raw = {'n_susceptible_agents': res[:, col['n_susceptible']].copy()}
+        if rows is None:
+            d = self.people._d
+            out, out_v = cvres.finalize_tables(
+                self._w['results'], self._w['results_f'], self.npts, int(self.pars['n_variants']),
+                self.pars['use_waning'], [self.pars['pop_infected']], [cvres.initial_period(self.pars, self.npts)],
+                [0, self.people.n], self.import_schedule()[None, :], d['date_infectious'], d['date_recovered'],
+                d['date_dead'])
+            counts = self._w['results'].cpu().numpy().view(np.uint32)
+            rows = (out[0].cpu().numpy(), out_v[0].cpu().numpy(), counts)
+        R = cvres.to_results(rows[0], rows[1])
+        self.raw = {'n_susceptible_agents': rows[2][:, _lib.R_COL['n_susceptible']].astype(np.float64)}
         self.results = R
-        self.compute_states()
         self.results_ready = True
         self.t = self.npts - 1
 
+    def import_schedule(self):
+        """Imported agents per day, from the variants' import events (immunity.py:117-130)."""
+        imp = np.zeros(self.npts)
+        days, vars_, nimp = self._imp_arrays
+        for j in range(len(days)):
+            imp[days[j]] += nimp[j]
+        return imp
+
     def compute_states(self):
-        """Prevalence, incidence and the recomputed susceptible count (sim.py:829-850)."""
-        R = self.results
-        count_recov = 1 - int(bool(self.pars['use_waning']))
-        pop = float(self.pars['pop_size'])
-        R['n_alive'] = Result('n_alive', values=pop - R['cum_deaths'].values)
-        R['n_naive'] = Result('n_naive', values=pop - R['cum_deaths'].values - R['n_recovered'].values
-                              - R['n_exposed'].values)
-        R['n_susceptible'].values = R['n_alive'].values - R['n_exposed'].values - count_recov * R['cum_recoveries'].values
-        R['n_preinfectious'] = Result('n_preinfectious', values=R['n_exposed'].values - R['n_infectious'].values)
-        R['n_removed'] = Result('n_removed', values=count_recov * R['cum_recoveries'].values + R['cum_deaths'].values)
-        with np.errstate(divide='ignore', invalid='ignore'):
-            R['prevalence'] = Result('prevalence', scale=False, values=R['n_exposed'].values / R['n_alive'].values)
-            R['incidence'] = Result('incidence', scale=False,
-                                    values=R['new_infections'].values / R['n_susceptible'].values)
+        """Prevalence, incidence and the recomputed susceptible count (sim.py:828-850): part of
+        finalize, kept for callers of the reference's method name."""
+        if not self.results_ready:
+            self.finalize()
+        return self.results
 
     @property
     def summary(self):

@@ -23,6 +23,8 @@
  *   cvk_run_days       covasim/sim.py:558-685   Sim.step, repeated (the day loop of Sim.run, sim.py:733-755)
  *   cvk_pop_*          covasim/population.py:143-364 make_randpop / make_*_contacts,
  *                      people.py:139-160 set_prognoses (population generation, before the day loop)
+ *   cvk_finalize       covasim/sim.py:764-1001 finalize, compute_states, compute_doubling, compute_r_eff
+ *   cvk_reduce         covasim/run.py:220-303  MultiSim.reduce (median / quantiles or mean / std)
  */
 #ifndef COVASIM_AMD_H
 #define COVASIM_AMD_H
@@ -384,6 +386,63 @@ int cvk_batch_copy_results(void* handle, uint32_t* results_host, double* results
 /* Synchronous copy of up to capacity infection records (source, target, day, layer | variant << 8
  * | sim << 12; source and target are indices within the sim); *count_out = records written. */
 int cvk_batch_copy_log(void* handle, int32_t* log_host, int64_t capacity, int64_t* count_out);
+/* Device pointers of the batch's result tables (valid while the handle lives): results
+ * [n_sims][npts][CVK_R_COLS] uint32 and results_f [n_sims][npts][CVK_RF_COLS] double. */
+int cvk_batch_results(void* handle, const uint32_t** results_out, const double** results_f_out);
+
+/* ---- results: finalisation and reduction over sims ----
+ * cvk_finalize replaces covasim/sim.py:764-850 (Sim.finalize: cumulative flows, compute_states),
+ * :872-895 (compute_doubling) and :898-1001 (compute_r_eff, method 'daily') for one sim or a batch,
+ * on the device. Output rows are doubles [n_sims][CVK_F_ROWS][npts] and, by variant,
+ * [n_sims][CVK_FV_ROWS][n_variants][npts]. Flows follow the order new_ / cum_ + infections,
+ * reinfections, infectious, symptomatic, severe, critical, recoveries, deaths, tests, diagnoses,
+ * known_deaths, quarantined, isolated, doses, vaccinated (defaults.py result_flows; flows the day
+ * loop does not produce are 0); stocks n_ + susceptible, exposed, infectious, symptomatic, severe,
+ * critical, recovered, dead, diagnosed, known_dead, quarantined, isolated, vaccinated. */
+#define CVK_F_N_FLOWS 15
+#define CVK_F_N_STOCKS 13
+enum {
+  CVK_F_NEW = 0,                              /* + flow */
+  CVK_F_CUM = CVK_F_NEW + CVK_F_N_FLOWS,      /* + flow */
+  CVK_F_STOCK = CVK_F_CUM + CVK_F_N_FLOWS,    /* + stock; n_susceptible recomputed as compute_states does */
+  CVK_F_N_ALIVE = CVK_F_STOCK + CVK_F_N_STOCKS,
+  CVK_F_N_NAIVE, CVK_F_N_PREINFECTIOUS, CVK_F_N_REMOVED, CVK_F_PREVALENCE, CVK_F_INCIDENCE,
+  CVK_F_FRAC_VACCINATED, CVK_F_N_IMPORTS, CVK_F_POP_NABS, CVK_F_POP_PROTECTION, CVK_F_POP_SYMP_PROTECTION,
+  CVK_F_DOUBLING_TIME, CVK_F_R_EFF,
+  CVK_F_ROWS
+};
+enum {
+  CVK_FV_NEW = 0,       /* + 0 infections, 1 symptomatic, 2 severe, 3 infectious (by variant) */
+  CVK_FV_CUM = 4,       /* + the same four */
+  CVK_FV_N_EXPOSED = 8, CVK_FV_N_INFECTIOUS, CVK_FV_INCIDENCE, CVK_FV_PREVALENCE,
+  CVK_FV_ROWS
+};
+
+typedef struct {
+  int32_t n_sims, npts, n_variants, use_waning;
+  int32_t r_eff_smoothing;          /* sc.smooth repeats of compute_r_eff (default 2) */
+  int32_t doubling_window;          /* compute_doubling window (default 3) */
+  double max_doubling_time;         /* default 30 */
+  const double* pop_infected_host;  /* [n_sims] */
+  const int32_t* initial_period_host;  /* [n_sims]: int(min(npts, exp2inf par1 + asym2rec par1)), sim.py:934-935 */
+  const int64_t* agent_offsets_host;   /* [n_sims + 1]: sim s owns agents [off[s], off[s+1]) */
+} cvk_finalize_desc;
+
+int64_t cvk_finalize_work_bytes(const cvk_finalize_desc* desc);
+/* results / results_f: the day loop's tables ([n_sims][npts][...], as cvk_batch_results or a
+ * cvk_work's). imports: [n_sims][npts] imported agents per day (NULL = none). Dates: the agents'
+ * float32 date_infectious, date_recovered, date_dead (NaN = undefined) over all sims' agents.
+ * Synchronous on `stream` (per-sim offsets are uploaded into `work`). */
+int cvk_finalize(const cvk_finalize_desc* desc, const uint32_t* results, const double* results_f,
+                 const double* imports, const float* date_infectious, const float* date_recovered,
+                 const float* date_dead, double* out, double* out_variant, void* work, int64_t work_bytes,
+                 void* stream);
+/* MultiSim.reduce (covasim/run.py:220-303) over n_sims rows of n_cols doubles (values[s][c]):
+ * use_mean = 0: center = median, low / high = quantiles q_low, q_high (numpy's 'linear' method; a
+ * column holding NaN gives NaN); use_mean = 1: center = mean, low / high = mean -/+ bounds * std
+ * (numpy's pairwise sums, ddof 0). n_sims <= CVK_BATCH_MAX_SIMS. */
+int cvk_reduce(int32_t n_sims, int64_t n_cols, const double* values, int32_t use_mean, double q_low, double q_high,
+               double bounds, double* center, double* low, double* high, void* stream);
 
 /* ---- population generation ----
  * Replaces covasim/population.py:143-226 make_randpop (ages, sexes), :241-284 make_random_contacts,

@@ -1,0 +1,193 @@
+"""Results finalisation and multi-sim reduction (covasim/sim.py:764-1001, covasim/run.py:220-303).
+
+CPU: the numpy restatement (oracle/results_ref.py) is pinned by closed forms: cumulative flows of
+constant series, the doubling time of an exact exponential, r_eff's mean infectious duration on a
+hand-built population, the smoothing kernel's binomial spread of an impulse.
+GPU: cvk_finalize equals the restatement on real runs (single sims and a batch): bit for bit on
+flows, stocks and derived states, within 1e-13 relative on the doubling time (log) and 1e-5
+relative on r_eff (the reference averages float32 dates in float32, the device in float64).
+cvk_reduce equals numpy's quantile / mean / std over the sims.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import results_ref as rr
+
+
+def test_smooth_spreads_an_impulse_binomially():
+    x = np.zeros(11)
+    x[5] = 16.0
+    y = rr.smooth(x, 2)
+    assert np.array_equal(y[3:8], [1, 4, 6, 4, 1])
+    assert y.sum() == 16.0
+    c = rr.smooth(np.full(9, 3.0), 1)
+    assert np.array_equal(c[1:-1], np.full(7, 3.0)) and c[0] == c[-1] == 2.25  # zero beyond the ends
+
+
+def test_doubling_time_of_an_exponential():
+    t = np.arange(30)
+    cum = 10.0 * 2.0 ** (t / 5.0)
+    d = rr.compute_doubling(cum)
+    assert np.all(np.isnan(d[:3]))
+    assert np.allclose(d[3:], 5.0, rtol=1e-12)
+    slow = rr.compute_doubling(10.0 * 2.0 ** (t / 100.0))
+    assert np.all(slow[3:] == 30)  # capped at max_doubling_time
+    flat = rr.compute_doubling(np.full(10, 7.0))
+    assert np.all(np.isnan(flat))
+
+
+def test_r_eff_mean_infectious_duration():
+    nan = np.nan
+    d_inf = np.array([2, 5, 8, nan], np.float32)
+    d_rec = np.array([10, nan, 20, nan], np.float32)
+    d_dead = np.array([nan, 15, nan, nan], np.float32)
+    new = np.array([4.0, 6.0, 0.0, 3.0, 1.0])
+    imp = np.array([0.0, 2.0, 0.0, 0.0, 0.0])
+    ninf = np.array([2.0, 4.0, 0.0, 3.0, 2.0])
+    # outcomes 10, 20, 15 (mean 15) of agents infectious on days 2, 8, 5 (mean 5): 10 days
+    r = rr.compute_r_eff(new, imp, ninf, d_inf, d_rec, d_dead, initial_period_par=0, smoothing=0)
+    assert np.allclose(r, 10 * np.array([2.0, 1.0, 0.0, 1.0, 0.5]))
+    r2 = rr.compute_r_eff(new, imp, ninf, d_inf, d_rec, d_dead, initial_period_par=2.5, smoothing=0)
+    assert np.allclose(r2, [15.0, 10.0, 0.0, 10.0, 5.0])  # day 0 averages days 0..1
+    r3 = rr.compute_r_eff(new, imp, ninf, d_inf, d_rec, d_dead, initial_period_par=0, smoothing=1)
+    assert np.allclose(r3, [20.0, 10.0, 5.0, 6.25, 5.0])  # ends kept, interior (0.25, 0.5, 0.25)
+    none = rr.compute_r_eff(new, imp, ninf, d_inf, np.full(4, nan, np.float32), np.full(4, nan, np.float32), 0, 2)
+    assert np.all(none == 0)
+
+
+def test_finalize_closed_forms():
+    from covasim_amd import _lib
+    npts, nv = 6, 1
+    counts = np.zeros((npts, _lib.R_COLS), np.uint32)
+    col = _lib.R_COL
+    counts[:, col['new_infections']] = 2
+    counts[:, col['new_deaths']] = [0, 0, 1, 0, 1, 0]
+    counts[:, col['new_recoveries']] = 1
+    counts[:, col['n_exposed']] = 10
+    counts[:, col['n_infectious']] = 4
+    counts[:, col['n_recovered']] = np.arange(npts)
+    counts[:, col['n_alive']] = 100
+    counts_f = np.zeros((npts, _lib.RF_COLS))
+    nan = np.full(5, np.nan, np.float32)
+    R = rr.finalize(counts, counts_f, col, _lib.R_BYVAR, _lib.MAX_VARIANTS, nv, 100.0, 5, False, None, nan, nan, nan,
+                    0.0)
+    assert np.array_equal(R['cum_infections'], 2.0 * np.arange(1, npts + 1) + 5)
+    assert np.array_equal(R['n_alive'], 100 - np.array([0, 0, 1, 1, 2, 2.0]))
+    assert np.array_equal(R['n_susceptible'], R['n_alive'] - 10 - np.arange(1, npts + 1))
+    assert np.array_equal(R['prevalence'], 10 / R['n_alive'])
+    assert np.array_equal(R['n_removed'], np.arange(1, npts + 1) + R['cum_deaths'])
+    W = rr.finalize(counts, counts_f, col, _lib.R_BYVAR, _lib.MAX_VARIANTS, nv, 100.0, 5, True, None, nan, nan, nan,
+                    0.0)
+    assert np.array_equal(W['n_susceptible'], W['n_alive'] - 10)  # waning: recovered agents count as susceptible
+
+
+# ---------------------------------------------------------------------------------------------
+# GPU
+# ---------------------------------------------------------------------------------------------
+def oracle_rows(sim, batch=None, index=0):
+    """The restatement applied to a finished covasim_amd Sim's tables and dates (read from the
+    batch that ran it, when given)."""
+    from covasim_amd import _lib
+    from covasim_amd import results as cvres
+    if batch is None:
+        counts = sim._w['results'].cpu().numpy().view(np.uint32)
+        counts_f = sim._w['results_f'].cpu().numpy()
+        get = sim.people.get
+    else:
+        res, resf = batch.raw_results()
+        counts, counts_f = res[index], resf[index]
+
+        def get(key):
+            return batch.get(key, index)
+    dur = sim.pars['dur']
+    return rr.finalize(counts, counts_f, _lib.R_COL, _lib.R_BYVAR, _lib.MAX_VARIANTS, int(sim.pars['n_variants']),
+                       float(sim.people.n), sim.pars['pop_infected'], sim.pars['use_waning'], sim.import_schedule(),
+                       get('date_infectious'), get('date_recovered'), get('date_dead'),
+                       dur['exp2inf']['par1'] + dur['asym2rec']['par1']), cvres
+
+
+def assert_rows_match(sim, want, cvres):
+    for name in cvres.F_ROW_NAMES:
+        got, exp = sim.results[name].values, np.asarray(want[name], dtype=float)
+        if name == 'doubling_time':
+            assert np.array_equal(np.isnan(got), np.isnan(exp)), name
+            np.testing.assert_allclose(got[~np.isnan(got)], exp[~np.isnan(exp)], rtol=1e-13, err_msg=name)
+        elif name == 'r_eff':
+            np.testing.assert_allclose(got, exp, rtol=1e-5, atol=1e-9, err_msg=name)
+        else:
+            assert np.array_equal(got, exp, equal_nan=True), (name, got[:5], exp[:5])
+    for name in cvres.FV_ROW_NAMES:
+        assert np.array_equal(sim.results['variant'][name].values, want['variant'][name], equal_nan=True), name
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', ['hybrid', 'variants_waning'])
+def test_device_finalize_matches_restatement(case):
+    import covasim_amd as cv
+    from tests.helpers import copy_pop, make_population
+    print(f'finalize case {case}', flush=True)
+    if case == 'hybrid':
+        pop = make_population(n=20000, pop_type='hybrid')
+        sim = cv.Sim(pop_size=20000, pop_type='hybrid', n_days=60, rand_seed=3, beta=0.03, popdict=copy_pop(pop),
+                     dynam_layer=dict(c=1))
+    else:
+        pop = make_population(n=20000, pop_type='random')
+        sim = cv.Sim(pop_size=20000, n_days=50, rand_seed=5, beta=0.02, use_waning=True, popdict=copy_pop(pop),
+                     variants=[cv.variant(dict(rel_beta=1.7), days=[10], n_imports=30)])
+    sim.run()
+    want, cvres = oracle_rows(sim)
+    assert_rows_match(sim, want, cvres)
+    assert sim.results['cum_infections'].values[-1] > 100
+    assert np.isfinite(sim.results['r_eff'].values).all()
+
+
+@pytest.mark.gpu
+def test_batch_finalize_and_reduce():
+    import torch
+    import covasim_amd as cv
+    from oracle import results_ref
+    from tests.helpers import copy_pop, make_population
+    sims = []
+    for i, n in enumerate([5000, 12000, 8000, 3000]):
+        pop = make_population(n=n, pop_type='hybrid', seed=i + 1)
+        sims.append(cv.Sim(pop_size=n, pop_type='hybrid', n_days=40, rand_seed=10 + i, beta=0.03 + 0.005 * i,
+                           popdict=copy_pop(pop), dynam_layer=dict(c=1)))
+    b = cv.BatchSim(sims)
+    b.run()
+    for i, sim in enumerate(sims):
+        want, cvres = oracle_rows(sim, batch=b, index=i)
+        assert_rows_match(sim, want, cvres)
+    red = b.reduce()
+    med = b.reduce(quantiles=[0.25, 0.75])
+    mean = b.reduce(use_mean=True, bounds=1.5)
+    stack = np.stack([np.stack([s.results[k].values for k in cvres.F_ROW_NAMES]) for s in sims])
+    for got, kw in [(red, {}), (med, dict(quantiles=(0.25, 0.75))), (mean, dict(use_mean=True, bounds=1.5))]:
+        c, lo, hi = results_ref.reduce(stack, **kw)
+        for i, k in enumerate(cvres.F_ROW_NAMES):
+            assert np.array_equal(got[k].values, c[i], equal_nan=True), k
+            assert np.array_equal(got[k].low, lo[i], equal_nan=True), k
+            assert np.array_equal(got[k].high, hi[i], equal_nan=True), k
+    torch.cuda.synchronize()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('S', [1, 2, 7, 256, 1000, 4096])
+def test_device_reduce_matches_numpy(S):
+    import torch
+    from covasim_amd import results as cvres
+    rng = np.random.default_rng(S)
+    vals = rng.normal(size=(S, 3, 37)) * 10.0 ** rng.integers(-3, 4, size=(1, 3, 37))
+    vals[:, 0, :5] = np.round(vals[:, 0, :5])  # ties
+    vals[rng.integers(S), 1, 3] = np.nan
+    vals[:, 2, 7] = np.inf if S > 1 else 1.0
+    t = torch.as_tensor(vals, device='cuda')
+    for kw in [dict(), dict(quantiles={'low': 0.05, 'high': 0.95}), dict(quantiles=[0.0, 1.0]),
+               dict(use_mean=True), dict(use_mean=True, bounds=3)]:
+        got = [x.cpu().numpy() for x in cvres.reduce_tables(t, **kw)]
+        q = kw.get('quantiles', (0.1, 0.9))
+        q = (q['low'], q['high']) if isinstance(q, dict) else tuple(q)
+        with np.errstate(invalid='ignore'):
+            want = rr.reduce(vals, quantiles=q, use_mean=kw.get('use_mean', False), bounds=kw.get('bounds', 2))
+        for g, w in zip(got, want):
+            assert np.array_equal(g, w, equal_nan=True), (kw, np.argwhere(~((g == w) | (np.isnan(g) & np.isnan(w))))[:5])
