@@ -158,7 +158,8 @@ def reset_workspace(w):
 
 class ImportsS(C.Structure):
     _fields_ = [('n', C.c_int32), ('days', C.POINTER(C.c_int32)), ('variants', C.POINTER(C.c_int32)),
-                ('n_imports', C.POINTER(C.c_int64))]
+                ('n_imports', C.POINTER(C.c_int64)), ('n_beta_days', C.c_int32), ('beta_v_days', C.c_void_p),
+                ('beta_layer_days', C.c_void_p)]
 
 
 BATCH_MAX_SIMS = 4096

@@ -50,6 +50,8 @@ class BatchSim:
                 raise ValueError('BatchSim takes sims that have not started running')
             if sim._custom_update:
                 raise NotImplementedError('custom Layer.update subclasses are not supported in a batch')
+            if sim.pars['interventions']:
+                raise NotImplementedError('interventions are supported on single sims, not in a batch')
         s0 = sims[0]
         lkeys = s0.layer_keys()
         for sim in sims[1:]:

@@ -887,7 +887,16 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
   // Day t+1's resample runs as extra workgroups of day t's infect pass (store-bound work beside
   // latency-bound work); the infect pass then takes dynamic-layer sources from the resample stream.
   const bool overlap = !gen && g_overlap_resample && t1 - t0 > 1 && has_dynamic(layers, params);
+  const bool sched = imports && imports->n_beta_days > 0 && imports->beta_v_days && imports->beta_layer_days;
+  cvk_params day_params = *params;
   for (int32_t t = t0; t < t1; ++t) {
+    if (sched) {  // this day's rates; kernel arguments are copied at launch (and into a captured graph)
+      const int32_t row = std::min<int32_t>(t, imports->n_beta_days - 1);
+      memcpy(day_params.beta_v, imports->beta_v_days + (int64_t)row * CVK_MAX_VARIANTS, sizeof(day_params.beta_v));
+      memcpy(day_params.beta_layer, imports->beta_layer_days + (int64_t)row * CVK_MAX_LAYERS,
+             sizeof(day_params.beta_layer));
+      params = &day_params;
+    }
     if (t == 0 && pop_infected > 0) {  // init_infections, sim.py:505-532 (flows not counted)
       rc = choose_impl(people, params, work, 0, STREAM_CHOOSE, pop_infected, 0, CVK_LAYER_SEED, 0, st);
       if (rc) return rc;

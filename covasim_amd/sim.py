@@ -15,6 +15,7 @@ import numpy as np
 
 from . import _lib
 from . import results as cvres
+from . import interventions as cviv
 from . import immunity as cvimm
 from . import parameters as cvpar
 from . import population as cvpop
@@ -103,8 +104,12 @@ class Sim:
             if self.pars.get(k, v) != v:
                 raise NotImplementedError(f'Parameter {k}={self.pars[k]!r} is outside the GPU transmission path '
                                           f'(supported value: {v!r})')
-        if self.pars['interventions'] or self.pars['analyzers']:
-            raise NotImplementedError('Interventions and analyzers are outside the GPU transmission path')
+        if self.pars['analyzers']:
+            raise NotImplementedError('Analyzers are outside the GPU transmission path')
+        for iv in self.pars['interventions']:
+            if not isinstance(iv, cviv.SUPPORTED):
+                raise NotImplementedError(f'Intervention {iv!r} is outside the GPU transmission path (supported: '
+                                          f'{", ".join(c.__name__ for c in cviv.SUPPORTED)})')
         if int(self.pars['n_days']) < 0:
             raise ValueError('n_days must be non-negative')
 
@@ -130,6 +135,8 @@ class Sim:
         _lib.lib()
         self.validate_pars()
         self.init_variants()
+        for iv in self.pars['interventions']:
+            iv.initialize(self)
         if self.pars['prognoses'] is None:
             self.pars['prognoses'] = cvpar.get_prognoses(self.pars['prog_by_age'])
         if self.pars['use_waning']:
@@ -204,14 +211,15 @@ class Sim:
         p.vl_lo = (lr * zero + one * one) / den
         p.asymp_factor = f32(pars['asymp_factor'])
         labels = [pars['variant_map'][v] for v in range(p.n_variants)]
+        beta_v, beta_layer = self._rates(pars)
         for v, lab in enumerate(labels):
             vp = pars['variant_pars'][lab]
-            p.beta_v[v] = f32(pars['beta'] * pars['rel_beta'] * vp['rel_beta'])  # sim.py:627
+            p.beta_v[v] = beta_v[v]
             for k in ('rel_symp_prob', 'rel_severe_prob', 'rel_crit_prob', 'rel_death_prob'):
                 val = pars[k] * vp[k] if v else pars[k]  # people.py:477-481
                 getattr(p, k)[v] = f32(val)
         for li, lk in enumerate(lkeys):
-            p.beta_layer[li] = f32(pars['beta_layer'].get(lk, 1.0))
+            p.beta_layer[li] = beta_layer[li]
             p.iso_factor[li] = f32(pars['iso_factor'].get(lk, 0.2))
             p.quar_factor[li] = f32(pars['quar_factor'].get(lk, 0.3))
         p.no_hosp_factor = f32(pars['no_hosp_factor'])
@@ -268,11 +276,34 @@ class Sim:
         imp.days = self._imp_arrays[0].ctypes.data_as(C.POINTER(C.c_int32))
         imp.variants = self._imp_arrays[1].ctypes.data_as(C.POINTER(C.c_int32))
         imp.n_imports = self._imp_arrays[2].ctypes.data_as(C.POINTER(C.c_int64))
+        # per-day rates of change_beta interventions (interventions.py:530-586)
+        self._beta_days = cviv.beta_schedule(self, self._rates)
+        if self._beta_days is not None:
+            imp.n_beta_days = self.npts
+            imp.beta_v_days = self._beta_days[0].ctypes.data
+            imp.beta_layer_days = self._beta_days[1].ctypes.data
         self._imports = imp
         self._custom_update = [lk for lk in lkeys if dyn.get(lk, 0) and type(people.contacts[lk]).update
                                is not Layer.update]
 
     # -- running --
+    def _rates(self, pars):
+        """float32 beta per variant (sim.py:627) and per layer (compute_trans_sus's beta_layer)."""
+        f32 = np.float32
+        beta_v = [f32(pars['beta'] * pars['rel_beta'] * pars['variant_pars'][pars['variant_map'][v]]['rel_beta'])
+                  for v in range(int(pars['n_variants']))]
+        beta_layer = [f32(pars['beta_layer'].get(lk, 1.0)) for lk in self.layer_keys()]
+        return beta_v, beta_layer
+
+    def _apply_schedule(self, t):
+        """Day t's rates from the interventions' per-day table, into the parameters of a manual day."""
+        if self._beta_days is None:
+            return
+        row = min(t, len(self._beta_days[0]) - 1)
+        for v in range(_lib.MAX_VARIANTS):
+            self._params.beta_v[v] = self._beta_days[0][row, v]
+        for li in range(_lib.MAX_LAYERS):
+            self._params.beta_layer[li] = self._beta_days[1][row, li]
     def _refresh_layers(self):
         for li, lk in enumerate(self.layer_keys()):
             lay = self.people.contacts[lk]
@@ -315,6 +346,7 @@ class Sim:
             raise AlreadyRunError('Simulation already complete (call sim.initialize() to re-run)')
         # parameters edited in place between days (e.g. beta_layer[k] *= f, interventions.py:575-586) apply
         self._build_params()
+        self._apply_schedule(self.t)
         self._manual_day(self.t)
         self.t += 1
         self.people.t = self.t

@@ -305,6 +305,13 @@ typedef struct {
   const int32_t* days;
   const int32_t* variants;
   const int64_t* n_imports;
+  /* Per-day transmission rates, as interventions set them (interventions.py:530-586 change_beta):
+   * day t uses row min(t, n_beta_days - 1) of beta_v_days [n_beta_days][CVK_MAX_VARIANTS] (the
+   * float32 beta * rel_beta * variant rel_beta of sim.py:627) and of beta_layer_days
+   * [n_beta_days][CVK_MAX_LAYERS]; n_beta_days = 0 (or NULL tables) keeps cvk_params' values. */
+  int32_t n_beta_days;
+  const float* beta_v_days;
+  const float* beta_layer_days;
 } cvk_imports;
 
 /* Runs days [t0, t1) (t0 == 0 also seeds pop_infected infections first). use_graph != 0

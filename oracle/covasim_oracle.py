@@ -725,6 +725,7 @@ class Sim:
         self.layers = {k: [np.asarray(v[0], np.int32).copy(), np.asarray(v[1], np.int32).copy(),
                            np.asarray(v[2], F32).copy()] for k, v in layers.items()}
         self.rng = rng
+        self.interventions = []  # called each day after the variant imports, as sim.py:591-593
         self.t = 0
         nv = pars['n_variants']
         self.results = {k: np.zeros(self.npts) for k in FLOW_KEYS}
@@ -758,6 +759,8 @@ class Sim:
                 imp = rng.choose_imports(sus_inds, n_imports, t, vi)
                 people.infect(inds=imp, rng=rng, layer='importation', variant=vi)
                 self.results['n_imports'][t] += n_imports
+        for intervention in self.interventions:  # sim.py:591-593
+            intervention(self)
 
         vd = pars['viral_dist']
         viral_load = compute_viral_load(t, people.date_infectious, people.date_recovered, people.date_dead,
