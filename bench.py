@@ -47,6 +47,11 @@ WORKLOADS = {
                     desc='differentiable relaxation on the hybrid 1M population (static layers), 60 days: '
                          'float32 forward + hand-written adjoint with checkpoints every 10 days, gradient wrt '
                          '4 layer betas, beta and 10 age-bin rel_sus; calibration to a stochastic incidence curve'),
+    'interv1m': dict(pop_size=1_000_000, pop_type='hybrid', n_days=180, households='categorical', interventions=True,
+                     pars=dict(beta=0.016, use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
+                     desc='hybrid1m with interventions: change_beta(30, 0.6) and clip_edges([40, 90], [0.5, 1.0], '
+                          'layers s, w); clip days run through the individual passes, the days between them as '
+                          'captured graphs; the change_beta-only run is one graph'),
     'popgen10m': dict(pop_size=10_000_000, pop_type='hybrid', households='categorical', popgen=True,
                       pars=dict(),
                       desc='population build of the hybrid 10M population on the device: ages, age-correlated '
@@ -346,6 +351,44 @@ def run_popgen(w, args):
     print(json.dumps(line))
 
 
+def run_interventions(w, args):
+    """hybrid1m with interventions: a change_beta-only run (one captured graph, per-day rates) and a
+    run that also clips edges (clip days through the individual passes, graphs between them). One
+    step = one day; value = the clip run's agent-days/s, wall time of Sim.run without finalisation."""
+    import torch
+    import covasim_amd as cv
+    pop = build_population(w, args.seed)
+    n = w['pop_size']
+
+    def copy():
+        out = {k: v for k, v in pop.items() if k != 'contacts'}
+        out['contacts'] = {k: {c: a.copy() for c, a in lay.items()} for k, lay in pop['contacts'].items()}
+        return out
+
+    def one(ivs):
+        sim = cv.Sim(pop_size=n, pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=args.seed, popdict=copy(),
+                     households=w['households'], interventions=ivs, **w['pars'])
+        sim.initialize()
+        torch.cuda.synchronize()
+        sim.run()
+        return sim
+    one([cv.change_beta(30, 0.6)])  # warm-up
+    beta_only = one([cv.change_beta(30, 0.6)])
+    clipped = one([cv.change_beta(30, 0.6), cv.clip_edges([40, 90], [0.5, 1.0], layers=['s', 'w'])])
+    days = clipped.npts
+    line = {
+        'metric': 'agent-days/sec at 1M hybrid with interventions', 'value': n * days / clipped.elapsed,
+        'unit': 'agent-days/s', 'n_gpus': 1, 'steps': days, 'warmup': days, 'ms_per_step': clipped.elapsed / days * 1e3,
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': w['desc'], 'agents': n, 'days': days},
+        'change_beta_only': {'ms_per_day': beta_only.elapsed / days * 1e3,
+                             'agent_days_per_s': n * days / beta_only.elapsed,
+                             'cum_infections': float(beta_only.results['cum_infections'].values[-1])},
+        'clip_run': {'cum_infections': float(clipped.results['cum_infections'].values[-1])},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
@@ -404,6 +447,8 @@ def main():
         return run_batch(w, args)
     if w.get('popgen'):
         return run_popgen(w, args)
+    if w.get('interventions'):
+        return run_interventions(w, args)
     if w.get('relax'):
         return run_relax(w, args)
     steps = args.steps or w['n_days']

@@ -13,10 +13,10 @@ from .base import Result, Layer, Contacts
 from .people import People
 from .sim import Sim, AlreadyRunError
 from .batch import BatchSim
-from .interventions import Intervention, change_beta
+from .interventions import Intervention, change_beta, clip_edges
 from . import utils
 from .utils import set_seed
 
-__all__ = ['__version__', 'Intervention', 'change_beta', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
+__all__ = ['__version__', 'Intervention', 'change_beta', 'clip_edges', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
            'get_cross_immunity', 'variant', 'Result', 'Layer', 'Contacts', 'People', 'Sim', 'AlreadyRunError', 'BatchSim',
            'utils', 'set_seed']

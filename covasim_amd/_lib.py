@@ -48,6 +48,7 @@ STREAM_INFECT = 0x20000000
 STREAM_CHOOSE = 0x30000000
 STREAM_IMPORT = 0x40000000
 STREAM_RESAMPLE = 0x50000000
+STREAM_CLIP = 0x80000000
 
 # result columns
 R_NAMES = ['new_infections', 'new_reinfections', 'new_infectious', 'new_symptomatic', 'new_severe',
@@ -276,6 +277,10 @@ def lib():
     L.cvk_finalize_work_bytes.restype = i64
     L.cvk_finalize.argtypes = [FD, vp, vp, vp, vp, vp, vp, vp, vp, vp, i64, vp]
     L.cvk_finalize.restype = C.c_int
+    L.cvk_move_edges_work_bytes.argtypes = [i64]
+    L.cvk_move_edges_work_bytes.restype = i64
+    L.cvk_move_edges.argtypes = [vp, vp, vp, i64, i64, u64, i32, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]
+    L.cvk_move_edges.restype = C.c_int
     L.cvk_reduce.argtypes = [i32, i64, vp, i32, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
     L.cvk_reduce.restype = C.c_int
     PG, PC = C.POINTER(PopGen), C.POINTER(PopCounts)

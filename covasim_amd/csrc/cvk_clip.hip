@@ -1,0 +1,139 @@
+// Edge moves of clip_edges (covasim/interventions.py:589-673): k of a layer's n edges, chosen
+// uniformly without replacement, leave it for another layer (the intervention's store, or back).
+//
+// Choice: edge e gets the 64-bit key (Philox word << 32) | e, distinct for every edge; the moved
+// edges are those whose key is at most the k-th smallest. The k-th smallest key comes from an 8-pass
+// radix select (one 8-bit digit per pass: a 256-bucket histogram of the keys that share the digits
+// found so far, then one workgroup picks the digit where the count reaches k). The edges then split
+// by one exclusive scan of the selection flags: kept edges in their order into keep_*, moved edges in
+// their order into move_*.
+#include <algorithm>
+#include <string.h>
+
+#include "cvk_device.h"
+#include "cvk_host.h"
+#include "cvk_scan.h"
+
+namespace cvk {
+namespace {
+
+constexpr int CLIP_BLOCK = 256;
+
+__device__ inline uint64_t edge_key(uint64_t seed, int32_t t, uint32_t slot, int64_t e) {
+  const u32x4 w = philox_seeded(seed, (uint32_t)e, (uint32_t)t, STREAM_CLIP | slot, (uint32_t)((uint64_t)e >> 32));
+  return ((uint64_t)w.x << 32) | (uint32_t)e;
+}
+
+// state: [0] digits found so far (prefix), [1] how many keys below the prefix's bucket remain to take
+__global__ __launch_bounds__(CLIP_BLOCK) void key_hist_kernel(uint64_t seed, int32_t t, uint32_t slot, int64_t n,
+                                                              int pass, const uint64_t* state, uint32_t* hist) {
+  __shared__ uint32_t s_hist[256];
+  s_hist[threadIdx.x] = 0;
+  __syncthreads();
+  const int shift = 56 - 8 * pass;
+  const uint64_t prefix = state[0];
+  const uint64_t high_mask = pass == 0 ? 0ull : ~0ull << (64 - 8 * pass);
+  for (int64_t e = (int64_t)blockIdx.x * CLIP_BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * CLIP_BLOCK) {
+    const uint64_t key = edge_key(seed, t, slot, e);
+    if ((key & high_mask) == prefix) atomicAdd(&s_hist[(key >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  if (s_hist[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
+}
+
+// One workgroup: the digit whose bucket holds the remaining k-th key; clears the histogram.
+__global__ __launch_bounds__(CLIP_BLOCK) void key_select_kernel(int pass, uint64_t* state, uint32_t* hist) {
+  if (threadIdx.x == 0) {
+    uint64_t remain = state[1];
+    uint32_t digit = 255;
+    for (uint32_t b = 0; b < 256; ++b) {
+      if (remain <= hist[b]) {
+        digit = b;
+        break;
+      }
+      remain -= hist[b];
+    }
+    state[0] |= (uint64_t)digit << (56 - 8 * pass);
+    state[1] = remain;
+  }
+  __syncthreads();
+  hist[threadIdx.x] = 0;
+}
+
+__global__ void key_init_kernel(uint64_t* state, uint32_t* hist, uint64_t k) {
+  if (threadIdx.x == 0) {
+    state[0] = 0;
+    state[1] = k;
+  }
+  hist[threadIdx.x] = 0;
+}
+
+struct Chosen {
+  uint64_t seed;
+  int32_t t;
+  uint32_t slot;
+  const uint64_t* kth;  // the k-th smallest key (all keys <= it are chosen); null: nothing chosen
+  __device__ int64_t operator()(int64_t e) const { return kth && edge_key(seed, t, slot, e) <= *kth ? 1 : 0; }
+};
+
+struct SplitEdges {
+  const int32_t *p1, *p2;
+  const float* beta;
+  int32_t *kp1, *kp2, *mp1, *mp2;
+  float *kbeta, *mbeta;
+  __device__ void operator()(int64_t e, int64_t moved_before, int64_t chosen) const {
+    if (chosen) {
+      mp1[moved_before] = p1[e];
+      mp2[moved_before] = p2[e];
+      mbeta[moved_before] = beta[e];
+    } else {
+      const int64_t j = e - moved_before;
+      kp1[j] = p1[e];
+      kp2[j] = p2[e];
+      kbeta[j] = beta[e];
+    }
+  }
+};
+
+inline int64_t align256(int64_t b) { return (b + 255) & ~(int64_t)255; }
+
+}  // namespace
+}  // namespace cvk
+
+using namespace cvk;
+
+extern "C" int64_t cvk_move_edges_work_bytes(int64_t n) {
+  if (n < 0) return 0;
+  return align256(16) + align256(256 * 4) + align256(8 * (scan_tiles(n) + 1));
+}
+
+extern "C" int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float* beta, int64_t n, int64_t k,
+                              uint64_t seed, int32_t t, int32_t slot, int32_t* keep_p1, int32_t* keep_p2,
+                              float* keep_beta, int32_t* move_p1, int32_t* move_p2, float* move_beta, void* work,
+                              int64_t work_bytes, void* stream) {
+  if (n < 0 || k < 0 || k > n || n > (1ll << 32) || slot < 0 || slot > 255)
+    return fail(CVK_E_INVALID, "cvk_move_edges: need 0 <= k <= n <= 2^32 and a slot in 0..255");
+  if (n > 0 && (!p1 || !p2 || !beta)) return fail(CVK_E_INVALID, "cvk_move_edges: null layer column");
+  if (n - k > 0 && (!keep_p1 || !keep_p2 || !keep_beta)) return fail(CVK_E_INVALID, "cvk_move_edges: null keep column");
+  if (k > 0 && (!move_p1 || !move_p2 || !move_beta)) return fail(CVK_E_INVALID, "cvk_move_edges: null move column");
+  if (!work || ((uintptr_t)work & 255)) return fail(CVK_E_ALIGN, "cvk_move_edges: workspace must be 256-byte aligned");
+  if (work_bytes < cvk_move_edges_work_bytes(n)) return fail(CVK_E_CAPACITY, "cvk_move_edges: workspace too small");
+  if (n == 0) return CVK_OK;
+  hipStream_t st = (hipStream_t)stream;
+  char* base = (char*)work;
+  uint64_t* state = (uint64_t*)base;
+  uint32_t* hist = (uint32_t*)(base + align256(16));
+  int64_t* tiles = (int64_t*)(base + align256(16) + align256(256 * 4));
+  const uint32_t slot_u = (uint32_t)slot;
+  if (k > 0) {
+    hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, state, hist, (uint64_t)k);
+    const uint32_t grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((n + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
+    for (int pass = 0; pass < 8; ++pass) {
+      hipLaunchKernelGGL(key_hist_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, seed, t, slot_u, n, pass,
+                         (const uint64_t*)state, hist);
+      hipLaunchKernelGGL(key_select_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, pass, state, hist);
+    }
+  }
+  SplitEdges split{p1, p2, beta, keep_p1, keep_p2, move_p1, move_p2, keep_beta, move_beta};
+  return scan(Chosen{seed, t, slot_u, k > 0 ? (const uint64_t*)state : nullptr}, split, n, tiles, nullptr, st);
+}

@@ -39,6 +39,8 @@ constexpr uint32_t STREAM_POP_HH_PERM = 0x70000400u;
 constexpr uint32_t STREAM_POP_COUNT = 0x70000500u;
 constexpr uint32_t STREAM_POP_EDGE = 0x70000600u;
 constexpr uint32_t STREAM_POP_EDGE_RETRY = 0x70000700u;
+// Edge moves of clip_edges (cvk_clip.hip); the low byte is the layer slot.
+constexpr uint32_t STREAM_CLIP = 0x80000000u;
 
 // Per-infection draw slots (counter word c3 of STREAM_INFECT).
 constexpr uint32_t SLOT_EXP2INF = 0;

@@ -1,7 +1,9 @@
 """Interventions that act on the transmission step (covasim/interventions.py).
 
 ``change_beta`` (interventions.py:530-586) rescales the overall beta or a layer's beta from given
-days on. The GPU day loop takes its effect as a per-day table of the float32 rates the kernels
+days on. ``clip_edges`` (interventions.py:589-673) moves a share of a layer's edges out of the sim
+into the intervention's store, and back, with the cvk_move_edges kernels; runs with it go day by
+day (the layer sizes change between days). The GPU day loop takes its effect as a per-day table of the float32 rates the kernels
 use (``cvk_imports.beta_v_days`` / ``beta_layer_days``), built once when the sim initialises by
 replaying the interventions over the days on a copy of the parameters. A run therefore stays one
 captured hipGraph with the rates of each day baked into that day's launches, and the sim's own
@@ -12,7 +14,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ['Intervention', 'change_beta', 'beta_schedule']
+__all__ = ['Intervention', 'change_beta', 'clip_edges', 'beta_schedule']
 
 
 class Intervention:
@@ -100,6 +102,81 @@ class change_beta(Intervention):
                     pars['beta_layer'][lkey] = new_beta
 
 
+def move_edges(cols, n, k, dest, seed, t, slot):
+    """Move k of the n edges in cols (p1, p2, beta tensors), chosen by cvk_move_edges, onto the end of
+    dest (p1, p2, beta tensors). Returns (kept columns, extended destination columns)."""
+    import ctypes as C
+    import torch
+    L = _lib.lib()
+    dev = cols[0].device
+    m = int(dest[0].numel())
+    keep = [torch.empty(n - k, dtype=c.dtype, device=dev) for c in cols]
+    out = [torch.empty(m + k, dtype=c.dtype, device=dev) for c in cols]
+    for o, d in zip(out, dest):
+        o[:m].copy_(d)
+    nbytes = int(L.cvk_move_edges_work_bytes(n))
+    buf = torch.empty(nbytes + 256, dtype=torch.uint8, device=dev)
+    work = C.c_void_p(buf.data_ptr() + (-buf.data_ptr()) % 256)
+    tail = [C.c_void_p(o.data_ptr() + m * o.element_size()) for o in out]
+    _lib.check(L.cvk_move_edges(_lib.ptr(cols[0]), _lib.ptr(cols[1]), _lib.ptr(cols[2]), n, k,
+                                int(seed) & 0xFFFFFFFFFFFFFFFF, int(t), int(slot), _lib.ptr(keep[0]), _lib.ptr(keep[1]),
+                                _lib.ptr(keep[2]), tail[0], tail[1], tail[2], work, nbytes, _lib.stream_ptr()),
+               'cvk_move_edges')
+    torch.cuda.current_stream().synchronize()  # before `buf` is released
+    return keep, out
+
+
+class clip_edges(Intervention):
+    """From ``days[i]`` on, keep the share ``changes[i]`` of each layer's edges in the sim: edges move
+    to the intervention's store or back, chosen uniformly without replacement
+    (interventions.py:589-673). Layers default to all of the sim's."""
+
+    def __init__(self, days, changes, layers=None, **kwargs):
+        super().__init__(**kwargs)
+        self.days = days
+        self.changes = changes
+        self.layers = layers
+        self.contacts = None
+
+    def initialize(self, sim):
+        import torch
+        super().initialize(sim)
+        self.days = process_days(sim, self.days)
+        self.changes = process_changes(self.changes, self.days)
+        keys = sim.layer_keys() if hasattr(sim, 'layer_keys') else list(sim.pars['beta_layer'])
+        self.layers = list(keys) if self.layers is None else (
+            [self.layers] if isinstance(self.layers, str) else list(self.layers))
+        dev = getattr(sim, 'device', None) or 'cuda'
+        self.contacts = {k: (torch.empty(0, dtype=torch.int32, device=dev), torch.empty(0, dtype=torch.int32, device=dev),
+                             torch.empty(0, dtype=torch.float32, device=dev)) for k in self.layers}
+
+    def apply(self, sim):
+        for ind, day in enumerate(self.days):
+            if day != sim.t:
+                continue
+            keys = sim.layer_keys()
+            for lkey in self.layers:
+                layer = sim.people.contacts[lkey]
+                held = self.contacts[lkey]
+                n_sim, n_int = len(layer), int(held[0].numel())
+                n = n_sim + n_int
+                if not n:
+                    continue
+                n_to_move = int((n_sim / n - self.changes[ind]) * n)
+                cols = (layer.tensor('p1'), layer.tensor('p2'), layer.tensor('beta'))
+                slot = keys.index(lkey)
+                if n_to_move > 0:  # out of the sim
+                    keep, self.contacts[lkey] = move_edges(cols, n_sim, n_to_move, held, sim.pars['rand_seed'],
+                                                           sim.t, slot)
+                    new = keep
+                elif n_to_move < 0:  # back into the sim
+                    kept_held, new = move_edges(held, n_int, -n_to_move, cols, sim.pars['rand_seed'], sim.t, slot)
+                    self.contacts[lkey] = tuple(kept_held)
+                else:
+                    continue
+                layer['p1'], layer['p2'], layer['beta'] = new[0], new[1], new[2]
+
+
 class _Replay:
     """What an intervention sees while the schedule is replayed: a day and a copy of the rates."""
 
@@ -129,4 +206,4 @@ def beta_schedule(sim, rates):
     return bv, bl
 
 
-SUPPORTED = (change_beta,)
+SUPPORTED = (change_beta, clip_edges)

@@ -135,8 +135,6 @@ class Sim:
         _lib.lib()
         self.validate_pars()
         self.init_variants()
-        for iv in self.pars['interventions']:
-            iv.initialize(self)
         if self.pars['prognoses'] is None:
             self.pars['prognoses'] = cvpar.get_prognoses(self.pars['prog_by_age'])
         if self.pars['use_waning']:
@@ -145,6 +143,8 @@ class Sim:
             self.pars['nab_kin'] = cvimm.precompute_waning(self.npts, self.pars['nab_decay'])
         self.init_people()
         self.init_workspace()
+        for iv in self.pars['interventions']:
+            iv.initialize(self)
         self._build_params()
         self.t = 0
         self.complete = False
@@ -285,6 +285,7 @@ class Sim:
         self._imports = imp
         self._custom_update = [lk for lk in lkeys if dyn.get(lk, 0) and type(people.contacts[lk]).update
                                is not Layer.update]
+        self._edge_interventions = [iv for iv in pars['interventions'] if isinstance(iv, cviv.clip_edges)]
 
     # -- running --
     def _rates(self, pars):
@@ -304,6 +305,7 @@ class Sim:
             self._params.beta_v[v] = self._beta_days[0][row, v]
         for li in range(_lib.MAX_LAYERS):
             self._params.beta_layer[li] = self._beta_days[1][row, li]
+
     def _refresh_layers(self):
         for li, lk in enumerate(self.layer_keys()):
             lay = self.people.contacts[lk]
@@ -334,6 +336,11 @@ class Sim:
             if days[j] == t:
                 _lib.check(L.cvk_choose(P, pr, w, t, _lib.STREAM_IMPORT | int(vars_[j]), int(nimp[j]), 1,
                                         _lib.LAYER_IMPORT, int(vars_[j]), st), 'variant imports')
+        if self._edge_interventions:  # after the contact update and the imports, as sim.py:591-593
+            self.t = t
+            for iv in self._edge_interventions:
+                iv(self)
+            self._refresh_layers()
         _lib.check(L.cvk_edge_pass(P, self._layers, pr, w, t, None, st), 'edge pass')
         _lib.check(L.cvk_infect_pass(P, self._layers, pr, w, t, 1, st), 'infect pass')
         _lib.check(L.cvk_count_pass(P, pr, w, t, st), 'count pass')
@@ -366,18 +373,30 @@ class Sim:
         if t1 <= self.t:
             raise AlreadyRunError(f'Simulation is already at day {self.t}; cannot run until day {until}')
         T = time.perf_counter()
-        if self._custom_update:
-            for t in range(self.t, t1):
+        stream = torch.cuda.current_stream()
+        graph = bool(use_graph) if use_graph is not None else True
+        if graph and stream.cuda_stream == 0:
+            graph = False
+        L = _lib.lib()
+        # days on which an edge intervention acts run through the individual passes (the layers
+        # change between the contact update and transmission); the days between them run as one
+        # day loop (a captured graph)
+        manual_days = set(range(self.t, t1)) if self._custom_update else {
+            d for iv in self._edge_interventions for d in iv.days if self.t <= d < t1}
+        t = self.t
+        while t < t1:
+            if t in manual_days:
+                self._apply_schedule(t)
                 self._manual_day(t)
-        else:
-            stream = torch.cuda.current_stream()
-            graph = bool(use_graph) if use_graph is not None else True
-            if graph and stream.cuda_stream == 0:
-                graph = False
-            L = _lib.lib()
-            _lib.check(L.cvk_run_days(self.people.struct(), self._layers, self._params, self._work, self.t, t1,
+                t += 1
+                continue
+            nxt = min([d for d in manual_days if d > t] + [t1])
+            if manual_days:
+                self._refresh_layers()
+            _lib.check(L.cvk_run_days(self.people.struct(), self._layers, self._params, self._work, t, nxt,
                                       int(self.pars['pop_infected']), self._imports, 1 if graph else 0,
                                       C.c_void_p(stream.cuda_stream)), 'run')
+            t = nxt
         torch.cuda.synchronize()
         self.elapsed = time.perf_counter() - T
         self.t = t1

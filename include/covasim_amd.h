@@ -25,6 +25,7 @@
  *                      people.py:139-160 set_prognoses (population generation, before the day loop)
  *   cvk_finalize       covasim/sim.py:764-1001 finalize, compute_states, compute_doubling, compute_r_eff
  *   cvk_reduce         covasim/run.py:220-303  MultiSim.reduce (median / quantiles or mean / std)
+ *   cvk_move_edges     covasim/interventions.py:589-673 clip_edges (Layer.pop_inds / append, base.py:1739-1768)
  */
 #ifndef COVASIM_AMD_H
 #define COVASIM_AMD_H
@@ -556,6 +557,19 @@ int cvk_pop_contacts_fill(const cvk_popgen* g, int32_t slot, int64_t n_members, 
 int cvk_pop_people(const cvk_popgen* g, int64_t n, const float* age, uint8_t* sex, float* symp_prob,
                    float* severe_prob, float* crit_prob, float* death_prob, float* rel_sus, float* rel_trans,
                    void* work, int64_t work_bytes, void* stream);
+
+/* ---- interventions ----
+ * change_beta (interventions.py:530-586) needs no entry point of its own: its per-day rates go to
+ * cvk_run_days / cvk_graph_create in cvk_imports.beta_v_days / beta_layer_days.
+ * cvk_move_edges is the edge move of clip_edges (interventions.py:589-673, Layer.pop_inds / append,
+ * base.py:1739-1768): k of a layer's n edges, chosen uniformly without replacement, move out. Edge e
+ * has the key (philox(seed; e lo, t, 0x80000000 | slot, e hi).x << 32) | e and the k smallest keys
+ * move. keep_* get the other n - k edges in their order, move_* the k moved edges in their order
+ * (p1, p2, beta columns; point move_* at the end of the destination layer to append them). */
+int64_t cvk_move_edges_work_bytes(int64_t n);
+int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float* beta, int64_t n, int64_t k, uint64_t seed,
+                   int32_t t, int32_t slot, int32_t* keep_p1, int32_t* keep_p2, float* keep_beta, int32_t* move_p1,
+                   int32_t* move_p2, float* move_beta, void* work, int64_t work_bytes, void* stream);
 
 /* ---- differentiable relaxation ----
  * No reference counterpart: an expected-infection relaxation of the transmission step
