@@ -376,6 +376,26 @@ def run_interventions(w, args):
     beta_only = one([cv.change_beta(30, 0.6)])
     clipped = one([cv.change_beta(30, 0.6), cv.clip_edges([40, 90], [0.5, 1.0], layers=['s', 'w'])])
     days = clipped.npts
+    # device-state snapshot mid-run (checkpoint.py): save at day 89, load, finish
+    import tempfile
+    sim = cv.Sim(pop_size=n, pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=args.seed, popdict=copy(),
+                 households=w['households'],
+                 interventions=[cv.change_beta(30, 0.6), cv.clip_edges([40, 90], [0.5, 1.0], layers=['s', 'w'])],
+                 **w['pars'])
+    sim.run(until=89)
+    with tempfile.TemporaryDirectory() as tmp:
+        fn = os.path.join(tmp, 'mid.sim')
+        t0 = time.perf_counter()
+        sim.save(fn)
+        save_s = time.perf_counter() - t0
+        size = os.path.getsize(fn)
+        t0 = time.perf_counter()
+        resumed = cv.load(fn)
+        load_s = time.perf_counter() - t0
+    resumed.run()
+    snapshot = {'save_s': save_s, 'load_s': load_s, 'bytes': size,
+                'resumed_equals_uninterrupted': bool(np.array_equal(resumed.results['new_infections'].values,
+                                                                    clipped.results['new_infections'].values))}
     line = {
         'metric': 'agent-days/sec at 1M hybrid with interventions', 'value': n * days / clipped.elapsed,
         'unit': 'agent-days/s', 'n_gpus': 1, 'steps': days, 'warmup': days, 'ms_per_step': clipped.elapsed / days * 1e3,
@@ -385,6 +405,7 @@ def run_interventions(w, args):
                              'agent_days_per_s': n * days / beta_only.elapsed,
                              'cum_infections': float(beta_only.results['cum_infections'].values[-1])},
         'clip_run': {'cum_infections': float(clipped.results['cum_infections'].values[-1])},
+        'snapshot_day_89': snapshot,
     }
     print(json.dumps(line), flush=True)
 
