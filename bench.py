@@ -125,7 +125,23 @@ def cpu_baseline(w, pop, seed, budget_s, omp=False):
     el = time.perf_counter() - t0
     return dict(value=w['pop_size'] * days / el, unit='agent-days/s', cores=threads, kind='port',
                 sample=f'oracle C restatement (gcc -O3{", OpenMP" if omp else ""}, Philox draws) on the same '
-                       f'population, days 0..{days - 1} of the workload, {el:.1f} s on {threads} host thread(s)')
+                       f'population, days 0..{days - 1} of the workload, {el:.1f} s on {threads} host thread(s)',
+                host=host_cpu())
+
+
+def host_cpu():
+    """The host CPU's model and logical core count (as lscpu / nproc report them)."""
+    model = ''
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    model = line.split(':', 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {'model': model, 'logical_cores': os.cpu_count(),
+            'usable_cores': len(os.sched_getaffinity(0)) if hasattr(os, 'sched_getaffinity') else None}
 
 
 def run_batch(w, args):
