@@ -51,7 +51,9 @@ WORKLOADS = {
                      pars=dict(beta=0.016, use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
                      desc='hybrid1m with interventions: change_beta(30, 0.6) and clip_edges([40, 90], [0.5, 1.0], '
                           'layers s, w); clip days run through the individual passes, the days between them as '
-                          'captured graphs; the change_beta-only run is one graph'),
+                          'captured graphs; the change_beta-only run is one graph; a third run tests '
+                          '(test_prob symp 0.1, asymp 0.001, delay 1) and traces contacts (h 1.0, s/w 0.5 after '
+                          '1 day, c 0.1 after 2 days) every day, one graph'),
     'popgen10m': dict(pop_size=10_000_000, pop_type='hybrid', households='categorical', popgen=True,
                       pars=dict(),
                       desc='population build of the hybrid 10M population on the device: ages, age-correlated '
@@ -368,8 +370,9 @@ def run_popgen(w, args):
 
 
 def run_interventions(w, args):
-    """hybrid1m with interventions: a change_beta-only run (one captured graph, per-day rates) and a
-    run that also clips edges (clip days through the individual passes, graphs between them). One
+    """hybrid1m with interventions: a change_beta-only run (one captured graph, per-day rates), a
+    run that also clips edges (clip days through the individual passes, graphs between them) and a
+    testing + contact-tracing run (one captured graph). One
     step = one day; value = the clip run's agent-days/s, wall time of Sim.run without finalisation."""
     import torch
     import covasim_amd as cv
@@ -392,6 +395,9 @@ def run_interventions(w, args):
     beta_only = one([cv.change_beta(30, 0.6)])
     clipped = one([cv.change_beta(30, 0.6), cv.clip_edges([40, 90], [0.5, 1.0], layers=['s', 'w'])])
     days = clipped.npts
+    # testing and contact tracing every day, one captured graph
+    traced = one([cv.test_prob(symp_prob=0.1, asymp_prob=0.001, test_delay=1),
+                  cv.contact_tracing(trace_probs=dict(h=1.0, s=0.5, w=0.5, c=0.1), trace_time=dict(h=0, s=1, w=1, c=2))])
     # device-state snapshot mid-run (checkpoint.py): save at day 89, load, finish
     import tempfile
     sim = cv.Sim(pop_size=n, pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=args.seed, popdict=copy(),
@@ -421,6 +427,9 @@ def run_interventions(w, args):
                              'agent_days_per_s': n * days / beta_only.elapsed,
                              'cum_infections': float(beta_only.results['cum_infections'].values[-1])},
         'clip_run': {'cum_infections': float(clipped.results['cum_infections'].values[-1])},
+        'testing_tracing': {'ms_per_day': traced.elapsed / days * 1e3, 'agent_days_per_s': n * days / traced.elapsed,
+                            **{k: float(traced.results[k].values[-1]) for k in
+                               ('cum_infections', 'cum_tests', 'cum_diagnoses', 'cum_quarantined')}},
         'snapshot_day_89': snapshot,
     }
     print(json.dumps(line), flush=True)

@@ -13,11 +13,11 @@ from .base import Result, Layer, Contacts
 from .people import People
 from .sim import Sim, AlreadyRunError
 from .batch import BatchSim
-from .interventions import Intervention, change_beta, clip_edges
+from .interventions import Intervention, change_beta, clip_edges, test_prob, contact_tracing
 from .checkpoint import load
 from . import utils
 from .utils import set_seed
 
-__all__ = ['__version__', 'Intervention', 'change_beta', 'clip_edges', 'load', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
+__all__ = ['__version__', 'Intervention', 'change_beta', 'clip_edges', 'test_prob', 'contact_tracing', 'load', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
            'get_cross_immunity', 'variant', 'Result', 'Layer', 'Contacts', 'People', 'Sim', 'AlreadyRunError', 'BatchSim',
            'utils', 'set_seed']

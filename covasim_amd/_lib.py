@@ -55,7 +55,8 @@ R_NAMES = ['new_infections', 'new_reinfections', 'new_infectious', 'new_symptoma
            'new_critical', 'new_recoveries', 'new_deaths', 'new_known_deaths',
            'n_susceptible', 'n_exposed', 'n_infectious', 'n_symptomatic', 'n_severe', 'n_critical',
            'n_recovered', 'n_dead', 'n_diagnosed', 'n_known_dead', 'n_quarantined', 'n_isolated',
-           'n_vaccinated', 'n_imports', 'mid_severe', 'mid_critical', 'n_alive']
+           'n_vaccinated', 'n_imports', 'mid_severe', 'mid_critical', 'n_alive',
+           'new_tests', 'new_diagnoses', 'new_quarantined', 'new_isolated']
 R_COL = {k: i for i, k in enumerate(R_NAMES)}
 R_BYVAR = len(R_NAMES)
 RV_NAMES = ['new_infections_by_variant', 'new_symptomatic_by_variant', 'new_severe_by_variant',
@@ -157,10 +158,37 @@ def reset_workspace(w):
         w[k].zero_()
 
 
+OP_TEST, OP_TRACE = 1, 2
+QP_NONE, QP_START, QP_END, QP_BOTH, QP_DAILY, QP_DAYS = range(6)
+MAX_QUAR_DAYS = 8
+PENDING_NONE = -2**31
+DIAG_CTL_WORDS = 8
+
+
+class DiagOp(C.Structure):
+    _fields_ = [('kind', C.c_int32), ('slot', C.c_int32), ('start_day', C.c_int32), ('end_day', C.c_int32),
+                ('symp_prob', C.c_double), ('asymp_prob', C.c_double), ('symp_quar_prob', C.c_double),
+                ('asymp_quar_prob', C.c_double), ('sensitivity', C.c_double), ('keep_prob', C.c_double),
+                ('test_delay', C.c_double), ('quar_policy', C.c_int32), ('n_quar_days', C.c_int32),
+                ('quar_days', C.c_int32 * MAX_QUAR_DAYS), ('trace_prob', C.c_double * MAX_LAYERS),
+                ('trace_time', C.c_double * MAX_LAYERS), ('trace_period', C.c_int32 * MAX_LAYERS),
+                ('presumptive', C.c_int32), ('pad', C.c_int32)]
+
+
+DIAG_DATE_FIELDS = ['date_tested', 'date_pos_test', 'date_known_contact', 'date_quarantined', 'date_end_quarantine']
+
+
+class DiagS(C.Structure):
+    _fields_ = ([(k, C.c_void_p) for k in DIAG_DATE_FIELDS] +
+                [('pending', C.c_void_p), ('cases', C.c_void_p), ('contacts', C.c_void_p), ('ctl', C.c_void_p),
+                 ('ring', C.c_int32), ('pad', C.c_int32)])
+
+
 class ImportsS(C.Structure):
     _fields_ = [('n', C.c_int32), ('days', C.POINTER(C.c_int32)), ('variants', C.POINTER(C.c_int32)),
                 ('n_imports', C.POINTER(C.c_int64)), ('n_beta_days', C.c_int32), ('beta_v_days', C.c_void_p),
-                ('beta_layer_days', C.c_void_p)]
+                ('beta_layer_days', C.c_void_p), ('n_diag_ops', C.c_int32), ('diag_ops', C.POINTER(DiagOp)),
+                ('diag', C.POINTER(DiagS))]
 
 
 BATCH_MAX_SIMS = 4096
@@ -282,6 +310,11 @@ def lib():
     L.cvk_move_edges.argtypes = [vp, vp, vp, i64, i64, u64, i32, i32, vp, vp, vp, vp, vp, vp, vp, i64, vp]
     L.cvk_move_edges.restype = C.c_int
     L.cvk_reduce.argtypes = [i32, i64, vp, i32, C.c_double, C.c_double, C.c_double, vp, vp, vp, vp]
+    DS, DO = C.POINTER(DiagS), C.POINTER(DiagOp)
+    L.cvk_diag_apply.argtypes = [P, Ly, Pa, W, DS, DO, i32, vp]
+    L.cvk_diag_apply.restype = C.c_int
+    L.cvk_states_post.argtypes = [P, Pa, W, DS, i32, vp]
+    L.cvk_states_post.restype = C.c_int
     L.cvk_reduce.restype = C.c_int
     PG, PC = C.POINTER(PopGen), C.POINTER(PopCounts)
     L.cvk_pop_work_bytes.argtypes = [i64, i64]
@@ -320,7 +353,8 @@ def _check_layout(L):
             C.sizeof(ImportsS), Params.dur.offset, Params.nab_init.offset, Params.rel_imm_symp.offset,
             Params.immunity.offset, Params.nab_kin.offset, PeopleS.sev_imm.offset, WorkS.log_capacity.offset,
             WorkS.npts.offset, R_COLS, RF_COLS, C.sizeof(BatchDesc), BatchDesc.people.offset, C.sizeof(BatchImports),
-            C.sizeof(BatchConfig), C.sizeof(RelaxDesc), C.sizeof(PopGen), PopGen.nb_p0.offset, C.sizeof(PopCounts)]
+            C.sizeof(BatchConfig), C.sizeof(RelaxDesc), C.sizeof(PopGen), PopGen.nb_p0.offset, C.sizeof(PopCounts),
+            C.sizeof(DiagOp), DiagOp.trace_prob.offset, C.sizeof(DiagS), ImportsS.diag.offset]
     if got != want:
         raise RuntimeError(f'covasim_amd: ctypes struct layout mismatch with the library: {got} != {want}')
 

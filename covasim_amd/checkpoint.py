@@ -3,7 +3,7 @@
 The reference pickles the whole Sim object. Here a snapshot is one safetensors file: every device
 array of the agents (state flags, dates, durations, immunity), the contact layers as they stand
 (resampled and clipped), the day loop's workspace (per-day result tables, infection log, control
-words) and the stores of clip_edges interventions, plus the day and the parameters as JSON in the
+words, scheduled quarantines) and the stores of clip_edges interventions, plus the day and the parameters as JSON in the
 file's metadata. Randomness is counter-based (Philox keyed by the seed and the day), so the seed
 and the day are the whole random state: a loaded sim continues exactly as the saved one would
 have. Loading executes nothing from the file (no pickle).
@@ -38,6 +38,8 @@ def _encode(obj):
             layers = None
         return {'__intervention__': type(obj).__name__, 'days': _encode(obj.days), 'changes': _encode(obj.changes),
                 'layers': _encode(layers), 'label': obj.label}
+    if isinstance(obj, cviv.DIAG_TYPES):  # rebuilt from the arguments it was made with
+        return {'__intervention__': type(obj).__name__, 'kwargs': _encode(obj._kwargs)}
     if obj is None or isinstance(obj, (bool, int, float, str)):
         return obj
     raise TypeError(f'cannot save a parameter of type {type(obj).__name__} in a snapshot')
@@ -58,6 +60,9 @@ def _decode(obj):
             return cvimm.variant(dict(_decode(d['p']), label=d['label']), _decode(d['days']),
                                  n_imports=d['n_imports'], rescale=d['rescale'])
         if '__intervention__' in obj:
+            if 'kwargs' in obj:
+                cls = {'test_prob': cviv.test_prob, 'contact_tracing': cviv.contact_tracing}[obj['__intervention__']]
+                return cls(**_decode(obj['kwargs']))
             cls = {'change_beta': cviv.change_beta, 'clip_edges': cviv.clip_edges}[obj['__intervention__']]
             return cls(_decode(obj['days']), _decode(obj['changes']), layers=_decode(obj['layers']),
                        label=obj['label'])

@@ -603,7 +603,8 @@ extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
       (int64_t)CVK_R_COLS, (int64_t)CVK_RF_COLS, (int64_t)sizeof(cvk_batch_desc),
       (int64_t)offsetof(cvk_batch_desc, people), (int64_t)sizeof(cvk_batch_imports), (int64_t)sizeof(cvk_batch_config),
       (int64_t)sizeof(cvk_relax_desc), (int64_t)sizeof(cvk_popgen), (int64_t)offsetof(cvk_popgen, nb_p0),
-      (int64_t)sizeof(cvk_pop_counts)};
+      (int64_t)sizeof(cvk_pop_counts), (int64_t)sizeof(cvk_diag_op), (int64_t)offsetof(cvk_diag_op, trace_prob),
+      (int64_t)sizeof(cvk_diag), (int64_t)offsetof(cvk_imports, diag)};
   const int32_t m = (int32_t)(sizeof(vals) / sizeof(vals[0]));
   if (!out || n < m) return fail(CVK_E_INVALID, "cvk_abi_layout: output too small");
   for (int32_t j = 0; j < m; ++j) out[j] = vals[j];
@@ -880,9 +881,12 @@ static bool has_dynamic(const cvk_layer* layers, const cvk_params* params) {
 static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
                          int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, hipStream_t st) {
   int rc;
-  // With generated dynamic layers, no day's resample is written until the last day is closed.
+  // With generated dynamic layers, no day's resample is written until the last day is closed
+  // (not with contact tracing, which reads each day's contacts).
+  const bool diag = imports && imports->diag;
   bool gen = false;
-  for (int l = 0; l < params->n_layers; ++l) gen = gen || (g_gen_dynamic && layers[l].dynamic && layers[l].n_edges > 0);
+  for (int l = 0; l < params->n_layers; ++l)
+    gen = gen || (g_gen_dynamic && !diag && layers[l].dynamic && layers[l].n_edges > 0);
   uint32_t infect_blocks = 0;  // infect partial rows not yet added to their day's row
   // Day t+1's resample runs as extra workgroups of day t's infect pass (store-bound work beside
   // latency-bound work); the infect pass then takes dynamic-layer sources from the resample stream.
@@ -920,6 +924,10 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
         if (rc) return rc;
       }
     }
+    if (diag) {  // interventions that test and trace, then update_states_post (sim.py:591-596)
+      rc = diag_day(people, layers, params, work, imports, t, st);
+      if (rc) return rc;
+    }
     rc = edge_pass_impl(people, layers, params, work, t, nullptr, st, &r, gen);  // block 0 reduces day_begin's rows
     if (rc) return rc;
     // tomorrow's resample (once today's edges have been read) with today's infections
@@ -937,12 +945,12 @@ static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_
 }
 
 static int check_run_args(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
-                          int32_t t0, int32_t t1) {
+                          int32_t t0, int32_t t1, const cvk_imports* imports) {
   int rc = validate(people, params, work, t0);
   if (rc) return rc;
   if (t1 < t0 || t1 > work->npts) return fail(CVK_E_INVALID, "bad day range");
   if (params->n_layers > 0 && !layers) return fail(CVK_E_INVALID, "layers is null");
-  return CVK_OK;
+  return check_diag_schedule(people, layers, params, work, imports);
 }
 
 extern "C" int cvk_graph_create(const cvk_people* people, cvk_layer* layers, const cvk_params* params,
@@ -954,7 +962,7 @@ extern "C" int cvk_graph_destroy(void* graph_exec);
 extern "C" int cvk_run_days(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
                             int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports,
                             int32_t use_graph, void* stream) {
-  int rc = check_run_args(people, layers, params, work, t0, t1);
+  int rc = check_run_args(people, layers, params, work, t0, t1, imports);
   if (rc) return rc;
   if (t1 == t0) return CVK_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -975,7 +983,7 @@ extern "C" int cvk_graph_create(const cvk_people* people, cvk_layer* layers, con
                                 const cvk_imports* imports, void* stream, void** graph_exec_out) {
   if (!graph_exec_out) return fail(CVK_E_INVALID, "cvk_graph_create: null output handle");
   *graph_exec_out = nullptr;
-  int rc = check_run_args(people, layers, params, work, t0, t1);
+  int rc = check_run_args(people, layers, params, work, t0, t1, imports);
   if (rc) return rc;
   if (t1 <= t0) return fail(CVK_E_INVALID, "cvk_graph_create: empty day range");
   hipStream_t st = (hipStream_t)stream;

@@ -41,6 +41,9 @@ constexpr uint32_t STREAM_POP_EDGE = 0x70000600u;
 constexpr uint32_t STREAM_POP_EDGE_RETRY = 0x70000700u;
 // Edge moves of clip_edges (cvk_clip.hip); the low byte is the layer slot.
 constexpr uint32_t STREAM_CLIP = 0x80000000u;
+// Testing (| op slot) and contact tracing (| op slot << 8 | layer), cvk_diag.hip.
+constexpr uint32_t STREAM_TEST = 0x90000000u;
+constexpr uint32_t STREAM_TRACE = 0xA0000000u;
 
 // Per-infection draw slots (counter word c3 of STREAM_INFECT).
 constexpr uint32_t SLOT_EXP2INF = 0;

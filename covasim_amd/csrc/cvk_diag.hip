@@ -1,0 +1,413 @@
+// Testing, contact tracing, quarantine and isolation on the device (gfx950, wave64).
+//
+//   test_kernel          test_prob.apply (interventions.py:919-962) + People.test (people.py:589-617):
+//                        one pass over the agents; the test, true-positive and loss draws of agent i
+//                        are Philox words keyed by (i, t, TEST | slot)
+//   trace_cases_kernel   contact_tracing.select_cases (interventions.py:1047-1066): bitmap of the
+//                        agents diagnosed today (or, presumptive, exposed and tested today)
+//   trace_edges_kernel   Layer.find_contacts (utils.py:131-147) for every traced layer at once: each
+//                        edge with a case at one end marks the other end's layer bit (atomicOr;
+//                        only the few edges at a case write)
+//   trace_notify_kernel  binomial_filter per (contact, layer) and notify_contacts
+//                        (interventions.py:1101-1118): known_contact, date_known_contact and the
+//                        quarantine scheduled to start on day int(t + trace_time)
+//   states_post_kernel   update_states_post (people.py:189-196): check_diagnosed, check_quar,
+//                        check_enter_iso, then the edge records of agents whose quarantine or
+//                        isolation changed (the transmission step of the same day reads them)
+//
+// Scheduled quarantines (People._pending_quarantine, people.py:116, 620-641) are a ring of per-agent
+// end days: row d % ring holds the largest end day among the quarantines scheduled to start on day d.
+// check_quar walks the day's list in order; the first entry of an agent starts or extends its
+// quarantine and the later ones only extend it, and whether it may start does not change within the
+// walk, so the walk's result is that of the single entry with the largest end day (and the agent is
+// counted once), which is what the ring keeps.
+#include <hip/hip_runtime.h>
+
+#include "cvk_host.h"
+#include "cvk_kernels.h"
+
+namespace cvk {
+namespace {
+
+constexpr int DIAG_BLOCK = 256;
+constexpr int DIAG_APT = 2;                 // agents per thread (contiguous 256-agent chunks)
+constexpr int DIAG_SPAN = DIAG_BLOCK * DIAG_APT;
+constexpr int TRACE_EDGE_BLOCK = 256;
+constexpr int TRACE_EDGE_GRID = 2048;       // grid-stride blocks per traced layer
+
+// Adds per-wave counts c[0..NC) (wave-uniform, from ballots) to row[cols[k]]: one atomic per block
+// and column (columns only these kernels write).
+template <int NC>
+__device__ inline void block_add(const uint32_t (&c)[NC], uint32_t* row, const int (&cols)[NC]) {
+  __shared__ uint32_t s[DIAG_BLOCK / 64][NC];
+  const int w = threadIdx.x >> 6;
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < NC; ++k) s[w][k] = c[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < NC) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int j = 0; j < DIAG_BLOCK / 64; ++j) tot += s[j][threadIdx.x];
+    if (tot) atomicAdd(row + cols[threadIdx.x], tot);
+  }
+}
+
+__device__ inline int64_t diag_agent(int r) {
+  return ((int64_t)blockIdx.x * DIAG_APT + r) * DIAG_BLOCK + threadIdx.x;
+}
+
+// get_quar_inds (interventions.py:682-710) for one agent.
+__device__ inline bool quar_test(const cvk_diag_op& op, uint32_t f, float dq, float deq, int32_t t) {
+  switch (op.quar_policy) {
+    case CVK_QP_START: return dq == (float)(t - 1);
+    case CVK_QP_END: return deq == (float)(t + 1);
+    case CVK_QP_BOTH: return dq == (float)(t - 1) || deq == (float)(t + 1);
+    case CVK_QP_DAILY: return (f & CVK_QUARANTINED) != 0;
+    case CVK_QP_DAYS: {
+      bool q = false;
+      for (int k = 0; k < op.n_quar_days; ++k) q = q || dq == (float)(t - 1 - op.quar_days[k]);
+      return q;
+    }
+    default: return false;
+  }
+}
+
+__global__ __launch_bounds__(DIAG_BLOCK) void test_kernel(cvk_people P, cvk_diag D, cvk_diag_op op, uint64_t seed,
+                                                          int32_t t, uint32_t* row) {
+  const float tf = (float)t;
+  const uint32_t stream = STREAM_TEST | (uint32_t)op.slot;
+  uint32_t c[1] = {0};
+#pragma unroll
+  for (int r = 0; r < DIAG_APT; ++r) {
+    const int64_t i = diag_agent(r);
+    bool tested = false;
+    if (i < P.n) {
+      const uint32_t f = P.flags[i];
+      if (!(f & CVK_DIAGNOSED)) {  // test_probs[diag_inds] = 0
+        const bool symp = (f & CVK_SYMPTOMATIC) != 0;
+        const bool q = op.quar_policy != CVK_QP_NONE && op.quar_policy != CVK_QP_DAILY
+                           ? quar_test(op, f, D.date_quarantined[i], D.date_end_quarantine[i], t)
+                           : quar_test(op, f, 0.0f, 0.0f, t);
+        const double p = q ? (symp ? op.symp_quar_prob : op.asymp_quar_prob) : (symp ? op.symp_prob : op.asymp_prob);
+        if (p > 0.0) {  // binomial_arr: u < p (never for p <= 0)
+          const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 0);
+          if (u01(w.x, w.y) < p) {
+            tested = true;
+            P.flags[i] = f | CVK_TESTED;
+            D.date_tested[i] = tf;
+            // true positive among the infectious, not yet holding a diagnosis date, not lost
+            if ((f & CVK_INFECTIOUS) && u01(w.z, w.w) < op.sensitivity && isnan(P.date_diagnosed[i])) {
+              const u32x4 w2 = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 1);
+              if (u01(w2.x, w2.y) < op.keep_prob) {
+                P.date_diagnosed[i] = (float)((double)t + op.test_delay);
+                D.date_pos_test[i] = tf;
+              }
+            }
+          }
+        }
+      }
+    }
+    c[0] += (uint32_t)__popcll(__ballot(tested));
+  }
+  const int cols[1] = {CVK_R_NEW_TESTS};
+  block_add<1>(c, row, cols);
+}
+
+__global__ __launch_bounds__(DIAG_BLOCK) void trace_cases_kernel(cvk_people P, cvk_diag D, int32_t presumptive,
+                                                                 int32_t t, uint32_t* count) {
+  const float tf = (float)t;
+  const int64_t n_words = (P.n + 63) / 64;
+  uint32_t c = 0;
+#pragma unroll
+  for (int r = 0; r < DIAG_APT; ++r) {
+    const int64_t i = diag_agent(r);
+    bool is_case = false;
+    if (i < P.n)
+      is_case = presumptive ? ((P.flags[i] & CVK_EXPOSED) && D.date_tested[i] == tf) : P.date_diagnosed[i] == tf;
+    const uint64_t b = __ballot(is_case);
+    const int64_t word = (((int64_t)blockIdx.x * DIAG_APT + r) * DIAG_BLOCK + (threadIdx.x & ~63u)) >> 6;
+    if (lane_id() == 0 && word < n_words) D.cases[word] = b;
+    c += (uint32_t)__popcll(b);
+  }
+  __shared__ uint32_t s[DIAG_BLOCK / 64];
+  if (lane_id() == 0) s[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t tot = 0;
+    for (int j = 0; j < DIAG_BLOCK / 64; ++j) tot += s[j];
+    if (tot) atomicAdd(count, tot);
+  }
+}
+
+struct TraceLayers {
+  const int32_t* p1[CVK_MAX_LAYERS];
+  const int32_t* p2[CVK_MAX_LAYERS];
+  int64_t n[CVK_MAX_LAYERS];
+  uint32_t layer[CVK_MAX_LAYERS];  // layer index (the contact bit)
+};
+
+__device__ inline bool case_bit(const uint64_t* cases, int32_t a) {
+  return (cases[(uint32_t)a >> 6] >> ((uint32_t)a & 63u)) & 1ull;
+}
+
+// blockIdx.y: the traced layer; every block of a layer strides over its edges.
+__global__ __launch_bounds__(TRACE_EDGE_BLOCK) void trace_edges_kernel(TraceLayers L, const uint64_t* cases,
+                                                                       uint32_t* contacts, const uint32_t* count) {
+  if (*count == 0) return;  // nobody to trace today
+  const int li = blockIdx.y;
+  const int32_t* p1 = L.p1[li];
+  const int32_t* p2 = L.p2[li];
+  const int64_t n = L.n[li];
+  const uint32_t bit = 1u << L.layer[li];
+  for (int64_t e = (int64_t)blockIdx.x * TRACE_EDGE_BLOCK + threadIdx.x; e < n;
+       e += (int64_t)gridDim.x * TRACE_EDGE_BLOCK) {
+    const int32_t a = p1[e], b = p2[e];
+    if (case_bit(cases, a)) atomicOr(contacts + b, bit);
+    if (case_bit(cases, b)) atomicOr(contacts + a, bit);
+  }
+}
+
+__global__ __launch_bounds__(DIAG_BLOCK) void trace_notify_kernel(cvk_people P, cvk_diag D, cvk_diag_op op,
+                                                                  uint64_t seed, int32_t t) {
+#pragma unroll
+  for (int r = 0; r < DIAG_APT; ++r) {
+    const int64_t i = diag_agent(r);
+    if (i >= P.n) continue;
+    const uint32_t m = D.contacts[i];
+    if (!m) continue;
+    D.contacts[i] = 0;
+    const uint32_t f = P.flags[i];
+    if (f & CVK_DEAD) continue;  // np.setdiff1d(contact_inds, is_dead)
+    float dkc = D.date_known_contact[i];
+    bool any = false;
+    for (uint32_t mm = m; mm; mm &= mm - 1) {
+      const int l = __ffs(mm) - 1;
+      const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t,
+                                    STREAM_TRACE | ((uint32_t)op.slot << 8) | (uint32_t)l, 0);
+      if (!(u01(w.x, w.y) < op.trace_prob[l])) continue;  // binomial_filter
+      any = true;
+      const double s = (double)t + op.trace_time[l];
+      dkc = fminf(dkc, (float)s);  // np.fmin: a NaN date gives way
+      const int32_t start = (int32_t)s;
+      const int32_t end = start + op.trace_period[l];
+      int32_t* pend = D.pending + (int64_t)(start % D.ring) * P.n + i;
+      if (end > *pend) *pend = end;
+    }
+    if (any) {
+      P.flags[i] = f | CVK_KNOWN_CONTACT;
+      D.date_known_contact[i] = dkc;
+    }
+  }
+}
+
+__global__ __launch_bounds__(DIAG_BLOCK) void states_post_kernel(cvk_people P, cvk_params prm, cvk_work W,
+                                                                 cvk_diag D, int32_t t, uint32_t* row) {
+  if (blockIdx.x == 0 && threadIdx.x < 8) D.ctl[threadIdx.x] = 0;  // the day's tracing ops are done
+  const float tf = (float)t;
+  float quar_max = 0.0f, iso_max = 0.0f;
+  for (int l = 0; l < prm.n_layers; ++l) {
+    quar_max = fmaxf(quar_max, prm.quar_factor[l]);
+    iso_max = fmaxf(iso_max, prm.iso_factor[l]);
+  }
+  int32_t* pend_row = D.pending + (int64_t)(t % D.ring) * P.n;
+  uint32_t c[3] = {0, 0, 0};
+#pragma unroll
+  for (int r = 0; r < DIAG_APT; ++r) {
+    const int64_t i = diag_agent(r);
+    bool new_diag = false, new_quar = false, new_iso = false;
+    if (i < P.n) {
+      const uint32_t f0 = P.flags[i];
+      uint32_t f = f0;
+      const float dd = P.date_diagnosed[i];
+      // check_diagnosed (people.py:315-332): positive tests counted, then diagnoses that are due
+      if (!(f & CVK_DIAGNOSED)) {
+        const float dpt = D.date_pos_test[i];
+        if (!isnan(dpt) && tf >= dpt) {
+          D.date_pos_test[i] = __int_as_float(0x7FC00000);
+          new_diag = true;
+        }
+        if (!isnan(dd) && tf >= dd) f |= CVK_DIAGNOSED;
+      }
+      // check_quar (people.py:335-358)
+      const int32_t e = pend_row[i];
+      float deq = 0.0f;
+      bool deq_loaded = false;
+      if (e != CVK_PENDING_NONE) {
+        pend_row[i] = CVK_PENDING_NONE;
+        if (f & CVK_QUARANTINED) {
+          deq = D.date_end_quarantine[i];
+          deq_loaded = true;
+          if ((float)e > deq) {  // max(date_end_quarantine, end_day)
+            deq = (float)e;
+            D.date_end_quarantine[i] = deq;
+          }
+        } else if (!(f & (CVK_DEAD | CVK_RECOVERED | CVK_DIAGNOSED | CVK_ISOLATED))) {
+          f |= CVK_QUARANTINED;
+          D.date_quarantined[i] = tf;
+          deq = (float)e;
+          deq_loaded = true;
+          D.date_end_quarantine[i] = deq;
+          new_quar = true;
+        }
+      }
+      if (f & CVK_QUARANTINED) {
+        if (!deq_loaded) deq = D.date_end_quarantine[i];
+        if (dd == tf) {  // diagnosed today: quarantine ends, isolation begins
+          deq = tf;
+          D.date_end_quarantine[i] = tf;
+        }
+        if (!isnan(deq) && tf >= deq) f &= ~CVK_QUARANTINED;
+      }
+      // check_enter_iso (people.py:361-365)
+      if (dd == tf) {
+        f |= CVK_ISOLATED;
+        P.date_end_isolation[i] = P.date_recovered[i];
+        new_iso = true;
+      }
+      if (f != f0) {
+        P.flags[i] = f;
+        if ((f ^ f0) & (CVK_QUARANTINED | CVK_ISOLATED)) {  // factors of today's transmission changed
+          const float rt = P.rel_trans[i], rs = P.rel_sus[i];
+          W.rec[i] = edge_record(f, rt, rs);
+          W.bound[i] = edge_bound(f, rt, rs, prm, quar_max, iso_max);
+        }
+      }
+    }
+    c[0] += (uint32_t)__popcll(__ballot(new_diag));
+    c[1] += (uint32_t)__popcll(__ballot(new_quar));
+    c[2] += (uint32_t)__popcll(__ballot(new_iso));
+  }
+  const int cols[3] = {CVK_R_NEW_DIAGNOSES, CVK_R_NEW_QUARANTINED, CVK_R_NEW_ISOLATED};
+  block_add<3>(c, row, cols);
+}
+
+int agent_grid(int64_t n) { return (int)((n + DIAG_SPAN - 1) / DIAG_SPAN); }
+
+bool diag_ok(const cvk_diag* D) {
+  return D && D->date_tested && D->date_pos_test && D->date_known_contact && D->date_quarantined &&
+         D->date_end_quarantine && D->pending && D->cases && D->contacts && D->ctl && D->ring >= 1;
+}
+
+int check_common(const cvk_people* P, const cvk_params* prm, const cvk_work* W, const cvk_diag* D, int32_t t) {
+  if (!P || !prm || !W) return fail(CVK_E_INVALID, "null people, params or work");
+  if (!P->flags || !P->date_diagnosed || !P->date_end_isolation || !P->date_recovered || !P->rel_trans ||
+      !P->rel_sus)
+    return fail(CVK_E_INVALID, "people arrays missing");
+  if (P->n <= 0 || P->n > INT32_MAX) return fail(CVK_E_INVALID, "agent count out of range");
+  if (!W->results || !W->rec || !W->bound) return fail(CVK_E_INVALID, "workspace arrays missing");
+  if (t < 0 || t >= W->npts) return fail(CVK_E_INVALID, "day out of range");
+  if (!diag_ok(D)) return fail(CVK_E_INVALID, "cvk_diag arrays missing or ring < 1");
+  return CVK_OK;
+}
+
+int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, const cvk_layer* layers) {
+  if (!op) return fail(CVK_E_INVALID, "null op");
+  if (op->slot < 0 || op->slot > 255) return fail(CVK_E_INVALID, "op slot must be 0..255");
+  if (op->kind == CVK_OP_TEST) {
+    if (op->quar_policy < CVK_QP_NONE || op->quar_policy > CVK_QP_DAYS)
+      return fail(CVK_E_INVALID, "unknown quarantine testing policy");
+    if (op->n_quar_days < 0 || op->n_quar_days > CVK_MAX_QUAR_DAYS)
+      return fail(CVK_E_INVALID, "n_quar_days out of range");
+    return CVK_OK;
+  }
+  if (op->kind != CVK_OP_TRACE) return fail(CVK_E_INVALID, "unknown op kind");
+  for (int l = 0; l < prm->n_layers; ++l) {
+    if (!(op->trace_prob[l] > 0.0)) continue;
+    if (!(op->trace_time[l] >= 0.0) || op->trace_time[l] >= (double)INT32_MAX / 2)
+      return fail(CVK_E_INVALID, "trace_time must be >= 0");
+    if ((int64_t)op->trace_time[l] + 1 >= D->ring) return fail(CVK_E_INVALID, "cvk_diag.ring too small for trace_time");
+    if (!layers) return fail(CVK_E_INVALID, "null layers");
+    if (layers[l].n_edges > 0 && (!layers[l].p1 || !layers[l].p2)) return fail(CVK_E_INVALID, "layer arrays missing");
+  }
+  return CVK_OK;
+}
+
+int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W, const cvk_diag* D,
+               const cvk_diag_op* op, int32_t t, hipStream_t st) {
+  if (t < op->start_day || (op->end_day >= 0 && t > op->end_day)) return CVK_OK;
+  uint32_t* row = W->results + (int64_t)t * CVK_R_COLS;
+  const int grid = agent_grid(P->n);
+  if (op->kind == CVK_OP_TEST) {
+    test_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t, row);
+    return check_launch("test_kernel");
+  }
+  TraceLayers L{};
+  int nl = 0;
+  int64_t n_max = 0;
+  for (int l = 0; l < prm->n_layers; ++l) {
+    if (!(op->trace_prob[l] > 0.0) || layers[l].n_edges <= 0) continue;  // trace_prob == 0: layer skipped
+    L.p1[nl] = layers[l].p1;
+    L.p2[nl] = layers[l].p2;
+    L.n[nl] = layers[l].n_edges;
+    L.layer[nl] = (uint32_t)l;
+    n_max = n_max > layers[l].n_edges ? n_max : layers[l].n_edges;
+    ++nl;
+  }
+  if (nl == 0) return CVK_OK;
+  uint32_t* count = D->ctl + (op->slot & 7);
+  trace_cases_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, op->presumptive, t, count);
+  int rc = check_launch("trace_cases_kernel");
+  if (rc) return rc;
+  const int64_t gx = (n_max + TRACE_EDGE_BLOCK - 1) / TRACE_EDGE_BLOCK;
+  trace_edges_kernel<<<dim3((unsigned)(gx < TRACE_EDGE_GRID ? gx : TRACE_EDGE_GRID), (unsigned)nl), TRACE_EDGE_BLOCK,
+                       0, st>>>(L, D->cases, D->contacts, count);
+  rc = check_launch("trace_edges_kernel");
+  if (rc) return rc;
+  trace_notify_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t);
+  return check_launch("trace_notify_kernel");
+}
+
+int post_impl(const cvk_people* P, const cvk_params* prm, cvk_work* W, const cvk_diag* D, int32_t t, hipStream_t st) {
+  uint32_t* row = W->results + (int64_t)t * CVK_R_COLS;
+  states_post_kernel<<<agent_grid(P->n), DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t, row);
+  return check_launch("states_post_kernel");
+}
+
+}  // namespace
+
+// The day loop's testing and tracing step (after the imports, before transmission).
+int diag_day(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W,
+             const cvk_imports* imports, int32_t t, hipStream_t st) {
+  if (!imports || !imports->diag) return CVK_OK;
+  for (int j = 0; j < imports->n_diag_ops; ++j) {
+    const int rc = apply_impl(P, layers, prm, W, imports->diag, imports->diag_ops + j, t, st);
+    if (rc) return rc;
+  }
+  return post_impl(P, prm, W, imports->diag, t, st);
+}
+
+int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, const cvk_work* W,
+                        const cvk_imports* imports) {
+  if (!imports || !imports->diag) return CVK_OK;
+  int rc = check_common(P, prm, W, imports->diag, 0);
+  if (rc) return rc;
+  if (imports->n_diag_ops < 0 || (imports->n_diag_ops > 0 && !imports->diag_ops))
+    return fail(CVK_E_INVALID, "bad diag_ops");
+  for (int j = 0; j < imports->n_diag_ops; ++j) {
+    rc = check_op(imports->diag_ops + j, imports->diag, prm, layers);
+    if (rc) return rc;
+  }
+  return CVK_OK;
+}
+
+}  // namespace cvk
+
+using namespace cvk;
+
+extern "C" int cvk_diag_apply(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
+                              cvk_work* work, const cvk_diag* diag, const cvk_diag_op* op, int32_t t, void* stream) {
+  int rc = check_common(people, params, work, diag, t);
+  if (rc) return rc;
+  rc = check_op(op, diag, params, layers);
+  if (rc) return rc;
+  return apply_impl(people, layers, params, work, diag, op, t, (hipStream_t)stream);
+}
+
+extern "C" int cvk_states_post(const cvk_people* people, const cvk_params* params, cvk_work* work,
+                               const cvk_diag* diag, int32_t t, void* stream) {
+  const int rc = check_common(people, params, work, diag, t);
+  if (rc) return rc;
+  return post_impl(people, params, work, diag, t, (hipStream_t)stream);
+}

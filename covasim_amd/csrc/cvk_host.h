@@ -14,6 +14,13 @@ char* last_error_buffer();
 // CVK_TUNE_OVERLAP_RESAMPLE as set by cvk_set_tuning (read by the batched day loop too).
 int overlap_resample_enabled();
 
+// Testing and tracing step of the day loop (cvk_diag.hip): the ops of `imports`, then
+// update_states_post; nothing when imports->diag is null.
+int diag_day(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W,
+             const cvk_imports* imports, int32_t t, hipStream_t st);
+int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, const cvk_work* W,
+                        const cvk_imports* imports);
+
 inline int fail(int code, const char* msg) {
   snprintf(last_error_buffer(), 512, "%s", msg);
   return code;

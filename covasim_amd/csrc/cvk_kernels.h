@@ -126,6 +126,30 @@ __device__ inline uint32_t bound_code(float x) {
 }
 __device__ inline float bound_decode(uint32_t code) { return __uint_as_float((code & 0x1FFFu) << 18); }
 
+// Edge-pass record of an agent with flags f: the flags, and rel_trans if infectious, else rel_sus.
+__device__ inline uint64_t edge_record(uint32_t f, float rt, float rs) {
+  const float val = (f & CVK_INFECTIOUS) ? rt : rs;
+  return ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
+}
+
+// Prefilter bound of the edge pass (wave_tile_pass): the source's factors for any layer (variant
+// in bits 13..15), or the target's; 0 for agents that neither transmit nor acquire.
+__device__ inline uint16_t edge_bound(uint32_t f, float rt, float rs, const cvk_params& prm, float quar_max,
+                                      float iso_max) {
+  uint32_t bc = 0;
+  if ((f & CVK_INFECTIOUS) && rt != 0.0f) {
+    float ub = rt;
+    if (f & CVK_QUARANTINED) ub = ub * quar_max;
+    if (!(f & CVK_SYMPTOMATIC)) ub = ub * prm.asymp_factor;
+    if (f & CVK_ISOLATED) ub = ub * iso_max;
+    ub = ub * ((f & CVK_EARLY) ? prm.vl_hi : prm.vl_lo);
+    bc = (inf_var(f) << 13) | bound_code(ub);
+  } else if ((f & CVK_SUSCEPTIBLE) && rs != 0.0f) {
+    bc = bound_code((f & CVK_QUARANTINED) ? rs * quar_max : rs);
+  }
+  return (uint16_t)bc;
+}
+
 // Per-agent inputs, all loaded before any is used (one memory round trip).
 struct AgentIn {
   uint32_t f, vs;
@@ -209,7 +233,7 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
     const int64_t li = idx[r];
     const int64_t g = S.g0 + li;
     const bool valid = li < S.n_local;
-    bool inf_bit = false, sus_bit = false;
+    bool inf_bit = false;
     uint32_t f_close = 0, vs_close = 0;  // state counted as the previous day's stocks
     uint32_t ev = 0;                     // this day's transition events (bits as the day columns)
     uint32_t ev_var = CVK_VAR_NONE;      // variant of a new infectious agent
@@ -298,22 +322,8 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         if (f != f0) P.flags[g] = f;
         if (vs != vs0) P.vstate[g] = vs;
         inf_bit = (f & CVK_INFECTIOUS) && x.rt != 0.0f;
-        sus_bit = (f & CVK_SUSCEPTIBLE) && x.rs != 0.0f;
-        const float val = (f & CVK_INFECTIOUS) ? x.rt : x.rs;
-        S.rec[li] = ((uint64_t)f << 32) | (uint64_t)__float_as_uint(val);
-        // prefilter bound (wave_tile_pass): the source's factors for any layer, or the target's
-        uint32_t bc = 0;
-        if (inf_bit) {
-          float ub = x.rt;
-          if (f & CVK_QUARANTINED) ub = ub * quar_max;
-          if (!(f & CVK_SYMPTOMATIC)) ub = ub * prm.asymp_factor;
-          if (f & CVK_ISOLATED) ub = ub * iso_max;
-          ub = ub * ((f & CVK_EARLY) ? prm.vl_hi : prm.vl_lo);
-          bc = (inf_var(f) << 13) | bound_code(ub);
-        } else if (sus_bit) {
-          bc = bound_code((f & CVK_QUARANTINED) ? x.rs * quar_max : x.rs);
-        }
-        S.bound[li] = (uint16_t)bc;
+        S.rec[li] = edge_record(f, x.rt, x.rs);
+        S.bound[li] = edge_bound(f, x.rt, x.rs, prm, quar_max, iso_max);
         if (f & CVK_SEVERE) ev |= 1u << 7;
         if (f & CVK_CRITICAL) ev |= 1u << 8;
       }

@@ -34,14 +34,18 @@ __device__ inline int flow_col(int f) {
     case 5: return CVK_R_NEW_CRITICAL;
     case 6: return CVK_R_NEW_RECOVERIES;
     case 7: return CVK_R_NEW_DEATHS;
+    case 8: return CVK_R_NEW_TESTS;
+    case 9: return CVK_R_NEW_DIAGNOSES;
     case 10: return CVK_R_NEW_KNOWN_DEATHS;
-    default: return -1;  // tests, diagnoses, quarantined, isolated, doses, vaccinated: not produced
+    case 11: return CVK_R_NEW_QUARANTINED;
+    case 12: return CVK_R_NEW_ISOLATED;
+    default: return -1;  // doses, vaccinated: not produced
   }
 }
 
 __device__ inline int stock_col(int k) { return CVK_R_N_SUSCEPTIBLE + k; }  // same order, n_susceptible..n_vaccinated
 
-constexpr int FLOW_RECOVERIES = 6, FLOW_DEATHS = 7, FLOW_INFECTIONS = 0;
+constexpr int FLOW_RECOVERIES = 6, FLOW_DEATHS = 7, FLOW_INFECTIONS = 0, FLOW_TESTS = 8, FLOW_DIAGNOSES = 9;
 constexpr int STOCK_EXPOSED = 1, STOCK_INFECTIOUS = 2, STOCK_RECOVERED = 6, STOCK_VACCINATED = 12;
 
 // numpy's pairwise summation of get(lo), .., get(lo + n - 1) (the order of np.add.reduce over a
@@ -209,6 +213,11 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(FinArgs a) {
                              (double)R[(int64_t)t * CVK_R_COLS + CVK_R_N_ALIVE_NAB];
     row(CVK_F_POP_PROTECTION)[t] = RF[(int64_t)t * CVK_RF_COLS + CVK_RF_SUS_IMM_SUM] / ((double)nv * pop);
     row(CVK_F_POP_SYMP_PROTECTION)[t] = RF[(int64_t)t * CVK_RF_COLS + CVK_RF_SYMP_IMM_SUM] / ((double)nv * pop);
+    // compute_yield (sim.py:853-868): positives per test, and relative to the undiagnosed alive
+    const double nt = row(CVK_F_NEW + FLOW_TESTS)[t];
+    const double ty = nt != 0.0 ? row(CVK_F_NEW + FLOW_DIAGNOSES)[t] / nt : 0.0;
+    row(CVK_F_TEST_YIELD)[t] = ty;
+    row(CVK_F_REL_TEST_YIELD)[t] = ni != 0.0 ? ty / (ni / (n_alive - row(CVK_F_CUM + FLOW_DIAGNOSES)[t])) : 0.0;
     const double inv_sus = 1.0 / n_sus, inv_alive = 1.0 / n_alive;
     for (int v = 0; v < nv; ++v) {
       const double x = vrow(CVK_FV_NEW + 0, v)[t];

@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ['Intervention', 'change_beta', 'clip_edges', 'beta_schedule']
+__all__ = ['Intervention', 'change_beta', 'clip_edges', 'test_prob', 'contact_tracing', 'beta_schedule']
 
 
 class Intervention:
@@ -177,6 +177,152 @@ class clip_edges(Intervention):
                 layer['p1'], layer['p2'], layer['beta'] = new[0], new[1], new[2]
 
 
+def _day_or_none(sim, day):
+    if day is None:
+        return None
+    return process_days(sim, day)[0]
+
+
+class test_prob(Intervention):
+    """Test each agent with a probability set by its symptoms and quarantine state; true positives
+    among the infectious are diagnosed test_delay days later unless lost to follow-up
+    (interventions.py:858-962, People.test people.py:589-617). Runs on the device (cvk_diag_apply);
+    subtarget, ili_prev and swab_delay are not supported."""
+
+    def __init__(self, symp_prob, asymp_prob=0.0, symp_quar_prob=None, asymp_quar_prob=None, quar_policy=None,
+                 subtarget=None, ili_prev=None, sensitivity=1.0, loss_prob=0.0, test_delay=0, start_day=0,
+                 end_day=None, swab_delay=None, **kwargs):
+        super().__init__(**kwargs)
+        for name, val in (('subtarget', subtarget), ('ili_prev', ili_prev), ('swab_delay', swab_delay)):
+            if val is not None:
+                raise NotImplementedError(f'test_prob: {name} is not supported by the device implementation')
+        self._kwargs = dict(symp_prob=symp_prob, asymp_prob=asymp_prob, symp_quar_prob=symp_quar_prob,
+                            asymp_quar_prob=asymp_quar_prob, quar_policy=quar_policy, sensitivity=sensitivity,
+                            loss_prob=loss_prob, test_delay=test_delay, start_day=start_day, end_day=end_day,
+                            label=kwargs.get('label'))
+        self.symp_prob = symp_prob
+        self.asymp_prob = asymp_prob
+        self.symp_quar_prob = symp_quar_prob if symp_quar_prob is not None else symp_prob
+        self.asymp_quar_prob = asymp_quar_prob if asymp_quar_prob is not None else asymp_prob
+        self.quar_policy = quar_policy if quar_policy else 'start'
+        self.sensitivity = sensitivity
+        self.loss_prob = loss_prob
+        self.test_delay = test_delay
+        self.start_day = start_day
+        self.end_day = end_day
+
+    def initialize(self, sim):
+        super().initialize(sim)
+        self.start_day = _day_or_none(sim, self.start_day)
+        self.end_day = _day_or_none(sim, self.end_day)
+        self.days = [self.start_day, self.end_day]
+
+    def op(self, slot, layer_keys):
+        """This intervention as a cvk_diag_op (slot: its index among the sim's interventions)."""
+        o = _lib.DiagOp()
+        o.kind = _lib.OP_TEST
+        o.slot = slot
+        o.start_day = self.start_day
+        o.end_day = -1 if self.end_day is None else self.end_day
+        o.symp_prob, o.asymp_prob = float(self.symp_prob), float(self.asymp_prob)
+        o.symp_quar_prob, o.asymp_quar_prob = float(self.symp_quar_prob), float(self.asymp_quar_prob)
+        o.sensitivity = float(self.sensitivity)
+        o.keep_prob = 1.0 - float(self.loss_prob)
+        o.test_delay = float(self.test_delay)
+        qp = self.quar_policy
+        named = {None: _lib.QP_NONE, 'start': _lib.QP_START, 'end': _lib.QP_END, 'both': _lib.QP_BOTH,
+                 'daily': _lib.QP_DAILY}
+        if isinstance(qp, str) or qp is None:
+            if qp not in named:
+                raise ValueError(f'Quarantine policy "{qp}" not recognized')
+            o.quar_policy = named[qp]
+        elif callable(qp):
+            raise NotImplementedError('test_prob: a callable quar_policy is not supported by the device implementation')
+        else:
+            qd = [int(q) for q in np.atleast_1d(qp)]
+            if not 0 < len(qd) <= _lib.MAX_QUAR_DAYS:
+                raise ValueError(f'test_prob: between 1 and {_lib.MAX_QUAR_DAYS} quarantine test days are supported')
+            o.quar_policy = _lib.QP_DAYS
+            o.n_quar_days = len(qd)
+            for j, q in enumerate(qd):
+                o.quar_days[j] = q
+        return o
+
+    def apply(self, sim):
+        raise RuntimeError('test_prob runs inside the sim\'s day loop')
+
+
+class contact_tracing(Intervention):
+    """Trace the contacts of the agents diagnosed today (presumptive: exposed and tested today) in
+    each layer with probability trace_probs[layer]; the living ones become known contacts and are
+    scheduled to quarantine trace_time[layer] days later (interventions.py:965-1118). Runs on the
+    device (cvk_diag_apply); capacity is not supported."""
+
+    def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,
+                 quar_period=None, capacity=None, **kwargs):
+        super().__init__(**kwargs)
+        if capacity is not None:
+            raise NotImplementedError('contact_tracing: capacity is not supported by the device implementation')
+        self._kwargs = dict(trace_probs=trace_probs, trace_time=trace_time, start_day=start_day, end_day=end_day,
+                            presumptive=presumptive, quar_period=quar_period, label=kwargs.get('label'))
+        self.trace_probs = trace_probs
+        self.trace_time = trace_time
+        self.start_day = start_day
+        self.end_day = end_day
+        self.presumptive = presumptive
+        self.quar_period = quar_period
+
+    def initialize(self, sim):
+        super().initialize(sim)
+        self.start_day = _day_or_none(sim, self.start_day)
+        self.end_day = _day_or_none(sim, self.end_day)
+        self.days = [self.start_day, self.end_day]
+        keys = sim.layer_keys()
+        if self.trace_probs is None:
+            self.trace_probs = 1.0
+        if self.trace_time is None:
+            self.trace_time = 0.0
+        if self.quar_period is None:
+            self.quar_period = sim.pars['quar_period']
+        if np.isscalar(self.trace_probs):
+            self.trace_probs = {k: self.trace_probs for k in keys}
+        if np.isscalar(self.trace_time):
+            self.trace_time = {k: self.trace_time for k in keys}
+        for lk, prob in self.trace_probs.items():
+            if lk not in keys:
+                raise ValueError(f'contact_tracing: layer "{lk}" not in {keys}')
+            if prob != 0 and lk not in self.trace_time:
+                raise ValueError(f'contact_tracing: no trace_time for layer "{lk}"')
+            if prob != 0 and not float(self.trace_time[lk]) >= 0:
+                raise ValueError('contact_tracing: trace_time must be >= 0')
+
+    def max_delay(self):
+        """The largest whole-day delay from tracing to the start of quarantine."""
+        return max([int(float(self.trace_time[lk])) for lk, p in self.trace_probs.items() if p != 0] + [0])
+
+    def op(self, slot, layer_keys):
+        o = _lib.DiagOp()
+        o.kind = _lib.OP_TRACE
+        o.slot = slot
+        o.start_day = self.start_day
+        o.end_day = -1 if self.end_day is None else self.end_day
+        o.presumptive = 1 if self.presumptive else 0
+        for li, lk in enumerate(layer_keys):
+            prob = float(self.trace_probs.get(lk, 0.0))
+            o.trace_prob[li] = prob
+            if prob != 0:
+                tt = float(self.trace_time[lk])
+                o.trace_time[li] = tt
+                o.trace_period[li] = int(self.quar_period - tt)  # schedule_quarantine's int(period)
+        return o
+
+    def apply(self, sim):
+        raise RuntimeError('contact_tracing runs inside the sim\'s day loop')
+
+
+DIAG_TYPES = (test_prob, contact_tracing)
+
+
 class _Replay:
     """What an intervention sees while the schedule is replayed: a day and a copy of the rates."""
 
@@ -206,4 +352,4 @@ def beta_schedule(sim, rates):
     return bv, bl
 
 
-SUPPORTED = (change_beta, clip_edges)
+SUPPORTED = (change_beta, clip_edges, test_prob, contact_tracing)

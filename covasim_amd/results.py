@@ -23,13 +23,14 @@ STOCKS = ['susceptible', 'exposed', 'infectious', 'symptomatic', 'severe', 'crit
           'diagnosed', 'known_dead', 'quarantined', 'isolated', 'vaccinated']  # defaults.py:145-159
 F_ROW_NAMES = ([f'new_{f}' for f in FLOWS] + [f'cum_{f}' for f in FLOWS] + [f'n_{k}' for k in STOCKS] +
                ['n_alive', 'n_naive', 'n_preinfectious', 'n_removed', 'prevalence', 'incidence', 'frac_vaccinated',
-                'n_imports', 'pop_nabs', 'pop_protection', 'pop_symp_protection', 'doubling_time', 'r_eff'])
+                'n_imports', 'pop_nabs', 'pop_protection', 'pop_symp_protection', 'doubling_time', 'r_eff',
+                'test_yield', 'rel_test_yield'])
 FV_ROW_NAMES = ['new_infections_by_variant', 'new_symptomatic_by_variant', 'new_severe_by_variant',
                 'new_infectious_by_variant', 'cum_infections_by_variant', 'cum_symptomatic_by_variant',
                 'cum_severe_by_variant', 'cum_infectious_by_variant', 'n_exposed_by_variant', 'n_infectious_by_variant',
                 'incidence_by_variant', 'prevalence_by_variant']
 UNSCALED = {'prevalence', 'incidence', 'frac_vaccinated', 'pop_nabs', 'pop_protection', 'pop_symp_protection',
-            'doubling_time', 'r_eff', 'incidence_by_variant', 'prevalence_by_variant'}
+            'doubling_time', 'r_eff', 'test_yield', 'rel_test_yield', 'incidence_by_variant', 'prevalence_by_variant'}
 
 
 def initial_period(pars, npts):

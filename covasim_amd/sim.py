@@ -145,6 +145,7 @@ class Sim:
         self.init_workspace()
         for iv in self.pars['interventions']:
             iv.initialize(self)
+        self._init_diag()
         self._build_params()
         self.t = 0
         self.complete = False
@@ -188,6 +189,35 @@ class Sim:
 
     def layer_keys(self):
         return list(self.people.contacts.keys())
+
+    def _init_diag(self):
+        """Testing and quarantine state when the sim has test_prob / contact_tracing interventions:
+        the five dates beside the agents, the ring of scheduled quarantines (one row per day of the
+        longest tracing delay, plus one), and the tracing scratch (cvk_diag)."""
+        import torch
+        ivs = [iv for iv in self.pars['interventions'] if isinstance(iv, cviv.DIAG_TYPES)]
+        self._diag = None
+        if not ivs:
+            return
+        people, dev = self.people, self.people.device
+        n = people.n
+        for k in _lib.DIAG_DATE_FIELDS:
+            people._d[k] = torch.full((n,), float('nan'), dtype=torch.float32, device=dev)
+        ring = 2 + max([iv.max_delay() for iv in ivs if isinstance(iv, cviv.contact_tracing)] + [0])
+        w = self._w
+        w['diag_pending'] = torch.full((ring, n), _lib.PENDING_NONE, dtype=torch.int32, device=dev)
+        w['diag_cases'] = torch.zeros(2 * ((n + 63) // 64), dtype=torch.int32, device=dev)
+        w['diag_contacts'] = torch.zeros(n, dtype=torch.int32, device=dev)
+        w['diag_ctl'] = torch.zeros(_lib.DIAG_CTL_WORDS, dtype=torch.int32, device=dev)
+        d = _lib.DiagS()
+        for k in _lib.DIAG_DATE_FIELDS:
+            setattr(d, k, people._d[k].data_ptr())
+        d.pending = w['diag_pending'].data_ptr()
+        d.cases = w['diag_cases'].data_ptr()
+        d.contacts = w['diag_contacts'].data_ptr()
+        d.ctl = w['diag_ctl'].data_ptr()
+        d.ring = ring
+        self._diag = d
 
     def _build_params(self):
         import torch
@@ -282,6 +312,14 @@ class Sim:
             imp.n_beta_days = self.npts
             imp.beta_v_days = self._beta_days[0].ctypes.data
             imp.beta_layer_days = self._beta_days[1].ctypes.data
+        # testing and tracing ops, in the sim's order; an op's stream slot is its intervention's index
+        self._diag_ops = [(iv, iv.op(i, lkeys)) for i, iv in enumerate(pars['interventions'])
+                          if isinstance(iv, cviv.DIAG_TYPES)]
+        if self._diag is not None:
+            self._diag_op_array = (_lib.DiagOp * max(1, len(self._diag_ops)))(*[o for _, o in self._diag_ops])
+            imp.n_diag_ops = len(self._diag_ops)
+            imp.diag_ops = self._diag_op_array
+            imp.diag = C.pointer(self._diag)
         self._imports = imp
         self._custom_update = [lk for lk in lkeys if dyn.get(lk, 0) and type(people.contacts[lk]).update
                                is not Layer.update]
@@ -336,11 +374,18 @@ class Sim:
             if days[j] == t:
                 _lib.check(L.cvk_choose(P, pr, w, t, _lib.STREAM_IMPORT | int(vars_[j]), int(nimp[j]), 1,
                                         _lib.LAYER_IMPORT, int(vars_[j]), st), 'variant imports')
-        if self._edge_interventions:  # after the contact update and the imports, as sim.py:591-593
-            self.t = t
-            for iv in self._edge_interventions:
-                iv(self)
-            self._refresh_layers()
+        if self._edge_interventions or self._diag is not None:  # after the contact update and the imports,
+            self.t = t                                          # in the sim's order (sim.py:591-596)
+            ops = {id(iv): o for iv, o in self._diag_ops}
+            for iv in self.pars['interventions']:
+                if isinstance(iv, cviv.clip_edges):
+                    iv(self)
+                    self._refresh_layers()
+                elif id(iv) in ops:
+                    _lib.check(L.cvk_diag_apply(P, self._layers, pr, w, C.byref(self._diag), C.byref(ops[id(iv)]), t,
+                                                st), iv.label)
+            if self._diag is not None:
+                _lib.check(L.cvk_states_post(P, pr, w, C.byref(self._diag), t, st), 'update_states_post')
         _lib.check(L.cvk_edge_pass(P, self._layers, pr, w, t, None, st), 'edge pass')
         _lib.check(L.cvk_infect_pass(P, self._layers, pr, w, t, 1, st), 'infect pass')
         _lib.check(L.cvk_count_pass(P, pr, w, t, st), 'count pass')

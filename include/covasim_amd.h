@@ -26,6 +26,11 @@
  *   cvk_finalize       covasim/sim.py:764-1001 finalize, compute_states, compute_doubling, compute_r_eff
  *   cvk_reduce         covasim/run.py:220-303  MultiSim.reduce (median / quantiles or mean / std)
  *   cvk_move_edges     covasim/interventions.py:589-673 clip_edges (Layer.pop_inds / append, base.py:1739-1768)
+ *   cvk_diag_apply     covasim/interventions.py:858-962 test_prob (People.test, people.py:589-617) and
+ *                      :965-1118 contact_tracing (Layer.find_contacts, utils.py:131-147;
+ *                      People.schedule_quarantine, people.py:620-641)
+ *   cvk_states_post    covasim/people.py:189-196 update_states_post (check_diagnosed :315-332,
+ *                      check_quar :335-358, check_enter_iso :361-365)
  */
 #ifndef COVASIM_AMD_H
 #define COVASIM_AMD_H
@@ -102,6 +107,7 @@ enum {
   CVK_R_N_QUARANTINED, CVK_R_N_ISOLATED, CVK_R_N_VACCINATED, CVK_R_N_IMPORTS,
   CVK_R_MID_SEVERE, CVK_R_MID_CRITICAL,   /* counts after update_states_pre, for bed limits */
   CVK_R_N_ALIVE_NAB,                      /* alive agents (denominator of pop_nabs) */
+  CVK_R_NEW_TESTS, CVK_R_NEW_DIAGNOSES, CVK_R_NEW_QUARANTINED, CVK_R_NEW_ISOLATED,  /* testing and tracing */
   CVK_R_BYVAR,                            /* start of 6 x CVK_MAX_VARIANTS by-variant columns */
   CVK_R_COLS = CVK_R_BYVAR + 6 * CVK_MAX_VARIANTS
 };
@@ -299,6 +305,71 @@ int cvk_infect_pass(const cvk_people* people, const cvk_layer* layers, const cvk
 int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work* work, int32_t t,
                    void* stream);
 
+/* ---- testing, contact tracing, quarantine and isolation ----
+ * One cvk_diag_op is one test_prob (interventions.py:858-962) or contact_tracing
+ * (interventions.py:965-1118) intervention; cvk_diag_apply runs it on day t (nothing outside its
+ * days), after the day's imports and before transmission, in the sim's intervention order;
+ * cvk_states_post then runs update_states_post (people.py:189-196) once per day.
+ * Draws (Philox4x32-10 keyed by the seed): the test of agent i on day t uses
+ * (i, t, 0x90000000 | slot, 0) -- words 0,1: tested (u < probability), words 2,3: true positive
+ * (u < sensitivity) -- and (i, t, 0x90000000 | slot, 1) words 0,1: not lost (u < 1 - loss_prob);
+ * the tracing of contact i found in layer l uses (i, t, 0xA0000000 | slot << 8 | l, 0), words 0,1
+ * (u < trace_prob[l]). Uniforms are 53-bit doubles, compared with the double probabilities. */
+#define CVK_OP_TEST 1
+#define CVK_OP_TRACE 2
+/* quarantine testing policies of get_quar_inds (interventions.py:682-710) */
+#define CVK_QP_NONE 0    /* nobody is tested as quarantined */
+#define CVK_QP_START 1   /* date_quarantined == t - 1 */
+#define CVK_QP_END 2     /* date_end_quarantine == t + 1 */
+#define CVK_QP_BOTH 3    /* either of the two */
+#define CVK_QP_DAILY 4   /* quarantined */
+#define CVK_QP_DAYS 5    /* date_quarantined == t - 1 - q for some q in quar_days */
+#define CVK_MAX_QUAR_DAYS 8
+#define CVK_PENDING_NONE INT32_MIN
+
+typedef struct {
+  int32_t kind;                 /* CVK_OP_TEST or CVK_OP_TRACE */
+  int32_t slot;                 /* 0..255, the Philox stream slot: the intervention's index in the sim */
+  int32_t start_day, end_day;   /* acts on days start_day <= t <= end_day; end_day < 0: no end */
+  /* CVK_OP_TEST: test probabilities by symptom and quarantine-test state (diagnosed agents: 0) */
+  double symp_prob, asymp_prob, symp_quar_prob, asymp_quar_prob;
+  double sensitivity;
+  double keep_prob;             /* 1 - loss_prob */
+  double test_delay;            /* date_diagnosed = float32(t + test_delay) */
+  int32_t quar_policy;          /* CVK_QP_* */
+  int32_t n_quar_days;
+  int32_t quar_days[CVK_MAX_QUAR_DAYS];
+  /* CVK_OP_TRACE */
+  double trace_prob[CVK_MAX_LAYERS];      /* 0: the layer is not traced */
+  double trace_time[CVK_MAX_LAYERS];      /* >= 0; quarantine starts on day int(t + trace_time) */
+  int32_t trace_period[CVK_MAX_LAYERS];   /* int(quar_period - trace_time): days of quarantine */
+  int32_t presumptive;          /* trace the exposed agents tested today, not those diagnosed today */
+  int32_t pad;
+} cvk_diag_op;
+
+/* Testing and quarantine state beside cvk_people (device arrays of n agents). */
+typedef struct {
+  float *date_tested, *date_pos_test, *date_known_contact, *date_quarantined, *date_end_quarantine;
+  int32_t* pending;    /* [ring][n]: row d % ring holds, per agent, the latest end day of the quarantines
+                          scheduled to start on day d (CVK_PENDING_NONE: none); initialise to NONE */
+  uint64_t* cases;     /* [(n + 63) / 64] bitmap of the cases a tracing op is tracing (scratch) */
+  uint32_t* contacts;  /* [n] bit l: found in layer l by the running tracing op; initialise to 0 (the op
+                          clears what it set) */
+  uint32_t* ctl;       /* [8] case counts of the day's tracing ops; initialise to 0 (cvk_states_post
+                          clears them) */
+  int32_t ring;        /* > the largest int(trace_time) of every tracing op */
+  int32_t pad;
+} cvk_diag;
+
+/* Runs one test_prob / contact_tracing op on day t (new tests go to the day's CVK_R_NEW_TESTS). */
+int cvk_diag_apply(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
+                   const cvk_diag* diag, const cvk_diag_op* op, int32_t t, void* stream);
+/* update_states_post on day t: check_diagnosed, check_quar, check_enter_iso (counted in
+ * CVK_R_NEW_DIAGNOSES / NEW_QUARANTINED / NEW_ISOLATED), and the edge records of the agents whose
+ * quarantine or isolation changed (so it runs after cvk_agent_pass and before cvk_edge_pass). */
+int cvk_states_post(const cvk_people* people, const cvk_params* params, cvk_work* work, const cvk_diag* diag,
+                    int32_t t, void* stream);
+
 /* Import schedule for cvk_run_days: on day days[i], n_imports[i] susceptible agents are
  * infected with variant variants[i] (immunity.py:117-130). Host arrays. */
 typedef struct {
@@ -313,6 +384,11 @@ typedef struct {
   int32_t n_beta_days;
   const float* beta_v_days;
   const float* beta_layer_days;
+  /* Testing and tracing: with diag != NULL each day runs diag_ops[0..n_diag_ops) (host array, the
+   * sim's order) after the imports, then cvk_states_post, before transmission. */
+  int32_t n_diag_ops;
+  const cvk_diag_op* diag_ops;
+  const cvk_diag* diag;
 } cvk_imports;
 
 /* Runs days [t0, t1) (t0 == 0 also seeds pop_infected infections first). use_graph != 0
@@ -416,7 +492,7 @@ enum {
   CVK_F_N_ALIVE = CVK_F_STOCK + CVK_F_N_STOCKS,
   CVK_F_N_NAIVE, CVK_F_N_PREINFECTIOUS, CVK_F_N_REMOVED, CVK_F_PREVALENCE, CVK_F_INCIDENCE,
   CVK_F_FRAC_VACCINATED, CVK_F_N_IMPORTS, CVK_F_POP_NABS, CVK_F_POP_PROTECTION, CVK_F_POP_SYMP_PROTECTION,
-  CVK_F_DOUBLING_TIME, CVK_F_R_EFF,
+  CVK_F_DOUBLING_TIME, CVK_F_R_EFF, CVK_F_TEST_YIELD, CVK_F_REL_TEST_YIELD,
   CVK_F_ROWS
 };
 enum {

@@ -29,6 +29,8 @@ Seed-for-seed equality with the reference's MT19937 trajectories is parity unpin
 distributional.
 """
 
+from collections import defaultdict
+
 import numpy as np
 
 from . import philox as px
@@ -108,7 +110,7 @@ def make_pars(**kwargs):
             crit2die=dict(dist='lognormal_int', par1=10.7, par2=4.8),
         ),
         rel_symp_prob=1.0, rel_severe_prob=1.0, rel_crit_prob=1.0, rel_death_prob=1.0,
-        n_beds_hosp=None, n_beds_icu=None, no_hosp_factor=2.0, no_icu_factor=2.0,
+        n_beds_hosp=None, n_beds_icu=None, no_hosp_factor=2.0, no_icu_factor=2.0, quar_period=14,
         variants=[],  # list of dict(rel_beta=..., days=[...], n_imports=..., ...)
     )
     # variant_pars['wild'] is captured from the defaults before kwargs apply (parameters.py:131-134)
@@ -455,6 +457,7 @@ class People:
         for key in DATE_KEYS + DUR_STATE_KEYS:
             setattr(self, key, np.full(n, np.nan, dtype=F32))
         self.infection_log = []
+        self._pending_quarantine = defaultdict(list)  # {start day: [(agent, end day)]}, people.py:116
         self.init_flows()
 
     def init_flows(self):
@@ -555,6 +558,67 @@ class People:
         end_inds = self.check_inds(~self.isolated, self.date_end_isolation, filter_inds=None)
         self.isolated[end_inds] = False
         return end_inds
+
+    # -- testing, quarantine and isolation (people.py:189-196, 315-375, 589-641) --
+
+    def update_states_post(self):
+        """people.py:189-196."""
+        self.flows['new_diagnoses'] += len(self.check_diagnosed())
+        self.flows['new_quarantined'] += self.check_quar()
+        self.flows['new_isolated'] += len(self.check_enter_iso())
+
+    def check_diagnosed(self):
+        """people.py:315-332: positive tests taken are counted; diagnoses that are due are set."""
+        test_pos_inds = self.check_inds(self.diagnosed, self.date_pos_test, filter_inds=None)
+        self.date_pos_test[test_pos_inds] = np.nan
+        diag_inds = self.check_inds(self.diagnosed, self.date_diagnosed, filter_inds=None)
+        self.diagnosed[diag_inds] = True
+        return test_pos_inds
+
+    def check_quar(self):
+        """people.py:335-358: the day's scheduled quarantines in order, then the ends."""
+        n_quarantined = 0
+        for ind, end_day in self._pending_quarantine.pop(self.t, []):
+            if self.quarantined[ind]:
+                self.date_end_quarantine[ind] = max(self.date_end_quarantine[ind], end_day)
+            elif not (self.dead[ind] or self.recovered[ind] or self.diagnosed[ind] or self.isolated[ind]):
+                self.quarantined[ind] = True
+                self.date_quarantined[ind] = self.t
+                self.date_end_quarantine[ind] = end_day
+                n_quarantined += 1
+        diag_inds = (self.quarantined & (self.date_diagnosed == self.t)).nonzero()[0]
+        self.date_end_quarantine[diag_inds] = self.t
+        end_inds = self.check_inds(~self.quarantined, self.date_end_quarantine, filter_inds=None)
+        self.quarantined[end_inds] = False
+        return n_quarantined
+
+    def check_enter_iso(self):
+        """people.py:361-365."""
+        iso_inds = (self.date_diagnosed == self.t).nonzero()[0]
+        self.isolated[iso_inds] = True
+        self.date_end_isolation[iso_inds] = self.date_recovered[iso_inds]
+        return iso_inds
+
+    def test(self, inds, pos_u, keep_u, test_sensitivity=1.0, loss_prob=0.0, test_delay=0):
+        """people.py:589-617 with the draws given per agent: pos_u[i] < sensitivity is a true
+        positive, keep_u[i] < 1 - loss_prob keeps the result."""
+        inds = np.unique(inds)
+        self.tested[inds] = True
+        self.date_tested[inds] = self.t
+        is_infectious = inds[self.infectious[inds]]
+        is_inf_pos = is_infectious[pos_u[is_infectious] < test_sensitivity]
+        not_diagnosed = is_inf_pos[np.isnan(self.date_diagnosed[is_inf_pos])]
+        final_inds = not_diagnosed[keep_u[not_diagnosed] < 1.0 - loss_prob]
+        self.date_diagnosed[final_inds] = self.t + test_delay
+        self.date_pos_test[final_inds] = self.t
+        return final_inds
+
+    def schedule_quarantine(self, inds, start_date=None, period=None):
+        """people.py:620-641."""
+        start_date = self.t if start_date is None else int(start_date)
+        period = self.pars['quar_period'] if period is None else int(period)
+        for ind in inds:
+            self._pending_quarantine[start_date].append((int(ind), start_date + period))
 
     def check_immunity(self):
         """immunity.py:303-350 (no vaccines on this path: vaccine_imm = 0)."""
@@ -759,8 +823,10 @@ class Sim:
                 imp = rng.choose_imports(sus_inds, n_imports, t, vi)
                 people.infect(inds=imp, rng=rng, layer='importation', variant=vi)
                 self.results['n_imports'][t] += n_imports
-        for intervention in self.interventions:  # sim.py:591-593
+        for i, intervention in enumerate(self.interventions):  # sim.py:591-593
+            self.iv_slot = i  # an intervention's stream slot is its index in the list
             intervention(self)
+        people.update_states_post()  # sim.py:596
 
         vd = pars['viral_dist']
         viral_load = compute_viral_load(t, people.date_infectious, people.date_recovered, people.date_dead,

@@ -96,3 +96,155 @@ class clip_edges:
                     for i in range(3):
                         lay[i] = np.concatenate([lay[i], held[i][mask]])
                         held[i] = held[i][~mask]
+
+
+# ---------------------------------------------------------------------------------------------
+# Testing and contact tracing (interventions.py:682-710, 858-1118)
+# ---------------------------------------------------------------------------------------------
+def _active(iv, t):
+    return t >= iv.start_day and (iv.end_day is None or t <= iv.end_day)
+
+
+def get_quar_inds(quar_policy, sim):
+    """interventions.py:682-710 (the policies given as a string or numbers)."""
+    import numpy as np
+    t, people = sim.t, sim.people
+    if quar_policy is None:
+        return np.array([], dtype=np.int64)
+    if quar_policy == 'start':
+        return (people.date_quarantined == t - 1).nonzero()[0]
+    if quar_policy == 'end':
+        return (people.date_end_quarantine == t + 1).nonzero()[0]
+    if quar_policy == 'both':
+        return np.concatenate([(people.date_quarantined == t - 1).nonzero()[0],
+                               (people.date_end_quarantine == t + 1).nonzero()[0]])
+    if quar_policy == 'daily':
+        return people.quarantined.nonzero()[0]
+    q = np.atleast_1d(np.asarray(quar_policy))
+    return np.unique(np.concatenate([(people.date_quarantined == t - 1 - v).nonzero()[0] for v in q]))
+
+
+class test_prob:
+    """interventions.py:858-962 (without subtarget, ili_prev and swab_delay): test probabilities
+    set piece by piece in the reference's order, one Bernoulli draw per agent, then People.test.
+    The draws of agent i are philox(i, t, 0x90000000 | slot, 0) (tested, true positive) and
+    philox(i, t, 0x90000000 | slot, 1) (not lost), slot = the intervention's index in the sim."""
+
+    def __init__(self, symp_prob, asymp_prob=0.0, symp_quar_prob=None, asymp_quar_prob=None, quar_policy=None,
+                 sensitivity=1.0, loss_prob=0.0, test_delay=0, start_day=0, end_day=None):
+        self.symp_prob = symp_prob
+        self.asymp_prob = asymp_prob
+        self.symp_quar_prob = symp_quar_prob if symp_quar_prob is not None else symp_prob
+        self.asymp_quar_prob = asymp_quar_prob if asymp_quar_prob is not None else asymp_prob
+        self.quar_policy = quar_policy if quar_policy else 'start'
+        self.sensitivity = sensitivity
+        self.loss_prob = loss_prob
+        self.test_delay = test_delay
+        self.start_day = start_day
+        self.end_day = end_day
+
+    def initialize(self, sim):
+        return self
+
+    def __call__(self, sim):
+        import numpy as np
+        from . import philox as px
+        t, people = sim.t, sim.people
+        if not _active(self, t):
+            return
+        n = people.n
+        symp_inds = people.symptomatic.nonzero()[0]
+        asymp_inds = np.setdiff1d(np.arange(n), symp_inds)
+        quar_test_inds = get_quar_inds(self.quar_policy, sim)
+        symp_quar_inds = np.intersect1d(quar_test_inds, symp_inds)
+        asymp_quar_inds = np.intersect1d(quar_test_inds, asymp_inds)
+        diag_inds = people.diagnosed.nonzero()[0]
+        test_probs = np.zeros(n)
+        test_probs[symp_inds] = self.symp_prob
+        test_probs[asymp_inds] = self.asymp_prob
+        test_probs[symp_quar_inds] = self.symp_quar_prob
+        test_probs[asymp_quar_inds] = self.asymp_quar_prob
+        test_probs[diag_inds] = 0.0
+        stream = px.STREAM_TEST | sim.iv_slot
+        u_test, u_pos = px.stream_uniforms(sim.pars['rand_seed'], t, np.arange(n), stream, 0)
+        u_keep, _ = px.stream_uniforms(sim.pars['rand_seed'], t, np.arange(n), stream, 1)
+        test_inds = (u_test < test_probs).nonzero()[0]
+        people.test(test_inds, u_pos, u_keep, test_sensitivity=self.sensitivity, loss_prob=self.loss_prob,
+                    test_delay=self.test_delay)
+        sim.results['new_tests'][t] += len(test_inds)
+        return test_inds
+
+
+def find_contacts(p1, p2, inds):
+    """utils.py:131-147 + Layer.find_contacts (base.py): the partners of inds, sorted and unique."""
+    import numpy as np
+    inds = set(int(i) for i in inds)
+    partners = set()
+    for a, b in zip(p1.tolist(), p2.tolist()):
+        if a in inds:
+            partners.add(b)
+        if b in inds:
+            partners.add(a)
+    return np.array(sorted(partners), dtype=np.int64)
+
+
+class contact_tracing:
+    """interventions.py:965-1118 (without capacity): the agents diagnosed today (or, presumptive,
+    exposed and tested today) are traced through each layer with trace_prob > 0; each contact found
+    in layer l is kept when philox(i, t, 0xA0000000 | slot << 8 | l, 0) < trace_prob[l], and the
+    living ones are notified and scheduled to quarantine on day int(t + trace_time[l])."""
+
+    def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,
+                 quar_period=None):
+        self.trace_probs = trace_probs
+        self.trace_time = trace_time
+        self.start_day = start_day
+        self.end_day = end_day
+        self.presumptive = presumptive
+        self.quar_period = quar_period
+
+    def initialize(self, sim):
+        keys = list(sim.layers.keys())
+        if self.trace_probs is None:
+            self.trace_probs = 1.0
+        if self.trace_time is None:
+            self.trace_time = 0.0
+        if self.quar_period is None:
+            self.quar_period = sim.pars['quar_period']
+        if isinstance(self.trace_probs, (int, float)):
+            self.trace_probs = {k: self.trace_probs for k in keys}
+        if isinstance(self.trace_time, (int, float)):
+            self.trace_time = {k: self.trace_time for k in keys}
+        return self
+
+    def __call__(self, sim):
+        import numpy as np
+        from . import philox as px
+        t, people = sim.t, sim.people
+        if not _active(self, t):
+            return
+        if not self.presumptive:
+            trace_inds = (people.date_diagnosed == t).nonzero()[0]
+        else:
+            just_tested = (people.date_tested == t).nonzero()[0]
+            trace_inds = just_tested[people.exposed[just_tested]]
+        if not len(trace_inds):
+            return {}
+        keys = list(sim.layers.keys())
+        contacts = {}
+        for lkey, prob in self.trace_probs.items():
+            if prob == 0:
+                continue
+            lay = sim.layers[lkey]
+            traceable = find_contacts(lay[0], lay[1], trace_inds)
+            if len(traceable):
+                u, _ = px.stream_uniforms(sim.pars['rand_seed'], t, traceable,
+                                          px.STREAM_TRACE | (sim.iv_slot << 8) | keys.index(lkey), 0)
+                contacts.setdefault(self.trace_time[lkey], []).extend(traceable[u < prob].tolist())
+        is_dead = people.dead.nonzero()[0]
+        for trace_time, inds in contacts.items():  # notify_contacts, interventions.py:1101-1118
+            inds = np.setdiff1d(np.array(inds, dtype=np.int64), is_dead)
+            people.known_contact[inds] = True
+            people.date_known_contact[inds] = np.fmin(people.date_known_contact[inds], t + trace_time)
+            people.schedule_quarantine(inds, start_date=t + trace_time, period=self.quar_period - trace_time)
+        return contacts

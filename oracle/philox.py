@@ -23,6 +23,9 @@ variant import choice      (agent, t, IMPORT | v, 0)       w0 = selection key
 dynamic-layer resample     (q_lo, t, RESAMPLE | l, q_hi)   edge 2q: w0->p1, w1->p2;
   (pair q = e >> 1)                                         edge 2q+1: w2->p1, w3->p2
 resample retry k of edge e (e_lo, t, RETRY | l, e_hi<<8|k)  w0->p1, w1->p2
+test of agent i (op s)     (i, t, TEST | s, 0)             (w0,w1)->tested, (w2,w3)->positive
+                           (i, t, TEST | s, 1)             (w0,w1)->not lost
+trace of contact i, layer  (i, t, TRACE | s<<8 | l, 0)     (w0,w1)->traced
 =========================  ==============================  =====================
 
 Resampled endpoints are unbiased bounded integers (Lemire's multiply-and-reject):
@@ -46,6 +49,8 @@ STREAM_IMPORT = 0x40000000
 STREAM_RESAMPLE = 0x50000000
 STREAM_RESAMPLE_RETRY = 0x60000000
 RESAMPLE_MAX_RETRY = 255
+STREAM_TEST = 0x90000000   # | slot
+STREAM_TRACE = 0xA0000000  # | slot << 8 | layer
 
 # Fixed per-infection draw slots (see People.infect restatement in covasim_oracle.py)
 SLOT_EXP2INF = 0
@@ -120,6 +125,13 @@ def slot_uniforms(seed, t, agents, slot):
     """(u0, u1) for an infection-outcome slot of each agent on day t."""
     k0, k1 = seed_key(seed)
     w = philox4x32(np.asarray(agents, dtype=np.int64), t, STREAM_INFECT, slot, k0, k1)
+    return u01(w[0], w[1]), u01(w[2], w[3])
+
+
+def stream_uniforms(seed, t, agents, stream, c3=0):
+    """(u0, u1) of the draw (agent, t, stream, c3) of each agent: words 0,1 and 2,3."""
+    k0, k1 = seed_key(seed)
+    w = philox4x32(np.asarray(agents, dtype=np.int64), t, stream, c3, k0, k1)
     return u01(w[0], w[1]), u01(w[2], w[3])
 
 

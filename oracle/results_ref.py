@@ -102,6 +102,14 @@ def finalize(counts, counts_f, col, byvar_base, max_variants, n_variants, pop_si
         R['pop_nabs'] = counts_f[:, 0] / counts[:, col['n_alive']]
         R['pop_protection'] = counts_f[:, 1] / (n_variants * pop_size)
         R['pop_symp_protection'] = counts_f[:, 2] / (n_variants * pop_size)
+    R['test_yield'] = np.zeros(npts)  # compute_yield, sim.py:853-868
+    inds = R['new_tests'].nonzero()[0]
+    R['test_yield'][inds] = R['new_diagnoses'][inds] / R['new_tests'][inds]
+    R['rel_test_yield'] = np.zeros(npts)
+    inds = R['n_infectious'].nonzero()[0]
+    with np.errstate(divide='ignore', invalid='ignore'):
+        denom = R['n_infectious'][inds] / (R['n_alive'][inds] - R['cum_diagnoses'][inds])
+        R['rel_test_yield'][inds] = R['test_yield'][inds] / denom
     R['doubling_time'] = compute_doubling(R['cum_infections'])
     R['r_eff'] = compute_r_eff(R['new_infections'], R['n_imports'], R['n_infectious'], date_infectious,
                                date_recovered, date_dead, initial_period_par, smoothing)

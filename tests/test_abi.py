@@ -37,9 +37,10 @@ def test_struct_layout_report():
     L = _lib.lib()
     out = (C.c_int64 * 32)()
     m = L.cvk_abi_layout(out, 32)
-    assert m == 24
+    assert m == 28
     assert out[0] == C.sizeof(_lib.Dur) == 40
     assert out[21] == C.sizeof(_lib.PopGen) and out[23] == C.sizeof(_lib.PopCounts) == 40
+    assert out[24] == C.sizeof(_lib.DiagOp) and out[26] == C.sizeof(_lib.DiagS) == 80
     assert L.cvk_abi_layout(out, 3) < 0  # too-small output is an error, not an overflow
 
 
@@ -62,6 +63,17 @@ def test_host_side_argument_errors():
     assert L.cvk_graph_launch(None, None) == _lib.E_INVALID
     with pytest.raises(ValueError):
         _lib.check(L.cvk_agent_pass(None, None, None, 0, None), 'agent pass')
+
+
+def test_diag_argument_errors():
+    """cvk_diag_apply / cvk_states_post reject missing state, bad ops and days before any device work."""
+    from covasim_amd import _lib
+    L = _lib.lib()
+    P, prm, w, d, op = _lib.PeopleS(), _lib.Params(), _lib.WorkS(), _lib.DiagS(), _lib.DiagOp()
+    assert L.cvk_states_post(None, None, None, None, 0, None) == _lib.E_INVALID
+    assert L.cvk_diag_apply(C.byref(P), None, C.byref(prm), C.byref(w), C.byref(d), C.byref(op), 0,
+                            None) == _lib.E_INVALID
+    assert b'people arrays' in L.cvk_last_error()
 
 
 def test_population_argument_errors():
