@@ -162,7 +162,7 @@ OP_TEST, OP_TRACE = 1, 2
 QP_NONE, QP_START, QP_END, QP_BOTH, QP_DAILY, QP_DAYS = range(6)
 MAX_QUAR_DAYS = 8
 PENDING_NONE = -2**31
-DIAG_CTL_WORDS = 8
+DIAG_MAX_BLOCKS = 4096
 
 
 class DiagOp(C.Structure):
@@ -180,8 +180,8 @@ DIAG_DATE_FIELDS = ['date_tested', 'date_pos_test', 'date_known_contact', 'date_
 
 class DiagS(C.Structure):
     _fields_ = ([(k, C.c_void_p) for k in DIAG_DATE_FIELDS] +
-                [('pending', C.c_void_p), ('cases', C.c_void_p), ('contacts', C.c_void_p), ('ctl', C.c_void_p),
-                 ('ring', C.c_int32), ('pad', C.c_int32)])
+                [('pending', C.c_void_p), ('cases', C.c_void_p), ('coarse', C.c_void_p), ('contacts', C.c_void_p),
+                 ('part', C.c_void_p), ('ring', C.c_int32), ('pad', C.c_int32)])
 
 
 class ImportsS(C.Structure):
@@ -315,6 +315,8 @@ def lib():
     L.cvk_diag_apply.restype = C.c_int
     L.cvk_states_post.argtypes = [P, Pa, W, DS, i32, vp]
     L.cvk_states_post.restype = C.c_int
+    L.cvk_diag_coarse_words.argtypes = [i64]
+    L.cvk_diag_coarse_words.restype = i64
     L.cvk_reduce.restype = C.c_int
     PG, PC = C.POINTER(PopGen), C.POINTER(PopCounts)
     L.cvk_pop_work_bytes.argtypes = [i64, i64]

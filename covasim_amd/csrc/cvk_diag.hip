@@ -23,6 +23,8 @@
 // counted once), which is what the ring keeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "cvk_host.h"
 #include "cvk_kernels.h"
 
@@ -30,15 +32,27 @@ namespace cvk {
 namespace {
 
 constexpr int DIAG_BLOCK = 256;
-constexpr int DIAG_APT = 2;                 // agents per thread (contiguous 256-agent chunks)
+constexpr int DIAG_APT = 2;                 // agents per thread and pass (contiguous 256-agent chunks)
 constexpr int DIAG_SPAN = DIAG_BLOCK * DIAG_APT;
 constexpr int TRACE_EDGE_BLOCK = 256;
-constexpr int TRACE_EDGE_GRID = 2048;       // grid-stride blocks per traced layer
+constexpr int TRACE_EDGE_GRID = 2048;       // grid-stride blocks of the edge scan
+constexpr int COARSE_MAX_WORDS = 2048;      // 8 KiB of LDS (a coarse bit per 4 agents at 262k agents, 64 at 4M)
+constexpr int N_PART = 4;                   // per-block count columns: tests, diagnoses, quarantined, isolated
 
-// Adds per-wave counts c[0..NC) (wave-uniform, from ballots) to row[cols[k]]: one atomic per block
-// and column (columns only these kernels write).
+__host__ __device__ inline int coarse_shift(int64_t n) {
+  int s = 0;
+  while (((n + ((int64_t)1 << s) - 1) >> s) > (int64_t)COARSE_MAX_WORDS * 32) ++s;
+  return s;
+}
+__host__ __device__ inline int64_t coarse_words(int64_t n) {
+  const int s = coarse_shift(n);
+  return (((n + ((int64_t)1 << s) - 1) >> s) + 31) / 32;
+}
+
+// Adds this block's wave counts c[0..NC) (wave-uniform, from ballots) to its entries of the
+// per-block count columns cols[k] (one writer per entry; kernels of a day run in stream order).
 template <int NC>
-__device__ inline void block_add(const uint32_t (&c)[NC], uint32_t* row, const int (&cols)[NC]) {
+__device__ inline void block_part(const uint32_t (&c)[NC], uint32_t* part, const int (&cols)[NC]) {
   __shared__ uint32_t s[DIAG_BLOCK / 64][NC];
   const int w = threadIdx.x >> 6;
   if (lane_id() == 0) {
@@ -50,22 +64,22 @@ __device__ inline void block_add(const uint32_t (&c)[NC], uint32_t* row, const i
     uint32_t tot = 0;
 #pragma unroll
     for (int j = 0; j < DIAG_BLOCK / 64; ++j) tot += s[j][threadIdx.x];
-    if (tot) atomicAdd(row + cols[threadIdx.x], tot);
+    if (tot) part[(int64_t)cols[threadIdx.x] * CVK_DIAG_MAX_BLOCKS + blockIdx.x] += tot;
   }
 }
 
-__device__ inline int64_t diag_agent(int r) {
-  return ((int64_t)blockIdx.x * DIAG_APT + r) * DIAG_BLOCK + threadIdx.x;
-}
+// Agent passes stride over spans of DIAG_SPAN agents; agent index of chunk r of the span at base.
+__device__ inline int64_t span_stride() { return (int64_t)gridDim.x * DIAG_SPAN; }
 
 // get_quar_inds (interventions.py:682-710) for one agent.
-__device__ inline bool quar_test(const cvk_diag_op& op, uint32_t f, float dq, float deq, int32_t t) {
+__device__ inline bool quar_test(const cvk_diag_op& op, uint32_t f, const cvk_diag& D, int64_t i, int32_t t) {
   switch (op.quar_policy) {
-    case CVK_QP_START: return dq == (float)(t - 1);
-    case CVK_QP_END: return deq == (float)(t + 1);
-    case CVK_QP_BOTH: return dq == (float)(t - 1) || deq == (float)(t + 1);
+    case CVK_QP_START: return D.date_quarantined[i] == (float)(t - 1);
+    case CVK_QP_END: return D.date_end_quarantine[i] == (float)(t + 1);
+    case CVK_QP_BOTH: return D.date_quarantined[i] == (float)(t - 1) || D.date_end_quarantine[i] == (float)(t + 1);
     case CVK_QP_DAILY: return (f & CVK_QUARANTINED) != 0;
     case CVK_QP_DAYS: {
+      const float dq = D.date_quarantined[i];
       bool q = false;
       for (int k = 0; k < op.n_quar_days; ++k) q = q || dq == (float)(t - 1 - op.quar_days[k]);
       return q;
@@ -75,69 +89,83 @@ __device__ inline bool quar_test(const cvk_diag_op& op, uint32_t f, float dq, fl
 }
 
 __global__ __launch_bounds__(DIAG_BLOCK) void test_kernel(cvk_people P, cvk_diag D, cvk_diag_op op, uint64_t seed,
-                                                          int32_t t, uint32_t* row) {
+                                                          int32_t t) {
   const float tf = (float)t;
   const uint32_t stream = STREAM_TEST | (uint32_t)op.slot;
   uint32_t c[1] = {0};
+  for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
-  for (int r = 0; r < DIAG_APT; ++r) {
-    const int64_t i = diag_agent(r);
-    bool tested = false;
-    if (i < P.n) {
-      const uint32_t f = P.flags[i];
-      if (!(f & CVK_DIAGNOSED)) {  // test_probs[diag_inds] = 0
-        const bool symp = (f & CVK_SYMPTOMATIC) != 0;
-        const bool q = op.quar_policy != CVK_QP_NONE && op.quar_policy != CVK_QP_DAILY
-                           ? quar_test(op, f, D.date_quarantined[i], D.date_end_quarantine[i], t)
-                           : quar_test(op, f, 0.0f, 0.0f, t);
-        const double p = q ? (symp ? op.symp_quar_prob : op.asymp_quar_prob) : (symp ? op.symp_prob : op.asymp_prob);
-        if (p > 0.0) {  // binomial_arr: u < p (never for p <= 0)
-          const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 0);
-          if (u01(w.x, w.y) < p) {
-            tested = true;
-            P.flags[i] = f | CVK_TESTED;
-            D.date_tested[i] = tf;
-            // true positive among the infectious, not yet holding a diagnosis date, not lost
-            if ((f & CVK_INFECTIOUS) && u01(w.z, w.w) < op.sensitivity && isnan(P.date_diagnosed[i])) {
-              const u32x4 w2 = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 1);
-              if (u01(w2.x, w2.y) < op.keep_prob) {
-                P.date_diagnosed[i] = (float)((double)t + op.test_delay);
-                D.date_pos_test[i] = tf;
+    for (int r = 0; r < DIAG_APT; ++r) {
+      const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
+      bool tested = false;
+      if (i < P.n) {
+        const uint32_t f = P.flags[i];
+        if (!(f & CVK_DIAGNOSED)) {  // test_probs[diag_inds] = 0
+          const bool symp = (f & CVK_SYMPTOMATIC) != 0;
+          const bool q = quar_test(op, f, D, i, t);
+          const double p =
+              q ? (symp ? op.symp_quar_prob : op.asymp_quar_prob) : (symp ? op.symp_prob : op.asymp_prob);
+          if (p > 0.0) {  // binomial_arr: u < p (never for p <= 0)
+            const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 0);
+            if (u01(w.x, w.y) < p) {
+              tested = true;
+              P.flags[i] = f | CVK_TESTED;
+              D.date_tested[i] = tf;
+              // true positive among the infectious, not yet holding a diagnosis date, not lost
+              if ((f & CVK_INFECTIOUS) && u01(w.z, w.w) < op.sensitivity && isnan(P.date_diagnosed[i])) {
+                const u32x4 w2 = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 1);
+                if (u01(w2.x, w2.y) < op.keep_prob) {
+                  P.date_diagnosed[i] = (float)((double)t + op.test_delay);
+                  D.date_pos_test[i] = tf;
+                }
               }
             }
           }
         }
       }
+      c[0] += (uint32_t)__popcll(__ballot(tested));
     }
-    c[0] += (uint32_t)__popcll(__ballot(tested));
   }
-  const int cols[1] = {CVK_R_NEW_TESTS};
-  block_add<1>(c, row, cols);
+  const int cols[1] = {0};
+  block_part<1>(c, D.part, cols);
 }
 
+// Bitmap of today's cases, and the coarse map (bit k: some case among agents [k << shift, ...)).
 __global__ __launch_bounds__(DIAG_BLOCK) void trace_cases_kernel(cvk_people P, cvk_diag D, int32_t presumptive,
-                                                                 int32_t t, uint32_t* count) {
+                                                                 int32_t t, int32_t shift) {
   const float tf = (float)t;
   const int64_t n_words = (P.n + 63) / 64;
-  uint32_t c = 0;
+  for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
-  for (int r = 0; r < DIAG_APT; ++r) {
-    const int64_t i = diag_agent(r);
-    bool is_case = false;
-    if (i < P.n)
-      is_case = presumptive ? ((P.flags[i] & CVK_EXPOSED) && D.date_tested[i] == tf) : P.date_diagnosed[i] == tf;
-    const uint64_t b = __ballot(is_case);
-    const int64_t word = (((int64_t)blockIdx.x * DIAG_APT + r) * DIAG_BLOCK + (threadIdx.x & ~63u)) >> 6;
-    if (lane_id() == 0 && word < n_words) D.cases[word] = b;
-    c += (uint32_t)__popcll(b);
-  }
-  __shared__ uint32_t s[DIAG_BLOCK / 64];
-  if (lane_id() == 0) s[threadIdx.x >> 6] = c;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t tot = 0;
-    for (int j = 0; j < DIAG_BLOCK / 64; ++j) tot += s[j];
-    if (tot) atomicAdd(count, tot);
+    for (int r = 0; r < DIAG_APT; ++r) {
+      const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
+      bool is_case = false;
+      if (i < P.n)
+        is_case = presumptive ? ((P.flags[i] & CVK_EXPOSED) && D.date_tested[i] == tf) : P.date_diagnosed[i] == tf;
+      const uint64_t b = __ballot(is_case);
+      const int64_t word = (base + r * DIAG_BLOCK + (threadIdx.x & ~63u)) >> 6;
+      if (lane_id() == 0 && word < n_words) {
+        D.cases[word] = b;
+        if (b) {
+          if (shift >= 6) {
+            const int64_t k = (word << 6) >> shift;
+            atomicOr(D.coarse + (k >> 5), 1u << (k & 31));
+          } else {
+            const int g = 1 << shift;
+            uint64_t cb = 0;
+            for (int j = 0; j < 64 / g; ++j)
+              if ((b >> (j * g)) & ((g == 64 ? 0ull : (1ull << g)) - 1ull)) cb |= 1ull << j;
+            const int64_t k0 = (word << 6) >> shift;  // a multiple of 64 / g
+            if (g == 1) {
+              atomicOr(D.coarse + (k0 >> 5), (uint32_t)cb);
+              atomicOr(D.coarse + (k0 >> 5) + 1, (uint32_t)(cb >> 32));
+            } else {
+              atomicOr(D.coarse + (k0 >> 5), (uint32_t)cb << (k0 & 31));
+            }
+          }
+        }
+      }
+    }
   }
 }
 
@@ -146,65 +174,98 @@ struct TraceLayers {
   const int32_t* p2[CVK_MAX_LAYERS];
   int64_t n[CVK_MAX_LAYERS];
   uint32_t layer[CVK_MAX_LAYERS];  // layer index (the contact bit)
+  int32_t nl;
 };
 
-__device__ inline bool case_bit(const uint64_t* cases, int32_t a) {
-  return (cases[(uint32_t)a >> 6] >> ((uint32_t)a & 63u)) & 1ull;
-}
-
-// blockIdx.y: the traced layer; every block of a layer strides over its edges.
+// Marks the partners of today's cases, every traced layer in turn (grid-stride, 4 edges per lane).
+// The coarse map sits in LDS; only endpoints whose coarse bit is set read the case bitmap.
 __global__ __launch_bounds__(TRACE_EDGE_BLOCK) void trace_edges_kernel(TraceLayers L, const uint64_t* cases,
-                                                                       uint32_t* contacts, const uint32_t* count) {
-  if (*count == 0) return;  // nobody to trace today
-  const int li = blockIdx.y;
-  const int32_t* p1 = L.p1[li];
-  const int32_t* p2 = L.p2[li];
-  const int64_t n = L.n[li];
-  const uint32_t bit = 1u << L.layer[li];
-  for (int64_t e = (int64_t)blockIdx.x * TRACE_EDGE_BLOCK + threadIdx.x; e < n;
-       e += (int64_t)gridDim.x * TRACE_EDGE_BLOCK) {
-    const int32_t a = p1[e], b = p2[e];
-    if (case_bit(cases, a)) atomicOr(contacts + b, bit);
-    if (case_bit(cases, b)) atomicOr(contacts + a, bit);
+                                                                       const uint32_t* coarse, int32_t shift,
+                                                                       int32_t n_cwords, uint32_t* contacts) {
+  extern __shared__ uint32_t s_coarse[];
+  int any = 0;
+  for (int k = threadIdx.x; k < n_cwords; k += TRACE_EDGE_BLOCK) {
+    const uint32_t v = coarse[k];
+    s_coarse[k] = v;
+    any |= v != 0u;
+  }
+  if (!__syncthreads_or(any)) return;  // nobody to trace today
+  auto is_case = [&](int32_t a) -> bool {
+    const uint32_t k = (uint32_t)a >> shift;
+    if (!((s_coarse[k >> 5] >> (k & 31u)) & 1u)) return false;
+    return (cases[(uint32_t)a >> 6] >> ((uint32_t)a & 63u)) & 1ull;
+  };
+  const int64_t stride = (int64_t)gridDim.x * TRACE_EDGE_BLOCK;
+#pragma unroll 1
+  for (int l = 0; l < CVK_MAX_LAYERS; ++l) {
+    if (l >= L.nl) break;
+    const int32_t* p1 = L.p1[l];
+    const int32_t* p2 = L.p2[l];
+    const int64_t n = L.n[l];
+    const uint32_t bit = 1u << L.layer[l];
+    // two quads per lane and pass: all four loads are in flight before the first lookup
+    for (int64_t q = (int64_t)blockIdx.x * TRACE_EDGE_BLOCK + threadIdx.x; q * 4 < n; q += 2 * stride) {
+      const int64_t e0 = q * 4, e1 = (q + stride) * 4;
+      const v4i a0 = load4(p1, e0, n), b0 = load4(p2, e0, n);
+      const v4i a1 = load4(p1, e1, n), b1 = load4(p2, e1, n);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (e0 + k < n) {
+          if (is_case(a0[k])) atomicOr(contacts + b0[k], bit);
+          if (is_case(b0[k])) atomicOr(contacts + a0[k], bit);
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        if (e1 + k < n) {
+          if (is_case(a1[k])) atomicOr(contacts + b1[k], bit);
+          if (is_case(b1[k])) atomicOr(contacts + a1[k], bit);
+        }
+      }
+    }
   }
 }
 
 __global__ __launch_bounds__(DIAG_BLOCK) void trace_notify_kernel(cvk_people P, cvk_diag D, cvk_diag_op op,
-                                                                  uint64_t seed, int32_t t) {
+                                                                  uint64_t seed, int32_t t, int32_t n_cwords) {
+  // the coarse map has been read: clear it for the next tracing op
+  for (int64_t k = (int64_t)blockIdx.x * DIAG_BLOCK + threadIdx.x; k < n_cwords; k += (int64_t)gridDim.x * DIAG_BLOCK)
+    D.coarse[k] = 0u;
+  for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
-  for (int r = 0; r < DIAG_APT; ++r) {
-    const int64_t i = diag_agent(r);
-    if (i >= P.n) continue;
-    const uint32_t m = D.contacts[i];
-    if (!m) continue;
-    D.contacts[i] = 0;
-    const uint32_t f = P.flags[i];
-    if (f & CVK_DEAD) continue;  // np.setdiff1d(contact_inds, is_dead)
-    float dkc = D.date_known_contact[i];
-    bool any = false;
-    for (uint32_t mm = m; mm; mm &= mm - 1) {
-      const int l = __ffs(mm) - 1;
-      const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t,
-                                    STREAM_TRACE | ((uint32_t)op.slot << 8) | (uint32_t)l, 0);
-      if (!(u01(w.x, w.y) < op.trace_prob[l])) continue;  // binomial_filter
-      any = true;
-      const double s = (double)t + op.trace_time[l];
-      dkc = fminf(dkc, (float)s);  // np.fmin: a NaN date gives way
-      const int32_t start = (int32_t)s;
-      const int32_t end = start + op.trace_period[l];
-      int32_t* pend = D.pending + (int64_t)(start % D.ring) * P.n + i;
-      if (end > *pend) *pend = end;
-    }
-    if (any) {
-      P.flags[i] = f | CVK_KNOWN_CONTACT;
-      D.date_known_contact[i] = dkc;
+    for (int r = 0; r < DIAG_APT; ++r) {
+      const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
+      if (i >= P.n) continue;
+      const uint32_t m = D.contacts[i];
+      if (!m) continue;
+      D.contacts[i] = 0;
+      const uint32_t f = P.flags[i];
+      if (f & CVK_DEAD) continue;  // np.setdiff1d(contact_inds, is_dead)
+      float dkc = D.date_known_contact[i];
+      bool any = false;
+      for (uint32_t mm = m; mm; mm &= mm - 1) {
+        const int l = __ffs(mm) - 1;
+        const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t,
+                                      STREAM_TRACE | ((uint32_t)op.slot << 8) | (uint32_t)l, 0);
+        if (!(u01(w.x, w.y) < op.trace_prob[l])) continue;  // binomial_filter
+        any = true;
+        const double s = (double)t + op.trace_time[l];
+        dkc = fminf(dkc, (float)s);  // np.fmin: a NaN date gives way
+        const int32_t start = (int32_t)s;
+        const int32_t end = start + op.trace_period[l];
+        int32_t* pend = D.pending + (int64_t)(start % D.ring) * P.n + i;
+        if (end > *pend) *pend = end;
+      }
+      if (any) {
+        P.flags[i] = f | CVK_KNOWN_CONTACT;
+        D.date_known_contact[i] = dkc;
+      }
     }
   }
 }
 
 __global__ __launch_bounds__(DIAG_BLOCK) void states_post_kernel(cvk_people P, cvk_params prm, cvk_work W,
-                                                                 cvk_diag D, int32_t t, uint32_t* row) {
-  if (blockIdx.x == 0 && threadIdx.x < 8) D.ctl[threadIdx.x] = 0;  // the day's tracing ops are done
+                                                                 cvk_diag D, int32_t t) {
   const float tf = (float)t;
   float quar_max = 0.0f, iso_max = 0.0f;
   for (int l = 0; l < prm.n_layers; ++l) {
@@ -213,81 +274,115 @@ __global__ __launch_bounds__(DIAG_BLOCK) void states_post_kernel(cvk_people P, c
   }
   int32_t* pend_row = D.pending + (int64_t)(t % D.ring) * P.n;
   uint32_t c[3] = {0, 0, 0};
+  for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
-  for (int r = 0; r < DIAG_APT; ++r) {
-    const int64_t i = diag_agent(r);
-    bool new_diag = false, new_quar = false, new_iso = false;
-    if (i < P.n) {
-      const uint32_t f0 = P.flags[i];
-      uint32_t f = f0;
-      const float dd = P.date_diagnosed[i];
-      // check_diagnosed (people.py:315-332): positive tests counted, then diagnoses that are due
-      if (!(f & CVK_DIAGNOSED)) {
-        const float dpt = D.date_pos_test[i];
-        if (!isnan(dpt) && tf >= dpt) {
-          D.date_pos_test[i] = __int_as_float(0x7FC00000);
-          new_diag = true;
-        }
-        if (!isnan(dd) && tf >= dd) f |= CVK_DIAGNOSED;
-      }
-      // check_quar (people.py:335-358)
-      const int32_t e = pend_row[i];
-      float deq = 0.0f;
-      bool deq_loaded = false;
-      if (e != CVK_PENDING_NONE) {
-        pend_row[i] = CVK_PENDING_NONE;
-        if (f & CVK_QUARANTINED) {
-          deq = D.date_end_quarantine[i];
-          deq_loaded = true;
-          if ((float)e > deq) {  // max(date_end_quarantine, end_day)
-            deq = (float)e;
-            D.date_end_quarantine[i] = deq;
+    for (int r = 0; r < DIAG_APT; ++r) {
+      const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
+      bool new_diag = false, new_quar = false, new_iso = false;
+      if (i < P.n) {
+        const uint32_t f0 = P.flags[i];
+        uint32_t f = f0;
+        const float dd = P.date_diagnosed[i];
+        // check_diagnosed (people.py:315-332): positive tests counted, then diagnoses that are due
+        if (!(f & CVK_DIAGNOSED)) {
+          const float dpt = D.date_pos_test[i];
+          if (!isnan(dpt) && tf >= dpt) {
+            D.date_pos_test[i] = __int_as_float(0x7FC00000);
+            new_diag = true;
           }
-        } else if (!(f & (CVK_DEAD | CVK_RECOVERED | CVK_DIAGNOSED | CVK_ISOLATED))) {
-          f |= CVK_QUARANTINED;
-          D.date_quarantined[i] = tf;
-          deq = (float)e;
-          deq_loaded = true;
-          D.date_end_quarantine[i] = deq;
-          new_quar = true;
+          if (!isnan(dd) && tf >= dd) f |= CVK_DIAGNOSED;
+        }
+        // check_quar (people.py:335-358)
+        const int32_t e = pend_row[i];
+        float deq = 0.0f;
+        bool deq_loaded = false;
+        if (e != CVK_PENDING_NONE) {
+          pend_row[i] = CVK_PENDING_NONE;
+          if (f & CVK_QUARANTINED) {
+            deq = D.date_end_quarantine[i];
+            deq_loaded = true;
+            if ((float)e > deq) {  // max(date_end_quarantine, end_day)
+              deq = (float)e;
+              D.date_end_quarantine[i] = deq;
+            }
+          } else if (!(f & (CVK_DEAD | CVK_RECOVERED | CVK_DIAGNOSED | CVK_ISOLATED))) {
+            f |= CVK_QUARANTINED;
+            D.date_quarantined[i] = tf;
+            deq = (float)e;
+            deq_loaded = true;
+            D.date_end_quarantine[i] = deq;
+            new_quar = true;
+          }
+        }
+        if (f & CVK_QUARANTINED) {
+          if (!deq_loaded) deq = D.date_end_quarantine[i];
+          if (dd == tf) {  // diagnosed today: quarantine ends, isolation begins
+            deq = tf;
+            D.date_end_quarantine[i] = tf;
+          }
+          if (!isnan(deq) && tf >= deq) f &= ~CVK_QUARANTINED;
+        }
+        // check_enter_iso (people.py:361-365)
+        if (dd == tf) {
+          f |= CVK_ISOLATED;
+          P.date_end_isolation[i] = P.date_recovered[i];
+          new_iso = true;
+        }
+        if (f != f0) {
+          P.flags[i] = f;
+          if ((f ^ f0) & (CVK_QUARANTINED | CVK_ISOLATED)) {  // factors of today's transmission changed
+            const float rt = P.rel_trans[i], rs = P.rel_sus[i];
+            W.rec[i] = edge_record(f, rt, rs);
+            W.bound[i] = edge_bound(f, rt, rs, prm, quar_max, iso_max);
+          }
         }
       }
-      if (f & CVK_QUARANTINED) {
-        if (!deq_loaded) deq = D.date_end_quarantine[i];
-        if (dd == tf) {  // diagnosed today: quarantine ends, isolation begins
-          deq = tf;
-          D.date_end_quarantine[i] = tf;
-        }
-        if (!isnan(deq) && tf >= deq) f &= ~CVK_QUARANTINED;
-      }
-      // check_enter_iso (people.py:361-365)
-      if (dd == tf) {
-        f |= CVK_ISOLATED;
-        P.date_end_isolation[i] = P.date_recovered[i];
-        new_iso = true;
-      }
-      if (f != f0) {
-        P.flags[i] = f;
-        if ((f ^ f0) & (CVK_QUARANTINED | CVK_ISOLATED)) {  // factors of today's transmission changed
-          const float rt = P.rel_trans[i], rs = P.rel_sus[i];
-          W.rec[i] = edge_record(f, rt, rs);
-          W.bound[i] = edge_bound(f, rt, rs, prm, quar_max, iso_max);
-        }
-      }
+      c[0] += (uint32_t)__popcll(__ballot(new_diag));
+      c[1] += (uint32_t)__popcll(__ballot(new_quar));
+      c[2] += (uint32_t)__popcll(__ballot(new_iso));
     }
-    c[0] += (uint32_t)__popcll(__ballot(new_diag));
-    c[1] += (uint32_t)__popcll(__ballot(new_quar));
-    c[2] += (uint32_t)__popcll(__ballot(new_iso));
   }
-  const int cols[3] = {CVK_R_NEW_DIAGNOSES, CVK_R_NEW_QUARANTINED, CVK_R_NEW_ISOLATED};
-  block_add<3>(c, row, cols);
+  const int cols[3] = {1, 2, 3};
+  block_part<3>(c, D.part, cols);
 }
 
-int agent_grid(int64_t n) { return (int)((n + DIAG_SPAN - 1) / DIAG_SPAN); }
+// The day's per-block counts into row t (one block), then cleared.
+constexpr int DIAG_REDUCE_BLOCK = 1024;
+__global__ __launch_bounds__(DIAG_REDUCE_BLOCK) void diag_reduce_kernel(uint32_t* part, int32_t n_blocks,
+                                                                        uint32_t* row) {
+  __shared__ uint32_t s[DIAG_REDUCE_BLOCK / 64][N_PART];
+  uint32_t acc[N_PART];
+#pragma unroll
+  for (int k = 0; k < N_PART; ++k) {
+    acc[k] = 0;
+    for (int b = threadIdx.x; b < n_blocks; b += DIAG_REDUCE_BLOCK) {
+      uint32_t* p = part + (int64_t)k * CVK_DIAG_MAX_BLOCKS + b;
+      acc[k] += *p;
+      *p = 0;
+    }
+    acc[k] = wave_sum(acc[k]);
+  }
+  if (lane_id() == 0) {
+#pragma unroll
+    for (int k = 0; k < N_PART; ++k) s[threadIdx.x >> 6][k] = acc[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < N_PART) {
+    uint32_t tot = 0;
+    for (int j = 0; j < DIAG_REDUCE_BLOCK / 64; ++j) tot += s[j][threadIdx.x];
+    const int cols[N_PART] = {CVK_R_NEW_TESTS, CVK_R_NEW_DIAGNOSES, CVK_R_NEW_QUARANTINED, CVK_R_NEW_ISOLATED};
+    row[cols[threadIdx.x]] = tot;
+  }
+}
+
+int agent_grid(int64_t n) {
+  const int64_t g = (n + DIAG_SPAN - 1) / DIAG_SPAN;
+  return (int)(g < CVK_DIAG_MAX_BLOCKS ? g : CVK_DIAG_MAX_BLOCKS);
+}
 
 bool diag_ok(const cvk_diag* D) {
   return D && D->date_tested && D->date_pos_test && D->date_known_contact && D->date_quarantined &&
-         D->date_end_quarantine && D->pending && D->cases && D->contacts && D->ctl && D->ring >= 1;
+         D->date_end_quarantine && D->pending && D->cases && D->coarse && D->contacts && D->part && D->ring >= 1;
 }
 
 int check_common(const cvk_people* P, const cvk_params* prm, const cvk_work* W, const cvk_diag* D, int32_t t) {
@@ -299,6 +394,7 @@ int check_common(const cvk_people* P, const cvk_params* prm, const cvk_work* W, 
   if (!W->results || !W->rec || !W->bound) return fail(CVK_E_INVALID, "workspace arrays missing");
   if (t < 0 || t >= W->npts) return fail(CVK_E_INVALID, "day out of range");
   if (!diag_ok(D)) return fail(CVK_E_INVALID, "cvk_diag arrays missing or ring < 1");
+  if ((uintptr_t)D->cases & 7) return fail(CVK_E_ALIGN, "cvk_diag.cases must be 8-byte aligned");
   return CVK_OK;
 }
 
@@ -320,6 +416,8 @@ int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, co
     if ((int64_t)op->trace_time[l] + 1 >= D->ring) return fail(CVK_E_INVALID, "cvk_diag.ring too small for trace_time");
     if (!layers) return fail(CVK_E_INVALID, "null layers");
     if (layers[l].n_edges > 0 && (!layers[l].p1 || !layers[l].p2)) return fail(CVK_E_INVALID, "layer arrays missing");
+    if (((uintptr_t)layers[l].p1 | (uintptr_t)layers[l].p2) & 15)
+      return fail(CVK_E_ALIGN, "traced layers: p1/p2 must be 16-byte aligned");
   }
   return CVK_OK;
 }
@@ -327,42 +425,46 @@ int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, co
 int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W, const cvk_diag* D,
                const cvk_diag_op* op, int32_t t, hipStream_t st) {
   if (t < op->start_day || (op->end_day >= 0 && t > op->end_day)) return CVK_OK;
-  uint32_t* row = W->results + (int64_t)t * CVK_R_COLS;
   const int grid = agent_grid(P->n);
   if (op->kind == CVK_OP_TEST) {
-    test_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t, row);
+    test_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t);
     return check_launch("test_kernel");
   }
   TraceLayers L{};
   int nl = 0;
-  int64_t n_max = 0;
+  int64_t quads = 0;
   for (int l = 0; l < prm->n_layers; ++l) {
     if (!(op->trace_prob[l] > 0.0) || layers[l].n_edges <= 0) continue;  // trace_prob == 0: layer skipped
     L.p1[nl] = layers[l].p1;
     L.p2[nl] = layers[l].p2;
     L.n[nl] = layers[l].n_edges;
     L.layer[nl] = (uint32_t)l;
-    n_max = n_max > layers[l].n_edges ? n_max : layers[l].n_edges;
+    quads = std::max<int64_t>(quads, (layers[l].n_edges + 3) / 4);
     ++nl;
   }
   if (nl == 0) return CVK_OK;
-  uint32_t* count = D->ctl + (op->slot & 7);
-  trace_cases_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, op->presumptive, t, count);
+  L.nl = nl;
+  const int shift = coarse_shift(P->n);
+  const int64_t n_cwords = coarse_words(P->n);
+  trace_cases_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, op->presumptive, t, shift);
   int rc = check_launch("trace_cases_kernel");
   if (rc) return rc;
-  const int64_t gx = (n_max + TRACE_EDGE_BLOCK - 1) / TRACE_EDGE_BLOCK;
-  trace_edges_kernel<<<dim3((unsigned)(gx < TRACE_EDGE_GRID ? gx : TRACE_EDGE_GRID), (unsigned)nl), TRACE_EDGE_BLOCK,
-                       0, st>>>(L, D->cases, D->contacts, count);
+  const int64_t gx = (quads + TRACE_EDGE_BLOCK - 1) / TRACE_EDGE_BLOCK;
+  trace_edges_kernel<<<(unsigned)std::min<int64_t>(gx, TRACE_EDGE_GRID), TRACE_EDGE_BLOCK, (size_t)n_cwords * 4,
+                       st>>>(L, D->cases, D->coarse, shift, (int32_t)n_cwords, D->contacts);
   rc = check_launch("trace_edges_kernel");
   if (rc) return rc;
-  trace_notify_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t);
+  trace_notify_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t, (int32_t)n_cwords);
   return check_launch("trace_notify_kernel");
 }
 
 int post_impl(const cvk_people* P, const cvk_params* prm, cvk_work* W, const cvk_diag* D, int32_t t, hipStream_t st) {
-  uint32_t* row = W->results + (int64_t)t * CVK_R_COLS;
-  states_post_kernel<<<agent_grid(P->n), DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t, row);
-  return check_launch("states_post_kernel");
+  const int grid = agent_grid(P->n);
+  states_post_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t);
+  const int rc = check_launch("states_post_kernel");
+  if (rc) return rc;
+  diag_reduce_kernel<<<1, DIAG_REDUCE_BLOCK, 0, st>>>(D->part, grid, W->results + (int64_t)t * CVK_R_COLS);
+  return check_launch("diag_reduce_kernel");
 }
 
 }  // namespace
@@ -395,6 +497,8 @@ int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_
 }  // namespace cvk
 
 using namespace cvk;
+
+extern "C" int64_t cvk_diag_coarse_words(int64_t n) { return n > 0 ? coarse_words(n) : 0; }
 
 extern "C" int cvk_diag_apply(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                               cvk_work* work, const cvk_diag* diag, const cvk_diag_op* op, int32_t t, void* stream) {

@@ -207,15 +207,17 @@ class Sim:
         w = self._w
         w['diag_pending'] = torch.full((ring, n), _lib.PENDING_NONE, dtype=torch.int32, device=dev)
         w['diag_cases'] = torch.zeros(2 * ((n + 63) // 64), dtype=torch.int32, device=dev)
+        w['diag_coarse'] = torch.zeros(int(_lib.lib().cvk_diag_coarse_words(n)), dtype=torch.int32, device=dev)
         w['diag_contacts'] = torch.zeros(n, dtype=torch.int32, device=dev)
-        w['diag_ctl'] = torch.zeros(_lib.DIAG_CTL_WORDS, dtype=torch.int32, device=dev)
+        w['diag_part'] = torch.zeros(4 * _lib.DIAG_MAX_BLOCKS, dtype=torch.int32, device=dev)
         d = _lib.DiagS()
         for k in _lib.DIAG_DATE_FIELDS:
             setattr(d, k, people._d[k].data_ptr())
         d.pending = w['diag_pending'].data_ptr()
         d.cases = w['diag_cases'].data_ptr()
+        d.coarse = w['diag_coarse'].data_ptr()
         d.contacts = w['diag_contacts'].data_ptr()
-        d.ctl = w['diag_ctl'].data_ptr()
+        d.part = w['diag_part'].data_ptr()
         d.ring = ring
         self._diag = d
 

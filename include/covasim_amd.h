@@ -353,20 +353,29 @@ typedef struct {
   int32_t* pending;    /* [ring][n]: row d % ring holds, per agent, the latest end day of the quarantines
                           scheduled to start on day d (CVK_PENDING_NONE: none); initialise to NONE */
   uint64_t* cases;     /* [(n + 63) / 64] bitmap of the cases a tracing op is tracing (scratch) */
+  uint32_t* coarse;    /* [cvk_diag_coarse_words(n)] bit k: some case among agents [k << c, (k + 1) << c),
+                          c = the smallest shift that fits the map in 8 KiB; initialise to 0 (each
+                          tracing op clears it) */
   uint32_t* contacts;  /* [n] bit l: found in layer l by the running tracing op; initialise to 0 (the op
                           clears what it set) */
-  uint32_t* ctl;       /* [8] case counts of the day's tracing ops; initialise to 0 (cvk_states_post
-                          clears them) */
+  uint32_t* part;      /* [4][CVK_DIAG_MAX_BLOCKS] per-block counts of the day (tests, diagnoses,
+                          quarantined, isolated); initialise to 0 (cvk_states_post sums them into the
+                          day's row and clears them) */
   int32_t ring;        /* > the largest int(trace_time) of every tracing op */
   int32_t pad;
 } cvk_diag;
 
-/* Runs one test_prob / contact_tracing op on day t (new tests go to the day's CVK_R_NEW_TESTS). */
+#define CVK_DIAG_MAX_BLOCKS 4096
+int64_t cvk_diag_coarse_words(int64_t n);
+
+/* Runs one test_prob / contact_tracing op on day t (its tests are counted by the next
+ * cvk_states_post of the day). */
 int cvk_diag_apply(const cvk_people* people, const cvk_layer* layers, const cvk_params* params, cvk_work* work,
                    const cvk_diag* diag, const cvk_diag_op* op, int32_t t, void* stream);
-/* update_states_post on day t: check_diagnosed, check_quar, check_enter_iso (counted in
- * CVK_R_NEW_DIAGNOSES / NEW_QUARANTINED / NEW_ISOLATED), and the edge records of the agents whose
- * quarantine or isolation changed (so it runs after cvk_agent_pass and before cvk_edge_pass). */
+/* update_states_post on day t: check_diagnosed, check_quar, check_enter_iso, and the edge records
+ * of the agents whose quarantine or isolation changed (so it runs after cvk_agent_pass and before
+ * cvk_edge_pass); then the day's counts (the ops' tests and its own diagnoses, quarantines and
+ * isolations) go to CVK_R_NEW_TESTS .. CVK_R_NEW_ISOLATED of row t. */
 int cvk_states_post(const cvk_people* people, const cvk_params* params, cvk_work* work, const cvk_diag* diag,
                     int32_t t, void* stream);
 

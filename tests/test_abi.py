@@ -40,7 +40,7 @@ def test_struct_layout_report():
     assert m == 28
     assert out[0] == C.sizeof(_lib.Dur) == 40
     assert out[21] == C.sizeof(_lib.PopGen) and out[23] == C.sizeof(_lib.PopCounts) == 40
-    assert out[24] == C.sizeof(_lib.DiagOp) and out[26] == C.sizeof(_lib.DiagS) == 80
+    assert out[24] == C.sizeof(_lib.DiagOp) and out[26] == C.sizeof(_lib.DiagS) == 88
     assert L.cvk_abi_layout(out, 3) < 0  # too-small output is an error, not an overflow
 
 

@@ -276,3 +276,39 @@ def test_tracing_with_clip_edges_and_snapshot(tmp_path):
     resumed.run()
     for key in COMPARE_KEYS + DIAG_ROWS + DIAG_STOCKS:
         assert np.array_equal(resumed.results[key].values, ores[key]), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('n', [20_000, 150_000, 1_000_000, 5_000_000])
+def test_trace_scan_finds_every_contact(n):
+    """The contact scan at sizes whose coarse case map has 1 to 128 agents a bit: with trace_probs 1
+    and trace_time 0, the partners of the agents diagnosed today become known contacts and, unless
+    diagnosed themselves, enter quarantine today (a numpy scan of the same edges is the reference)."""
+    import covasim_amd as cv
+    print(f'trace scan n={n}', flush=True)
+    rng = np.random.default_rng(n)
+    E = 3 * n
+    p1 = rng.integers(0, n, E).astype(np.int32)
+    p2 = rng.integers(0, n, E).astype(np.int32)
+    age = rng.uniform(0, 90, n).astype(np.float32)
+    pop = dict(age=age, contacts={'a': dict(p1=p1, p2=p2, beta=np.ones(E, np.float32))})
+    sim = cv.Sim(pop_size=n, pop_type='random', n_days=3, pop_infected=0, rand_seed=1, popdict=pop,
+                 interventions=[cv.contact_tracing(trace_probs=1.0, trace_time=0)])
+    sim.initialize()
+    cases = rng.choice(n, max(1, n // 2000), replace=False)
+    dd = np.full(n, np.nan, np.float32)
+    dd[cases] = 0
+    sim.people.set('date_diagnosed', dd)
+    sim.step()
+    is_case = np.zeros(n, bool)
+    is_case[cases] = True
+    contact = np.zeros(n, bool)
+    contact[p2[is_case[p1]]] = True
+    contact[p1[is_case[p2]]] = True
+    assert np.array_equal(sim.people.known_contact, contact)
+    assert np.array_equal(sim.people.quarantined, contact & ~is_case)
+    assert np.array_equal(sim.people.isolated, is_case) and np.array_equal(sim.people.diagnosed, is_case)
+    from covasim_amd import _lib
+    row = sim._w['results'][0].cpu().numpy()
+    assert row[_lib.R_COL['new_quarantined']] == np.count_nonzero(contact & ~is_case)
+    assert row[_lib.R_COL['new_isolated']] == len(cases)
