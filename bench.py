@@ -369,6 +369,26 @@ def run_popgen(w, args):
     print(json.dumps(line))
 
 
+def device_copy_gbs(mib=1024, reps=5):
+    """Measured device-to-device copy bandwidth (bytes read + written per second, best of reps), for
+    context beside the 8 TB/s peak."""
+    import torch
+    src = torch.ones((mib << 20) // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None else min(best, ms)
+    del src, dst
+    return round(2 * (mib << 20) / (best / 1e3) / 1e9, 1)
+
+
 def run_interventions(w, args):
     """hybrid1m with interventions: a change_beta-only run (one captured graph, per-day rates), a
     run that also clips edges (clip days through the individual passes, graphs between them) and a
@@ -623,6 +643,7 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         out['copy_probe_bytes'] = nbytes
+    out['roofline']['device_copy_gbs'] = device_copy_gbs()
     if not args.no_cpu_baseline:
         out['cpu_baseline'] = cpu_baseline(w, pop, args.seed, args.cpu_budget)
         if args.cpu_omp:

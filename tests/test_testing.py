@@ -6,8 +6,9 @@ symptomatic agent is diagnosed on its symptom day, or test_delay days later, and
 tests with zero sensitivity or certain loss diagnose nobody), by hand-worked quarantine cases of
 check_quar and by a brute-force find_contacts.
 GPU: runs with test_prob and contact_tracing (per-layer trace times, presumptive tracing, the
-quarantine testing policies, clip_edges days between) equal the oracle bit for bit: results rows,
-dates and flags, in a captured graph, eagerly and stepped day by day.
+quarantine testing policies, waning and a second variant, clip_edges days between, a snapshot) equal
+the oracle bit for bit: results rows, dates and flags, in a captured graph, eagerly and stepped day by
+day; the contact scan equals a numpy scan at 20k to 5M agents.
 """
 
 import numpy as np
@@ -212,7 +213,8 @@ CASES = {
         ivs=lambda m: [m.test_prob(symp_prob=0.15, quar_policy=[0, 4], test_delay=1),
                        m.contact_tracing(trace_probs=dict(h=0.9, w=0.4), trace_time=dict(h=1, w=2), start_day=10),
                        m.test_prob(symp_prob=0.05, asymp_prob=0.02, quar_policy='daily', sensitivity=0.8)],
-        pars=dict(n_days=45, beta=0.035, use_waning=False)),
+        pars=dict(n_days=45, beta=0.035, use_waning=True),
+        variants=[dict(rel_beta=1.7, days=[12], n_imports=30)]),
 }
 
 
@@ -238,7 +240,10 @@ def test_testing_tracing_matches_oracle(case, mode):
     spec = CASES[case]
     pop = make_population(n=20000, pop_type='hybrid')
     kw = dict(pop_type='hybrid', dynam_layer=dict(c=1), **spec['pars'])
-    sim = cv.Sim(pop_size=20000, rand_seed=7, popdict=copy_pop(pop), interventions=spec['ivs'](cv), **kw)
+    variants = spec.get('variants', [])
+    gvars = [cv.variant(dict(rel_beta=v['rel_beta']), days=v['days'], n_imports=v['n_imports']) for v in variants]
+    sim = cv.Sim(pop_size=20000, rand_seed=7, popdict=copy_pop(pop), interventions=spec['ivs'](cv),
+                 variants=gvars, **kw)
     if mode == 'step':
         sim.initialize()
         while not sim.complete:
@@ -246,7 +251,7 @@ def test_testing_tracing_matches_oracle(case, mode):
         sim.finalize()
     else:
         sim.run(use_graph=(mode == 'graph'))
-    osim, ores = run_oracle(pop, spec['ivs'](oiv), seed=7, **kw)
+    osim, ores = run_oracle(pop, spec['ivs'](oiv), seed=7, variants=variants, **kw)
     assert ores['new_tests'].sum() > 0 and ores['new_diagnoses'].sum() > 0
     if case != 'presumptive':
         assert ores['new_quarantined'].sum() > 0
