@@ -150,25 +150,21 @@ __device__ inline uint16_t edge_bound(uint32_t f, float rt, float rs, const cvk_
   return (uint16_t)bc;
 }
 
-// Per-agent inputs, all loaded before any is used (one memory round trip).
+// Per-agent inputs, loaded in two round trips: the state words (and the NAb fields), then only the
+// dates and factors the agent's state can use. Those of an agent that is not exposed are NaN / 0
+// except rel_sus (susceptible), date_recovered (ever infected, with waning: check_immunity) and
+// date_end_isolation (isolated): its transitions and viral-load phase then come out as before (none,
+// and not early), and its edge record and bound are those of an agent that neither transmits nor
+// acquires. A line of 32 agents none of whom is exposed is not fetched at all.
 struct AgentIn {
   uint32_t f, vs;
   float d_inf, d_sym, d_sev, d_crit, d_rec, d_dead, d_iso, rt, rs, peak, nab;
   int32_t tne;
 };
 
-__device__ inline void load_agent(const cvk_people& P, int64_t g, bool waning, AgentIn& x) {
+__device__ inline void load_agent_state(const cvk_people& P, int64_t g, bool waning, AgentIn& x) {
   x.f = P.flags[g];
   x.vs = P.vstate[g];
-  x.d_inf = P.date_infectious[g];
-  x.d_sym = P.date_symptomatic[g];
-  x.d_sev = P.date_severe[g];
-  x.d_crit = P.date_critical[g];
-  x.d_rec = P.date_recovered[g];
-  x.d_dead = P.date_dead[g];
-  x.d_iso = P.date_end_isolation[g];
-  x.rt = P.rel_trans[g];
-  x.rs = P.rel_sus[g];
   if (waning) {
     x.peak = P.peak_nab[g];
     x.nab = P.nab[g];
@@ -178,6 +174,20 @@ __device__ inline void load_agent(const cvk_people& P, int64_t g, bool waning, A
     x.nab = 0.0f;
     x.tne = 0;
   }
+}
+
+__device__ inline void load_agent_dates(const cvk_people& P, int64_t g, bool waning, AgentIn& x) {
+  const float fnan = __builtin_nanf("");
+  const bool exp = (x.f & CVK_EXPOSED) != 0;
+  x.d_inf = exp ? P.date_infectious[g] : fnan;
+  x.d_sym = exp ? P.date_symptomatic[g] : fnan;
+  x.d_sev = exp ? P.date_severe[g] : fnan;
+  x.d_crit = exp ? P.date_critical[g] : fnan;
+  x.d_dead = exp ? P.date_dead[g] : fnan;
+  x.d_rec = (exp || (waning && !(x.f & CVK_NAIVE))) ? P.date_recovered[g] : fnan;
+  x.d_iso = (x.f & CVK_ISOLATED) ? P.date_end_isolation[g] : fnan;
+  x.rt = exp ? P.rel_trans[g] : 0.0f;
+  x.rs = (exp || (x.f & CVK_SUSCEPTIBLE)) ? P.rel_sus[g] : 0.0f;
 }
 
 // The agents one block of the agent pass handles: AGENT_BLOCK * APT consecutive agents of one sim.
@@ -226,7 +236,12 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
 #pragma unroll
   for (int r = 0; r < APT; ++r) {
     idx[r] = S.li0 + (int64_t)r * AGENT_BLOCK + threadIdx.x;
-    if (idx[r] < S.n_local) load_agent(P, S.g0 + idx[r], waning, in[r]);
+    if (idx[r] < S.n_local) load_agent_state(P, S.g0 + idx[r], waning, in[r]);
+  }
+  if (transitions) {  // a count-only pass reads the state words alone
+#pragma unroll
+    for (int r = 0; r < APT; ++r)
+      if (idx[r] < S.n_local) load_agent_dates(P, S.g0 + idx[r], waning, in[r]);
   }
 #pragma unroll
   for (int r = 0; r < APT; ++r) {
