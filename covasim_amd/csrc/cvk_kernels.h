@@ -316,13 +316,21 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
           const double nab = (double)in[r].nab;
           const bool was_inf = tf >= d_rec;
           const uint32_t rv = rec_var(vs);
+          double prev_eff = -1.0;  // efficacies of the last distinct effective level (levels are >= 0)
+          float ve_sus = 0.0f, ve_symp = 0.0f, ve_sev = 0.0f;
           for (int v = 0; v < nv; ++v) {
             const double imm = (was_inf && rv < MAXV) ? (double)prm.immunity[v][rv] : 0.0;
             const double eff = nab * imm;
+            if (eff != prev_eff) {  // variants with the same cross-immunity share the three powers
+              ve_sus = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
+              ve_symp = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
+              ve_sev = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
+              prev_eff = eff;
+            }
             const int64_t off = (int64_t)v * S.n_stride + g;
-            P.sus_imm[off] = calc_ve(eff, prm.ve_exp_alpha[0], prm.ve_beta[0]);
-            P.symp_imm[off] = calc_ve(eff, prm.ve_exp_alpha[1], prm.ve_beta[1]);
-            P.sev_imm[off] = calc_ve(eff, prm.ve_exp_alpha[2], prm.ve_beta[2]);
+            P.sus_imm[off] = ve_sus;
+            P.symp_imm[off] = ve_symp;
+            P.sev_imm[off] = ve_sev;
           }
         }
         // viral-load phase (utils.py:58-78): early iff (t - start)/total < switch point
