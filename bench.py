@@ -474,7 +474,8 @@ def main():
     ap.add_argument('--overlap-resample', type=int, default=None,
                     help='1: the next day\'s resample runs on a side branch beside the infect pass; 0: in the agent pass')
     ap.add_argument('--generate-dynamic', type=int, default=None,
-                    help='1: the day loop computes dynamic-layer endpoints in place; 0: writes and reads them daily')
+                    help='1: the day loop computes dynamic-layer endpoints in place; 0: writes and reads them daily; '
+                         '2 (default): 1 from 2^25 dynamic edges up')
     ap.add_argument('--eager-only', action='store_true',
                     help='counter-collection mode: skip the graph replays; value is then the eager loop rate')
     ap.add_argument('--copy-probe', type=int, default=0,
@@ -623,7 +624,12 @@ def main():
         'ms_per_step': wall / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': w['desc'], 'agents': n, 'days': steps, 'edges': int(n_edges),
-                   'dynamic_edges': int(dyn_edges), 'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
+                   'dynamic_edges': int(dyn_edges),
+                   'dynamic_layers': ('generated from the resample stream'
+                                      if tuning.get('generate_dynamic', 2) == 1 or
+                                      (tuning.get('generate_dynamic', 2) == 2 and dyn_edges >= 2 ** 25)
+                                      else 'resampled into the layer arrays daily'),
+                   'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
                    'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {}),
                    **({'mode': 'eager launches (counter collection); value is not the graph-replay rate'}
                       if args.eager_only else {})},

@@ -231,8 +231,9 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 #define CVK_TUNE_EDGE_CHUNKS 2
 /* CVK_TUNE_GENERATE_DYNAMIC: 1 = inside cvk_run_days / graphs, the edge and infect passes compute
  * each day's dynamic-layer endpoints from the resample stream, and the layer arrays are written
- * once, with the last day's contacts, when the run closes its last day; 0 (default) = every day's
- * resample is written to the layer and read back. The results are identical. */
+ * once, with the last day's contacts, when the run closes its last day; 0 = every day's resample is
+ * written to the layer and read back; 2 (default) = 1 from 2^25 dynamic edges up, else 0 (never with
+ * testing and tracing, which read each day's contacts). The results are identical. */
 #define CVK_TUNE_GENERATE_DYNAMIC 4
 /* CVK_TUNE_OVERLAP_RESAMPLE: 1 (default) = inside cvk_run_days / graphs, day t+1's resample of the
  * dynamic layers runs as extra workgroups of day t's infect pass, and that pass takes dynamic-layer

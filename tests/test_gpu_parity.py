@@ -214,7 +214,7 @@ def test_full_sim_matches_oracle(gpu, name):
     pars['pop_type'] = cfg['pop']['pop_type']
     variants = pars.pop('variants', [])
     keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 2),
-                generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 0),
+                generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 2),
                 overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1),
                 edge_prefetch=(_lib.TUNE_EDGE_PREFETCH, 0))
     for k, v in cfg.get('tuning', {}).items():
