@@ -13,7 +13,8 @@ edge indices, so every sim's results equal those of running it alone (``Sim.run`
 After ``run()`` each sim's ``results`` are finalised exactly as ``Sim.finalize`` does.
 
 ``autotune()`` is the launch-shape autotuner. It times a few days of the batch for each
-candidate shape, restoring the initial state between trials, and keeps the fastest.
+candidate shape, from the initial state and from the state a third of the way through the
+run (restored between trials), and keeps the fastest.
 The shapes are the edge-pass workgroup size, the tile depth, the workgroups per CU and
 the agents per thread in the agent pass.
 """
@@ -297,24 +298,36 @@ class BatchSim:
 
     # -- autotuner --
     def autotune(self, days=8, repeats=1, verbose=False):
-        """Pick the fastest launch shape by timing `days` days per candidate from the initial state."""
+        """Pick the fastest launch shape by timing, per candidate, `days` days from the initial state
+        (quiet days) plus `days` days from the state a third of the way through the run (busier days)."""
         import torch
         snap = self.snapshot_state()
         st = torch.cuda.Stream()
         trials = {}
+        lib = _lib.lib()
+        mid = min(max(self.npts // 3, days), max(self.npts - days, 0))
+
+        def run(t0, t1):
+            _lib.check(lib.cvk_batch_run_days(self._handle, t0, t1, self.pop_infected.ctypes.data,
+                                              C.byref(self._imports), 0, C.c_void_p(st.cuda_stream)), 'autotune')
+
+        run(0, mid)  # the busier state, with the shape in place now
+        st.synchronize()
+        snap_mid = self.snapshot_state() if mid > 0 else None
+
+        def timed(state, t0):
+            self.restore_state(state)
+            torch.cuda.synchronize()
+            c0 = time.perf_counter()
+            run(t0, min(t0 + days, self.npts))
+            st.synchronize()
+            return time.perf_counter() - c0
 
         def trial(shape):
             self.set_config(*shape)
             best = None
             for _ in range(repeats):
-                self.restore_state(snap)
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                _lib.check(_lib.lib().cvk_batch_run_days(self._handle, 0, min(days, self.npts),
-                                                         self.pop_infected.ctypes.data, C.byref(self._imports), 0,
-                                                         C.c_void_p(st.cuda_stream)), 'autotune')
-                st.synchronize()
-                el = time.perf_counter() - t0
+                el = timed(snap, 0) + (timed(snap_mid, mid) if snap_mid is not None else 0.0)
                 best = el if best is None else min(best, el)
             trials[shape] = best
             if verbose:
@@ -327,8 +340,8 @@ class BatchSim:
         best = edge + (apt,)
         self.set_config(*best)
         self.restore_state(snap)
-        del snap
-        self.tuning = dict(best=best, days=days, trials_ms={'x'.join(map(str, k)): round(v * 1e3, 3)
+        del snap, snap_mid
+        self.tuning = dict(best=best, days=days, from_days=[0, mid], trials_ms={'x'.join(map(str, k)): round(v * 1e3, 3)
                                                               for k, v in trials.items()})
         return best
 

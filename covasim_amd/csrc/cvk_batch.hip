@@ -106,7 +106,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
   S.partials_f = C.partials_f;
   S.part_stride = C.part_stride;
   S.part_col = blockIdx.x;
-  agent_block<APT>(C.P, C.params[s], S, a.t, a.prologue, a.transitions);
+  agent_block<APT, false>(C.P, C.params[s], S, a.t, a.prologue, a.transitions);
 }
 
 struct BatchReduce {

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   S.partials_f = W.partials_f;
   S.part_stride = CVK_MAX_REDUCE_BLOCKS;
   S.part_col = blockIdx.x;
-  agent_block<AGENTS_PER_THREAD>(P, prm, S, a.t, a.prologue, a.transitions);
+  agent_block<AGENTS_PER_THREAD, true>(P, prm, S, a.t, a.prologue, a.transitions);
 }
 
 // Writes the day rows from the summed partial rows of day_begin (block-wide).

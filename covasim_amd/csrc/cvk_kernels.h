@@ -176,6 +176,19 @@ __device__ inline void load_agent_state(const cvk_people& P, int64_t g, bool wan
   }
 }
 
+// Every date and factor, independent of the state words (issued with them: one round trip).
+__device__ inline void load_agent_all(const cvk_people& P, int64_t g, AgentIn& x) {
+  x.d_inf = P.date_infectious[g];
+  x.d_sym = P.date_symptomatic[g];
+  x.d_sev = P.date_severe[g];
+  x.d_crit = P.date_critical[g];
+  x.d_rec = P.date_recovered[g];
+  x.d_dead = P.date_dead[g];
+  x.d_iso = P.date_end_isolation[g];
+  x.rt = P.rel_trans[g];
+  x.rs = P.rel_sus[g];
+}
+
 __device__ inline void load_agent_dates(const cvk_people& P, int64_t g, bool waning, AgentIn& x) {
   const float fnan = __builtin_nanf("");
   const bool exp = (x.f & CVK_EXPOSED) != 0;
@@ -210,7 +223,9 @@ struct AgentSpan {
 // update_states_pre (people.py:164-312, 368-374) + check_immunity (immunity.py:303-350) + the
 // viral-load phase (utils.py:58-78) for day t, preceded (closing != 0) by update_nab and the
 // stock counts of day tc = transitions ? t - 1 : t (sim.py:651-674). Block-wide.
-template <int APT>
+// LAZY: load the dates and factors in a second round trip, only where the state uses them (fewer
+// bytes on quiet days); otherwise every field is loaded in one round trip.
+template <int APT, bool LAZY>
 __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const AgentSpan& S, int32_t t,
                             int32_t prologue, int32_t transitions) {
   // per-wave counters: sums of per-agent booleans via ballot (wave-uniform, no shuffles)
@@ -236,9 +251,12 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
 #pragma unroll
   for (int r = 0; r < APT; ++r) {
     idx[r] = S.li0 + (int64_t)r * AGENT_BLOCK + threadIdx.x;
-    if (idx[r] < S.n_local) load_agent_state(P, S.g0 + idx[r], waning, in[r]);
+    if (idx[r] < S.n_local) {
+      load_agent_state(P, S.g0 + idx[r], waning, in[r]);
+      if (!LAZY && transitions) load_agent_all(P, S.g0 + idx[r], in[r]);
+    }
   }
-  if (transitions) {  // a count-only pass reads the state words alone
+  if (LAZY && transitions) {  // a count-only pass reads the state words alone
 #pragma unroll
     for (int r = 0; r < APT; ++r)
       if (idx[r] < S.n_local) load_agent_dates(P, S.g0 + idx[r], waning, in[r]);
