@@ -1,19 +1,22 @@
 // Testing, contact tracing, quarantine and isolation on the device (gfx950, wave64).
 //
-//   test_kernel          test_prob.apply (interventions.py:919-962) + People.test (people.py:589-617):
+//   test_kernel          test_prob.apply (interventions.py:921-962) + People.test (people.py:589-617):
 //                        one pass over the agents; the test, true-positive and loss draws of agent i
 //                        are Philox words keyed by (i, t, TEST | slot)
-//   trace_cases_kernel   contact_tracing.select_cases (interventions.py:1047-1066): bitmap of the
+//   trace_cases_kernel   contact_tracing.select_cases (interventions.py:1069-1086): bitmap of the
 //                        agents diagnosed today (or, presumptive, exposed and tested today)
-//   trace_edges_kernel   Layer.find_contacts (utils.py:131-147) for every traced layer at once: each
+//   trace_edges_kernel   Layer.find_contacts (base.py:1808, utils.py:131-147) for every traced layer at once: each
 //                        edge with a case at one end marks the other end's layer bit (atomicOr;
 //                        only the few edges at a case write)
 //   trace_notify_kernel  binomial_filter per (contact, layer) and notify_contacts
-//                        (interventions.py:1101-1118): known_contact, date_known_contact and the
+//                        (interventions.py:1125-1145): known_contact, date_known_contact and the
 //                        quarantine scheduled to start on day int(t + trace_time)
 //   states_post_kernel   update_states_post (people.py:189-196): check_diagnosed, check_quar,
 //                        check_enter_iso, then the edge records of agents whose quarantine or
 //                        isolation changed (the transmission step of the same day reads them)
+//   diag_reduce_kernel   the day's per-block counts (tests, diagnoses, quarantined, isolated) into the
+//                        day's result row: every agent pass writes its block's counts to its own
+//                        entries, so no result word is the target of one atomic per block
 //
 // Scheduled quarantines (People._pending_quarantine, people.py:116, 620-641) are a ring of per-agent
 // end days: row d % ring holds the largest end day among the quarantines scheduled to start on day d.

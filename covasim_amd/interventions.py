@@ -255,7 +255,7 @@ class test_prob(Intervention):
 class contact_tracing(Intervention):
     """Trace the contacts of the agents diagnosed today (presumptive: exposed and tested today) in
     each layer with probability trace_probs[layer]; the living ones become known contacts and are
-    scheduled to quarantine trace_time[layer] days later (interventions.py:965-1118). Runs on the
+    scheduled to quarantine trace_time[layer] days later (interventions.py:965-1145). Runs on the
     device (cvk_diag_apply); capacity is not supported."""
 
     def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,

@@ -27,7 +27,7 @@
  *   cvk_reduce         covasim/run.py:220-303  MultiSim.reduce (median / quantiles or mean / std)
  *   cvk_move_edges     covasim/interventions.py:589-673 clip_edges (Layer.pop_inds / append, base.py:1739-1768)
  *   cvk_diag_apply     covasim/interventions.py:858-962 test_prob (People.test, people.py:589-617) and
- *                      :965-1118 contact_tracing (Layer.find_contacts, utils.py:131-147;
+ *                      :965-1145 contact_tracing (Layer.find_contacts, utils.py:131-147;
  *                      People.schedule_quarantine, people.py:620-641)
  *   cvk_states_post    covasim/people.py:189-196 update_states_post (check_diagnosed :315-332,
  *                      check_quar :335-358, check_enter_iso :361-365)
@@ -308,7 +308,7 @@ int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work*
 
 /* ---- testing, contact tracing, quarantine and isolation ----
  * One cvk_diag_op is one test_prob (interventions.py:858-962) or contact_tracing
- * (interventions.py:965-1118) intervention; cvk_diag_apply runs it on day t (nothing outside its
+ * (interventions.py:965-1145) intervention; cvk_diag_apply runs it on day t (nothing outside its
  * days), after the day's imports and before transmission, in the sim's intervention order;
  * cvk_states_post then runs update_states_post (people.py:189-196) once per day.
  * Draws (Philox4x32-10 keyed by the seed): the test of agent i on day t uses

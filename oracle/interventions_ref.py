@@ -189,7 +189,7 @@ def find_contacts(p1, p2, inds):
 
 
 class contact_tracing:
-    """interventions.py:965-1118 (without capacity): the agents diagnosed today (or, presumptive,
+    """interventions.py:965-1145 (without capacity): the agents diagnosed today (or, presumptive,
     exposed and tested today) are traced through each layer with trace_prob > 0; each contact found
     in layer l is kept when philox(i, t, 0xA0000000 | slot << 8 | l, 0) < trace_prob[l], and the
     living ones are notified and scheduled to quarantine on day int(t + trace_time[l])."""
@@ -242,7 +242,7 @@ class contact_tracing:
                                           px.STREAM_TRACE | (sim.iv_slot << 8) | keys.index(lkey), 0)
                 contacts.setdefault(self.trace_time[lkey], []).extend(traceable[u < prob].tolist())
         is_dead = people.dead.nonzero()[0]
-        for trace_time, inds in contacts.items():  # notify_contacts, interventions.py:1101-1118
+        for trace_time, inds in contacts.items():  # notify_contacts, interventions.py:1125-1145
             inds = np.setdiff1d(np.array(inds, dtype=np.int64), is_dead)
             people.known_contact[inds] = True
             people.date_known_contact[inds] = np.fmin(people.date_known_contact[inds], t + trace_time)

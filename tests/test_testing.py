@@ -1,4 +1,4 @@
-"""Testing, contact tracing, quarantine and isolation (covasim/interventions.py:682-1118,
+"""Testing, contact tracing, quarantine and isolation (covasim/interventions.py:682-1145,
 people.py:189-196, 315-375, 589-641).
 
 CPU: the oracle's restatement is pinned by closed forms of whole runs (with certain tests every
