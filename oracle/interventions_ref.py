@@ -99,7 +99,7 @@ class clip_edges:
 
 
 # ---------------------------------------------------------------------------------------------
-# Testing and contact tracing (interventions.py:682-710, 858-1118)
+# Testing and contact tracing (interventions.py:682-710, 858-1145)
 # ---------------------------------------------------------------------------------------------
 def _active(iv, t):
     return t >= iv.start_day and (iv.end_day is None or t <= iv.end_day)
