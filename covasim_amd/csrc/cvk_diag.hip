@@ -229,46 +229,61 @@ __global__ __launch_bounds__(TRACE_EDGE_BLOCK) void trace_edges_kernel(TraceLaye
   }
 }
 
-__global__ __launch_bounds__(DIAG_BLOCK) void trace_notify_kernel(cvk_people P, cvk_diag D, cvk_diag_op op,
-                                                                  uint64_t seed, int32_t t, int32_t n_cwords) {
-  // the coarse map has been read: clear it for the next tracing op
+// notify_contacts for agent i (a contact of today's cases in the layers of its contact mask).
+__device__ inline void notify_agent(const cvk_people& P, const cvk_diag& D, const cvk_diag_op& op, uint64_t seed,
+                                    int32_t t, int64_t i) {
+  const uint32_t m = D.contacts[i];
+  if (!m) return;
+  D.contacts[i] = 0;
+  const uint32_t f = P.flags[i];
+  if (f & CVK_DEAD) return;  // np.setdiff1d(contact_inds, is_dead)
+  float dkc = D.date_known_contact[i];
+  bool any = false;
+  for (uint32_t mm = m; mm; mm &= mm - 1) {
+    const int l = __ffs(mm) - 1;
+    const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t,
+                                  STREAM_TRACE | ((uint32_t)op.slot << 8) | (uint32_t)l, 0);
+    if (!(u01(w.x, w.y) < op.trace_prob[l])) continue;  // binomial_filter
+    any = true;
+    const double s = (double)t + op.trace_time[l];
+    dkc = fminf(dkc, (float)s);  // np.fmin: a NaN date gives way
+    const int32_t start = (int32_t)s;
+    const int32_t end = start + op.trace_period[l];
+    int32_t* pend = D.pending + (int64_t)(start % D.ring) * P.n + i;
+    if (end > *pend) *pend = end;
+  }
+  if (any) {
+    P.flags[i] = f | CVK_KNOWN_CONTACT;
+    D.date_known_contact[i] = dkc;
+  }
+}
+
+// The coarse map has been read by the scan: clear it for the next tracing op.
+__device__ inline void clear_coarse(const cvk_diag& D, int32_t n_cwords) {
   for (int64_t k = (int64_t)blockIdx.x * DIAG_BLOCK + threadIdx.x; k < n_cwords; k += (int64_t)gridDim.x * DIAG_BLOCK)
     D.coarse[k] = 0u;
+}
+
+__global__ __launch_bounds__(DIAG_BLOCK) void trace_notify_kernel(cvk_people P, cvk_diag D, cvk_diag_op op,
+                                                                  uint64_t seed, int32_t t, int32_t n_cwords) {
+  clear_coarse(D, n_cwords);
   for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
     for (int r = 0; r < DIAG_APT; ++r) {
       const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
-      if (i >= P.n) continue;
-      const uint32_t m = D.contacts[i];
-      if (!m) continue;
-      D.contacts[i] = 0;
-      const uint32_t f = P.flags[i];
-      if (f & CVK_DEAD) continue;  // np.setdiff1d(contact_inds, is_dead)
-      float dkc = D.date_known_contact[i];
-      bool any = false;
-      for (uint32_t mm = m; mm; mm &= mm - 1) {
-        const int l = __ffs(mm) - 1;
-        const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t,
-                                      STREAM_TRACE | ((uint32_t)op.slot << 8) | (uint32_t)l, 0);
-        if (!(u01(w.x, w.y) < op.trace_prob[l])) continue;  // binomial_filter
-        any = true;
-        const double s = (double)t + op.trace_time[l];
-        dkc = fminf(dkc, (float)s);  // np.fmin: a NaN date gives way
-        const int32_t start = (int32_t)s;
-        const int32_t end = start + op.trace_period[l];
-        int32_t* pend = D.pending + (int64_t)(start % D.ring) * P.n + i;
-        if (end > *pend) *pend = end;
-      }
-      if (any) {
-        P.flags[i] = f | CVK_KNOWN_CONTACT;
-        D.date_known_contact[i] = dkc;
-      }
+      if (i < P.n) notify_agent(P, D, op, seed, t, i);
     }
   }
 }
 
+// NOTIFY: the day's last op is a tracing op whose notify pass is done here, agent by agent, before
+// the agent's update_states_post (each agent's steps touch only its own state, so the order between
+// agents does not matter; the quarantine it schedules for today is read back by the same thread).
+template <bool NOTIFY>
 __global__ __launch_bounds__(DIAG_BLOCK) void states_post_kernel(cvk_people P, cvk_params prm, cvk_work W,
-                                                                 cvk_diag D, int32_t t) {
+                                                                 cvk_diag D, int32_t t, cvk_diag_op op,
+                                                                 int32_t n_cwords) {
+  if (NOTIFY) clear_coarse(D, n_cwords);
   const float tf = (float)t;
   float quar_max = 0.0f, iso_max = 0.0f;
   for (int l = 0; l < prm.n_layers; ++l) {
@@ -283,6 +298,7 @@ __global__ __launch_bounds__(DIAG_BLOCK) void states_post_kernel(cvk_people P, c
       const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
       bool new_diag = false, new_quar = false, new_iso = false;
       if (i < P.n) {
+        if (NOTIFY) notify_agent(P, D, op, prm.seed, t, i);
         const uint32_t f0 = P.flags[i];
         uint32_t f = f0;
         const float dd = P.date_diagnosed[i];
@@ -425,8 +441,9 @@ int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, co
   return CVK_OK;
 }
 
+// defer_notify: leave the tracing op's notify pass to the day's update_states_post launch.
 int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W, const cvk_diag* D,
-               const cvk_diag_op* op, int32_t t, hipStream_t st) {
+               const cvk_diag_op* op, int32_t t, hipStream_t st, bool defer_notify = false) {
   if (t < op->start_day || (op->end_day >= 0 && t > op->end_day)) return CVK_OK;
   const int grid = agent_grid(P->n);
   if (op->kind == CVK_OP_TEST) {
@@ -456,14 +473,27 @@ int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* p
   trace_edges_kernel<<<(unsigned)std::min<int64_t>(gx, TRACE_EDGE_GRID), TRACE_EDGE_BLOCK, (size_t)n_cwords * 4,
                        st>>>(L, D->cases, D->coarse, shift, (int32_t)n_cwords, D->contacts);
   rc = check_launch("trace_edges_kernel");
-  if (rc) return rc;
+  if (rc || defer_notify) return rc;
   trace_notify_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t, (int32_t)n_cwords);
   return check_launch("trace_notify_kernel");
 }
 
-int post_impl(const cvk_people* P, const cvk_params* prm, cvk_work* W, const cvk_diag* D, int32_t t, hipStream_t st) {
+// Whether op runs its contact scan on day t (active, with a traced layer that has edges).
+bool traces_on(const cvk_diag_op& op, const cvk_layer* layers, const cvk_params* prm, int32_t t) {
+  if (op.kind != CVK_OP_TRACE || t < op.start_day || (op.end_day >= 0 && t > op.end_day)) return false;
+  for (int l = 0; l < prm->n_layers; ++l)
+    if (op.trace_prob[l] > 0.0 && layers[l].n_edges > 0) return true;
+  return false;
+}
+
+// notify: a tracing op whose notify pass this launch does first (its scan has run), or null.
+int post_impl(const cvk_people* P, const cvk_params* prm, cvk_work* W, const cvk_diag* D, int32_t t, hipStream_t st,
+              const cvk_diag_op* notify = nullptr) {
   const int grid = agent_grid(P->n);
-  states_post_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t);
+  if (notify)
+    states_post_kernel<true><<<grid, DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t, *notify, (int32_t)coarse_words(P->n));
+  else
+    states_post_kernel<false><<<grid, DIAG_BLOCK, 0, st>>>(*P, *prm, *W, *D, t, cvk_diag_op{}, 0);
   const int rc = check_launch("states_post_kernel");
   if (rc) return rc;
   diag_reduce_kernel<<<1, DIAG_REDUCE_BLOCK, 0, st>>>(D->part, grid, W->results + (int64_t)t * CVK_R_COLS);
@@ -476,11 +506,15 @@ int post_impl(const cvk_people* P, const cvk_params* prm, cvk_work* W, const cvk
 int diag_day(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, cvk_work* W,
              const cvk_imports* imports, int32_t t, hipStream_t st) {
   if (!imports || !imports->diag) return CVK_OK;
-  for (int j = 0; j < imports->n_diag_ops; ++j) {
-    const int rc = apply_impl(P, layers, prm, W, imports->diag, imports->diag_ops + j, t, st);
+  // the day's last op, when it traces, has its notify pass fused into update_states_post
+  const int n = imports->n_diag_ops;
+  const cvk_diag_op* last = n > 0 ? imports->diag_ops + (n - 1) : nullptr;
+  const bool fuse = last && traces_on(*last, layers, prm, t);
+  for (int j = 0; j < n; ++j) {
+    const int rc = apply_impl(P, layers, prm, W, imports->diag, imports->diag_ops + j, t, st, fuse && j == n - 1);
     if (rc) return rc;
   }
-  return post_impl(P, prm, W, imports->diag, t, st);
+  return post_impl(P, prm, W, imports->diag, t, st, fuse ? last : nullptr);
 }
 
 int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_params* prm, const cvk_work* W,
