@@ -76,6 +76,17 @@ def test_diag_argument_errors():
     assert b'people arrays' in L.cvk_last_error()
 
 
+def test_diag_coarse_map_sizes():
+    """The coarse case map is the smallest power-of-two grouping that fits 2048 words (8 KiB of LDS)."""
+    from covasim_amd import _lib
+    L = _lib.lib()
+    for n in [1, 63, 64, 20_000, 65_536, 65_537, 150_000, 1_000_000, 5_000_000, 10_000_000, 2**31 - 1]:
+        words = L.cvk_diag_coarse_words(n)
+        shift = next(s for s in range(32) if -(-n // (1 << s)) <= 2048 * 32)
+        assert words == -(-(-(-n // (1 << shift))) // 32) and 1 <= words <= 2048, n
+    assert L.cvk_diag_coarse_words(0) == 0
+
+
 def test_population_argument_errors():
     """The cvk_pop_* entry points reject bad tables, sizes and workspaces before any device work."""
     from covasim_amd import _lib
