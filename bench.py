@@ -69,7 +69,7 @@ def measured_traffic(workload, tuning):
 
     The summary holds FETCH_SIZE and WRITE_SIZE means over the edge-pass dispatches of this workload,
     already scaled by what the same counters read for a device copy of known size; it counts only
-    if it was taken with the same tuning and library version (cvk_version) as this run.
+    if it was taken with the same tuning and on a library built from the same sources (cvk_source_hash).
     """
     path = os.path.join(ROOT, 'profiles', f'edge_pass_traffic_{workload}.txt')
     if not os.path.exists(path):
@@ -82,7 +82,7 @@ def measured_traffic(workload, tuning):
     if vals.get('tuning', 'default') != (json.dumps(tuning, sort_keys=True) if tuning else 'default'):
         return None
     from covasim_amd import _lib
-    if vals.get('library_version') != str(_lib.lib().cvk_version()):  # counted on another kernel build
+    if vals.get('source_hash') != f'{_lib.lib().cvk_source_hash():016x}':  # counted on another kernel build
         return None
     try:
         return float(vals['fetch_bytes_per_launch']) + float(vals['write_bytes_per_launch'])

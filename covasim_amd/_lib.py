@@ -264,6 +264,7 @@ def lib():
     vp, i32, i64, u32, u64, f32 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float
     L.cvk_last_error.restype = C.c_char_p
     L.cvk_version.restype = C.c_int
+    L.cvk_source_hash.restype = C.c_uint64
     L.cvk_abi_layout.argtypes = [C.POINTER(C.c_int64), i32]
     L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cvk_set_tuning.argtypes = [i32, i32]
@@ -281,6 +282,7 @@ def lib():
     L.cvk_infect_pass.argtypes = [P, Ly, Pa, W, i32, i32, vp]
     L.cvk_count_pass.argtypes = [P, Pa, W, i32, vp]
     L.cvk_run_days.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), i32, vp]
+    L.cvk_day_plan.argtypes = [Ly, Pa, C.POINTER(ImportsS), i32, C.POINTER(C.c_int32)]
     L.cvk_graph_create.argtypes = [P, Ly, Pa, W, i32, i32, i64, C.POINTER(ImportsS), vp, C.POINTER(C.c_void_p)]
     L.cvk_graph_launch.argtypes = [vp, vp]
     L.cvk_graph_destroy.argtypes = [vp]

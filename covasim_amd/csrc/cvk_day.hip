@@ -596,7 +596,11 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
 }
-extern "C" int cvk_version(void) { return 5; }
+extern "C" int cvk_version(void) { return CVK_ABI_VERSION; }
+#ifndef CVK_SOURCE_HASH
+#define CVK_SOURCE_HASH 0ull
+#endif
+extern "C" uint64_t cvk_source_hash(void) { return CVK_SOURCE_HASH; }
 
 // Sizes and selected field offsets of the ABI structs, so bindings can verify their mirrors.
 extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
@@ -885,21 +889,39 @@ static bool has_dynamic(const cvk_layer* layers, const cvk_params* params) {
   return false;
 }
 
-static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
-                         int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, hipStream_t st) {
-  int rc;
-  // With generated dynamic layers, no day's resample is written until the last day is closed
-  // (not with contact tracing, which reads each day's contacts).
+// How a run of days [t0, t1) treats dynamic layers. gen: no day's resample is written until the last
+// day is closed; the edge and infect passes compute the endpoints from the resample stream (never
+// with contact tracing, which reads each day's contacts). overlap: day t+1's resample runs as extra
+// workgroups of day t's infect pass (store-bound work beside latency-bound work), which then takes
+// dynamic-layer sources from the resample stream.
+static void day_plan(const cvk_layer* layers, const cvk_params* params, const cvk_imports* imports, int32_t n_days,
+                     bool* gen, bool* overlap) {
   const bool diag = imports && imports->diag;
   int64_t dyn_edges = 0;
   for (int l = 0; l < params->n_layers; ++l) dyn_edges += layers[l].dynamic ? layers[l].n_edges : 0;
   const bool gen_on = !diag && (g_gen_dynamic == 1 || (g_gen_dynamic == 2 && dyn_edges >= GEN_AUTO_MIN_EDGES));
-  bool gen = false;
-  for (int l = 0; l < params->n_layers; ++l) gen = gen || (gen_on && layers[l].dynamic && layers[l].n_edges > 0);
+  *gen = gen_on && has_dynamic(layers, params);
+  *overlap = !*gen && g_overlap_resample && n_days > 1 && has_dynamic(layers, params);
+}
+
+extern "C" int cvk_day_plan(const cvk_layer* layers, const cvk_params* params, const cvk_imports* imports,
+                            int32_t n_days, int32_t* out) {
+  if (!params || !out || (params->n_layers > 0 && !layers)) return fail(CVK_E_INVALID, "cvk_day_plan: null argument");
+  if (params->n_layers < 0 || params->n_layers > CVK_MAX_LAYERS) return fail(CVK_E_INVALID, "cvk_day_plan: n_layers");
+  bool gen, overlap;
+  day_plan(layers, params, imports, n_days, &gen, &overlap);
+  out[0] = gen ? 1 : 0;
+  out[1] = overlap ? 1 : 0;
+  return CVK_OK;
+}
+
+static int run_days_impl(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,
+                         int32_t t0, int32_t t1, int64_t pop_infected, const cvk_imports* imports, hipStream_t st) {
+  int rc;
+  const bool diag = imports && imports->diag;
+  bool gen, overlap;
+  day_plan(layers, params, imports, t1 - t0, &gen, &overlap);
   uint32_t infect_blocks = 0;  // infect partial rows not yet added to their day's row
-  // Day t+1's resample runs as extra workgroups of day t's infect pass (store-bound work beside
-  // latency-bound work); the infect pass then takes dynamic-layer sources from the resample stream.
-  const bool overlap = !gen && g_overlap_resample && t1 - t0 > 1 && has_dynamic(layers, params);
   const bool sched = imports && imports->n_beta_days > 0 && imports->beta_v_days && imports->beta_layer_days;
   cvk_params day_params = *params;
   for (int32_t t = t0; t < t1; ++t) {

@@ -161,7 +161,9 @@ __global__ __launch_bounds__(DIAG_BLOCK) void trace_cases_kernel(cvk_people P, c
             const int64_t k0 = (word << 6) >> shift;  // a multiple of 64 / g
             if (g == 1) {
               atomicOr(D.coarse + (k0 >> 5), (uint32_t)cb);
-              atomicOr(D.coarse + (k0 >> 5) + 1, (uint32_t)(cb >> 32));
+              // the upper half exists only where the map has a word for it (the last bitmap word of
+              // an n with n % 64 in 1..32 has none)
+              if ((cb >> 32) != 0) atomicOr(D.coarse + (k0 >> 5) + 1, (uint32_t)(cb >> 32));
             } else {
               atomicOr(D.coarse + (k0 >> 5), (uint32_t)cb << (k0 & 31));
             }

@@ -183,7 +183,8 @@ __global__ __launch_bounds__(FIN_BLOCK) void finalize_kernel(FinArgs a) {
         if (k < 4) {
           vrow(CVK_FV_NEW + k, v)[t] = x;
           acc += x;
-          vrow(CVK_FV_CUM + k, v)[t] = (k == 0 && v == 0) ? acc + pi : acc;
+          // sim.py:785-786 adds pop_infected to the whole (n_variants, npts) cumulative array
+          vrow(CVK_FV_CUM + k, v)[t] = k == 0 ? acc + pi : acc;
         } else {
           vrow(k == 4 ? CVK_FV_N_EXPOSED : CVK_FV_N_INFECTIOUS, v)[t] = x;
         }

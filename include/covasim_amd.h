@@ -220,7 +220,13 @@ typedef struct {
 
 /* ---- library ---- */
 const char* cvk_last_error(void);
+/* ABI version: changes whenever a public struct, an entry point's signature or a tunable's meaning
+ * changes. Bindings compare it with the value they were written for. */
+#define CVK_ABI_VERSION 6
 int cvk_version(void);
+/* No reference counterpart: a 64-bit hash of the HIP and header sources the library was built from
+ * (set by the build script), so measurements taken on one kernel build can be told from another's. */
+uint64_t cvk_source_hash(void);
 /* No reference counterpart: ABI self-description for bindings. Writes sizeof/offsetof values of
  * the structs above into out[0..m) and returns m (see cvk_day.hip for the order). */
 int cvk_abi_layout(int64_t* out, int32_t n);
@@ -401,6 +407,12 @@ typedef struct {
   const cvk_diag* diag;
 } cvk_imports;
 
+/* No reference counterpart: how cvk_run_days / cvk_graph_create would run n_days days of these layers
+ * under the current tunables. out[0] = 1 if dynamic-layer endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC),
+ * out[1] = 1 if each next day's resample runs beside the infect pass (CVK_TUNE_OVERLAP_RESAMPLE).
+ * Host-only; reads no device memory. */
+int cvk_day_plan(const cvk_layer* layers, const cvk_params* params, const cvk_imports* imports, int32_t n_days,
+                 int32_t* out);
 /* Runs days [t0, t1) (t0 == 0 also seeds pop_infected infections first). use_graph != 0
  * captures the launches into one hipGraph and launches it once. */
 int cvk_run_days(const cvk_people* people, cvk_layer* layers, const cvk_params* params, cvk_work* work,

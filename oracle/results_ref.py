@@ -85,7 +85,7 @@ def finalize(counts, counts_f, col, byvar_base, max_variants, n_variants, pop_si
         V[name] = counts[:, base:base + n_variants].T.copy()
     for f in VARIANT_FLOWS:
         V[f'cum_{f}'] = np.cumsum(V[f'new_{f}'], axis=1)
-    V['cum_infections_by_variant'][0] += pop_infected
+    V['cum_infections_by_variant'] += pop_infected  # every variant row, sim.py:785-786 (base.py:146-147)
     count_recov = 1 - int(bool(use_waning))  # sim.py:836-850
     with np.errstate(divide='ignore', invalid='ignore'):
         R['n_alive'] = pop_size - R['cum_deaths']

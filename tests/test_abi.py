@@ -42,6 +42,11 @@ def test_struct_layout_report():
     assert out[21] == C.sizeof(_lib.PopGen) and out[23] == C.sizeof(_lib.PopCounts) == 40
     assert out[24] == C.sizeof(_lib.DiagOp) and out[26] == C.sizeof(_lib.DiagS) == 88
     assert L.cvk_abi_layout(out, 3) < 0  # too-small output is an error, not an overflow
+    # the struct sizes above belong to one ABI version: the header's macro and the library agree
+    hdr = open(os.path.join(ROOT, 'include', 'covasim_amd.h')).read()
+    version = int(re.search(r'#define CVK_ABI_VERSION (\d+)', hdr).group(1))
+    assert L.cvk_version() == version == 6
+    assert L.cvk_source_hash() != 0  # set by the build script
 
 
 def test_host_side_argument_errors():
@@ -113,3 +118,41 @@ def test_population_argument_errors():
                                    None) == _lib.E_INVALID  # slot above 255
     g.nb_p0 = 0.0
     assert L.cvk_pop_people(C.byref(g), 10, C.c_void_p(256), *([C.c_void_p(256)] * 7), None, 0, None) == _lib.E_INVALID
+
+
+def test_day_plan_generate_dynamic_switch():
+    """The default generate-dynamic mode (2) generates dynamic-layer endpoints from 2^25 dynamic
+    edges up and writes them below; 0 and 1 force either way; testing and tracing always write."""
+    from covasim_amd import _lib
+    L = _lib.lib()
+    prm = _lib.Params()
+    prm.n_layers = 2
+    layers = (_lib.LayerS * 2)()
+    layers[0].n_edges, layers[0].dynamic = 5_000_000, 0
+    layers[1].dynamic = 1
+    out = (C.c_int32 * 2)()
+
+    def plan(n_dyn, days=10, imports=None):
+        layers[1].n_edges = n_dyn
+        _lib.check(L.cvk_day_plan(layers, C.byref(prm), imports, days, out), 'plan')
+        return tuple(out)
+    try:
+        assert plan(2 ** 25 - 1) == (0, 1)  # written, with the next day's resample beside the infect pass
+        assert plan(2 ** 25) == (1, 0)      # generated
+        assert plan(2 ** 25, days=1) == (1, 0)
+        assert plan(2 ** 25 - 1, days=1) == (0, 0)  # a single day has no next day to resample
+        assert plan(0) == (0, 0)             # no dynamic edges at all
+        diag = _lib.DiagS()
+        imp = _lib.ImportsS()
+        imp.diag = C.pointer(diag)
+        assert plan(2 ** 26, imports=C.byref(imp)) == (0, 1)  # tracing reads each day's contacts
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 1), 'tuning')
+        assert plan(1000) == (1, 0)
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 0), 'tuning')
+        assert plan(2 ** 27) == (0, 1)
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_OVERLAP_RESAMPLE, 0), 'tuning')
+        assert plan(2 ** 27) == (0, 0)
+    finally:
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 2), 'tuning')
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_OVERLAP_RESAMPLE, 1), 'tuning')
+    assert L.cvk_day_plan(layers, None, None, 1, out) == _lib.E_INVALID

@@ -43,11 +43,23 @@ def full_state(sim):
     return out
 
 
+@pytest.fixture
+def generate_dynamic(request):
+    """Runs the test under CVK_TUNE_GENERATE_DYNAMIC = request.param (restores the default, 2)."""
+    from covasim_amd import _lib
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, request.param), 'tuning')
+    yield request.param
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 2), 'tuning')
+
+
 @pytest.mark.gpu
+@pytest.mark.parametrize('generate_dynamic', [2, 1], indirect=True)
 @pytest.mark.parametrize('until', [0, 17, 32])
-def test_resume_from_snapshot_equals_uninterrupted(tmp_path, until):
+def test_resume_from_snapshot_equals_uninterrupted(tmp_path, until, generate_dynamic):
+    """generate_dynamic 1: the dynamic layer's endpoints are generated inside each run and written
+    once when the run (here: the run up to the snapshot) closes, so the snapshot carries them."""
     import covasim_amd as cv
-    print(f'snapshot until={until}', flush=True)
+    print(f'snapshot until={until} generate_dynamic={generate_dynamic}', flush=True)
     pop = make_population(n=15000, pop_type='hybrid')
 
     def make():

@@ -82,6 +82,26 @@ def test_finalize_closed_forms():
     assert np.array_equal(W['n_susceptible'], W['n_alive'] - 10)  # waning: recovered agents count as susceptible
 
 
+def test_finalize_pop_infected_on_every_variant_row():
+    """sim.py:785-786 adds pop_infected to cum_infections_by_variant as a whole (n_variants, npts)
+    array, so every variant's row carries it, not only the wild type's."""
+    from covasim_amd import _lib
+    npts, nv = 5, 3
+    counts = np.zeros((npts, _lib.R_COLS), np.uint32)
+    base = _lib.R_BYVAR  # new_infections_by_variant columns
+    counts[:, base + 0] = 3
+    counts[:, base + 1] = [0, 0, 1, 2, 0]
+    counts[:, base + 2] = 0
+    counts_f = np.zeros((npts, _lib.RF_COLS))
+    nan = np.full(4, np.nan, np.float32)
+    R = rr.finalize(counts, counts_f, _lib.R_COL, _lib.R_BYVAR, _lib.MAX_VARIANTS, nv, 50.0, 7, False, None, nan,
+                    nan, nan, 0.0)
+    cum = R['variant']['cum_infections_by_variant']
+    assert np.array_equal(cum[0], 3.0 * np.arange(1, npts + 1) + 7)
+    assert np.array_equal(cum[1], np.array([0, 0, 1, 3, 3.0]) + 7)
+    assert np.array_equal(cum[2], np.full(npts, 7.0))
+
+
 # ---------------------------------------------------------------------------------------------
 # GPU
 # ---------------------------------------------------------------------------------------------
