@@ -15,6 +15,7 @@ POP_COUNT_POISSON, POP_COUNT_NEGBIN = 0, 1
 N_DURS = 9
 
 OK, E_INVALID, E_CAPACITY, E_HIP, E_ALIGN = 0, -1, -2, -3, -4
+INFECTIONS_NOALLOC_MAX_SLOTS = 65536 * 2048  # CVK_INFECTIONS_NOALLOC_MAX_SLOTS
 
 # state bits (flags word)
 SUSCEPTIBLE, NAIVE, EXPOSED, INFECTIOUS, SYMPTOMATIC, SEVERE, CRITICAL, TESTED = (1 << k for k in range(8))
@@ -274,6 +275,7 @@ def lib():
     L.cvk_trans_sus.argtypes = [vp, vp, vp, vp, f32, vp, vp, vp, vp, f32, f32, f32, vp, i64, vp, vp, vp]
     L.cvk_infections.argtypes = [f32, vp, vp, vp, i64, vp, vp, i64, vp, u64, i32, i32, i32, i32, vp, vp, vp, i64,
                                  C.POINTER(C.c_int64), vp]
+    L.cvk_infections_noalloc.argtypes = L.cvk_infections.argtypes
     P, Pa, W, Ly = C.POINTER(PeopleS), C.POINTER(Params), C.POINTER(WorkS), C.POINTER(LayerS)
     L.cvk_choose.argtypes = [P, Pa, W, i32, u32, i64, i32, i32, i32, vp]
     L.cvk_agent_pass.argtypes = [P, Pa, W, i32, vp]

@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <stdarg.h>
 #include <stdio.h>
 
 #include "../../include/covasim_amd.h"
@@ -23,6 +24,15 @@ int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_
 
 inline int fail(int code, const char* msg) {
   snprintf(last_error_buffer(), 512, "%s", msg);
+  return code;
+}
+
+// fail() with a printf-style message.
+inline int failf(int code, const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(last_error_buffer(), 512, fmt, ap);
+  va_end(ap);
   return code;
 }
 

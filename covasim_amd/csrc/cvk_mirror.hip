@@ -3,6 +3,8 @@
 // output order. The production day loop (cvk_day.hip) fuses these steps.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "cvk_device.h"
 #include "cvk_host.h"
 
@@ -216,21 +218,20 @@ extern "C" int cvk_trans_sus(const float* rel_trans, const float* rel_sus, const
   return check_launch("cvk_trans_sus");
 }
 
-extern "C" int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
-                              int64_t n_edges, const float* rel_trans, const float* rel_sus, int64_t n_agents,
-                              const double* uniforms, uint64_t seed, int32_t t, int32_t variant, int32_t layer,
-                              int32_t legacy, int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity,
-                              int64_t* count_host, void* stream) {
-  if (!count_host) return fail(CVK_E_INVALID, "cvk_infections: count_host is null");
+// Arguments of cvk_infections / cvk_infections_noalloc, validated; fills a. Returns CVK_OK, an
+// error, or 1 when there is nothing to do (no edges).
+static int infections_args(InfArgs& a, float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
+                           int64_t n_edges, const float* rel_trans, const float* rel_sus, int64_t n_agents,
+                           const double* uniforms, uint64_t seed, int32_t t, int32_t variant, int32_t layer,
+                           int32_t legacy, const int32_t* src_out, const int32_t* tgt_out, int64_t capacity,
+                           int64_t* count_host, const char* name) {
+  if (!count_host) return failf(CVK_E_INVALID, "%s: count_host is null", name);
   *count_host = 0;
-  if (n_edges < 0 || n_agents < 0 || capacity < 0)
-    return fail(CVK_E_INVALID, "cvk_infections: negative size");
-  if (n_edges == 0) return CVK_OK;
+  if (n_edges < 0 || n_agents < 0 || capacity < 0) return failf(CVK_E_INVALID, "%s: negative size", name);
+  if (n_edges == 0) return 1;
   if (!p1 || !p2 || !layer_betas || !rel_trans || !rel_sus || (capacity > 0 && (!src_out || !tgt_out)))
-    return fail(CVK_E_INVALID, "cvk_infections: null pointer");
-  if (n_edges >= (1ll << 31)) return fail(CVK_E_INVALID, "cvk_infections: more than 2^31-1 edges");
-  hipStream_t st = (hipStream_t)stream;
-  InfArgs a;
+    return failf(CVK_E_INVALID, "%s: null pointer", name);
+  if (n_edges >= (1ll << 31)) return failf(CVK_E_INVALID, "%s: more than 2^31-1 edges", name);
   a.beta = beta;
   a.p1 = p1;
   a.p2 = p2;
@@ -244,31 +245,102 @@ extern "C" int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, 
   a.variant = variant;
   a.layer = layer;
   a.n_slots = legacy ? n_edges : 2 * n_edges;
-  const int64_t n_blocks = (a.n_slots + INF_CHUNK - 1) / INF_CHUNK;
-  uint32_t* d_counts = nullptr;
-  uint64_t* d_total = nullptr;
-  if (hipMalloc((void**)&d_counts, (size_t)n_blocks * sizeof(uint32_t) + 64) != hipSuccess)
-    return fail(CVK_E_HIP, "cvk_infections: hipMalloc failed");
-  d_total = (uint64_t*)((char*)d_counts + ((n_blocks * sizeof(uint32_t) + 7) / 8) * 8);
+  return CVK_OK;
+}
+
+// Count, scan, read the total back, write the ordered output. d_counts holds n_blocks block counts
+// (overwritten, so it need not start at zero), d_total one 64-bit word.
+static int infections_run(const InfArgs& a, int64_t n_blocks, uint32_t* d_counts, uint64_t* d_total,
+                          int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity, int64_t* count_host,
+                          hipStream_t st, const char* name) {
   hipLaunchKernelGGL(infections_count_kernel, dim3((unsigned)n_blocks), dim3(INF_BLOCK), 0, st, a, d_counts);
   hipLaunchKernelGGL(scan_counts_kernel, dim3(1), dim3(1024), 0, st, d_counts, n_blocks, d_total);
   uint64_t total = 0;
   if (hipMemcpyAsync(&total, d_total, sizeof(uint64_t), hipMemcpyDeviceToHost, st) != hipSuccess ||
-      hipStreamSynchronize(st) != hipSuccess) {
-    (void)hipFree(d_counts);
-    return fail(CVK_E_HIP, "cvk_infections: count readback failed");
-  }
+      hipStreamSynchronize(st) != hipSuccess)
+    return failf(CVK_E_HIP, "%s: count readback failed", name);
   *count_host = (int64_t)total;
-  if ((int64_t)total > capacity) {
-    (void)hipFree(d_counts);
-    return fail(CVK_E_CAPACITY, "cvk_infections: output capacity too small");
-  }
+  if ((int64_t)total > capacity) return failf(CVK_E_CAPACITY, "%s: output capacity too small", name);
   if (total > 0)
     hipLaunchKernelGGL(infections_write_kernel, dim3((unsigned)n_blocks), dim3(INF_BLOCK), 0, st, a, d_counts,
                        capacity, src_out, tgt_out, prob_out);
-  const int rc = check_launch("cvk_infections");
+  const int rc = check_launch(name);
   const hipError_t se = hipStreamSynchronize(st);
-  (void)hipFree(d_counts);
   if (rc != CVK_OK) return rc;
-  return se == hipSuccess ? CVK_OK : fail(CVK_E_HIP, "cvk_infections: stream synchronize failed");
+  return se == hipSuccess ? CVK_OK : failf(CVK_E_HIP, "%s: stream synchronize failed", name);
+}
+
+extern "C" int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
+                              int64_t n_edges, const float* rel_trans, const float* rel_sus, int64_t n_agents,
+                              const double* uniforms, uint64_t seed, int32_t t, int32_t variant, int32_t layer,
+                              int32_t legacy, int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity,
+                              int64_t* count_host, void* stream) {
+  InfArgs a;
+  int rc = infections_args(a, beta, p1, p2, layer_betas, n_edges, rel_trans, rel_sus, n_agents, uniforms, seed, t,
+                           variant, layer, legacy, src_out, tgt_out, capacity, count_host, "cvk_infections");
+  if (rc != CVK_OK) return rc == 1 ? CVK_OK : rc;
+  const int64_t n_blocks = (a.n_slots + INF_CHUNK - 1) / INF_CHUNK;
+  uint32_t* d_counts = nullptr;
+  if (hipMalloc((void**)&d_counts, (size_t)n_blocks * sizeof(uint32_t) + 64) != hipSuccess)
+    return fail(CVK_E_HIP, "cvk_infections: hipMalloc failed");
+  uint64_t* d_total = (uint64_t*)((char*)d_counts + ((n_blocks * sizeof(uint32_t) + 7) / 8) * 8);
+  rc = infections_run(a, n_blocks, d_counts, d_total, src_out, tgt_out, prob_out, capacity, count_host,
+                      (hipStream_t)stream, "cvk_infections");
+  (void)hipFree(d_counts);
+  return rc;
+}
+
+// ---- the same computation without allocation ----
+// Scratch of cvk_infections_noalloc: the per-block counts (CVK_INFECTIONS_NOALLOC_MAX_BLOCKS words),
+// then the 64-bit total. Every GPU the library runs on holds its own copy until the process ends.
+// In host code `scratch` only names the symbol; its device address comes from hipGetSymbolAddress,
+// once per device, and only that address is passed to kernels.
+static_assert(CVK_INFECTIONS_NOALLOC_MAX_SLOTS == CVK_INFECTIONS_NOALLOC_MAX_BLOCKS * INF_CHUNK,
+              "header limit and scratch size disagree");
+static __device__ __align__(256) uint32_t scratch[CVK_INFECTIONS_NOALLOC_MAX_BLOCKS + 64];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+// The current device's address of `scratch` (resolved on the first call on that device).
+static int scratch_for_current_device(uint32_t** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(CVK_E_HIP, "cvk_infections_noalloc: no current device");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return failf(CVK_E_HIP, "cvk_infections_noalloc: hipGetSymbolAddress failed (%s)", hipGetErrorString(e));
+    g_scratch_ptr[dev] = (uint32_t*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CVK_OK;
+}
+
+extern "C" int cvk_infections_noalloc(float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
+                                      int64_t n_edges, const float* rel_trans, const float* rel_sus,
+                                      int64_t n_agents, const double* uniforms, uint64_t seed, int32_t t,
+                                      int32_t variant, int32_t layer, int32_t legacy, int32_t* src_out,
+                                      int32_t* tgt_out, float* prob_out, int64_t capacity, int64_t* count_host,
+                                      void* stream) {
+  InfArgs a;
+  int rc = infections_args(a, beta, p1, p2, layer_betas, n_edges, rel_trans, rel_sus, n_agents, uniforms, seed, t,
+                           variant, layer, legacy, src_out, tgt_out, capacity, count_host, "cvk_infections_noalloc");
+  if (rc != CVK_OK) return rc == 1 ? CVK_OK : rc;
+  if (a.n_slots > CVK_INFECTIONS_NOALLOC_MAX_SLOTS)
+    return failf(CVK_E_CAPACITY, "cvk_infections_noalloc: %lld edge slots exceed the fixed scratch's limit of %lld "
+                "(use cvk_infections)",
+                (long long)a.n_slots, (long long)CVK_INFECTIONS_NOALLOC_MAX_SLOTS);
+  uint32_t* base = nullptr;
+  rc = scratch_for_current_device(&base);
+  if (rc != CVK_OK) return rc;
+  const int64_t n_blocks = (a.n_slots + INF_CHUNK - 1) / INF_CHUNK;
+  uint64_t* d_total = (uint64_t*)(base + CVK_INFECTIONS_NOALLOC_MAX_BLOCKS);  // 8-byte aligned: the block is a multiple of 64 words
+  return infections_run(a, n_blocks, base, d_total, src_out, tgt_out, prob_out, capacity, count_host,
+                        (hipStream_t)stream, "cvk_infections_noalloc");
 }

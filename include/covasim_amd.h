@@ -288,6 +288,21 @@ int cvk_infections(float beta, const int32_t* p1, const int32_t* p2, const float
                    int32_t legacy, int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity,
                    int64_t* count_host, void* stream);
 
+/* No reference counterpart: cvk_infections without any allocation, for callers that cannot call
+ * hipMalloc (serving loops, harnesses that forbid allocation). Same arguments, results and errors
+ * as cvk_infections; its per-block counts live in a fixed device array of the library (256 KiB per
+ * GPU, held until the process ends) instead of a per-call allocation. Limit: at most
+ * CVK_INFECTIONS_NOALLOC_MAX_SLOTS edge slots (2 * n_edges, or n_edges with legacy != 0); larger
+ * calls return CVK_E_CAPACITY before any device work. Calls that overlap on one GPU (two host
+ * threads, or two streams) share that array and race: serialise them. */
+#define CVK_INFECTIONS_NOALLOC_MAX_BLOCKS 65536
+#define CVK_INFECTIONS_NOALLOC_MAX_SLOTS (CVK_INFECTIONS_NOALLOC_MAX_BLOCKS * 2048LL)
+int cvk_infections_noalloc(float beta, const int32_t* p1, const int32_t* p2, const float* layer_betas,
+                           int64_t n_edges, const float* rel_trans, const float* rel_sus, int64_t n_agents,
+                           const double* uniforms, uint64_t seed, int32_t t, int32_t variant, int32_t layer,
+                           int32_t legacy, int32_t* src_out, int32_t* tgt_out, float* prob_out, int64_t capacity,
+                           int64_t* count_host, void* stream);
+
 /* ---- production path: one day is agent_pass -> resample -> choose(imports) -> edge_pass ->
  *      infect_pass -> count_pass; cvk_run_days chains them for a range of days ---- */
 

@@ -156,3 +156,12 @@ def test_day_plan_generate_dynamic_switch():
         _lib.check(L.cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 2), 'tuning')
         _lib.check(L.cvk_set_tuning(_lib.TUNE_OVERLAP_RESAMPLE, 1), 'tuning')
     assert L.cvk_day_plan(layers, None, None, 1, out) == _lib.E_INVALID
+
+
+def test_noalloc_limit_matches_header():
+    from covasim_amd import _lib
+    hdr = open(HEADER).read()
+    blocks = int(re.search(r'#define CVK_INFECTIONS_NOALLOC_MAX_BLOCKS (\d+)', hdr).group(1))
+    chunk = int(re.search(r'#define CVK_INFECTIONS_NOALLOC_MAX_SLOTS \(CVK_INFECTIONS_NOALLOC_MAX_BLOCKS \* (\d+)LL\)',
+                          hdr).group(1))
+    assert _lib.INFECTIONS_NOALLOC_MAX_SLOTS == blocks * chunk
