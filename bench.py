@@ -1,14 +1,16 @@
 """Benchmark of the daily transmission step on one MI355X.
 
-A step is one simulated day of the plain hybrid population (1M agents; layers
-h/s/w/c with betas 3.0/0.6/0.6/0.3, community resampled daily) run through the
-C-ABI day loop captured as one hipGraph. ``--steps K`` days are timed from a
-fresh state after ``--warmup W`` untimed days (replays of the same graph).
+A step is one simulated day of the plain hybrid population (1M agents; layers h/s/w/c with betas
+3.0/0.6/0.6/0.3, community resampled daily, 180 days). ``--steps K`` days spread evenly over the
+whole run are timed in place, each as a one-day captured graph with HIP events around its launch,
+after ``--warmup`` untimed passes; the days between them run untimed. The whole run as one graph
+replay is reported beside them (``full_run``).
 
-Prints one JSON line: agent-days/s for the whole job, the edge-pass roofline
-(algorithmic bytes per launch over its HIP-event-measured duration, against the
-8 TB/s HBM3E peak) and the CPU baseline (the oracle restatement timed on a bounded
-sample of the same workload on this host).
+Prints one JSON line: agent-days/s over the K timed days, the edge-pass roofline (algorithmic bytes
+per launch over its HIP-event-measured duration, every day of the run, against the 8 TB/s HBM3E
+peak), the whole-day roofline (algorithmic bytes of the timed days over their time) and the CPU
+baselines (the oracle's C restatement, one thread and OpenMP, timed on the same sampled days of the
+same population on this host).
 """
 
 import argparse
@@ -103,12 +105,15 @@ def build_population(w, seed):
     return pop
 
 
-def cpu_baseline(w, pop, seed, budget_s, omp=False):
-    """The oracle's compiled C restatement on the same population until budget_s elapses.
+def cpu_baseline_days(w, pop, seed, days, budget_s=90.0):
+    """The oracle's compiled C restatement on the same population, timed on the same sampled days
+    as the GPU.
 
-    One host thread (numba's default single-threaded mode) unless omp: then OpenMP over the
-    host's cores. Days are run from day 0, so the sample covers the epidemic's growth phase.
-    """
+    The run advances through the unsampled days untimed with the OpenMP build; each sampled day is
+    run once by the single-threaded build (numba's default mode, numba_parallel='none') and, from
+    the same state, once by the OpenMP build over the host's cores (numba's parallel modes). Both
+    builds give identical results, so the state stays that of one run. Stops early, on a sampled-day
+    boundary, once budget_s has elapsed."""
     from oracle import covasim_oracle as co
     from oracle.cport.port import PortSim, load
     opars = co.make_pars(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=w['n_days'], rand_seed=seed,
@@ -117,18 +122,44 @@ def cpu_baseline(w, pop, seed, budget_s, omp=False):
     init = dict(age=pop['age'], **{k: pop[k] for k in keys})
     layers = {k: (pop['contacts'][k]['p1'], pop['contacts'][k]['p2'], pop['contacts'][k]['beta'])
               for k in opars['beta_layer']}
-    sim = PortSim(opars, init, layers, seed, omp=omp)
-    threads = load(omp).port_threads()
+    sim = PortSim(opars, init, layers, seed, omp=True)
+    one, omp = load(False), load(True)
+    threads = omp.port_threads()
+    want = set(days)
+    t_one, t_omp = {}, {}
     t0 = time.perf_counter()
-    days = 0
-    while days < sim.npts and time.perf_counter() - t0 < budget_s:
+    while sim.t < sim.npts and time.perf_counter() - t0 < budget_s:
+        t = sim.t
+        if t not in want:
+            sim.lib = omp
+            sim.step()
+            continue
+        state = {k: v.copy() for k, v in sim.a.items()}
+        lay = [[c.copy() for c in l] for l in sim.layers_np]
+        sim.lib = one
+        c0 = time.perf_counter()
         sim.step()
-        days += 1
-    el = time.perf_counter() - t0
-    return dict(value=w['pop_size'] * days / el, unit='agent-days/s', cores=threads, kind='port',
-                sample=f'oracle C restatement (gcc -O3{", OpenMP" if omp else ""}, Philox draws) on the same '
-                       f'population, days 0..{days - 1} of the workload, {el:.1f} s on {threads} host thread(s)',
-                host=host_cpu())
+        t_one[t] = time.perf_counter() - c0
+        for k, v in state.items():
+            np.copyto(sim.a[k], v)
+        for l, saved in zip(sim.layers_np, lay):
+            for c, v in zip(l, saved):
+                np.copyto(c, v)
+        sim.t = t
+        sim.lib = omp
+        c0 = time.perf_counter()
+        sim.step()
+        t_omp[t] = time.perf_counter() - c0
+    done = [d for d in days if d in t_one]
+    n = w['pop_size']
+    host = host_cpu()
+    def line(times, cores, label):
+        el = sum(times[d] for d in done)
+        return dict(value=n * len(done) / el if el > 0 else None, unit='agent-days/s', cores=cores, kind='port',
+                    sample=f'oracle C restatement (gcc -O3{label}, Philox draws) on the same population, the same '
+                           f'{len(done)} sampled days as the GPU ({done[0] if done else "-"}..'
+                           f'{done[-1] if done else "-"}), {el:.2f} s on {cores} host thread(s)', host=host)
+    return line(t_one, 1, ''), line(t_omp, threads, ', OpenMP')
 
 
 def host_cpu():
@@ -206,6 +237,25 @@ def run_batch(w, args):
         for e in execs:
             L.cvk_graph_destroy(e)
     single_inf = np.array([int(sim._w['results'][:, _lib.R_COL['new_infections']].sum().item()) for sim in sims])
+    # edge-pass roofline: the same run eagerly, HIP events around each day's batched edge launch
+    batch.restore_state(snap)
+    _lib.check(L.cvk_batch_time_edges(batch._handle, 1), 'time edges')
+    with torch.cuda.stream(stream):
+        _lib.check(L.cvk_batch_run_days(batch._handle, 0, steps, batch.pop_infected.ctypes.data,
+                                        C.byref(batch._imports), 0, C.c_void_p(stream.cuda_stream)), 'batch eager run')
+    torch.cuda.synchronize()
+    edge_ms = (C.c_float * steps)()
+    got = L.cvk_batch_edge_times(batch._handle, edge_ms, steps)
+    _lib.check(0 if got == steps else got, 'edge times')
+    _lib.check(L.cvk_batch_time_edges(batch._handle, 0), 'time edges')
+    edge_ms = np.array(edge_ms[:steps], dtype=np.float64)
+    eager_inf = batch.raw_results()[0][:, :, _lib.R_COL['new_infections']].sum(axis=1).astype(np.int64)
+    dyn_keys = [k for k, v in w['pars'].get('dynam_layer', {}).items() if v]
+    dyn_total = int(sum(batch.edge_counts[batch.layer_keys.index(k)].sum() for k in dyn_keys))
+    mode = 2 if args.generate_dynamic is None else args.generate_dynamic  # CVK_TUNE_GENERATE_DYNAMIC's rule
+    gen = dyn_total > 0 and (mode == 1 or (mode == 2 and dyn_total >= 2 ** 25))
+    edge_bytes = 8 * (n_edges - (dyn_total if gen else 0)) + n_agents / 8
+    edge_achieved = edge_bytes / (float(edge_ms.mean()) / 1e3) / 1e9
     # results of all sims: device finalisation (sim.py:764-1001) and reduction over sims (run.py:220-303)
     batch.device_results()
     torch.cuda.synchronize()
@@ -241,6 +291,8 @@ def run_batch(w, args):
                                       'and the three date arrays to the host')
     value = n_agents * steps / wall_batch
     single = n_agents * steps / wall_single
+    b_agent = 56
+    day_bytes_total = steps * (n_agents * b_agent + edge_bytes + (0 if gen else 8 * dyn_total)) + 64 * int(batch_inf.sum())
     out = {
         'metric': METRIC, 'value': value, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': steps, 'warmup': steps,
         'ms_per_step': wall_batch / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
@@ -250,11 +302,56 @@ def run_batch(w, args):
         'batched': {'sims_per_s': S / wall_batch, 'agent_days_per_s': value, 'wall_s': wall_batch,
                     'one_at_a_time_sims_per_s': S / wall_single, 'one_at_a_time_agent_days_per_s': single,
                     'one_at_a_time_wall_s': wall_single, 'speedup': wall_single / wall_batch,
-                    'identical_infection_totals': bool(np.array_equal(batch_inf, single_inf)),
+                    'identical_infection_totals': bool(np.array_equal(batch_inf, single_inf) and
+                                                       np.array_equal(batch_inf, eager_inf)),
+                    'dynamic_layers': 'generated from the resample stream' if gen else 'resampled daily',
                     'autotuned_shape': list(tuned), 'autotune_trials_ms': batch.tuning['trials_ms']},
+        'roofline': {'bound': 'hbm', 'kernel': 'batch_edge_kernel', 'achieved': edge_achieved, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': edge_achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'avg_launch_us': float(edge_ms.mean()) * 1e3, 'launches': int(steps),
+                     'algorithmic_bytes_per_launch': edge_bytes,
+                     'launch_us_p10_p50_p90_max': [round(float(x) * 1e3, 1) for x in
+                                                   np.percentile(edge_ms, [10, 50, 90, 100])],
+                     'whole_day': {'bytes_per_agent': b_agent, 'achieved': day_bytes_total / wall_batch / 1e9,
+                                   'frac': day_bytes_total / wall_batch / 1e9 / HBM_PEAK_GBS,
+                                   'formula': 'per day N*56 + edge-pass bytes + 8*E_dyn written (0 generated) + '
+                                              '64*new infections, over the batch graph wall time'}},
         'results': results_line,
     }
+    if not args.no_cpu_baseline:
+        out['cpu_baseline'], out['cpu_baseline_omp'] = cpu_baseline_sims(w, sims, sizes, betas, args, 30.0)
     print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_sims(w, sims, sizes, betas, args, budget_s):
+    """The batch's sims one after another through the oracle's C restatement (as the reference's
+    MultiSim runs independent sims), one thread and OpenMP, until budget_s each; agent-days/s."""
+    from oracle.cport.port import PortSim, load
+    from oracle import covasim_oracle as co
+    out = []
+    for omp in (False, True):
+        t0 = time.perf_counter()
+        agent_days, done = 0, 0
+        for i, sim in enumerate(sims):
+            if time.perf_counter() - t0 > budget_s:
+                break
+            people = sim.people
+            opars = co.make_pars(pop_size=int(sizes[i]), pop_type=w['pop_type'], n_days=w['n_days'],
+                                 rand_seed=args.seed + i, beta=float(betas[i]), **w['pars'])
+            init = dict(age=people.age, **{k: people.get(k) for k in ('symp_prob', 'severe_prob', 'crit_prob',
+                                                                       'death_prob', 'rel_sus', 'rel_trans')})
+            layers = {k: tuple(people.contacts[k][c] for c in ('p1', 'p2', 'beta')) for k in opars['beta_layer']}
+            PortSim(opars, init, layers, args.seed + i, omp=omp).run()
+            agent_days += int(sizes[i]) * (w['n_days'] + 1)
+            done += 1
+        el = time.perf_counter() - t0
+        cores = load(omp).port_threads()
+        out.append(dict(value=agent_days / el, unit='agent-days/s', cores=cores, kind='port',
+                        sample=f'oracle C restatement (gcc -O3{", OpenMP" if omp else ""}) running the first {done} '
+                               f'of the batch\'s sims one after another, all {w["n_days"] + 1} days each, '
+                               f'{el:.1f} s on {cores} host thread(s), set-up included',
+                        host=host_cpu()))
+    return out[0], out[1]
 
 
 def run_relax(w, args):
@@ -458,13 +555,14 @@ def run_interventions(w, args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=None, help='timed days (default: the full run)')
+    ap.add_argument('--steps', type=int, default=None,
+                    help='timed days, spread evenly over the run (default 20)')
     ap.add_argument('--warmup', type=int, default=None)
     ap.add_argument('--workload', default='hybrid1m', choices=sorted(WORKLOADS))
     ap.add_argument('--seed', type=int, default=1)
-    ap.add_argument('--cpu-budget', type=float, default=15.0)
+    ap.add_argument('--cpu-budget', type=float, default=90.0,
+                    help='seconds of CPU restatement work at most (stops on a sampled-day boundary)')
     ap.add_argument('--no-cpu-baseline', action='store_true')
-    ap.add_argument('--cpu-omp', action='store_true', help='also time the OpenMP build of the CPU restatement')
     ap.add_argument('--edge-chunks', type=int, default=None, help='edge-pass wave tile depth (2 or 4 chunks of 4 edges per lane)')
     ap.add_argument('--edge-lds', type=int, default=None, help='1: stage the infectious bitmap in LDS, 0: read it from L2')
     ap.add_argument('--edge-prefetch', type=int, default=None,
@@ -476,8 +574,12 @@ def main():
     ap.add_argument('--generate-dynamic', type=int, default=None,
                     help='1: the day loop computes dynamic-layer endpoints in place; 0: writes and reads them daily; '
                          '2 (default): 1 from 2^25 dynamic edges up')
+    ap.add_argument('--source-lists', type=int, default=None,
+                    help='static layers walked through source lists: 0 never, 1 always, 2 (default) on quiet days')
+    ap.add_argument('--source-permille', type=int, default=None,
+                    help='with --source-lists 2: most infectious agents per thousand for which lists are walked')
     ap.add_argument('--eager-only', action='store_true',
-                    help='counter-collection mode: skip the graph replays; value is then the eager loop rate')
+                    help='counter-collection mode: only the eager pass over every day (no graphs, no CPU baseline)')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='MiB of one device-to-device copy after the run (a known-size kernel for counter scaling)')
     args = ap.parse_args()
@@ -507,6 +609,12 @@ def main():
     if args.edge_coarse is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_COARSE, args.edge_coarse), 'tuning')
         tuning['edge_coarse'] = args.edge_coarse
+    if args.source_lists is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_SOURCE_LISTS, args.source_lists), 'tuning')
+        tuning['source_lists'] = args.source_lists
+    if args.source_permille is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_SOURCE_PERMILLE, args.source_permille), 'tuning')
+        tuning['source_permille'] = args.source_permille
     if args.edge_lds is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, args.edge_lds), 'tuning')
         tuning['edge_lds'] = args.edge_lds
@@ -518,14 +626,38 @@ def main():
         return run_interventions(w, args)
     if w.get('relax'):
         return run_relax(w, args)
-    steps = args.steps or w['n_days']
-    warmup = steps if args.warmup is None else args.warmup
-    n_days = max(w['n_days'], steps)
+    return run_single(w, args, tuning)
+
+
+def sample_days(npts, k):
+    """k days spread evenly over a run of npts days (the middle of k equal slices)."""
+    return [min(npts - 1, (2 * i + 1) * npts // (2 * k)) for i in range(k)]
+
+
+def day_bytes(n, b_agent, edge_read, resample_write, new_inf):
+    """Algorithmic HBM bytes of one simulated day: the agent pass (b_agent per agent), the edge pass
+    (edge_read: 8 B per stored edge read, 12 B where the layer carries weights other than 1, none for
+    a generated dynamic layer; plus the N/8-byte bitmap), the dynamic layer's resample writes (8 B
+    per written edge) and 64 B per new infection."""
+    return n * b_agent + edge_read + n / 8 + resample_write + 64 * new_inf
+
+
+def run_single(w, args, tuning):
+    """One sim: K one-day steps spread evenly over the whole run, timed in place (HIP events
+    around each day's captured graph), beside the full run as one graph replay."""
+    import torch
+    import covasim_amd as cv
+    from covasim_amd import _lib
+    n_days = w['n_days']
+    npts = n_days + 1
+    K = args.steps if args.steps else 20
+    W = 5 if args.warmup is None else args.warmup
     t_build = time.perf_counter()
     print(f'building population ({args.workload})', file=sys.stderr, flush=True)
     pop = build_population(w, args.seed)
     n_edges = sum(len(l['p1']) for l in pop['contacts'].values())
-    dyn_edges = sum(len(pop['contacts'][k]['p1']) for k, v in w['pars'].get('dynam_layer', {}).items() if v)
+    dyn_keys = [k for k, v in w['pars'].get('dynam_layer', {}).items() if v]
+    dyn_edges = sum(len(pop['contacts'][k]['p1']) for k in dyn_keys)
     variants = [cv.variant(dict(rel_beta=v['rel_beta']), days=v['days'], n_imports=v['n_imports'])
                 for v in w.get('variants', [])]
     sim = cv.Sim(pop_size=w['pop_size'], pop_type=w['pop_type'], n_days=n_days, rand_seed=args.seed,
@@ -534,9 +666,16 @@ def main():
     build_s = time.perf_counter() - t_build
     L = _lib.lib()
     stream = torch.cuda.Stream()
+    st = C.c_void_p(stream.cuda_stream)
     people = sim.people
+    P = people.struct()
+    pr, wk = sim._params, sim._work
+    pop_inf = int(w['pars']['pop_infected'])
     init_state = {k: v.clone() for k, v in people._d.items()}
     init_layers = {lk: {c: lay.tensor(c).clone() for c in ('p1', 'p2', 'beta')} for lk, lay in people.contacts.items()}
+    plan = (C.c_int32 * 2)()
+    _lib.check(L.cvk_day_plan(sim._layers, pr, sim._imports, npts, plan), 'day plan')
+    gen = bool(plan[0])
 
     def reset():
         for k, v in init_state.items():
@@ -546,101 +685,163 @@ def main():
                 lay.tensor(c).copy_(init_layers[lk][c])
         _lib.reset_workspace(sim._w)
 
-    P = people.struct()
-
-    def graph_run():
-        """Capture all days as one graph, replay it for warmup, then time one replay from a fresh state."""
-        reset()
-        torch.cuda.synchronize()
-        gexec = C.c_void_p()
-        _lib.check(L.cvk_graph_create(P, sim._layers, sim._params, sim._work, 0, steps,
-                                      int(w['pars']['pop_infected']), sim._imports, C.c_void_p(stream.cuda_stream),
-                                      C.byref(gexec)), 'graph create')
-        for _ in range(max(0, -(-warmup // steps))):  # warmup days, as whole-graph replays
-            reset()
-            _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'graph launch')
-        torch.cuda.synchronize()
-        reset()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _lib.check(L.cvk_graph_launch(gexec, C.c_void_p(stream.cuda_stream)), 'graph launch')
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        print(f'timed {steps} days in {wall * 1e3:.1f} ms', file=sys.stderr, flush=True)
+    def counts():
         res = sim._w['results'].cpu().numpy().view(np.uint32)
-        cum = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
-        _lib.check(L.cvk_graph_destroy(gexec), 'graph destroy')
-        return wall, res.copy(), cum
+        return res.copy()
 
-    with torch.cuda.stream(stream):
-        if not args.eager_only:
-            wall, res, cum_inf = graph_run()
-        # Edge-pass duration: the same days again, eager, with HIP events on this stream around each edge pass.
-        reset()
-        torch.cuda.synchronize()
-        t_eager = time.perf_counter()
-        edge_ms = []
-        st = C.c_void_p(stream.cuda_stream)
-        pr, wk = sim._params, sim._work
-        for t in range(steps):
-            if t == 0:
-                _lib.check(L.cvk_choose(P, pr, wk, 0, _lib.STREAM_CHOOSE, int(w['pars']['pop_infected']), 0,
-                                        _lib.LAYER_SEED, 0, st), 'seed')
-                _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, 0, 0, st), 'seed')
-            _lib.check(L.cvk_agent_pass(P, pr, wk, t, st), 'agent')
-            for li in range(pr.n_layers):
-                if sim._layers[li].dynamic:
-                    _lib.check(L.cvk_resample_layer(C.byref(sim._layers[li]), li, pr, t, st), 'resample')
-            days_i, vars_i, nimp_i = sim._imp_arrays
-            for j in range(len(days_i)):
-                if days_i[j] == t:
-                    _lib.check(L.cvk_choose(P, pr, wk, t, _lib.STREAM_IMPORT | int(vars_i[j]), int(nimp_i[j]), 1,
-                                            _lib.LAYER_IMPORT, int(vars_i[j]), st), 'imports')
+    def eager_day(t, events=None):
+        """Day t through the individual entry points (Sim.step's sequence); HIP events around the
+        edge pass when events is a list."""
+        if t == 0:
+            _lib.check(L.cvk_choose(P, pr, wk, 0, _lib.STREAM_CHOOSE, pop_inf, 0, _lib.LAYER_SEED, 0, st), 'seed')
+            _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, 0, 0, st), 'seed')
+        _lib.check(L.cvk_agent_pass(P, pr, wk, t, st), 'agent')
+        for li in range(pr.n_layers):
+            if sim._layers[li].dynamic:
+                _lib.check(L.cvk_resample_layer(C.byref(sim._layers[li]), li, pr, t, st), 'resample')
+        days_i, vars_i, nimp_i = sim._imp_arrays
+        for j in range(len(days_i)):
+            if days_i[j] == t:
+                _lib.check(L.cvk_choose(P, pr, wk, t, _lib.STREAM_IMPORT | int(vars_i[j]), int(nimp_i[j]), 1,
+                                        _lib.LAYER_IMPORT, int(vars_i[j]), st), 'imports')
+        if events is not None:
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             e0.record(stream)
-            _lib.check(L.cvk_edge_pass(P, sim._layers, pr, wk, t, None, st), 'edge')
+        _lib.check(L.cvk_edge_pass(P, sim._layers, pr, wk, t, None, st), 'edge')
+        if events is not None:
             e1.record(stream)
-            _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, t, 1, st), 'infect')
-            _lib.check(L.cvk_count_pass(P, pr, wk, t, st), 'count')
-            edge_ms.append((e0, e1))
-        torch.cuda.synchronize()
-        eager_wall = time.perf_counter() - t_eager
-        edge_ms = [a.elapsed_time(b) for a, b in edge_ms]
-        res2 = sim._w['results'].cpu().numpy().view(np.uint32)
-        if args.eager_only:  # counter-collection mode: no graph run to compare with
-            wall, res = eager_wall, res2
-            cum_inf = int(res[:steps, _lib.R_COL['new_infections']].sum()) + int(w['pars']['pop_infected'])
-            eager_matches_graph = None
-        else:
-            eager_matches_graph = bool(np.array_equal(res[:steps], res2[:steps]))
+            events.append((e0, e1))
+        _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, t, 1, st), 'infect')
+        _lib.check(L.cvk_count_pass(P, pr, wk, t, st), 'count')
 
+    def graph(t0, t1):
+        g = C.c_void_p()
+        _lib.check(L.cvk_graph_create(P, sim._layers, pr, wk, t0, t1, pop_inf, sim._imports, st, C.byref(g)),
+                   'graph create')
+        return g
+
+    days = sample_days(npts, K)
+    clock = sim._w['day_clock']
+    hz = C.c_int64(0)
+    _lib.check(L.cvk_wall_clock_rate(C.byref(hz)), 'wall clock rate')
+    tick_s = 1.0 / hz.value
+    if args.eager_only:  # counter collection: only the eager pass (2) below, every launch in the trace
+        with torch.cuda.stream(stream):
+            reset()
+            edge_ev = []
+            for t in range(npts):
+                eager_day(t, edge_ev)
+            torch.cuda.synchronize()
+        edge_ms = np.array([a.elapsed_time(b) for a, b in edge_ev])
+        line = {'mode': 'eager pass only (counter collection)', 'days': npts, 'edge_avg_launch_us':
+                float(edge_ms.mean()) * 1e3, 'source_hash': f'{L.cvk_source_hash():016x}',
+                'cum_infections': int(counts()[:, _lib.R_COL['new_infections']].sum()) + pop_inf}
+        if args.copy_probe:
+            nbytes = args.copy_probe << 20
+            src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+            dst = torch.empty_like(src)
+            torch.cuda.synchronize()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            line['copy_probe_bytes'] = nbytes
+        print(json.dumps(line), flush=True)
+        return
+    with torch.cuda.stream(stream):
+        # (1) the whole run as one captured graph: warmup replays, then one timed replay; every day's
+        # first pass stamps the device wall clock, so each day's duration inside the run is known
+        full = graph(0, npts)
+        for _ in range(max(1, -(-W // npts))):
+            reset()
+            _lib.check(L.cvk_graph_launch(full, st), 'graph launch')
+        reset()
+        clock.zero_()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _lib.check(L.cvk_graph_launch(full, st), 'graph launch')
+        torch.cuda.synchronize()
+        full_wall = time.perf_counter() - t0
+        _lib.check(L.cvk_graph_destroy(full), 'graph destroy')
+        res_full = counts()
+        ticks = clock.cpu().numpy().astype(np.int64)
+        day_s = np.diff(ticks) * tick_s  # day t: from its first pass to the next day's (or the closing pass)
+        print(f'full run: {npts} days in {full_wall * 1e3:.1f} ms', file=sys.stderr, flush=True)
+        # (2) every day eagerly, HIP events around each edge pass (the roofline kernel) and the source-list
+        # counters read after each day
+        reset()
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_COUNT_SOURCES, 1), 'tuning')
+        edge_ev = []
+        ctl = sim._w['ctl']
+        src_counts = np.zeros((npts, 2), dtype=np.int64)
+        for t in range(npts):
+            eager_day(t, edge_ev)
+            c = ctl[_lib.CTL_SRC_SLOTS:_lib.CTL_SRC_SLOTS + 4].cpu().numpy().view(np.int64)
+            src_counts[t] = c
+        _lib.check(L.cvk_set_tuning(_lib.TUNE_COUNT_SOURCES, 0), 'tuning')
+        torch.cuda.synchronize()
+        edge_ms = np.array([a.elapsed_time(b) for a, b in edge_ev])
+        res_eager = counts()
+    src_day = np.diff(np.vstack([np.zeros((1, 2), np.int64), src_counts]), axis=0)  # slots, lists walked per day
+    step_s = np.array([day_s[d] for d in days])
     n = w['pop_size']
-    avg_edge_s = float(np.mean(edge_ms)) / 1e3
-    edge_bytes = 8 * n_edges + n / 8  # p1+p2 int32 streams + the infectious bitmap (per launch)
-    achieved = edge_bytes / avg_edge_s / 1e9
-    value = n * steps / wall
+    new_inf = res_full[:, _lib.R_COL['new_infections']].astype(np.int64)
+    cum_inf = int(new_inf.sum()) + pop_inf
+    timed_s = float(step_s.sum())
+    value = n * K / timed_s
+    # roofline of the dominant kernel: the edge pass over every day of the run. Algorithmic bytes per
+    # launch: 8 per streamed edge (12 where the layer carries weights), none for a generated layer, 8 per
+    # source-list slot walked (target + edge) and per list (two offsets), and the N/8-byte bitmap
+    listed = set(sim._sources)
+    streamed = [k for k in pop['contacts'] if k not in listed]
+    weighted = sum(len(pop['contacts'][k]['p1']) for k in streamed if not np.all(pop['contacts'][k]['beta'] == 1))
+    stream_edges = sum(len(pop['contacts'][k]['p1']) for k in streamed) - (dyn_edges if gen else 0)
+    edge_bytes_day = 8 * stream_edges + 4 * weighted + n / 8 + 8 * src_day[:, 0] + 8 * src_day[:, 1]
+    achieved = float(edge_bytes_day.sum()) / (float(edge_ms.sum()) / 1e3) / 1e9
+    nv = 1 + len(w.get('variants', []))
+    b_agent = 56 + 8 * (nv - 1) + ((20 + 12 * nv) if w['pars'].get('use_waning') else 0)
+    bday = np.array([n * b_agent + edge_bytes_day[t] + (0 if gen else 8 * dyn_edges) + 64 * int(new_inf[t])
+                     for t in range(npts)])
+    whole_sampled = float(sum(bday[d] for d in days)) / timed_s / 1e9
+    whole_full = float(bday.sum()) / full_wall / 1e9
     out = {
-        'metric': METRIC, 'value': value, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': steps, 'warmup': warmup,
-        'ms_per_step': wall / steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'metric': METRIC, 'value': value, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': K, 'warmup': W,
+        'ms_per_step': timed_s / K * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
-        'config': {'workload': w['desc'], 'agents': n, 'days': steps, 'edges': int(n_edges),
-                   'dynamic_edges': int(dyn_edges),
-                   'dynamic_layers': ('generated from the resample stream'
-                                      if tuning.get('generate_dynamic', 2) == 1 or
-                                      (tuning.get('generate_dynamic', 2) == 2 and dyn_edges >= 2 ** 25)
-                                      else 'resampled into the layer arrays daily'),
-                   'cum_infections': cum_inf, 'eager_matches_graph': eager_matches_graph,
-                   'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {}),
-                   **({'mode': 'eager launches (counter collection); value is not the graph-replay rate'}
-                      if args.eager_only else {})},
+        'config': {'workload': w['desc'], 'agents': n, 'days': npts, 'edges': int(n_edges),
+                   'dynamic_edges': int(dyn_edges), 'source_listed_layers': sorted(listed),
+                   'dynamic_layers': 'generated from the resample stream' if gen else 'resampled into the layer arrays daily',
+                   'step': 'one simulated day; the K timed days are spread evenly over the whole run and timed in '
+                           'place inside one replay of the whole run as a captured graph (host synchronisation '
+                           'before and after it), each from its first pass to the next day\'s, by the device '
+                           'wall clock that the day loop stamps',
+                   'timed_days': days, 'cum_infections': cum_inf,
+                   'runs_identical': bool(np.array_equal(res_full, res_eager)),
+                   'setup_s': round(build_s, 2), **({'tuning': tuning} if tuning else {})},
+        'full_run': {'days': npts, 'wall_ms': full_wall * 1e3, 'ms_per_day': full_wall / npts * 1e3,
+                     'agent_days_per_s': n * npts / full_wall, 'device_clock_ms': float(day_s.sum()) * 1e3,
+                     'note': 'all days as one captured graph, one launch, host clock with synchronisation'},
         'roofline': {'bound': 'hbm', 'kernel': 'edge_pass_kernel', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                      'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
                      'traffic': measured_traffic(args.workload, tuning),
-                     'avg_launch_us': avg_edge_s * 1e6, 'algorithmic_bytes_per_launch': edge_bytes,
+                     'avg_launch_us': float(edge_ms.mean()) * 1e3, 'launches': int(len(edge_ms)),
+                     'algorithmic_bytes_per_launch': float(edge_bytes_day.mean()),
                      'launch_us_p10_p50_p90_max': [round(float(x) * 1e3, 1) for x in
-                                                   np.percentile(edge_ms, [10, 50, 90, 100])]},
+                                                   np.percentile(edge_ms, [10, 50, 90, 100])],
+                     'timed_days_avg_launch_us': float(np.mean([edge_ms[d] for d in days])) * 1e3,
+                     'source_slots_per_day_mean': float(src_day[:, 0].mean()),
+                     'whole_day': {'bytes_per_agent': b_agent, 'achieved_timed_days': whole_sampled,
+                                   'frac_timed_days': whole_sampled / HBM_PEAK_GBS,
+                                   'achieved_full_run': whole_full, 'frac_full_run': whole_full / HBM_PEAK_GBS,
+                                   'bytes_full_run': float(bday.sum()),
+                                   'formula': 'B_day = N*b_agent + edge-pass bytes + 8*E_dyn written + '
+                                              '64*new infections'}},
     }
     print('edge pass per-day us: ' + ' '.join(f'{x * 1e3:.0f}' for x in edge_ms), file=sys.stderr, flush=True)
+    print('infectious per thousand: ' + ' '.join(f'{x * 1e3 / n:.1f}' for x in
+                                                  res_full[:, _lib.R_COL['n_infectious']]), file=sys.stderr, flush=True)
+    print('source slots per day: ' + ' '.join(str(int(x)) for x in src_day[:, 0]), file=sys.stderr, flush=True)
+    print('day us (device clock): ' + ' '.join(f'{x * 1e6:.0f}' for x in day_s), file=sys.stderr, flush=True)
+    print('timed day us: ' + ' '.join(f'{d}:{x * 1e6:.0f}' for d, x in zip(days, step_s)), file=sys.stderr,
+          flush=True)
     if args.copy_probe:
         nbytes = args.copy_probe << 20
         src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
@@ -651,11 +852,10 @@ def main():
         out['copy_probe_bytes'] = nbytes
     out['roofline']['device_copy_gbs'] = device_copy_gbs()
     if not args.no_cpu_baseline:
-        out['cpu_baseline'] = cpu_baseline(w, pop, args.seed, args.cpu_budget)
-        if args.cpu_omp:
-            out['cpu_baseline_omp'] = cpu_baseline(w, pop, args.seed, args.cpu_budget, omp=True)
+        one, omp = cpu_baseline_days(w, pop, args.seed, days, args.cpu_budget)
+        out['cpu_baseline'] = one
+        out['cpu_baseline_omp'] = omp
     print(json.dumps(out), flush=True)
-
 
 if __name__ == '__main__':
     main()

@@ -16,8 +16,8 @@ from .batch import BatchSim
 from .interventions import Intervention, change_beta, clip_edges, test_prob, contact_tracing
 from .checkpoint import load
 from . import utils
-from .utils import set_seed
+from .utils import set_seed, choose, choose_r, default_int, default_float
 
 __all__ = ['__version__', 'Intervention', 'change_beta', 'clip_edges', 'test_prob', 'contact_tracing', 'load', 'make_pars', 'reset_layer_pars', 'get_prognoses', 'get_variant_pars',
            'get_cross_immunity', 'variant', 'Result', 'Layer', 'Contacts', 'People', 'Sim', 'AlreadyRunError', 'BatchSim',
-           'utils', 'set_seed']
+           'utils', 'set_seed', 'choose', 'choose_r', 'default_int', 'default_float']

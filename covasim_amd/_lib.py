@@ -40,6 +40,9 @@ TUNE_GENERATE_DYNAMIC = 4
 TUNE_OVERLAP_RESAMPLE = 5
 TUNE_EDGE_COARSE = 6
 TUNE_EDGE_PREFETCH = 7
+TUNE_SOURCE_LISTS = 8
+TUNE_COUNT_SOURCES = 9
+TUNE_SOURCE_PERMILLE = 10
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15
@@ -66,6 +69,7 @@ R_COLS = R_BYVAR + len(RV_NAMES) * MAX_VARIANTS
 RF_COLS = 4
 CTL_WORDS = 64
 CTL_LOG_COUNT, CTL_LOG_OVERFLOW, CTL_SEL_SHORT = 2, 4, 10
+CTL_SRC_SLOTS, CTL_SRC_AGENTS = 12, 14
 
 
 class Dur(C.Structure):
@@ -106,9 +110,13 @@ class PeopleS(C.Structure):
     _fields_ = [('n', C.c_int64)] + [(k, C.c_void_p) for k in PEOPLE_PTR_FIELDS]
 
 
+class SourcesS(C.Structure):
+    _fields_ = [('indptr', C.c_void_p), ('target', C.c_void_p), ('edge_dir', C.c_void_p), ('n_slots', C.c_int64)]
+
+
 class LayerS(C.Structure):
     _fields_ = [('p1', C.c_void_p), ('p2', C.c_void_p), ('beta', C.c_void_p), ('n_edges', C.c_int64),
-                ('dynamic', C.c_int32), ('beta_ones', C.c_int32)]
+                ('dynamic', C.c_int32), ('beta_ones', C.c_int32), ('sources', C.POINTER(SourcesS))]
 
 
 class WorkS(C.Structure):
@@ -116,13 +124,13 @@ class WorkS(C.Structure):
                 ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
                 ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('coarse', C.c_void_p),
                 ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p), ('bound', C.c_void_p),
-                ('npts', C.c_int32), ('pad', C.c_int32)]
+                ('npts', C.c_int32), ('pad', C.c_int32), ('day_clock', C.c_void_p)]
 
 
 MAX_REDUCE_BLOCKS = 65536
 PARTIAL_COLS = 64
 WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'coarse',
-                   'rec', 'partials', 'partials_f', 'bound']
+                   'rec', 'partials', 'partials_f', 'bound', 'day_clock']
 
 
 def make_workspace(n, npts, log_capacity, device):
@@ -143,6 +151,7 @@ def make_workspace(n, npts, log_capacity, device):
         partials=torch.zeros((MAX_REDUCE_BLOCKS, PARTIAL_COLS), dtype=torch.int32, device=device),
         partials_f=torch.zeros((MAX_REDUCE_BLOCKS, RF_COLS), dtype=torch.float64, device=device),
         bound=torch.zeros(n, dtype=torch.int16, device=device),
+        day_clock=torch.zeros(npts + 1, dtype=torch.int64, device=device),
     )
     s = WorkS()
     for k in WORK_PTR_FIELDS:
@@ -266,6 +275,7 @@ def lib():
     L.cvk_last_error.restype = C.c_char_p
     L.cvk_version.restype = C.c_int
     L.cvk_source_hash.restype = C.c_uint64
+    L.cvk_wall_clock_rate.argtypes = [C.POINTER(C.c_int64)]
     L.cvk_abi_layout.argtypes = [C.POINTER(C.c_int64), i32]
     L.cvk_device_info.argtypes = [C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cvk_set_tuning.argtypes = [i32, i32]
@@ -280,6 +290,11 @@ def lib():
     L.cvk_choose.argtypes = [P, Pa, W, i32, u32, i64, i32, i32, i32, vp]
     L.cvk_agent_pass.argtypes = [P, Pa, W, i32, vp]
     L.cvk_resample_layer.argtypes = [Ly, i32, Pa, i32, vp]
+    L.cvk_resample_frac_work_bytes.restype = i64
+    L.cvk_sources_work_bytes.argtypes = [i64]
+    L.cvk_sources_work_bytes.restype = i64
+    L.cvk_sources_build.argtypes = [Ly, i64, vp, vp, vp, i64, C.POINTER(C.c_int64), vp, i64, vp]
+    L.cvk_resample_layer_frac.argtypes = [Ly, i32, Pa, i32, i64, vp, i64, vp]
     L.cvk_edge_pass.argtypes = [P, Ly, Pa, W, i32, C.POINTER(C.c_void_p), vp]
     L.cvk_infect_pass.argtypes = [P, Ly, Pa, W, i32, i32, vp]
     L.cvk_count_pass.argtypes = [P, Pa, W, i32, vp]
@@ -289,6 +304,8 @@ def lib():
     L.cvk_graph_launch.argtypes = [vp, vp]
     L.cvk_graph_destroy.argtypes = [vp]
     BD, BI, BC = C.POINTER(BatchDesc), C.POINTER(BatchImports), C.POINTER(BatchConfig)
+    L.cvk_batch_time_edges.argtypes = [vp, i32]
+    L.cvk_batch_edge_times.argtypes = [vp, C.POINTER(C.c_float), i32]
     L.cvk_batch_create.argtypes = [BD, i32, i64, C.POINTER(C.c_void_p)]
     L.cvk_batch_destroy.argtypes = [vp]
     L.cvk_batch_set_config.argtypes = [vp, BC]

@@ -63,24 +63,43 @@ class Layer:
         if 'beta' not in data:
             self['beta'] = np.ones(len(self), dtype=np.float32)
 
+    def _columns(self):
+        """The device columns; created empty when a subclass's __init__ filled the layer item by item
+        without calling Layer.__init__ (as examples/t11_dynamic_layers.py:12-17 does)."""
+        d = self.__dict__.get('_d')
+        if d is None:
+            import torch
+            self.__dict__.setdefault('label', None)
+            self.__dict__.setdefault('device', torch.device('cuda'))
+            self.__dict__.setdefault('n_agents', None)
+            d = {k: torch.empty(0, dtype=torch.int32 if k != 'beta' else torch.float32, device=self.device)
+                 for k in self.meta}
+            self.__dict__['_d'] = d
+        return d
+
     # -- dict-like interface --
     def keys(self):
         return list(self.meta.keys())
 
+    def items(self):
+        return [(k, self[k]) for k in self.meta]
+
     def __len__(self):
-        return int(self._d['p1'].numel())
+        return int(self._columns()['p1'].numel())
 
     def __contains__(self, item):
         p1, p2 = self['p1'], self['p2']
         return bool((item == p1).any() or (item == p2).any())
 
     def __getitem__(self, key):
-        if key not in self._d:
+        d = self._columns()
+        if key not in d:
             raise KeyError(key)
-        return self._d[key].cpu().numpy()
+        return d[key].cpu().numpy()
 
     def __setitem__(self, key, value):
         import torch
+        self._columns()
         if key not in self.meta:
             raise KeyError(f'Layer keys are {list(self.meta)}, not "{key}"')
         if isinstance(value, torch.Tensor):
@@ -95,11 +114,11 @@ class Layer:
 
     def tensor(self, key):
         """The device tensor backing a column."""
-        return self._d[key]
+        return self._columns()[key]
 
     def beta_all_ones(self):
         """True when every edge weight is exactly 1.0 (checked on the device, cached per column tensor)."""
-        t = self._d['beta']
+        t = self._columns()['beta']
         key = (t.data_ptr(), t.numel(), t._version)
         if getattr(self, '_ones_key', None) != key:
             self._ones = bool((t == 1.0).all()) if t.numel() else True
@@ -110,13 +129,13 @@ class Layer:
         """Equal lengths for every column (base.py:1716-1736)."""
         n = len(self)
         for k in self.meta:
-            if self._d[k].numel() != n:
-                raise TypeError(f'Expecting length {n} for layer key "{k}"; got {self._d[k].numel()}')
+            if self._columns()[k].numel() != n:
+                raise TypeError(f'Expecting length {n} for layer key "{k}"; got {self._columns()[k].numel()}')
 
     def set_n_agents(self, n):
         self.n_agents = int(n)
         for k in ('p1', 'p2'):
-            t = self._d[k]
+            t = self._columns()[k]
             if t.numel() and (int(t.min()) < 0 or int(t.max()) >= n):
                 raise ValueError(f'Layer "{self.label}" column {k} has indices outside [0, {n})')
 
@@ -147,15 +166,26 @@ class Layer:
         return out
 
     def update(self, people, frac=1.0):
-        """Regenerate the layer's edges (base.py:1849-1876). The default resample runs on the GPU."""
+        """Regenerate the layer's edges (base.py:1849-1876) on the GPU: round(n_edges * frac) edges,
+        a uniform subset (all of them for frac = 1), get new endpoints drawn uniformly with
+        replacement and weight 1."""
+        import torch
         from . import _lib
-        if frac != 1.0:
-            raise NotImplementedError('only frac=1 (full resample) is supported on the device path')
         L = _lib.lib()
-        s = _lib.LayerS(_lib.ptr(self._d['p1']).value, _lib.ptr(self._d['p2']).value,
-                        _lib.ptr(self._d['beta']).value, len(self), 1, 0)
-        _lib.check(L.cvk_resample_layer(s, int(people._layer_index(self.label)), people._params, int(people.t),
-                                        _lib.stream_ptr()), 'Layer.update')
+        n = len(self)
+        k = int(np.round(n * frac))  # base.py:1868
+        if not 0 <= k <= n:
+            raise ValueError(f'frac must be in [0, 1], not {frac}')
+        s = _lib.LayerS(_lib.ptr(self._columns()['p1']).value, _lib.ptr(self._columns()['p2']).value,
+                        _lib.ptr(self._columns()['beta']).value, n, 1, 0)
+        li, t = int(people._layer_index(self.label)), int(people.t)
+        if k == n:
+            _lib.check(L.cvk_resample_layer(s, li, people._params, t, _lib.stream_ptr()), 'Layer.update')
+        elif k > 0:
+            nb = int(L.cvk_resample_frac_work_bytes())
+            work = torch.empty(nb // 4, dtype=torch.int32, device=self.device)
+            _lib.check(L.cvk_resample_layer_frac(s, li, people._params, t, k, _lib.ptr(work), nb, _lib.stream_ptr()),
+                       'Layer.update')
 
     def __repr__(self):
         return f'Layer("{self.label}", n_edges={len(self)})'
@@ -163,6 +193,16 @@ class Layer:
 
 class Contacts(dict):
     """Ordered dict of layers (base.py:1509-1607)."""
+
+    def __setitem__(self, lkey, layer):
+        if isinstance(layer, Layer):  # a layer put in place after the sim was built (examples/t11_dynamic_layers.py:43)
+            layer._columns()
+            if layer.label is None:
+                layer.label = lkey
+            old = self.get(lkey)
+            if old is not None and layer.n_agents is None:
+                layer.n_agents = old.n_agents
+        super().__setitem__(lkey, layer)
 
     def __repr__(self):
         return 'Contacts(' + ', '.join(f'{k}: {len(v)}' for k, v in self.items()) + ')'

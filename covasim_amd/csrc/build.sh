@@ -10,7 +10,7 @@ src_hash="0x$(cat "${here}"/*.hip "${here}"/*.h "${here}/../../include/covasim_a
   -ffp-contract=off -fhip-fp32-correctly-rounded-divide-sqrt -fno-gpu-rdc \
   -Xclang -target-feature -Xclang -packed-fp32-ops \
   -Wall -Wno-unused-function -DCVK_SOURCE_HASH="${src_hash}" \
-  "${here}/cvk_mirror.hip" "${here}/cvk_day.hip" "${here}/cvk_batch.hip" "${here}/cvk_relax.hip" "${here}/cvk_popgen.hip" "${here}/cvk_results.hip" "${here}/cvk_clip.hip" "${here}/cvk_diag.hip" \
+  "${here}/cvk_mirror.hip" "${here}/cvk_day.hip" "${here}/cvk_batch.hip" "${here}/cvk_relax.hip" "${here}/cvk_popgen.hip" "${here}/cvk_results.hip" "${here}/cvk_clip.hip" "${here}/cvk_diag.hip" "${here}/cvk_sources.hip" \
   -o "${out}.tmp"
 mv "${out}.tmp" "${out}"
 echo "built ${out}"

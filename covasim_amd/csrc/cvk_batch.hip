@@ -139,14 +139,16 @@ struct BatchEdgeArgs {
   int64_t n_tiles;
   int32_t lds_words;           // bitmap words the LDS stage holds (larger sims read L2)
   int32_t t;
+  uint32_t gen_mask;           // bit l: layer l's endpoints of day t come from the resample stream (GEN)
   BatchReduce reduce;
 };
 
 // Workgroup b owns the wave tiles [T*b/G, T*(b+1)/G) of the day, ordered by (sim, layer). It walks
 // them one sim at a time: the sim's infectious bitmap is staged in LDS (when it fits), then the
 // block's waves take the sim's tiles from an LDS counter, as in the single-sim pass. A wave's exact
-// queue is drained when its tiles change segment and when the sim is done.
-template <int BLOCK, int K>
+// queue is drained when its tiles change segment and when the sim is done. GEN: the dynamic layers'
+// endpoints are computed from the day's resample stream instead of read (a.gen_mask).
+template <int BLOCK, int K, bool GEN>
 __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchEdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
@@ -175,6 +177,7 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
     uint32_t n_x = 0;
     int32_t q_seg = -1;
     SegView L = uniform_copy(a.segs[a.tile_seg[tile0]]);  // segment of the wave's tiles and queued tests
+    if (GEN) L.gen = (int32_t)((a.gen_mask >> L.l) & 1u);
     for (;;) {
       uint32_t k = 0;
       if (lane_id() == 0) k = atomicAdd(&s_next, 1u);
@@ -190,8 +193,9 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
         n_x = 0;
         q_seg = seg;
         L = uniform_copy(a.segs[seg]);
+        if (GEN) L.gen = (int32_t)((a.gen_mask >> L.l) & 1u);
       }
-      load_wave_edges<K>(S, L, a.t, T);
+      load_wave_edges<K, GEN>(S, L, a.t, T);
       if (in_lds) wave_tile_pass<BITS_LDS, K>(S, L, a.t, s_mem, Q, T, n_x);
       else wave_tile_pass<BITS_L2, K>(S, L, a.t, nullptr, Q, T, n_x);
     }
@@ -486,6 +490,7 @@ struct Batch {
   std::vector<BatchSim> sims;          // [S]
   cvk_batch_config cfg{1024, 4, 1, 2};
   int64_t n_tiles = 0, n_rs_chunks = 0;
+  int64_t dyn_edges = 0;  // edges of all sims' dynamic layers
   uint32_t agent_blocks = 0;
   int64_t part_stride = 0;
   int64_t bitmap_words = 0;
@@ -520,6 +525,10 @@ struct Batch {
   uint32_t* d_short = nullptr;
   int64_t* d_k = nullptr;        // [k_slots][S] counts for choose calls
   int64_t k_slots = 0;
+  // edge-pass timing (cvk_batch_time_edges): an event pair per day around the edge launch, eager runs only
+  bool time_edges = false;
+  std::vector<hipEvent_t> ev_begin, ev_end;
+  std::vector<uint8_t> ev_used;
 
   ~Batch() {
     void* ptrs[] = {d_sims, d_params, d_segs, d_views, d_a_off, d_seg_tile_start, d_tile_seg, d_block_sim,
@@ -528,6 +537,8 @@ struct Batch {
                     d_all, d_short, d_k};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
+    for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_end) (void)hipEventDestroy(e);
   }
 
   BatchCommon common() const {
@@ -616,12 +627,17 @@ static int edge_lds_budget(int max_lds, int64_t* budget) {
   int dev = 0;
   (void)hipGetDevice(&dev);
   if (configured != dev) {
-    hipFuncAttributes attr;
-    const void* fn = (const void*)batch_edge_kernel<BLOCK, CHUNKS>;
-    if (hipFuncGetAttributes(&attr, fn) != hipSuccess) return fail(CVK_E_HIP, "cannot query the batch edge pass");
-    const int64_t b = (int64_t)max_lds - (int64_t)attr.sharedSizeBytes;
-    if (b <= 0 || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess)
-      return fail(CVK_E_HIP, "cannot raise the LDS limit of the batch edge pass");
+    const void* fns[2] = {(const void*)batch_edge_kernel<BLOCK, CHUNKS, false>,
+                          (const void*)batch_edge_kernel<BLOCK, CHUNKS, true>};
+    int64_t b = max_lds;
+    for (const void* fn : fns) {  // both instantiations get the same dynamic LDS limit
+      hipFuncAttributes attr;
+      if (hipFuncGetAttributes(&attr, fn) != hipSuccess) return fail(CVK_E_HIP, "cannot query the batch edge pass");
+      b = std::min<int64_t>(b, (int64_t)max_lds - (int64_t)attr.sharedSizeBytes);
+    }
+    for (const void* fn : fns)
+      if (b <= 0 || hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)b) != hipSuccess)
+        return fail(CVK_E_HIP, "cannot raise the LDS limit of the batch edge pass");
     value = b;
     configured = dev;
   }
@@ -661,7 +677,7 @@ static int launch_day_begin(Batch& b, int32_t t, int prologue, int transitions, 
   return check_launch("cvk_batch day_begin");
 }
 
-static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t st) {
+static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t st, bool gen) {
   int64_t budget = 0;
   int rc = edge_budget_for(b, &budget);
   if (rc) return rc;
@@ -681,23 +697,34 @@ static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t 
   a.lds_words = (int32_t)lds_words;
   a.t = t;
   a.reduce = red;
+  if (gen)
+    for (int l = 0; l < b.L; ++l)
+      if (b.dynamic[l]) a.gen_mask |= 1u << l;
   const size_t shmem = (size_t)(lds_words * 4 + queue_bytes);
   const dim3 grid((unsigned)std::max(1, b.n_cu * c.edge_blocks_per_cu));
   const BatchCommon C = b.common();
-  if (c.edge_block == 1024 && c.edge_chunks == 4)
-    hipLaunchKernelGGL((batch_edge_kernel<1024, 4>), grid, dim3(1024), shmem, st, C, a);
-  else if (c.edge_block == 1024)
-    hipLaunchKernelGGL((batch_edge_kernel<1024, 2>), grid, dim3(1024), shmem, st, C, a);
-  else if (c.edge_chunks == 4)
-    hipLaunchKernelGGL((batch_edge_kernel<512, 4>), grid, dim3(512), shmem, st, C, a);
-  else
-    hipLaunchKernelGGL((batch_edge_kernel<512, 2>), grid, dim3(512), shmem, st, C, a);
+  const bool g = a.gen_mask != 0;
+  if (c.edge_block == 1024 && c.edge_chunks == 4) {
+    if (g) hipLaunchKernelGGL((batch_edge_kernel<1024, 4, true>), grid, dim3(1024), shmem, st, C, a);
+    else hipLaunchKernelGGL((batch_edge_kernel<1024, 4, false>), grid, dim3(1024), shmem, st, C, a);
+  } else if (c.edge_block == 1024) {
+    if (g) hipLaunchKernelGGL((batch_edge_kernel<1024, 2, true>), grid, dim3(1024), shmem, st, C, a);
+    else hipLaunchKernelGGL((batch_edge_kernel<1024, 2, false>), grid, dim3(1024), shmem, st, C, a);
+  } else if (c.edge_chunks == 4) {
+    if (g) hipLaunchKernelGGL((batch_edge_kernel<512, 4, true>), grid, dim3(512), shmem, st, C, a);
+    else hipLaunchKernelGGL((batch_edge_kernel<512, 4, false>), grid, dim3(512), shmem, st, C, a);
+  } else {
+    if (g) hipLaunchKernelGGL((batch_edge_kernel<512, 2, true>), grid, dim3(512), shmem, st, C, a);
+    else hipLaunchKernelGGL((batch_edge_kernel<512, 2, false>), grid, dim3(512), shmem, st, C, a);
+  }
   return check_launch("cvk_batch edge_pass");
 }
 
 // resample_t >= 0: the launch also resamples the dynamic layers for day resample_t (extra workgroups),
-// and the sources of dynamic-layer infections come from the resample stream.
-static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st, int32_t resample_t = -1) {
+// and the sources of dynamic-layer infections come from the resample stream; gen: they come from it
+// because the day's dynamic contacts were generated, not written.
+static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st, int32_t resample_t = -1,
+                         bool gen = false) {
   BatchInfectArgs ia;
   memset(&ia, 0, sizeof(ia));
   ia.a_off = b.d_a_off;
@@ -719,6 +746,9 @@ static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st, i
       if (b.dynamic[l]) ia.gen_mask |= 1u << l;
     rs_blocks = (uint32_t)std::min<int64_t>(b.n_rs_chunks, 8 * b.n_cu);
   }
+  if (gen)
+    for (int l = 0; l < b.L; ++l)
+      if (b.dynamic[l]) ia.gen_mask |= 1u << l;
   hipLaunchKernelGGL(batch_infect_kernel, dim3(ia.infect_blocks + rs_blocks), dim3(INFECT_BLOCK), 0, st, b.common(),
                      ia);
   return check_launch("cvk_batch infect_pass");
@@ -781,10 +811,15 @@ static int upload_counts(Batch& b, const int64_t* pop_infected, const cvk_batch_
   return upload(b.d_k, k.data(), slots * b.S);
 }
 
-static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk_batch_imports* imp, hipStream_t st) {
+static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk_batch_imports* imp, hipStream_t st,
+                         bool captured = false) {
   int rc;
-  // as in the single-sim loop: day t+1's resample runs as extra workgroups of day t's infect pass
-  const bool overlap = overlap_resample_enabled() && b.n_rs_chunks > 0 && t1 - t0 > 1;
+  const bool timed = b.time_edges && !captured;
+  // as in the single-sim loop: the dynamic layers' endpoints are generated inside the edge and infect
+  // passes and written once, with the last day's contacts, when the run closes (gen); otherwise day
+  // t+1's resample runs as extra workgroups of day t's infect pass (overlap)
+  const bool gen = b.n_rs_chunks > 0 && generate_dynamic_for(b.dyn_edges, false);
+  const bool overlap = !gen && overlap_resample_enabled() && b.n_rs_chunks > 0 && t1 - t0 > 1;
   for (int32_t t = t0; t < t1; ++t) {
     if (t == 0 && seeds) {
       rc = launch_choose(b, 0, 0, STREAM_CHOOSE, 0, CVK_LAYER_SEED, 0, st);
@@ -792,7 +827,7 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       rc = launch_infect(b, 0, 0, st);
       if (rc) return rc;
     }
-    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, t == t0 ? 2 : (overlap ? 0 : 1), st);  // weights: first resample
+    rc = launch_day_begin(b, t, t > t0 ? 1 : 0, 1, gen ? 0 : (t == t0 ? 2 : (overlap ? 0 : 1)), st);  // 2: weights too
     if (rc) return rc;
     if (imp) {
       for (int32_t j = 0; j < imp->n_events; ++j) {
@@ -803,12 +838,17 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       }
     }
     BatchReduce red{1, t, t > t0 ? 1 : 0, 1};
-    rc = launch_edge(b, t, red, st);
+    if (timed && hipEventRecord(b.ev_begin[t], st) != hipSuccess) return fail(CVK_E_HIP, "cvk_batch: event record failed");
+    rc = launch_edge(b, t, red, st, gen);
     if (rc) return rc;
-    rc = launch_infect(b, t, 1, st, overlap && t + 1 < t1 ? t + 1 : -1);
+    if (timed) {
+      if (hipEventRecord(b.ev_end[t], st) != hipSuccess) return fail(CVK_E_HIP, "cvk_batch: event record failed");
+      b.ev_used[t] = 1;
+    }
+    rc = launch_infect(b, t, 1, st, overlap && t + 1 < t1 ? t + 1 : -1, gen);
     if (rc) return rc;
   }
-  rc = launch_day_begin(b, t1 - 1, 0, 0, 0, st);
+  rc = launch_day_begin(b, t1 - 1, 0, 0, gen ? 2 : 0, st);  // generated: the last day's contacts, weights 1
   if (rc) return rc;
   BatchReduce red{1, t1 - 1, 0, 0};
   hipLaunchKernelGGL(batch_reduce_kernel, dim3(std::min(b.S, 8 * b.n_cu)), dim3(1024), 0, st, b.common(), red);
@@ -907,6 +947,7 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
       G.quar_factor = b->params[s].quar_factor[l];
       G.beta_ones = (d->beta_ones[l] || b->dynamic[l]) ? 1 : 0;
       G.l = l;
+      if (b->dynamic[l]) b->dyn_edges += G.n;
       if (b->dynamic[l])
         for (int64_t e = 0; e < G.n; e += RS_CHUNK) {
           rs_seg.push_back(s * b->L + l);
@@ -1039,7 +1080,7 @@ extern "C" int cvk_batch_graph_create(void* handle, int32_t t0, int32_t t1, cons
   if (rc) return rc;
   if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return fail(CVK_E_HIP, "cvk_batch_graph_create: hipStreamBeginCapture failed");
-  rc = run_days_impl(b, t0, t1, seeds_any(b, pop_infected_host), imports, st);
+  rc = run_days_impl(b, t0, t1, seeds_any(b, pop_infected_host), imports, st, true);
   hipGraph_t graph = nullptr;
   const hipError_t ec = hipStreamEndCapture(st, &graph);
   if (rc) {
@@ -1111,4 +1152,32 @@ extern "C" int cvk_batch_copy_log(void* handle, int32_t* log_host, int64_t capac
   }
   if (written > b.log_capacity) return fail(CVK_E_CAPACITY, "cvk_batch_copy_log: the log overflowed its capacity");
   return CVK_OK;
+}
+
+extern "C" int cvk_batch_time_edges(void* handle, int32_t enable) {
+  if (!handle) return fail(CVK_E_INVALID, "cvk_batch_time_edges: null handle");
+  Batch& b = *(Batch*)handle;
+  if (enable && b.ev_begin.empty()) {
+    b.ev_begin.assign(b.npts, nullptr);
+    b.ev_end.assign(b.npts, nullptr);
+    for (int32_t t = 0; t < b.npts; ++t)
+      if (hipEventCreate(&b.ev_begin[t]) != hipSuccess || hipEventCreate(&b.ev_end[t]) != hipSuccess)
+        return fail(CVK_E_HIP, "cvk_batch_time_edges: hipEventCreate failed");
+  }
+  b.ev_used.assign(b.npts, 0);
+  b.time_edges = enable != 0;
+  return CVK_OK;
+}
+
+extern "C" int cvk_batch_edge_times(void* handle, float* ms_out, int32_t n) {
+  if (!handle || !ms_out) return fail(CVK_E_INVALID, "cvk_batch_edge_times: null argument");
+  Batch& b = *(Batch*)handle;
+  if (hipDeviceSynchronize() != hipSuccess) return fail(CVK_E_HIP, "cvk_batch_edge_times: device error");
+  const int32_t m = std::min<int32_t>(n, (int32_t)b.ev_used.size());
+  for (int32_t t = 0; t < m; ++t) {
+    ms_out[t] = -1.0f;
+    if (b.ev_used[t] && hipEventElapsedTime(&ms_out[t], b.ev_begin[t], b.ev_end[t]) != hipSuccess)
+      return fail(CVK_E_HIP, "cvk_batch_edge_times: hipEventElapsedTime failed");
+  }
+  return m;
 }

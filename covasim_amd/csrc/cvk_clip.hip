@@ -1,5 +1,7 @@
 // Edge moves of clip_edges (covasim/interventions.py:589-673): k of a layer's n edges, chosen
 // uniformly without replacement, leave it for another layer (the intervention's store, or back).
+// The same choice drives the partial resample of a dynamic layer (Layer.update with frac < 1,
+// covasim/base.py:1866-1875): the chosen edges get new iid uniform endpoints and weight 1.
 //
 // Choice: edge e gets the 64-bit key (Philox word << 32) | e, distinct for every edge; the moved
 // edges are those whose key is at most the k-th smallest. The k-th smallest key comes from an 8-pass
@@ -12,6 +14,7 @@
 
 #include "cvk_device.h"
 #include "cvk_host.h"
+#include "cvk_kernels.h"
 #include "cvk_scan.h"
 
 namespace cvk {
@@ -19,13 +22,14 @@ namespace {
 
 constexpr int CLIP_BLOCK = 256;
 
-__device__ inline uint64_t edge_key(uint64_t seed, int32_t t, uint32_t slot, int64_t e) {
-  const u32x4 w = philox_seeded(seed, (uint32_t)e, (uint32_t)t, STREAM_CLIP | slot, (uint32_t)((uint64_t)e >> 32));
+// stream_id: STREAM_CLIP | slot (clip_edges) or STREAM_PARTIAL | layer (partial resample)
+__device__ inline uint64_t edge_key(uint64_t seed, int32_t t, uint32_t stream_id, int64_t e) {
+  const u32x4 w = philox_seeded(seed, (uint32_t)e, (uint32_t)t, stream_id, (uint32_t)((uint64_t)e >> 32));
   return ((uint64_t)w.x << 32) | (uint32_t)e;
 }
 
 // state: [0] digits found so far (prefix), [1] how many keys below the prefix's bucket remain to take
-__global__ __launch_bounds__(CLIP_BLOCK) void key_hist_kernel(uint64_t seed, int32_t t, uint32_t slot, int64_t n,
+__global__ __launch_bounds__(CLIP_BLOCK) void key_hist_kernel(uint64_t seed, int32_t t, uint32_t stream_id, int64_t n,
                                                               int pass, const uint64_t* state, uint32_t* hist) {
   __shared__ uint32_t s_hist[256];
   s_hist[threadIdx.x] = 0;
@@ -34,7 +38,7 @@ __global__ __launch_bounds__(CLIP_BLOCK) void key_hist_kernel(uint64_t seed, int
   const uint64_t prefix = state[0];
   const uint64_t high_mask = pass == 0 ? 0ull : ~0ull << (64 - 8 * pass);
   for (int64_t e = (int64_t)blockIdx.x * CLIP_BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * CLIP_BLOCK) {
-    const uint64_t key = edge_key(seed, t, slot, e);
+    const uint64_t key = edge_key(seed, t, stream_id, e);
     if ((key & high_mask) == prefix) atomicAdd(&s_hist[(key >> shift) & 255u], 1u);
   }
   __syncthreads();
@@ -71,10 +75,38 @@ __global__ void key_init_kernel(uint64_t* state, uint32_t* hist, uint64_t k) {
 struct Chosen {
   uint64_t seed;
   int32_t t;
-  uint32_t slot;
+  uint32_t stream_id;
   const uint64_t* kth;  // the k-th smallest key (all keys <= it are chosen); null: nothing chosen
-  __device__ int64_t operator()(int64_t e) const { return kth && edge_key(seed, t, slot, e) <= *kth ? 1 : 0; }
+  __device__ int64_t operator()(int64_t e) const { return kth && edge_key(seed, t, stream_id, e) <= *kth ? 1 : 0; }
 };
+
+// The k-th smallest key of edges [0, n) into state[0] (8-pass radix select; k >= 1).
+inline void select_kth(uint64_t seed, int32_t t, uint32_t stream_id, int64_t n, int64_t k, uint64_t* state,
+                       uint32_t* hist, hipStream_t st) {
+  hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, state, hist, (uint64_t)k);
+  const uint32_t grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((n + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
+  for (int pass = 0; pass < 8; ++pass) {
+    hipLaunchKernelGGL(key_hist_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, seed, t, stream_id, n, pass,
+                       (const uint64_t*)state, hist);
+    hipLaunchKernelGGL(key_select_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, pass, state, hist);
+  }
+}
+
+// Partial resample: every chosen edge (key <= *kth) gets the endpoints the full resample of day t
+// gives it (dynamic_endpoints) and weight 1; the other edges are left as they are.
+__global__ __launch_bounds__(CLIP_BLOCK) void partial_resample_kernel(uint64_t seed, int32_t t, uint32_t layer,
+                                                                      uint32_t n_agents, int64_t n,
+                                                                      const uint64_t* kth, int32_t* p1, int32_t* p2,
+                                                                      float* beta) {
+  const uint64_t thr = *kth;
+  for (int64_t e = (int64_t)blockIdx.x * CLIP_BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * CLIP_BLOCK) {
+    if (edge_key(seed, t, STREAM_PARTIAL | layer, e) > thr) continue;
+    const EdgeEnds d = dynamic_endpoints(seed, t, layer, n_agents, e);
+    p1[e] = d.p1;
+    p2[e] = d.p2;
+    beta[e] = 1.0f;
+  }
+}
 
 struct SplitEdges {
   const int32_t *p1, *p2;
@@ -124,16 +156,38 @@ extern "C" int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float*
   uint64_t* state = (uint64_t*)base;
   uint32_t* hist = (uint32_t*)(base + align256(16));
   int64_t* tiles = (int64_t*)(base + align256(16) + align256(256 * 4));
-  const uint32_t slot_u = (uint32_t)slot;
-  if (k > 0) {
-    hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, state, hist, (uint64_t)k);
-    const uint32_t grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((n + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
-    for (int pass = 0; pass < 8; ++pass) {
-      hipLaunchKernelGGL(key_hist_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, seed, t, slot_u, n, pass,
-                         (const uint64_t*)state, hist);
-      hipLaunchKernelGGL(key_select_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, pass, state, hist);
-    }
-  }
+  const uint32_t sid = STREAM_CLIP | (uint32_t)slot;
+  if (k > 0) select_kth(seed, t, sid, n, k, state, hist, st);
   SplitEdges split{p1, p2, beta, keep_p1, keep_p2, move_p1, move_p2, keep_beta, move_beta};
-  return scan(Chosen{seed, t, slot_u, k > 0 ? (const uint64_t*)state : nullptr}, split, n, tiles, nullptr, st);
+  return scan(Chosen{seed, t, sid, k > 0 ? (const uint64_t*)state : nullptr}, split, n, tiles, nullptr, st);
+}
+
+extern "C" int64_t cvk_resample_frac_work_bytes(void) { return align256(16) + align256(256 * 4); }
+
+extern "C" int cvk_resample_layer_frac(const cvk_layer* layer, int32_t layer_index, const cvk_params* params,
+                                       int32_t t, int64_t k, void* work, int64_t work_bytes, void* stream) {
+  if (!layer || !params || layer->n_edges < 0 || (layer->n_edges > 0 && (!layer->p1 || !layer->p2 || !layer->beta)))
+    return fail(CVK_E_INVALID, "cvk_resample_layer_frac: invalid layer");
+  if (layer_index < 0 || layer_index >= CVK_MAX_LAYERS)
+    return fail(CVK_E_INVALID, "cvk_resample_layer_frac: bad index");
+  if (params->n_agents <= 0 || params->n_agents >= (1ll << 31))
+    return fail(CVK_E_INVALID, "cvk_resample_layer_frac: bad n_agents");
+  if (k < 0 || k > layer->n_edges || layer->n_edges > (1ll << 32))
+    return fail(CVK_E_INVALID, "cvk_resample_layer_frac: need 0 <= k <= n_edges <= 2^32");
+  if (k == 0) return CVK_OK;
+  if (!work || ((uintptr_t)work & 255))
+    return fail(CVK_E_ALIGN, "cvk_resample_layer_frac: workspace must be 256-byte aligned");
+  if (work_bytes < cvk_resample_frac_work_bytes())
+    return fail(CVK_E_CAPACITY, "cvk_resample_layer_frac: workspace too small");
+  hipStream_t st = (hipStream_t)stream;
+  uint64_t* state = (uint64_t*)work;
+  uint32_t* hist = (uint32_t*)((char*)work + align256(16));
+  const uint32_t l = (uint32_t)layer_index;
+  select_kth(params->seed, t, STREAM_PARTIAL | l, layer->n_edges, k, state, hist, st);
+  const uint32_t grid =
+      (uint32_t)std::max<int64_t>(1, std::min<int64_t>((layer->n_edges + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
+  hipLaunchKernelGGL(partial_resample_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, params->seed, t, l,
+                     (uint32_t)params->n_agents, layer->n_edges, (const uint64_t*)state, layer->p1, layer->p2,
+                     layer->beta);
+  return check_launch("cvk_resample_layer_frac");
 }

@@ -51,7 +51,7 @@ struct DayBeginArgs {
   int32_t transitions;  // 0: count-only finish pass for day t
   int32_t n_resample;
   uint32_t agent_blocks;
-  uint32_t pad;
+  int32_t clock_slot;   // >= 0: the launch's start time goes to work.day_clock[clock_slot]
   ResampleLayer rs[CVK_MAX_LAYERS];
 };
 
@@ -60,6 +60,9 @@ static_assert(AGENT_BLOCK * AGENTS_PER_THREAD == 512, "one coarse-map word per a
 
 __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cvk_params prm, cvk_work W,
                                                                 DayBeginArgs a) {
+  // the day's start on the device's constant-rate wall clock (work.day_clock, optional)
+  if (blockIdx.x == 0 && threadIdx.x == 0 && W.day_clock && a.clock_slot >= 0)
+    W.day_clock[a.clock_slot] = (uint64_t)wall_clock64();
   if (blockIdx.x >= a.agent_blocks) {
     const int64_t stride = (int64_t)(gridDim.x - a.agent_blocks) * AGENT_BLOCK;
     const int64_t start = (int64_t)(blockIdx.x - a.agent_blocks) * AGENT_BLOCK + threadIdx.x;
@@ -208,10 +211,39 @@ __global__ __launch_bounds__(256) void choose_mark_kernel(cvk_people P, cvk_work
 constexpr int EDGE_BLOCK = 1024;
 constexpr int EDGE_WAVES = EDGE_BLOCK / 64;
 
+// Walk decision of the listed (source-list) layers, block-wide: walk (mask) unless `automatic` and the agent
+// pass set more than max_permille infectious bits per thousand agents; every workgroup sums the same
+// per-block counts (partial column DC_BITS of agent_blocks blocks), so all of them decide alike.
+template <int BLOCK>
+__device__ inline uint32_t walk_decision(const cvk_work& W, uint32_t mask, int32_t automatic, int32_t max_permille,
+                                         uint32_t agent_blocks, uint32_t n_agents) {
+  __shared__ uint32_t s_inf;
+  if (!mask || !automatic) return mask;  // uniform over the grid
+  if (threadIdx.x == 0) s_inf = 0;
+  __syncthreads();
+  uint32_t cnt = 0;
+  for (uint32_t b = threadIdx.x; b < agent_blocks; b += BLOCK) cnt += W.partials[(int64_t)DC_BITS * CVK_MAX_REDUCE_BLOCKS + b];
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd(&s_inf, cnt);
+  __syncthreads();
+  return (uint64_t)s_inf * 1000u > (uint64_t)max_permille * n_agents ? 0u : mask;
+}
+
 struct EdgeArgs {
   SimView sim;
   SegView seg[CVK_MAX_LAYERS];
-  int64_t wave_tile_start[CVK_MAX_LAYERS + 1];  // wave tiles (64 x 4 x K edges) per layer, prefix
+  // wave tiles (64 x 4 x K edges) in tile order: the listed layers (src_mask) first, then the others, each
+  // group in layer order; position p of that order is layer pos_layer[p], its tiles start at tile_start[p]
+  int64_t tile_start[CVK_MAX_LAYERS + 1];
+  int32_t pos_layer[CVK_MAX_LAYERS];
+  int64_t listed_tiles;  // tiles of the listed layers (tile_start[n_listed])
+  SrcView src[CVK_MAX_LAYERS];             // source lists of the layers in src_mask
+  int64_t item_start[CVK_MAX_LAYERS + 1];  // source work items (SRC_ITEM_AGENTS agents) per layer, prefix
+  int32_t count_src;                       // add the slots and lists walked to the ctl counters
+  uint32_t src_mask;     // bit l: layer l has source lists; on days they are walked its tiles are skipped
+  int32_t src_auto;      // walk_decision's rule (0: the lists are always walked)
+  int32_t src_max_permille;
+  uint32_t agent_blocks;
   int32_t n_layers;
   int32_t lds_words;  // words of the LDS map (fine bitmap or coarse map, a multiple of 4)
   int32_t t;
@@ -220,10 +252,11 @@ struct EdgeArgs {
   DayReduce reduce;  // day_begin partial rows to reduce in block 0 (n_blocks == 0: none)
 };
 
-// Workgroup b owns wave tiles [T*b/G, T*(b+1)/G) of the day (T tiles over all layers, G workgroups);
-// its waves take the next tile from an LDS counter, so no wave waits for another and the global
-// memory system sees no tile atomics. A wave's exact-test queue is drained when its tiles change layer
-// and at the end.
+// On days the source lists are walked (walk_decision), every wave first walks its source items (item i
+// goes to wave i mod the grid's working waves) and the listed layers' tiles are skipped. Then workgroup b
+// takes its share [T*b/G, T*(b+1)/G) of the (other) layers' tiles (G workgroups); its waves take the next
+// tile from an LDS counter, so no wave waits for another and the global memory system sees no tile
+// atomics. A wave's exact-test queue is drained when its work changes layer and at the end.
 // MODE: BITS_LDS / BITS_COARSE / BITS_L2 (the LDS map is a.lds_words words of the fine bitmap or of
 // the coarse map). GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other
 // instantiations do not carry that code. PF: the next tile's endpoint loads are issued before this
@@ -245,27 +278,53 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     uint4* dst = (uint4*)s_mem;
     for (int j = threadIdx.x; j < (nbw >> 2); j += EDGE_BLOCK) dst[j] = src[j];
   }
+  const bool walked = walk_decision<EDGE_BLOCK>(a.work, a.src_mask, a.src_auto, a.src_max_permille, a.agent_blocks,
+                                                a.sim.n_agents) != 0;
   __syncthreads();
   const uint32_t* bits = s_mem;
-  const int64_t n_wt = a.wave_tile_start[a.n_layers];
+  const int64_t n_wt = a.tile_start[a.n_layers];
   const uint32_t skip = reducing && gridDim.x > 1 ? 1u : 0u;
   const int64_t workers = (int64_t)(gridDim.x - skip);
   const int64_t wb = (int64_t)blockIdx.x - skip;
-  const int64_t first = wb < 0 ? 0 : n_wt * wb / workers, last = wb < 0 ? 0 : n_wt * (wb + 1) / workers;
+  const int64_t tl = a.listed_tiles, to = n_wt - tl;
+  // this workgroup's listed share [l0, l1) (empty on walked days) and other share [o0, o1)
+  const int64_t l0 = wb < 0 || walked ? 0 : tl * wb / workers, l1 = wb < 0 || walked ? 0 : tl * (wb + 1) / workers;
+  const int64_t o0 = wb < 0 ? 0 : tl + to * wb / workers, o1 = wb < 0 ? 0 : tl + to * (wb + 1) / workers;
+  const int64_t n_mine = (l1 - l0) + (o1 - o0);
   uint32_t n_x = 0;
-  int32_t q_seg = 0;
+  int32_t q_seg = a.pos_layer[0];
+  if (walked && wb >= 0) {  // source items, one wave at a time over the grid's working waves
+    uint64_t cnt[2] = {0, 0};
+    const int64_t n_items = a.item_start[a.n_layers];
+    const int64_t stride = workers * EDGE_WAVES;
+    for (int64_t it = wb * EDGE_WAVES + (threadIdx.x >> 6); it < n_items; it += stride) {
+      int l = 0;
+      while (l + 1 < a.n_layers && it >= a.item_start[l + 1]) ++l;
+      if (l != q_seg) {
+        if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
+        n_x = 0;
+        q_seg = l;
+      }
+      source_item_pass<MODE>(a.sim, a.seg[l], a.src[l], a.t, bits, Q, (it - a.item_start[l]) * (int64_t)SRC_ITEM_AGENTS,
+                             n_x, cnt);
+    }
+    if (a.count_src && lane_id() == 0 && (cnt[0] | cnt[1])) {
+      atomicAdd((unsigned long long*)(a.work.ctl + CVK_CTL_SRC_SLOTS), (unsigned long long)cnt[0]);
+      atomicAdd((unsigned long long*)(a.work.ctl + CVK_CTL_SRC_AGENTS), (unsigned long long)cnt[1]);
+    }
+  }
   // the wave's next tile from the block's counter: its layer and first edge (seg -1: none left)
   auto grab = [&](WaveTile<K>& T) {
     uint32_t k = 0;
     if (lane_id() == 0) k = atomicAdd(&s_next, 1u);
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
-    const int64_t tile = first + k;
     T.seg = -1;
-    if (tile >= last) return;
-    int l = 0;
-    while (l + 1 < a.n_layers && tile >= a.wave_tile_start[l + 1]) ++l;
-    T.seg = l;
-    T.e0 = (tile - a.wave_tile_start[l]) * (int64_t)(64 * 4 * K);
+    if ((int64_t)k >= n_mine) return;
+    const int64_t tile = (int64_t)k < l1 - l0 ? l0 + k : o0 + (k - (l1 - l0));
+    int p = 0;
+    while (p + 1 < a.n_layers && tile >= a.tile_start[p + 1]) ++p;
+    T.seg = a.pos_layer[p];
+    T.e0 = (tile - a.tile_start[p]) * (int64_t)(64 * 4 * K);
   };
   WaveTile<K> cur, nxt;
   grab(cur);
@@ -439,10 +498,17 @@ static int g_gen_dynamic = 2;  // tunable: the fused day loop computes dynamic l
 // Measured: 10M dynamic edges (hybrid 1M) 0.092 ms/day writing against 0.097 generating; 100M
 // (hybrid 10M) 1.70 against 1.48. Generation is chosen from 2^25 dynamic edges up.
 constexpr int64_t GEN_AUTO_MIN_EDGES = (int64_t)1 << 25;
+static int g_source_lists = 2;   // tunable: walk static layers through their source lists when given:
+                                 // 0 never, 1 always, 2 on days with few infectious agents (g_source_permille)
+static int g_source_permille = 30;  // tunable: the most infectious agents per thousand for which lists are walked
+static int g_count_sources = 0;  // tunable: count the slots and lists the edge pass walks (ctl words)
 static int g_overlap_resample = 1;  // tunable: the fused day loop resamples inside the previous infect pass
 static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
 static int g_edge_prefetch = 0;     // tunable: issue the next wave tile's endpoint loads before the candidates
 int overlap_resample_enabled() { return g_overlap_resample; }
+bool generate_dynamic_for(int64_t dyn_edges, bool diag) {
+  return !diag && dyn_edges > 0 && (g_gen_dynamic == 1 || (g_gen_dynamic == 2 && dyn_edges >= GEN_AUTO_MIN_EDGES));
+}
 
 struct DeviceCaps {
   int device = -1;
@@ -550,6 +616,7 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
   a.prologue = prologue;
   a.transitions = transitions;
   a.agent_blocks = agent_blocks_for(people->n);
+  a.clock_slot = transitions ? t : t + 1;  // a closing pass marks the end of day t
   uint32_t rs_blocks = 0;
   int rc = fill_resample(a, layers, params, resample, &rs_blocks);
   if (rc) return rc;
@@ -592,6 +659,17 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
     case CVK_TUNE_EDGE_PREFETCH:
       g_edge_prefetch = value ? 1 : 0;
       return CVK_OK;
+    case CVK_TUNE_SOURCE_LISTS:
+      if (value < 0 || value > 2) return fail(CVK_E_INVALID, "cvk_set_tuning: source lists must be 0, 1 or 2");
+      g_source_lists = value;
+      return CVK_OK;
+    case CVK_TUNE_SOURCE_PERMILLE:
+      if (value < 0 || value > 1000) return fail(CVK_E_INVALID, "cvk_set_tuning: permille must be in 0..1000");
+      g_source_permille = value;
+      return CVK_OK;
+    case CVK_TUNE_COUNT_SOURCES:
+      g_count_sources = value ? 1 : 0;
+      return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
   }
@@ -620,6 +698,18 @@ extern "C" int cvk_abi_layout(int64_t* out, int32_t n) {
   if (!out || n < m) return fail(CVK_E_INVALID, "cvk_abi_layout: output too small");
   for (int32_t j = 0; j < m; ++j) out[j] = vals[j];
   return m;
+}
+
+// Frequency of the device wall clock that stamps work.day_clock.
+extern "C" int cvk_wall_clock_rate(int64_t* hz_out) {
+  if (!hz_out) return fail(CVK_E_INVALID, "cvk_wall_clock_rate: null output");
+  DeviceCaps& c = caps();
+  if (c.device < 0) return fail(CVK_E_HIP, "cvk_wall_clock_rate: no HIP device");
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device) != hipSuccess || khz <= 0)
+    return fail(CVK_E_HIP, "cvk_wall_clock_rate: attribute query failed");
+  *hz_out = (int64_t)khz * 1000;
+  return CVK_OK;
 }
 
 // Device properties the launch configuration depends on: CUs, LDS bytes per block.
@@ -797,12 +887,23 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   else if (g_edge_coarse && work->coarse && coarse_words * 4 + queue_bytes <= budget) mode = BITS_COARSE;
   a.lds_words = (int32_t)(mode == BITS_LDS ? fine_words : mode == BITS_COARSE ? coarse_words : 0);
   const int64_t wave_tile = 64 * 4 * (int64_t)g_edge_chunks;
-  int64_t wave_tiles = 0;
+  int64_t src_items = 0;
   for (int l = 0; l < params->n_layers; ++l) {
     const cvk_layer& L = layers[l];
     if (L.n_edges < 0 || (L.n_edges > 0 && (!L.p1 || !L.p2 || !L.beta)))
       return fail(CVK_E_INVALID, "cvk_edge_pass: invalid layer");
     if (((uintptr_t)L.p1 | (uintptr_t)L.p2) & 15) return fail(CVK_E_ALIGN, "cvk_edge_pass: p1/p2 must be 16-byte aligned");
+    const bool listed = g_source_lists && L.sources && !L.dynamic && L.n_edges > 0;
+    if (listed && (!L.sources->indptr || (L.sources->n_slots > 0 && (!L.sources->target || !L.sources->edge_dir))))
+      return fail(CVK_E_INVALID, "cvk_edge_pass: invalid source lists");
+    a.item_start[l] = src_items;
+    if (listed) {
+      a.src_mask |= 1u << l;
+      a.src[l].indptr = L.sources->indptr;
+      a.src[l].target = L.sources->target;
+      a.src[l].edge_dir = L.sources->edge_dir;
+      src_items += (people->n + SRC_ITEM_AGENTS - 1) / SRC_ITEM_AGENTS;
+    }
     SegView& G = a.seg[l];
     G.p1 = L.p1;
     G.p2 = L.p2;
@@ -815,13 +916,32 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
     G.gen = generate && L.dynamic ? 1 : 0;
     G.beta_ones = L.beta_ones || G.gen;  // a resampled layer's weights are all 1
     G.l = l;
-    a.wave_tile_start[l] = wave_tiles;
-    wave_tiles += (L.n_edges + wave_tile - 1) / wave_tile;
   }
-  for (int l = params->n_layers; l <= CVK_MAX_LAYERS; ++l) a.wave_tile_start[l] = wave_tiles;
-  if (wave_tiles == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
-  // one workgroup per CU (the LDS bitmap takes most of a CU's LDS), at least one wave tile each
-  const int grid = (int)std::min<int64_t>((wave_tiles + EDGE_WAVES - 1) / EDGE_WAVES, (int64_t)std::max(c.n_cu, 1));
+  for (int l = params->n_layers; l <= CVK_MAX_LAYERS; ++l) a.item_start[l] = src_items;
+  a.count_src = g_count_sources;
+  // tile order: the listed layers (always walked: no tiles), then the others
+  const bool always = g_source_lists == 1;
+  int64_t wave_tiles = 0;
+  int p = 0;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int l = 0; l < params->n_layers; ++l) {
+      const bool listed = (a.src_mask >> l) & 1u;
+      if (listed != (pass == 0)) continue;
+      a.pos_layer[p] = l;
+      a.tile_start[p++] = wave_tiles;
+      if (!(listed && always)) wave_tiles += (layers[l].n_edges + wave_tile - 1) / wave_tile;
+    }
+  for (int q = p; q <= CVK_MAX_LAYERS; ++q) a.tile_start[q] = wave_tiles;
+  a.listed_tiles = 0;
+  for (int q = 0; q < p; ++q)
+    if ((a.src_mask >> a.pos_layer[q]) & 1u) a.listed_tiles = a.tile_start[q + 1];
+  a.src_auto = g_source_lists == 2 ? 1 : 0;
+  a.src_max_permille = g_source_permille;
+  a.agent_blocks = (uint32_t)agent_blocks_for(people->n);
+  const int64_t work_items = std::max(wave_tiles, src_items);
+  if (work_items == 0) return reduce ? day_reduce_launch(work, *reduce, st) : CVK_OK;
+  // one workgroup per CU (the LDS bitmap takes most of a CU's LDS), at least one work item each
+  const int grid = (int)std::min<int64_t>((work_items + EDGE_WAVES - 1) / EDGE_WAVES, (int64_t)std::max(c.n_cu, 1));
   const size_t shmem = (size_t)((int64_t)a.lds_words * 4 + queue_bytes);
   bool any_gen = false;
   for (int l = 0; l < params->n_layers; ++l) any_gen = any_gen || a.seg[l].gen;
@@ -899,8 +1019,7 @@ static void day_plan(const cvk_layer* layers, const cvk_params* params, const cv
   const bool diag = imports && imports->diag;
   int64_t dyn_edges = 0;
   for (int l = 0; l < params->n_layers; ++l) dyn_edges += layers[l].dynamic ? layers[l].n_edges : 0;
-  const bool gen_on = !diag && (g_gen_dynamic == 1 || (g_gen_dynamic == 2 && dyn_edges >= GEN_AUTO_MIN_EDGES));
-  *gen = gen_on && has_dynamic(layers, params);
+  *gen = generate_dynamic_for(dyn_edges, diag) && has_dynamic(layers, params);
   *overlap = !*gen && g_overlap_resample && n_days > 1 && has_dynamic(layers, params);
 }
 

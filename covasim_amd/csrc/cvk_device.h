@@ -41,6 +41,7 @@ constexpr uint32_t STREAM_POP_EDGE = 0x70000600u;
 constexpr uint32_t STREAM_POP_EDGE_RETRY = 0x70000700u;
 // Edge moves of clip_edges (cvk_clip.hip); the low byte is the layer slot.
 constexpr uint32_t STREAM_CLIP = 0x80000000u;
+constexpr uint32_t STREAM_PARTIAL = 0xB0000000u;  // | layer: edge choice of a partial resample (frac < 1)
 // Testing (| op slot) and contact tracing (| op slot << 8 | layer), cvk_diag.hip.
 constexpr uint32_t STREAM_TEST = 0x90000000u;
 constexpr uint32_t STREAM_TRACE = 0xA0000000u;

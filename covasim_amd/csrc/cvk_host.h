@@ -14,6 +14,9 @@ char* last_error_buffer();
 
 // CVK_TUNE_OVERLAP_RESAMPLE as set by cvk_set_tuning (read by the batched day loop too).
 int overlap_resample_enabled();
+// CVK_TUNE_GENERATE_DYNAMIC applied to a run with dyn_edges dynamic-layer edges in all (diag: the run
+// tests and traces, which reads each day's contacts, so they are written).
+bool generate_dynamic_for(int64_t dyn_edges, bool diag);
 
 // Testing and tracing step of the day loop (cvk_diag.hip): the ops of `imports`, then
 // update_states_post; nothing when imports->diag is null.

@@ -113,7 +113,10 @@ __device__ void sum_partials(const uint32_t* partials, const double* partials_f,
 // ===========================================================================
 // stock columns (0..29) and day columns (30..46) of the partial rows
 constexpr int SC_ALIVE = 13, SC_EXPV = 14, SC_INFV = 14 + MAXV, N_STOCK_COLS = 14 + 2 * MAXV;
-constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV, N_AGENT_COLS = N_STOCK_COLS + N_DAY_COLS;
+constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV;
+// column of the agents whose infectious bit this pass set (the edge pass's choice between walking
+// source lists and streaming reads it)
+constexpr int DC_BITS = N_STOCK_COLS + N_DAY_COLS, N_AGENT_COLS = DC_BITS + 1;
 constexpr int N_AGENT_FCOLS = 3;
 static_assert(N_AGENT_COLS <= CVK_PARTIAL_COLS, "partial row too small");
 
@@ -393,6 +396,7 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         if (v < nv) c[DC_BASE + 9 + v] += (uint32_t)__popcll(__ballot(ev_var == (uint32_t)v));
       // one 64-bit ballot per wave covers 64 consecutive agents
       const uint64_t ib = __ballot(inf_bit);
+      c[DC_BITS] += (uint32_t)__popcll(ib);
       const int64_t word = li >> 6;
       if (lane_id() == 0 && word < n_words) S.bitmap[word] = ib;
       // coarse bits of this ballot's four 16-agent groups, at their place in the block's 512 agents
@@ -753,6 +757,42 @@ __device__ inline void flush_exact(const SimView& S, const SegView& L, int32_t t
   wave_sync_lds();
 }
 
+// The slots of this round that passed the prefilter (m = ballot of pass) go to the wave's exact queue,
+// or are tested at once when more than the queue holds passed together. Wave-uniform call.
+__device__ inline void queue_exact(const SimView& S, const SegView& L, int32_t t, const WaveQ& Q, uint64_t m, bool pass,
+                                   int32_t src, int32_t tgt, int64_t e, uint32_t dir, uint32_t vq, double u,
+                                   uint32_t& n_x) {
+  const uint32_t k = (uint32_t)__popcll(m);
+  if (k > (uint32_t)EDGE_XQ) {
+    if (pass) exact_test(S, L, t, src, tgt, e, dir, vq, float_rz(u), Q.beta);
+    return;
+  }
+  if (n_x + k > (uint32_t)EDGE_XQ) {
+    flush_exact(S, L, t, Q, n_x);
+    n_x = 0;
+  }
+  if (pass) {
+    const uint32_t pos = n_x + lanes_below(m);
+    Q.xs[pos] = (uint32_t)src;
+    Q.xt[pos] = (uint32_t)tgt;
+    Q.xe[pos] = (uint32_t)e;
+    Q.xm[pos] = dir | (vq << 1) | ((uint32_t)((uint64_t)e >> 32) << 4);
+    Q.xu[pos] = __float_as_uint(float_rz(u));
+  }
+  n_x += k;
+}
+
+// The prefilter bound code of a candidate slot (source code bs, target code bt != 0, variant vq = bs >> 13):
+// p_ub rounded up, PCODE_INF for weighted layers and products that may have underflowed, 0 when the
+// variant's or the layer's beta is 0.
+__device__ inline uint32_t candidate_pcode(const SegView& L, const WaveQ& Q, uint32_t bs, uint32_t bt, uint32_t vq) {
+  if (!L.beta_ones) return PCODE_INF;  // weighted layers: per-edge beta is not in the bound
+  const float bv = Q.beta[vq];
+  const float p_ub = (((bv * L.beta_layer) * bound_decode(bs)) * bound_decode(bt)) * BOUND_MARGIN;
+  // p_ub == 0 with nonzero factors can only be underflow: leave such slots to the exact test
+  return p_ub >= 0x1p-100f ? pcode_up(p_ub) : ((bv == 0.0f || L.beta_layer == 0.0f) ? 0u : PCODE_INF);
+}
+
 // One round: the top min(64, n_c) stacked candidates draw their uniform; those below p_ub are queued
 // for the exact test (or tested at once when more than the queue holds pass together).
 template <int K>
@@ -790,24 +830,7 @@ __device__ inline void candidate_round(const SimView& S, const SegView& L, int32
     }
   }
   const int32_t src = dir ? yv : xv, tgt = dir ? xv : yv;
-  const uint32_t k = (uint32_t)__popcll(m);
-  if (k > (uint32_t)EDGE_XQ) {
-    if (pass) exact_test(S, L, t, src, tgt, e, dir, vq, float_rz(u), Q.beta);
-    return;
-  }
-  if (n_x + k > (uint32_t)EDGE_XQ) {
-    flush_exact(S, L, t, Q, n_x);
-    n_x = 0;
-  }
-  if (pass) {
-    const uint32_t pos = n_x + lanes_below(m);
-    Q.xs[pos] = (uint32_t)src;
-    Q.xt[pos] = (uint32_t)tgt;
-    Q.xe[pos] = (uint32_t)e;
-    Q.xm[pos] = dir | (vq << 1) | ((uint32_t)((uint64_t)e >> 32) << 4);
-    Q.xu[pos] = __float_as_uint(float_rz(u));
-  }
-  n_x += k;
+  queue_exact(S, L, t, Q, m, pass, src, tgt, e, dir, vq, u, n_x);
 }
 
 // Where the edge pass finds the infectious bits: the fine bitmap staged in LDS; a coarse map in LDS
@@ -905,13 +928,7 @@ __device__ inline void tile_candidates(const SimView& S, const SegView& L, int32
       const uint32_t vq = bs[j] >> 13;
       // a non-infectious target's code is its susceptibility bound, or 0 if it cannot be infected
       if (((mixed >> j) & 1u) && vq < (uint32_t)S.n_variants && bt[j] != 0u) {
-        uint32_t pc = PCODE_INF;  // weighted layers: per-edge beta is not in the bound
-        if (L.beta_ones) {
-          const float bv = Q.beta[vq];
-          const float p_ub = (((bv * L.beta_layer) * bound_decode(bs[j])) * bound_decode(bt[j])) * BOUND_MARGIN;
-          // p_ub == 0 with nonzero factors can only be underflow: leave such slots to the exact test
-          pc = p_ub >= 0x1p-100f ? pcode_up(p_ub) : ((bv == 0.0f || L.beta_layer == 0.0f) ? 0u : PCODE_INF);
-        }
+        const uint32_t pc = candidate_pcode(L, Q, bs[j], bt[j], vq);
         if (pc) {
           has[q] = true;
           const uint32_t dir = ((from_x >> j) & 1u) ? 0u : 1u;
@@ -937,6 +954,155 @@ __device__ inline void wave_tile_pass(const SimView& S, const SegView& L, int32_
   TileBounds<K> B;
   tile_bits_and_bounds<MODE, K>(S, lds_bits, T, B);
   tile_candidates<K>(S, L, t, Q, T, B, n_x);
+}
+
+// ---------------------------------------------------------------------------
+// Source lists of a static layer: the edge pass walks the lists of the day's infectious agents
+// instead of streaming the layer. Slot k of agent i's list [indptr[i], indptr[i + 1]) is one
+// direction of one edge: target[k], and edge_dir[k] = edge << 1 | dir (dir 0: i is p1). Every
+// (source, target, edge, direction) slot with an infectious source is offered exactly as the
+// streamed pass offers it (same prefilter, uniform and exact test), so the results are the same.
+// ---------------------------------------------------------------------------
+struct SrcView {
+  const int32_t* indptr;
+  const int32_t* target;
+  const uint32_t* edge_dir;
+};
+
+constexpr int SRC_ITEM_WORDS = 64;                    // bitmap words per source work item (one per lane)
+constexpr int SRC_ITEM_AGENTS = SRC_ITEM_WORDS * 32;  // agents per source work item
+constexpr int SRC_UNROLL = 4;                         // slots per lane whose loads are in flight together
+
+// The position of the j-th set bit (j < popcount) of x.
+__device__ inline uint32_t nth_set_bit(uint32_t x, uint32_t j) {
+  uint32_t pos = 0;
+#pragma unroll
+  for (uint32_t half = 16; half; half >>= 1) {
+    const uint32_t c = (uint32_t)__popc(x & ((1u << half) - 1u));
+    if (j >= c) {
+      j -= c;
+      x >>= half;
+      pos += half;
+    }
+  }
+  return pos;
+}
+
+// Source work item: agents [a0, a0 + SRC_ITEM_AGENTS) of static layer L; lds_bits: the fine bitmap in LDS
+// (MODE == BITS_LDS), otherwise it is read from global memory. Rounds of up to 64 infectious agents: lane k takes the (64 r + k)-th set bit of the
+// item's words (k-th-bit selection over the words' prefix counts), loads its list bounds and bound
+// code, then the round's slots are dealt over the lanes (each lane finds its slot's agent by a binary
+// search over the round's prefix sums in LDS), SRC_UNROLL slots per lane with their loads in flight
+// together. cnt: slots and lists walked (added to by lane 0).
+template <int MODE>
+__device__ inline void source_item_pass(const SimView& S, const SegView& L, const SrcView& R, int32_t t,
+                                        const uint32_t* lds_bits, const WaveQ& Q, int64_t a0, uint32_t& n_x,
+                                        uint64_t* cnt) {
+  const uint32_t lane = lane_id();
+  const int64_t n_words = ((int64_t)S.n_agents + 31) / 32;
+  const int64_t w = (a0 >> 5) + lane;
+  uint32_t bits = 0;
+  if (lane < (uint32_t)SRC_ITEM_WORDS && w < n_words) bits = MODE == BITS_LDS ? lds_bits[w] : gmem(S.bitmap)[w];
+  if (w == n_words - 1 && (S.n_agents & 31u)) bits &= (1u << (S.n_agents & 31u)) - 1u;  // agents < n only
+  const uint32_t wc = (uint32_t)__popc(bits);
+  uint32_t wincl = wc;  // inclusive scan of the words' counts
+#pragma unroll
+  for (int off = 1; off < SRC_ITEM_WORDS; off <<= 1) {
+    const uint32_t y = __shfl_up(wincl, off, 64);
+    if ((int)lane >= off) wincl += y;
+  }
+  const uint32_t n_agents = __shfl(wincl, SRC_ITEM_WORDS - 1, 64);
+  if (n_agents == 0) return;  // wave-uniform
+  uint32_t* s_off = Q.cand;         // [64] exclusive slot offsets of the round's agents
+  uint32_t* s_first = Q.cand + 64;  // [64] first slot of each agent's list
+  uint32_t* s_src = Q.cand + 128;   // [64] the agent
+  uint32_t* s_bs = Q.cand + 192;    // [64] its bound code
+  uint32_t* s_wex = Q.cand + 256;   // [SRC_ITEM_WORDS] exclusive prefix of the words' counts
+  wave_sync_lds();
+  if (lane < (uint32_t)SRC_ITEM_WORDS) s_wex[lane] = wincl - wc;
+  wave_sync_lds();
+  const CVK_GLOBAL int32_t* indptr = gmem(R.indptr);
+  const CVK_GLOBAL int32_t* target = gmem(R.target);
+  const CVK_GLOBAL uint32_t* edge_dir = gmem(R.edge_dir);
+  const CVK_GLOBAL uint16_t* bound = gmem(S.bound);
+  for (uint32_t r0 = 0; r0 < n_agents; r0 += 64) {  // wave-uniform
+    const uint32_t g = r0 + lane;
+    uint32_t wd = 0;  // the last word with s_wex <= g (zero-count words share their successor's prefix)
+#pragma unroll
+    for (uint32_t step = SRC_ITEM_WORDS / 2; step; step >>= 1)
+      if (s_wex[wd + step] <= g) wd += step;
+    const uint32_t word_bits = (uint32_t)__shfl(bits, (int)wd, 64);  // every lane active
+    int32_t a = -1;
+    if (g < n_agents) a = (int32_t)(a0 + (int64_t)wd * 32 + nth_set_bit(word_bits, g - s_wex[wd]));
+    uint32_t first = 0, deg = 0, bs = 0;
+    if (a >= 0) {
+      first = (uint32_t)indptr[a];
+      deg = (uint32_t)indptr[a + 1] - first;
+      bs = bound[a];
+    }
+    uint32_t incl = deg;  // inclusive scan of the degrees over the wave
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_up(incl, off, 64);
+      if ((int)lane >= off) incl += y;
+    }
+    const uint32_t total = __shfl(incl, 63, 64);
+    if (lane == 0) {
+      cnt[0] += total;
+      cnt[1] += (uint32_t)__popcll(__ballot(a >= 0));
+    }
+    wave_sync_lds();
+    s_off[lane] = incl - deg;
+    s_first[lane] = first;
+    s_src[lane] = (uint32_t)a;
+    s_bs[lane] = bs;
+    wave_sync_lds();
+    for (uint32_t base = 0; base < total; base += 64 * SRC_UNROLL) {  // wave-uniform
+      uint32_t slot[SRC_UNROLL], bsj[SRC_UNROLL];
+      int32_t srcj[SRC_UNROLL], tgt[SRC_UNROLL];
+      uint32_t ed[SRC_UNROLL], bt[SRC_UNROLL];
+#pragma unroll
+      for (int j = 0; j < SRC_UNROLL; ++j) {
+        const uint32_t sidx = base + 64 * j + lane;
+        uint32_t lo = 0;  // the last lane with s_off <= sidx (zero-degree lanes share offsets)
+#pragma unroll
+        for (uint32_t step = 32; step; step >>= 1)
+          if (s_off[lo + step] <= sidx) lo += step;
+        slot[j] = sidx < total ? s_first[lo] + (sidx - s_off[lo]) : 0xFFFFFFFFu;
+        srcj[j] = (int32_t)s_src[lo];
+        bsj[j] = s_bs[lo];
+      }
+#pragma unroll
+      for (int j = 0; j < SRC_UNROLL; ++j) {
+        tgt[j] = 0;
+        ed[j] = 0;
+        if (slot[j] != 0xFFFFFFFFu) {
+          tgt[j] = target[slot[j]];
+          ed[j] = edge_dir[slot[j]];
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < SRC_UNROLL; ++j) bt[j] = slot[j] != 0xFFFFFFFFu ? bound[tgt[j]] : 0u;
+#pragma unroll
+      for (int j = 0; j < SRC_UNROLL; ++j) {
+        if (base + 64 * j >= total) break;  // wave-uniform
+        const int64_t e = (int64_t)(ed[j] >> 1);
+        const uint32_t dir = ed[j] & 1u;
+        const uint32_t vq = bsj[j] >> 13;
+        bool pass = false;
+        double u = 1.0;
+        if (slot[j] != 0xFFFFFFFFu && vq < (uint32_t)S.n_variants && bt[j] != 0u) {
+          const uint32_t pc = candidate_pcode(L, Q, bsj[j], bt[j], vq);
+          if (pc) {
+            u = edge_uniform(S, L, t, vq, (int)dir, e);
+            pass = u < (double)pcode_decode(pc);
+          }
+        }
+        const uint64_t m = __ballot(pass);
+        if (m) queue_exact(S, L, t, Q, m, pass, srcj[j], tgt[j], e, dir, vq, u, n_x);
+      }
+    }
+  }
 }
 
 // ===========================================================================

@@ -40,7 +40,8 @@ namespace cvk {
 constexpr int RELAX_BLOCK = 256;
 constexpr int RELAX_BINS = 10;
 constexpr int RELAX_MAX_PARAMS = 1 + CVK_MAX_LAYERS + RELAX_BINS;
-constexpr double RELAX_XMAX = 0.999999;  // per-edge probabilities are capped below 1 (log1p(-x) stays finite)
+constexpr double RELAX_XMAX = 0.999999;
+constexpr int RELAX_UNROLL = 8;  // neighbour-list entries whose gathers are in flight together  // per-edge probabilities are capped below 1 (log1p(-x) stays finite)
 
 template <class R>
 struct RelaxTheta {
@@ -114,11 +115,24 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_forward_kernel(RelaxDev<R> 
     R L = 0;
     for (int l = 0; l < D.n_layers; ++l) {
       const R bb = (th.beta * th.bl[l]) * sig;
+      const int32_t* nbr = D.nbr[l];
       const int64_t k1 = D.indptr[l][i + 1];
-      for (int64_t k = D.indptr[l][i]; k < k1; ++k) {
-        R x = bb * tr[D.nbr[l][k]];
-        x = x < xmax ? x : xmax;
-        L += r_log1p(-x);
+      // RELAX_UNROLL neighbours at a time: their indices, then their tr values, all in flight together;
+      // the sum runs in list order as before
+      for (int64_t k = D.indptr[l][i]; k < k1; k += RELAX_UNROLL) {
+        int32_t j[RELAX_UNROLL];
+        R trj[RELAX_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) j[u] = k + u < k1 ? nbr[k + u] : -1;
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) trj[u] = j[u] >= 0 ? tr[j[u]] : (R)0;
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) {
+          if (j[u] < 0) break;
+          R x = bb * trj[u];
+          x = x < xmax ? x : xmax;
+          L += r_log1p(-x);
+        }
       }
     }
     const R s0 = D.s[i];
@@ -175,15 +189,31 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_edge_kernel(RelaxDev<R>
       const R bb = th.beta * th.bl[l];
       const R bbs = bb * sig_a;
       R c = 0, src = 0;
+      const int32_t* nbr = D.nbr[l];
       const int64_t k1 = D.indptr[l][a + 1];
-      for (int64_t k = D.indptr[l][a]; k < k1; ++k) {
-        const int32_t j = D.nbr[l][k];
-        const R tr_j = tr[j];
-        const R x_in = bbs * tr_j;  // j -> a
-        if (x_in < xmax) c += tr_j / (x_in - (R)1);
-        const R sig_j = th.sig[D.bin[j]];
-        const R x_out = (bb * sig_j) * tr_a;  // a -> j
-        if (x_out < xmax) src += (D.lbar[j] * sig_j) / (x_out - (R)1);
+      for (int64_t k = D.indptr[l][a]; k < k1; k += RELAX_UNROLL) {  // gathers in flight together, sums in order
+        int32_t jj[RELAX_UNROLL];
+        R trj[RELAX_UNROLL], lbj[RELAX_UNROLL];
+        uint8_t bj[RELAX_UNROLL];
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) jj[u] = k + u < k1 ? nbr[k + u] : -1;
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) {
+          const bool ok = jj[u] >= 0;
+          trj[u] = ok ? tr[jj[u]] : (R)0;
+          lbj[u] = ok ? D.lbar[jj[u]] : (R)0;
+          bj[u] = ok ? D.bin[jj[u]] : (uint8_t)0;
+        }
+#pragma unroll
+        for (int u = 0; u < RELAX_UNROLL; ++u) {
+          if (jj[u] < 0) break;
+          const R tr_j = trj[u];
+          const R x_in = bbs * tr_j;  // j -> a
+          if (x_in < xmax) c += tr_j / (x_in - (R)1);
+          const R sig_j = th.sig[bj[u]];
+          const R x_out = (bb * sig_j) * tr_a;  // a -> j
+          if (x_out < xmax) src += (lbj[u] * sig_j) / (x_out - (R)1);
+        }
       }
       ibar += bb * src;
       sum_blc += th.bl[l] * c;

@@ -186,7 +186,7 @@ def _day_or_none(sim, day):
 class test_prob(Intervention):
     """Test each agent with a probability set by its symptoms and quarantine state; true positives
     among the infectious are diagnosed test_delay days later unless lost to follow-up
-    (interventions.py:858-962, People.test people.py:589-617). Runs on the device (cvk_diag_apply);
+    (interventions.py:858-981, People.test people.py:589-617). Runs on the device (cvk_diag_apply);
     subtarget, ili_prev and swab_delay are not supported."""
 
     def __init__(self, symp_prob, asymp_prob=0.0, symp_quar_prob=None, asymp_quar_prob=None, quar_policy=None,
@@ -255,7 +255,7 @@ class test_prob(Intervention):
 class contact_tracing(Intervention):
     """Trace the contacts of the agents diagnosed today (presumptive: exposed and tested today) in
     each layer with probability trace_probs[layer]; the living ones become known contacts and are
-    scheduled to quarantine trace_time[layer] days later (interventions.py:965-1145). Runs on the
+    scheduled to quarantine trace_time[layer] days later (interventions.py:984-1151). Runs on the
     device (cvk_diag_apply); capacity is not supported."""
 
     def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,

@@ -57,7 +57,7 @@ class Sim:
     """A single simulation on one GPU (sim.py:49-80)."""
 
     def __init__(self, pars=None, label=None, people=None, popdict=None, households='poisson', device=None,
-                 log_capacity=None, popgen='host', **kwargs):
+                 log_capacity=None, popgen='host', source_lists=True, **kwargs):
         merged = dict(pars or {})
         merged.update(kwargs)
         variants = merged.pop('variants', [])
@@ -70,6 +70,10 @@ class Sim:
         if popgen not in ('host', 'device'):
             raise ValueError(f'popgen must be "host" or "device", not "{popgen}"')
         self.popgen = popgen  # 'device': build the population with the cvk_pop_* kernels
+        # static layers get source lists (cvk_sources): the edge pass then walks the infectious agents'
+        # contacts instead of streaming the layer (16 bytes per stored edge of device memory)
+        self.source_lists = bool(source_lists)
+        self._sources = {}
         self.device = device
         self.log_capacity = log_capacity
         self.initialized = False
@@ -295,6 +299,7 @@ class Sim:
             layers[li].dynamic = 1 if (dyn.get(lk, 0) and type(lay).update is Layer.update) else 0
             layers[li].beta_ones = 1 if (layers[li].dynamic or lay.beta_all_ones()) else 0
         self._layers = layers
+        self._attach_sources()
         # import schedule of the variants (immunity.py:117-130)
         days, vars_, nimp = [], [], []
         for var in pars['variants']:
@@ -323,9 +328,16 @@ class Sim:
             imp.diag_ops = self._diag_op_array
             imp.diag = C.pointer(self._diag)
         self._imports = imp
-        self._custom_update = [lk for lk in lkeys if dyn.get(lk, 0) and type(people.contacts[lk]).update
-                               is not Layer.update]
         self._edge_interventions = [iv for iv in pars['interventions'] if isinstance(iv, cviv.clip_edges)]
+
+    @property
+    def _custom_update(self):
+        """Dynamic layers whose update() is not the default resample (Layer subclasses, which may be
+        put in place after initialize(), examples/t11_dynamic_layers.py:41-43): days then run
+        through the individual entry points, with update(people) on the host."""
+        dyn = self.pars.get('dynam_layer') or {}
+        return [lk for lk in self.layer_keys() if dyn.get(lk, 0) and
+                type(self.people.contacts[lk]).update is not Layer.update]
 
     # -- running --
     def _rates(self, pars):
@@ -354,6 +366,41 @@ class Sim:
             self._layers[li].beta = lay.tensor('beta').data_ptr()
             self._layers[li].n_edges = len(lay)
             self._layers[li].beta_ones = 1 if lay.beta_all_ones() else 0
+        self._attach_sources()
+
+    def _attach_sources(self):
+        """Source lists of the static layers (built on the device, rebuilt when a layer's endpoint tensors
+        change, e.g. after clip_edges); dynamic and custom-updated layers are streamed."""
+        import torch
+        L = _lib.lib()
+        dyn = self.pars.get('dynam_layer') or {}
+        for li, lk in enumerate(self.layer_keys()):
+            lay = self.people.contacts[lk]
+            static = not dyn.get(lk, 0)
+            if not (self.source_lists and static and len(lay) > 0):
+                self._layers[li].sources = None
+                self._sources.pop(lk, None)
+                continue
+            p1, p2 = lay.tensor('p1'), lay.tensor('p2')
+            key = (p1.data_ptr(), p2.data_ptr(), p1.numel(), p1._version, p2._version)
+            have = self._sources.get(lk)
+            if have is None or have[0] != key:
+                n, E = self.people.n, len(lay)
+                dev = p1.device
+                indptr = torch.empty(n + 1, dtype=torch.int32, device=dev)
+                target = torch.empty(2 * E, dtype=torch.int32, device=dev)
+                edge_dir = torch.empty(2 * E, dtype=torch.int32, device=dev)
+                nb = int(L.cvk_sources_work_bytes(n))
+                work = torch.empty((nb + 3) // 4, dtype=torch.int32, device=dev)
+                slots = C.c_int64(0)
+                _lib.check(L.cvk_sources_build(C.byref(self._layers[li]), n, indptr.data_ptr(), target.data_ptr(),
+                                               edge_dir.data_ptr(), 2 * E, C.byref(slots), work.data_ptr(), nb,
+                                               _lib.stream_ptr()), f'source lists of layer {lk}')
+                del work
+                s = _lib.SourcesS(indptr.data_ptr(), target.data_ptr(), edge_dir.data_ptr(), slots.value)
+                have = (key, s, indptr, target, edge_dir)
+                self._sources[lk] = have
+            self._layers[li].sources = C.pointer(have[1])
 
     def _manual_day(self, t):
         """One day through the individual entry points (used by step() and custom layer updates)."""

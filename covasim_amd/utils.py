@@ -12,7 +12,11 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ['compute_viral_load', 'compute_trans_sus', 'compute_infections', 'set_seed', 'get_seed']
+__all__ = ['compute_viral_load', 'compute_trans_sus', 'compute_infections', 'set_seed', 'get_seed', 'choose',
+           'choose_r', 'default_int', 'default_float']
+
+default_int = np.int32    # defaults.py:26-30 (precision 32)
+default_float = np.float32
 
 _SEED = [1]
 
@@ -131,3 +135,14 @@ def compute_infections(beta, p1, p2, layer_betas, rel_trans, rel_sus, legacy=Fal
     if return_probs:
         out = out + (_out(prob[:k], is_t),)
     return out
+
+
+def choose(max_n, n):
+    """n distinct integers from [0, max_n), uniformly (utils.py:430-443), on the host: for Layer
+    subclasses that build their own contacts (examples/t11_dynamic_layers.py:19-25)."""
+    return np.random.choice(max_n, n, replace=False)
+
+
+def choose_r(max_n, n):
+    """n integers from [0, max_n) with replacement (utils.py:446-459), on the host."""
+    return np.random.choice(max_n, n, replace=True)

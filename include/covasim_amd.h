@@ -15,6 +15,7 @@
  *   cvk_agent_pass     covasim/people.py:164-186 People.update_states_pre (+ check_* :211-312, :368-374)
  *                      covasim/immunity.py:303-350 check_immunity, utils.py:39-79 (viral-load phase)
  *   cvk_resample_layer covasim/base.py:1849-1876 Layer.update (frac = 1)
+ *   cvk_resample_layer_frac covasim/base.py:1866-1875 Layer.update (frac < 1)
  *   cvk_choose         covasim/utils.py:430-443 choose / immunity.py:117-130 variant import choice
  *   cvk_edge_pass      covasim/sim.py:611-649   variant x layer loop of compute_trans_sus + compute_infections,
  *                      with People.infect's first-occurrence dedup (people.py:465) fused as a min-key
@@ -26,8 +27,8 @@
  *   cvk_finalize       covasim/sim.py:764-1001 finalize, compute_states, compute_doubling, compute_r_eff
  *   cvk_reduce         covasim/run.py:220-303  MultiSim.reduce (median / quantiles or mean / std)
  *   cvk_move_edges     covasim/interventions.py:589-673 clip_edges (Layer.pop_inds / append, base.py:1739-1768)
- *   cvk_diag_apply     covasim/interventions.py:858-962 test_prob (People.test, people.py:589-617) and
- *                      :965-1145 contact_tracing (Layer.find_contacts, utils.py:131-147;
+ *   cvk_diag_apply     covasim/interventions.py:858-981 test_prob (People.test, people.py:589-617) and
+ *                      :984-1151 contact_tracing (Layer.find_contacts, utils.py:131-147;
  *                      People.schedule_quarantine, people.py:620-641)
  *   cvk_states_post    covasim/people.py:189-196 update_states_post (check_diagnosed :315-332,
  *                      check_quar :335-358, check_enter_iso :361-365)
@@ -170,6 +171,18 @@ typedef struct {
   float *sus_imm, *symp_imm, *sev_imm;   /* [n_variants][n] */
 } cvk_people;
 
+/* No reference counterpart: source lists of a static layer, both directions of every stored pair
+ * grouped by source agent. The slots [indptr[i], indptr[i + 1]) of agent i hold target[k] and
+ * edge_dir[k] = edge index << 1 | direction (0: agent i is p1, 1: it is p2). Self-loops and edges
+ * with an endpoint outside [0, n) are left out (they cannot transmit). Built by cvk_sources_build;
+ * the order of an agent's slots is unspecified (results do not depend on it). */
+typedef struct {
+  const int32_t* indptr;     /* [n + 1] */
+  const int32_t* target;     /* [n_slots] */
+  const uint32_t* edge_dir;  /* [n_slots] */
+  int64_t n_slots;
+} cvk_sources;
+
 /* One contact layer: int32 endpoints and float32 per-edge beta (base.py:1610-1676). */
 typedef struct {
   int32_t* p1;
@@ -178,6 +191,8 @@ typedef struct {
   int64_t n_edges;
   int32_t dynamic;     /* resampled each day (dynam_layer); the resample sets every beta to 1 */
   int32_t beta_ones;   /* nonzero when every beta[e] is 1.0f (the edge pass then skips the beta gather) */
+  const cvk_sources* sources;  /* optional (static layers): the edge pass walks the lists of the day's
+                                  infectious agents instead of streaming p1/p2; must describe p1/p2 */
 } cvk_layer;
 
 /* Device workspace (sizes from cvk_workspace_sizes). */
@@ -203,6 +218,10 @@ typedef struct {
                             of rel_sus x quarantine factor, others 0; code = float bits >> 18 rounded up */
   int32_t npts;
   int32_t pad;
+  uint64_t* day_clock;   /* optional [npts + 1] (null: off): the device wall clock when day t's first pass
+                            starts (entry t), and when the closing pass of a run's last day starts (entry
+                            t1); a day's duration in a run is day_clock[t + 1] - day_clock[t] ticks of
+                            cvk_wall_clock_rate Hz. No reference counterpart (measurement). */
 } cvk_work;
 
 #define CVK_MAX_REDUCE_BLOCKS 65536
@@ -216,14 +235,19 @@ typedef struct {
 #define CVK_CTL_SEL_REMAIN 8
 #define CVK_CTL_SEL_ALL 9
 #define CVK_CTL_SEL_SHORT 10      /* set when fewer agents were eligible than requested */
+#define CVK_CTL_SRC_SLOTS 12      /* 64-bit count (words 12, 13) of source-list slots walked by edge passes,
+                                     when CVK_TUNE_COUNT_SOURCES is on */
+#define CVK_CTL_SRC_AGENTS 14     /* 64-bit count (words 14, 15) of (agent, layer) lists walked */
 #define CVK_CTL_WORDS 64
 
 /* ---- library ---- */
 const char* cvk_last_error(void);
 /* ABI version: changes whenever a public struct, an entry point's signature or a tunable's meaning
  * changes. Bindings compare it with the value they were written for. */
-#define CVK_ABI_VERSION 6
+#define CVK_ABI_VERSION 7
 int cvk_version(void);
+/* No reference counterpart: the frequency (Hz) of the device wall clock of cvk_work.day_clock. */
+int cvk_wall_clock_rate(int64_t* hz_out);
 /* No reference counterpart: a 64-bit hash of the HIP and header sources the library was built from
  * (set by the build script), so measurements taken on one kernel build can be told from another's. */
 uint64_t cvk_source_hash(void);
@@ -245,6 +269,13 @@ int cvk_abi_layout(int64_t* out, int32_t n);
  * dynamic layers runs as extra workgroups of day t's infect pass, and that pass takes dynamic-layer
  * sources from the resample stream; 0 = each day's resample runs inside that day's agent pass. */
 #define CVK_TUNE_OVERLAP_RESAMPLE 5
+/* CVK_TUNE_SOURCE_LISTS: static layers with source lists are walked through the day's infectious agents'
+ * lists: 2 (default) on days when at most CVK_TUNE_SOURCE_PERMILLE (default 30) agents per thousand are
+ * infectious (streamed on the others), 1 always, 0 never. CVK_TUNE_COUNT_SOURCES: 1 = edge passes add the
+ * slots and lists they walk to the CVK_CTL_SRC_* counters (0 default). The results are identical either way. */
+#define CVK_TUNE_SOURCE_LISTS 8
+#define CVK_TUNE_COUNT_SOURCES 9
+#define CVK_TUNE_SOURCE_PERMILLE 10
 /* CVK_TUNE_EDGE_COARSE: 1 (default) = when the infectious bitmap does not fit in LDS, the edge pass
  * stages the coarse map there and reads bitmap words from L2 only for endpoints whose 16-agent group
  * has an infectious agent; 0 = every bitmap word is read from L2. */
@@ -318,6 +349,14 @@ int cvk_agent_pass(const cvk_people* people, const cvk_params* params, cvk_work*
                    void* stream);
 int cvk_resample_layer(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
                        void* stream);
+/* Layer.update with frac < 1 (covasim/base.py:1866-1875): k = round(n_edges * frac) edges, a uniform
+ * k-subset (the k smallest keys (Philox word of (edge, t, layer) << 32) | edge), get new endpoints,
+ * uniform over [0, n_agents) with replacement, and weight 1; the other edges keep theirs. A chosen
+ * edge gets exactly what cvk_resample_layer gives it on day t, so k = n_edges is the full resample.
+ * work: cvk_resample_frac_work_bytes() bytes of device memory, 256-byte aligned. */
+int64_t cvk_resample_frac_work_bytes(void);
+int cvk_resample_layer_frac(const cvk_layer* layer, int32_t layer_index, const cvk_params* params, int32_t t,
+                            int64_t k, void* work, int64_t work_bytes, void* stream);
 /* uniforms: NULL (Philox) or per layer a device float64 array [n_variants][2][n_edges].
  * Must follow cvk_agent_pass of the same day (it reads that pass's bitmaps, records and bound codes). */
 int cvk_edge_pass(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
@@ -328,8 +367,8 @@ int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work*
                    void* stream);
 
 /* ---- testing, contact tracing, quarantine and isolation ----
- * One cvk_diag_op is one test_prob (interventions.py:858-962) or contact_tracing
- * (interventions.py:965-1145) intervention; cvk_diag_apply runs it on day t (nothing outside its
+ * One cvk_diag_op is one test_prob (interventions.py:858-981) or contact_tracing
+ * (interventions.py:984-1151) intervention; cvk_diag_apply runs it on day t (nothing outside its
  * days), after the day's imports and before transmission, in the sim's intervention order;
  * cvk_states_post then runs update_states_post (people.py:189-196) once per day.
  * Draws (Philox4x32-10 keyed by the seed): the test of agent i on day t uses
@@ -501,6 +540,12 @@ int cvk_batch_run_days(void* handle, int32_t t0, int32_t t1, const int64_t* pop_
  * cvk_graph_launch / cvk_graph_destroy). */
 int cvk_batch_graph_create(void* handle, int32_t t0, int32_t t1, const int64_t* pop_infected_host,
                            const cvk_batch_imports* imports, void* stream, void** graph_exec_out);
+/* No reference counterpart (measurement): enable != 0 makes eager cvk_batch_run_days calls record a HIP
+ * event pair around each day's edge pass; cvk_batch_edge_times then writes the durations (ms) of days
+ * [0, n) into ms_out (-1 where a day was not timed since the last enable) and returns how many it wrote.
+ * Graph captures are never timed. */
+int cvk_batch_time_edges(void* handle, int32_t enable);
+int cvk_batch_edge_times(void* handle, float* ms_out, int32_t n);
 /* Synchronous copies of the result tables: results [n_sims][npts][CVK_R_COLS] uint32 and
  * results_f [n_sims][npts][CVK_RF_COLS] (either may be NULL). */
 int cvk_batch_copy_results(void* handle, uint32_t* results_host, double* results_f_host);
@@ -679,6 +724,12 @@ int cvk_pop_people(const cvk_popgen* g, int64_t n, const float* age, uint8_t* se
  * has the key (philox(seed; e lo, t, 0x80000000 | slot, e hi).x << 32) | e and the k smallest keys
  * move. keep_* get the other n - k edges in their order, move_* the k moved edges in their order
  * (p1, p2, beta columns; point move_* at the end of the destination layer to append them). */
+/* No reference counterpart: builds the source lists of a layer of n_agents agents (cvk_sources).
+ * target / edge_dir need room for 2 * n_edges slots (capacity); *n_slots_out receives the count (a
+ * synchronous read-back). work: cvk_sources_work_bytes(n_agents) bytes, 256-byte aligned. */
+int64_t cvk_sources_work_bytes(int64_t n_agents);
+int cvk_sources_build(const cvk_layer* layer, int64_t n_agents, int32_t* indptr, int32_t* target, uint32_t* edge_dir,
+                      int64_t capacity, int64_t* n_slots_out, void* work, int64_t work_bytes, void* stream);
 int64_t cvk_move_edges_work_bytes(int64_t n);
 int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float* beta, int64_t n, int64_t k, uint64_t seed,
                    int32_t t, int32_t slot, int32_t* keep_p1, int32_t* keep_p2, float* keep_beta, int32_t* move_p1,

@@ -790,6 +790,9 @@ class Sim:
                            np.asarray(v[2], F32).copy()] for k, v in layers.items()}
         self.rng = rng
         self.interventions = []  # called each day after the variant imports, as sim.py:591-593
+        # per dynamic layer key, optionally, the update of a Layer subclass (examples/t11_dynamic_layers.py:9-26):
+        # fn(t, layer index, (p1, p2, beta), n_agents) -> the day's (p1, p2, beta), of any length
+        self.layer_updates = {}
         self.t = 0
         nv = pars['n_variants']
         self.results = {k: np.zeros(self.npts) for k in FLOW_KEYS}
@@ -811,7 +814,13 @@ class Sim:
             people.infect(inds=inds, rng=rng, layer='seed_infection')
         people.update_states_pre(t=t)  # sim.py:577
         for li, (lkey, lay) in enumerate(self.layers.items()):  # people.update_contacts, people.py:199-206
-            if pars['dynam_layer'].get(lkey, 0):
+            if not pars['dynam_layer'].get(lkey, 0):
+                continue
+            if lkey in self.layer_updates:  # a Layer subclass's update(people): new (p1, p2, beta)
+                new = self.layer_updates[lkey](t, li, tuple(lay), people.n)
+                lay[0], lay[1], lay[2] = (np.asarray(new[0], np.int32), np.asarray(new[1], np.int32),
+                                          np.asarray(new[2], F32))
+            else:
                 lay[0], lay[1] = rng.resample(t, li, len(lay[0]), people.n)
                 lay[2] = np.ones(len(lay[0]), dtype=F32)
         hosp_max = people.severe.sum() > pars['n_beds_hosp'] if pars['n_beds_hosp'] is not None else False

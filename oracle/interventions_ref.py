@@ -125,7 +125,7 @@ def get_quar_inds(quar_policy, sim):
 
 
 class test_prob:
-    """interventions.py:858-962 (without subtarget, ili_prev and swab_delay): test probabilities
+    """interventions.py:858-981 (without subtarget, ili_prev and swab_delay): test probabilities
     set piece by piece in the reference's order, one Bernoulli draw per agent, then People.test.
     The draws of agent i are philox(i, t, 0x90000000 | slot, 0) (tested, true positive) and
     philox(i, t, 0x90000000 | slot, 1) (not lost), slot = the intervention's index in the sim."""
@@ -189,7 +189,7 @@ def find_contacts(p1, p2, inds):
 
 
 class contact_tracing:
-    """interventions.py:965-1145 (without capacity): the agents diagnosed today (or, presumptive,
+    """interventions.py:984-1151 (without capacity): the agents diagnosed today (or, presumptive,
     exposed and tested today) are traced through each layer with trace_prob > 0; each contact found
     in layer l is kept when philox(i, t, 0xA0000000 | slot << 8 | l, 0) < trace_prob[l], and the
     living ones are notified and scheduled to quarantine on day int(t + trace_time[l])."""

@@ -50,6 +50,7 @@ STREAM_RESAMPLE = 0x50000000
 STREAM_RESAMPLE_RETRY = 0x60000000
 RESAMPLE_MAX_RETRY = 255
 STREAM_TEST = 0x90000000   # | slot
+STREAM_PARTIAL = 0xB0000000  # | layer: edge choice of a partial resample
 STREAM_TRACE = 0xA0000000  # | slot << 8 | layer
 
 # Fixed per-infection draw slots (see People.infect restatement in covasim_oracle.py)
@@ -186,6 +187,26 @@ def resample_edges(seed, t, l, n_edges, n_agents):
             ok[bad] = ok2 | last
         out.append(val.astype(np.int32))
     return out[0], out[1]
+
+
+def partial_resample(seed, t, l, p1, p2, beta, n_agents, frac):
+    """Layer.update(people, frac) of base.py:1866-1875: k = round(n * frac) edges (np.round, half to
+    even), a uniform k-subset, get new uniform endpoints and weight 1; the others keep theirs.
+    The subset is the k smallest keys (Philox word 0 of (edge, t, STREAM_PARTIAL | l) << 32 | edge);
+    a chosen edge's endpoints are those the full resample of day t gives it. Returns new arrays."""
+    n = len(p1)
+    k = int(np.round(n * frac))
+    p1, p2, beta = np.array(p1, dtype=np.int32), np.array(p2, dtype=np.int32), np.array(beta, dtype=np.float32)
+    if k == 0:
+        return p1, p2, beta
+    k0, k1 = seed_key(seed)
+    e = np.arange(n, dtype=np.uint64)
+    w = philox4x32(e & MASK32, t, STREAM_PARTIAL | l, e >> np.uint64(32), k0, k1)
+    keys = (w[0] << np.uint64(32)) | (e & MASK32)
+    sel = np.argpartition(keys, k - 1)[:k] if k < n else np.arange(n)
+    q1, q2 = resample_edges(seed, t, l, n, n_agents)
+    p1[sel], p2[sel], beta[sel] = q1[sel], q2[sel], 1.0
+    return p1, p2, beta
 
 
 def box_muller(u0, u1):

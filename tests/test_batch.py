@@ -57,16 +57,34 @@ HYBRID_SPEC = [(1500, 0.03, 3, []), (2600, 0.02, 4, []), (803, 0.05, 5, []), (40
                (3101, 0.04, 7, [])]
 
 
+@pytest.fixture
+def batch_generate(request):
+    """CVK_TUNE_GENERATE_DYNAMIC for the batch only (1: the dynamic layer's endpoints are generated
+    inside the batched passes, written once at the end); restored to the default (2) after."""
+    from covasim_amd import _lib
+    yield request.param
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, 2), 'tuning')
+
+
 @pytest.mark.gpu
-def test_batch_equals_individual_runs_hybrid_dynamic(gpu):
+@pytest.mark.parametrize('batch_generate', [2, 1], indirect=True)
+def test_batch_equals_individual_runs_hybrid_dynamic(gpu, batch_generate):
     import covasim_amd as cv
+    from covasim_amd import _lib
     common = dict(use_waning=False, dynam_layer=dict(c=1))
-    alone = run_alone(HYBRID_SPEC, 35, 'hybrid', **common)
+    alone = run_alone(HYBRID_SPEC, 35, 'hybrid', **common)  # written layers (below the automatic switch)
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_GENERATE_DYNAMIC, batch_generate), 'tuning')
     batch = cv.BatchSim(make_sims(HYBRID_SPEC, 35, 'hybrid', **common)).run()
     for si, sim in enumerate(batch.sims):
         assert sim.results['cum_infections'].values[-1] > 30
         assert_same_results(sim, alone[si])
         assert_same_state(batch, si, alone[si])
+        # the dynamic layer after the run holds the last day's contacts, weights 1
+        li = batch.layer_keys.index('c')
+        a, c = int(batch.edge_starts[li, si]), int(batch.edge_counts[li, si])
+        for col in ('p1', 'p2', 'beta'):
+            got = batch._layers['c'][col][a:a + c].cpu().numpy()
+            assert np.array_equal(got, alone[si].people.contacts['c'][col]), (si, col)
 
 
 @pytest.mark.gpu

@@ -199,6 +199,26 @@ CONFIGS = {
                                               tuning=dict(generate_dynamic=1)),
     # p_ub >= 1 for most slots: whole rounds pass the prefilter (exact tests in place and queue drains)
     'dense_exact_tests': dict(pop=dict(n=3000, pop_type='random'), pars=dict(use_waning=False, beta=0.9)),
+    # every static layer streamed instead of walked through its source lists
+    'hybrid_dynamic_streamed': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                    pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                    tuning=dict(source_lists=0)),
+    'variants_waning_streamed_l2': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                        pars=dict(use_waning=True, beta=0.02,
+                                                  variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
+                                        tuning=dict(source_lists=0, edge_lds=0)),
+    # static layers walked through their source lists on every day, busy ones included
+    'hybrid_waning_always_walked': dict(pop=dict(n=4000, pop_type='hybrid', households='categorical'),
+                                        pars=dict(use_waning=True, beta=0.04, dynam_layer=dict(c=1)),
+                                        tuning=dict(source_lists=1)),
+    # the choice between walking and streaming switching back and forth (threshold at 5 per thousand)
+    'hybrid_walk_switching': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                  pars=dict(use_waning=False, beta=0.03, dynam_layer=dict(c=1)),
+                                  tuning=dict(source_lists=2, source_permille=5)),
+    # source lists with the bitmap read from L2 / filtered by the coarse map, and weighted static layers
+    'weighted_sources_coarse': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
+                                    pars=dict(use_waning=True, beta=0.05, dynam_layer=dict(c=1)),
+                                    tuning=dict(edge_lds=0, edge_coarse=1, source_lists=1)),
     'beds_limited': dict(pop=dict(n=3000, pop_type='random'),
                          pars=dict(use_waning=False, beta=0.08, n_beds_hosp=2, n_beds_icu=1,
                                    rel_severe_prob=20.0, rel_crit_prob=5.0)),
@@ -216,7 +236,8 @@ def test_full_sim_matches_oracle(gpu, name):
     keys = dict(edge_lds=(_lib.TUNE_EDGE_LDS, 1), edge_chunks=(_lib.TUNE_EDGE_CHUNKS, 2),
                 generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 2),
                 overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1),
-                edge_prefetch=(_lib.TUNE_EDGE_PREFETCH, 0))
+                edge_prefetch=(_lib.TUNE_EDGE_PREFETCH, 0), source_lists=(_lib.TUNE_SOURCE_LISTS, 2),
+                source_permille=(_lib.TUNE_SOURCE_PERMILLE, 30))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:
@@ -264,3 +285,48 @@ def test_infection_log_attribution(gpu):
                   for r in sim.infection_log())
     olog = sorted((d, tg, -1 if s < 0 else s, l) for (s, tg, d, l, v) in osim.people.infection_log)
     assert glog == olog
+
+
+@pytest.mark.parametrize('n, self_loops', [(5000, False), (70_001, True)])
+def test_source_lists_hold_both_directions_of_every_edge(gpu, n, self_loops):
+    """cvk_sources_build: agent i's slots are exactly (j, e, dir) for every edge e = (i, j) (dir 0) and
+    e = (j, i) (dir 1), self-loops left out; offsets are the running degree counts."""
+    import ctypes as C
+    import torch
+    from covasim_amd import _lib
+    rng = np.random.default_rng(n)
+    E = 3 * n + 7
+    p1 = rng.integers(0, n, E).astype(np.int32)
+    p2 = rng.integers(0, n, E).astype(np.int32)
+    if self_loops:
+        p1[::97] = p2[::97]
+    d1, d2 = torch.from_numpy(p1).cuda(), torch.from_numpy(p2).cuda()
+    beta = torch.ones(E, dtype=torch.float32, device='cuda')
+    lay = _lib.LayerS(d1.data_ptr(), d2.data_ptr(), beta.data_ptr(), E, 0, 1)
+    L = _lib.lib()
+    indptr = torch.empty(n + 1, dtype=torch.int32, device='cuda')
+    target = torch.empty(2 * E, dtype=torch.int32, device='cuda')
+    edge_dir = torch.empty(2 * E, dtype=torch.int32, device='cuda')
+    nb = int(L.cvk_sources_work_bytes(n))
+    work = torch.empty((nb + 3) // 4, dtype=torch.int32, device='cuda')
+    slots = C.c_int64(-1)
+    _lib.check(L.cvk_sources_build(C.byref(lay), n, indptr.data_ptr(), target.data_ptr(), edge_dir.data_ptr(),
+                                   2 * E, C.byref(slots), work.data_ptr(), nb, _lib.stream_ptr()), 'sources')
+    torch.cuda.synchronize()
+    keep = p1 != p2
+    assert slots.value == 2 * int(keep.sum())
+    ip = indptr.cpu().numpy().astype(np.int64)
+    deg = np.bincount(np.concatenate([p1[keep], p2[keep]]), minlength=n)
+    assert ip[0] == 0 and np.array_equal(np.diff(ip), deg)
+    tg = target.cpu().numpy()[:slots.value]
+    ed = edge_dir.cpu().numpy()[:slots.value].view(np.uint32).astype(np.int64)
+    src = np.repeat(np.arange(n), deg)
+    got = np.lexsort((ed, src))
+    e_all = np.flatnonzero(keep)
+    want_src = np.concatenate([p1[e_all], p2[e_all]])
+    want_tgt = np.concatenate([p2[e_all], p1[e_all]])
+    want_ed = np.concatenate([e_all << 1, (e_all << 1) | 1])
+    order = np.lexsort((want_ed, want_src))
+    assert np.array_equal(src[got], want_src[order])
+    assert np.array_equal(tg[got], want_tgt[order])
+    assert np.array_equal(ed[got], want_ed[order])

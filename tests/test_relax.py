@@ -227,7 +227,47 @@ def test_relax_1m_adjoint_matches_fp64_central_differences(gpu):
 
 
 @pytest.mark.gpu
-def test_relax_calibration_recovers_the_incidence_curve(gpu):
+def test_relax_identifiability_on_its_own_curve(gpu):
+    """Calibration against a curve the relaxation itself produced at known theta. The expected
+    incidence depends on the parameters only through the products beta * beta_l * sigma_b, so
+    (c beta, beta_l / c, sigma) and (c beta, beta_l, sigma / c) give the same curve: two exact null
+    directions, checked here to float64 rounding. With the layer betas and rel_sus fixed, beta is
+    recovered to 1e-4 relative; with all 15 free, the fit reproduces the curve (loss below 1e-3 of the
+    start's) without pinning the individual parameters."""
+    from covasim_amd.relax import Relaxation, calibrate
+    n, T = 20000, 40
+    relax = Relaxation(_sim(n, T, seed=7), precision=64)
+    truth = relax.theta0()
+    curve = relax.forward_np(truth)
+    assert curve.sum() > 1000
+    nl = relax.n_layers
+    for scale_layers in (True, False):
+        th = truth.copy()
+        th[0] *= 1.3
+        if scale_layers:
+            th[1:1 + nl] /= 1.3
+        else:
+            th[1 + nl:] /= 1.3
+        np.testing.assert_allclose(relax.forward_np(th), curve, rtol=1e-10)
+    r32 = Relaxation(relax.sim, precision=32)
+    target = r32.forward_np(truth)
+    mask = np.zeros(relax.n_params)
+    mask[0] = 1
+    start = truth.copy()
+    start[0] *= 1.4
+    fitted, hist = calibrate(r32, target, start, iters=300, lr=0.03, fit_mask=mask)
+    assert abs(fitted[0] / truth[0] - 1) < 1e-4, fitted[0] / truth[0]
+    np.testing.assert_allclose(fitted[1:], truth[1:], rtol=1e-13)  # held (exp(log(x)) rounding only)
+    start = truth * np.random.default_rng(3).uniform(0.7, 1.4, relax.n_params)
+    fitted, hist = calibrate(r32, target, start, iters=300, lr=0.03)
+    assert hist[-1] < 1e-3 * hist[0], (hist[0], hist[-1])
+
+
+@pytest.mark.gpu
+def test_relax_calibration_reduces_loss_against_the_stochastic_curve(gpu):
+    """Fitting the relaxation to the mean incidence of stochastic runs at known parameters lowers the
+    loss fivefold; the fitted parameters absorb the relaxation's bias (it ignores the depletion of
+    susceptible contacts around each case) and are not the stochastic sim's."""
     from covasim_amd.relax import Relaxation, calibrate
     n, T = 20000, 40
     observed = []
