@@ -689,16 +689,20 @@ def run_single(w, args, tuning):
         res = sim._w['results'].cpu().numpy().view(np.uint32)
         return res.copy()
 
-    def eager_day(t, events=None):
-        """Day t through the individual entry points (Sim.step's sequence); HIP events around the
-        edge pass when events is a list."""
-        if t == 0:
-            _lib.check(L.cvk_choose(P, pr, wk, 0, _lib.STREAM_CHOOSE, pop_inf, 0, _lib.LAYER_SEED, 0, st), 'seed')
-            _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, 0, 0, st), 'seed')
-        _lib.check(L.cvk_agent_pass(P, pr, wk, t, st), 'agent')
+    def resample(t):
         for li in range(pr.n_layers):
             if sim._layers[li].dynamic:
                 _lib.check(L.cvk_resample_layer(C.byref(sim._layers[li]), li, pr, t, st), 'resample')
+
+    def eager_day(t, events=None):
+        """Day t through the individual entry points; HIP events around the edge pass when events is a
+        list. Day t+1's contact resample runs right after day t's infect pass, where the captured day loop
+        runs it (beside that pass), so each edge pass meets the caches as it does in a graph run."""
+        if t == 0:
+            _lib.check(L.cvk_choose(P, pr, wk, 0, _lib.STREAM_CHOOSE, pop_inf, 0, _lib.LAYER_SEED, 0, st), 'seed')
+            _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, 0, 0, st), 'seed')
+            resample(0)
+        _lib.check(L.cvk_agent_pass(P, pr, wk, t, st), 'agent')
         days_i, vars_i, nimp_i = sim._imp_arrays
         for j in range(len(days_i)):
             if days_i[j] == t:
@@ -712,6 +716,8 @@ def run_single(w, args, tuning):
             e1.record(stream)
             events.append((e0, e1))
         _lib.check(L.cvk_infect_pass(P, sim._layers, pr, wk, t, 1, st), 'infect')
+        if t + 1 < npts:
+            resample(t + 1)
         _lib.check(L.cvk_count_pass(P, pr, wk, t, st), 'count')
 
     def graph(t0, t1):
