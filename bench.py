@@ -1,16 +1,19 @@
 """Benchmark of the daily transmission step on one MI355X.
 
 A step is one simulated day of the plain hybrid population (1M agents; layers h/s/w/c with betas
-3.0/0.6/0.6/0.3, community resampled daily, 180 days). ``--steps K`` days spread evenly over the
-whole run are timed in place, each as a one-day captured graph with HIP events around its launch,
-after ``--warmup`` untimed passes; the days between them run untimed. The whole run as one graph
-replay is reported beside them (``full_run``).
+3.0/0.6/0.6/0.3, community resampled daily, 180 days). The whole run is captured as one graph; after
+``--warmup`` untimed replays (at least one), one replay is timed, and inside it ``--steps K`` days
+spread evenly over the run are timed in place by the device wall clock that every day's first pass
+stamps (a day: from its first pass to the next day's). The whole replay's host-clock time is
+reported beside them (``full_run``).
 
 Prints one JSON line: agent-days/s over the K timed days, the edge-pass roofline (algorithmic bytes
-per launch over its HIP-event-measured duration, every day of the run, against the 8 TB/s HBM3E
-peak), the whole-day roofline (algorithmic bytes of the timed days over their time) and the CPU
-baselines (the oracle's C restatement, one thread and OpenMP, timed on the same sampled days of the
-same population on this host).
+per launch over the launch's duration, every day of the timed replay, against the 8 TB/s HBM3E peak;
+the duration is the launch's span from its first workgroup's start to its last one's end by the
+device wall clock; an eager run of the same days gives the HIP-event figure beside it),
+the whole-day roofline (algorithmic bytes of the days over their time) and the CPU baselines (the
+oracle's C restatement, one thread and OpenMP, timed on the same sampled days of the same population
+on this host).
 """
 
 import argparse
@@ -643,8 +646,8 @@ def day_bytes(n, b_agent, edge_read, resample_write, new_inf):
 
 
 def run_single(w, args, tuning):
-    """One sim: K one-day steps spread evenly over the whole run, timed in place (HIP events
-    around each day's captured graph), beside the full run as one graph replay."""
+    """One sim: K days spread evenly over the whole run, timed in place inside one replay of the run's
+    captured graph by the device wall clock, beside that replay's host-clock time."""
     import torch
     import covasim_amd as cv
     from covasim_amd import _lib
@@ -761,6 +764,7 @@ def run_single(w, args, tuning):
             _lib.check(L.cvk_graph_launch(full, st), 'graph launch')
         reset()
         clock.zero_()
+        clock[npts + 1::2] = -1  # edge-pass spans: earliest workgroup start ~0, latest end 0
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         _lib.check(L.cvk_graph_launch(full, st), 'graph launch')
@@ -769,24 +773,30 @@ def run_single(w, args, tuning):
         _lib.check(L.cvk_graph_destroy(full), 'graph destroy')
         res_full = counts()
         ticks = clock.cpu().numpy().astype(np.int64)
-        day_s = np.diff(ticks) * tick_s  # day t: from its first pass to the next day's (or the closing pass)
+        day_s = np.diff(ticks[:npts + 1]) * tick_s  # day t: from its first pass to the next day's (or the closing pass)
+        # each day's edge pass inside the same replay: its first workgroup's start to its last one's end
+        edge_ms = (ticks[npts + 2::2] - ticks[npts + 1::2]) * tick_s * 1e3
         print(f'full run: {npts} days in {full_wall * 1e3:.1f} ms', file=sys.stderr, flush=True)
-        # (2) every day eagerly, HIP events around each edge pass (the roofline kernel) and the source-list
-        # counters read after each day
+        # (2) every day eagerly: HIP events around each edge pass (the roofline kernel), and the
+        # source-list counters read after each day
         reset()
+        clock[npts + 1::2] = -1
+        clock[npts + 2::2] = 0
         _lib.check(L.cvk_set_tuning(_lib.TUNE_COUNT_SOURCES, 1), 'tuning')
         edge_ev = []
         ctl = sim._w['ctl']
-        src_counts = np.zeros((npts, 2), dtype=np.int64)
+        src_counts = np.zeros((npts, 3), dtype=np.int64)
         for t in range(npts):
             eager_day(t, edge_ev)
-            c = ctl[_lib.CTL_SRC_SLOTS:_lib.CTL_SRC_SLOTS + 4].cpu().numpy().view(np.int64)
-            src_counts[t] = c
+            src_counts[t] = ctl[_lib.CTL_SRC_SLOTS:_lib.CTL_SRC_SLOTS + 6].cpu().numpy().view(np.int64)
         _lib.check(L.cvk_set_tuning(_lib.TUNE_COUNT_SOURCES, 0), 'tuning')
         torch.cuda.synchronize()
-        edge_ms = np.array([a.elapsed_time(b) for a, b in edge_ev])
+        edge_ev_ms = np.array([a.elapsed_time(b) for a, b in edge_ev])
+        ticks_eager = clock.cpu().numpy().astype(np.int64)
+        edge_eager_ms = (ticks_eager[npts + 2::2] - ticks_eager[npts + 1::2]) * tick_s * 1e3
         res_eager = counts()
-    src_day = np.diff(np.vstack([np.zeros((1, 2), np.int64), src_counts]), axis=0)  # slots, lists walked per day
+    # per day: source-list slots walked, lists walked, 1 if the listed layers were streamed instead
+    src_day = np.diff(np.vstack([np.zeros((1, 3), np.int64), src_counts]), axis=0)
     step_s = np.array([day_s[d] for d in days])
     n = w['pop_size']
     new_inf = res_full[:, _lib.R_COL['new_infections']].astype(np.int64)
@@ -795,12 +805,17 @@ def run_single(w, args, tuning):
     value = n * K / timed_s
     # roofline of the dominant kernel: the edge pass over every day of the run. Algorithmic bytes per
     # launch: 8 per streamed edge (12 where the layer carries weights), none for a generated layer, 8 per
-    # source-list slot walked (target + edge) and per list (two offsets), and the N/8-byte bitmap
-    listed = set(sim._sources)
+    # source-list slot walked (target + edge) and per list (two offsets), and the N/8-byte bitmap. The
+    # layers with source lists count as streamed on the days the walk decision streams them.
+    listed = set(sim._sources) if tuning.get('source_lists', 2) else set()
+
+    def layer_bytes(keys):
+        return sum((12 if not np.all(pop['contacts'][k]['beta'] == 1) else 8) * len(pop['contacts'][k]['p1'])
+                   for k in keys)
     streamed = [k for k in pop['contacts'] if k not in listed]
-    weighted = sum(len(pop['contacts'][k]['p1']) for k in streamed if not np.all(pop['contacts'][k]['beta'] == 1))
-    stream_edges = sum(len(pop['contacts'][k]['p1']) for k in streamed) - (dyn_edges if gen else 0)
-    edge_bytes_day = 8 * stream_edges + 4 * weighted + n / 8 + 8 * src_day[:, 0] + 8 * src_day[:, 1]
+    stream_bytes = layer_bytes(streamed) - (8 * dyn_edges if gen else 0)
+    edge_bytes_day = (stream_bytes + n / 8 + 8 * src_day[:, 0] + 8 * src_day[:, 1]
+                      + layer_bytes(listed) * (src_day[:, 2] > 0))
     achieved = float(edge_bytes_day.sum()) / (float(edge_ms.sum()) / 1e3) / 1e9
     nv = 1 + len(w.get('variants', []))
     b_agent = 56 + 8 * (nv - 1) + ((20 + 12 * nv) if w['pars'].get('use_waning') else 0)
@@ -833,6 +848,11 @@ def run_single(w, args, tuning):
                      'launch_us_p10_p50_p90_max': [round(float(x) * 1e3, 1) for x in
                                                    np.percentile(edge_ms, [10, 50, 90, 100])],
                      'timed_days_avg_launch_us': float(np.mean([edge_ms[d] for d in days])) * 1e3,
+                     'launch_timing': 'each day\'s launch inside the timed replay of the whole run, from its '
+                                      'first workgroup start to its last workgroup end by the device wall clock',
+                     'eager_run_avg_launch_us': float(edge_eager_ms.mean()) * 1e3,
+                     'eager_run_avg_launch_us_hip_events': float(edge_ev_ms.mean()) * 1e3,
+                     'listed_layers_streamed_days': int((src_day[:, 2] > 0).sum()),
                      'source_slots_per_day_mean': float(src_day[:, 0].mean()),
                      'whole_day': {'bytes_per_agent': b_agent, 'achieved_timed_days': whole_sampled,
                                    'frac_timed_days': whole_sampled / HBM_PEAK_GBS,

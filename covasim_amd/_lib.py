@@ -69,7 +69,7 @@ R_COLS = R_BYVAR + len(RV_NAMES) * MAX_VARIANTS
 RF_COLS = 4
 CTL_WORDS = 64
 CTL_LOG_COUNT, CTL_LOG_OVERFLOW, CTL_SEL_SHORT = 2, 4, 10
-CTL_SRC_SLOTS, CTL_SRC_AGENTS = 12, 14
+CTL_SRC_SLOTS, CTL_SRC_AGENTS, CTL_SRC_STREAMED = 12, 14, 16
 
 
 class Dur(C.Structure):
@@ -151,7 +151,7 @@ def make_workspace(n, npts, log_capacity, device):
         partials=torch.zeros((MAX_REDUCE_BLOCKS, PARTIAL_COLS), dtype=torch.int32, device=device),
         partials_f=torch.zeros((MAX_REDUCE_BLOCKS, RF_COLS), dtype=torch.float64, device=device),
         bound=torch.zeros(n, dtype=torch.int16, device=device),
-        day_clock=torch.zeros(npts + 1, dtype=torch.int64, device=device),
+        day_clock=torch.zeros(3 * npts + 1, dtype=torch.int64, device=device),
     )
     s = WorkS()
     for k in WORK_PTR_FIELDS:
@@ -180,9 +180,12 @@ class DiagOp(C.Structure):
                 ('symp_prob', C.c_double), ('asymp_prob', C.c_double), ('symp_quar_prob', C.c_double),
                 ('asymp_quar_prob', C.c_double), ('sensitivity', C.c_double), ('keep_prob', C.c_double),
                 ('test_delay', C.c_double), ('quar_policy', C.c_int32), ('n_quar_days', C.c_int32),
-                ('quar_days', C.c_int32 * MAX_QUAR_DAYS), ('trace_prob', C.c_double * MAX_LAYERS),
+                ('quar_days', C.c_int32 * MAX_QUAR_DAYS),
+                ('sub_mask', C.c_void_p), ('sub_vals', C.c_void_p), ('sub_val', C.c_double),
+                ('ili_counts', C.c_void_p), ('n_ili_days', C.c_int32), ('n_swab_bins', C.c_int32),
+                ('swab_base', C.c_void_p), ('trace_prob', C.c_double * MAX_LAYERS),
                 ('trace_time', C.c_double * MAX_LAYERS), ('trace_period', C.c_int32 * MAX_LAYERS),
-                ('presumptive', C.c_int32), ('pad', C.c_int32)]
+                ('presumptive', C.c_int32), ('pad1', C.c_int32), ('capacity', C.c_int64)]
 
 
 DIAG_DATE_FIELDS = ['date_tested', 'date_pos_test', 'date_known_contact', 'date_quarantined', 'date_end_quarantine']
@@ -191,7 +194,8 @@ DIAG_DATE_FIELDS = ['date_tested', 'date_pos_test', 'date_known_contact', 'date_
 class DiagS(C.Structure):
     _fields_ = ([(k, C.c_void_p) for k in DIAG_DATE_FIELDS] +
                 [('pending', C.c_void_p), ('cases', C.c_void_p), ('coarse', C.c_void_p), ('contacts', C.c_void_p),
-                 ('part', C.c_void_p), ('ring', C.c_int32), ('pad', C.c_int32)])
+                 ('part', C.c_void_p), ('ring', C.c_int32), ('swab_bins', C.c_int32), ('select', C.c_void_p),
+                 ('swab_hist', C.c_void_p), ('swab_prob', C.c_void_p)])
 
 
 class ImportsS(C.Structure):

@@ -4,11 +4,9 @@
 // covasim/base.py:1866-1875): the chosen edges get new iid uniform endpoints and weight 1.
 //
 // Choice: edge e gets the 64-bit key (Philox word << 32) | e, distinct for every edge; the moved
-// edges are those whose key is at most the k-th smallest. The k-th smallest key comes from an 8-pass
-// radix select (one 8-bit digit per pass: a 256-bucket histogram of the keys that share the digits
-// found so far, then one workgroup picks the digit where the count reaches k). The edges then split
-// by one exclusive scan of the selection flags: kept edges in their order into keep_*, moved edges in
-// their order into move_*.
+// edges are those whose key is at most the k-th smallest (radix select, cvk_select.h). The edges then
+// split by one exclusive scan of the selection flags: kept edges in their order into keep_*, moved
+// edges in their order into move_*.
 #include <algorithm>
 #include <string.h>
 
@@ -16,6 +14,7 @@
 #include "cvk_host.h"
 #include "cvk_kernels.h"
 #include "cvk_scan.h"
+#include "cvk_select.h"
 
 namespace cvk {
 namespace {
@@ -24,52 +23,13 @@ constexpr int CLIP_BLOCK = 256;
 
 // stream_id: STREAM_CLIP | slot (clip_edges) or STREAM_PARTIAL | layer (partial resample)
 __device__ inline uint64_t edge_key(uint64_t seed, int32_t t, uint32_t stream_id, int64_t e) {
-  const u32x4 w = philox_seeded(seed, (uint32_t)e, (uint32_t)t, stream_id, (uint32_t)((uint64_t)e >> 32));
-  return ((uint64_t)w.x << 32) | (uint32_t)e;
+  return item_key(seed, t, stream_id, e);
 }
 
-// state: [0] digits found so far (prefix), [1] how many keys below the prefix's bucket remain to take
-__global__ __launch_bounds__(CLIP_BLOCK) void key_hist_kernel(uint64_t seed, int32_t t, uint32_t stream_id, int64_t n,
-                                                              int pass, const uint64_t* state, uint32_t* hist) {
-  __shared__ uint32_t s_hist[256];
-  s_hist[threadIdx.x] = 0;
-  __syncthreads();
-  const int shift = 56 - 8 * pass;
-  const uint64_t prefix = state[0];
-  const uint64_t high_mask = pass == 0 ? 0ull : ~0ull << (64 - 8 * pass);
-  for (int64_t e = (int64_t)blockIdx.x * CLIP_BLOCK + threadIdx.x; e < n; e += (int64_t)gridDim.x * CLIP_BLOCK) {
-    const uint64_t key = edge_key(seed, t, stream_id, e);
-    if ((key & high_mask) == prefix) atomicAdd(&s_hist[(key >> shift) & 255u], 1u);
-  }
-  __syncthreads();
-  if (s_hist[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_hist[threadIdx.x]);
-}
-
-// One workgroup: the digit whose bucket holds the remaining k-th key; clears the histogram.
-__global__ __launch_bounds__(CLIP_BLOCK) void key_select_kernel(int pass, uint64_t* state, uint32_t* hist) {
-  if (threadIdx.x == 0) {
-    uint64_t remain = state[1];
-    uint32_t digit = 255;
-    for (uint32_t b = 0; b < 256; ++b) {
-      if (remain <= hist[b]) {
-        digit = b;
-        break;
-      }
-      remain -= hist[b];
-    }
-    state[0] |= (uint64_t)digit << (56 - 8 * pass);
-    state[1] = remain;
-  }
-  __syncthreads();
-  hist[threadIdx.x] = 0;
-}
-
-__global__ void key_init_kernel(uint64_t* state, uint32_t* hist, uint64_t k) {
-  if (threadIdx.x == 0) {
-    state[0] = 0;
-    state[1] = k;
-  }
-  hist[threadIdx.x] = 0;
+// The k-th smallest key of edges [0, n) into state[0] (k >= 1).
+inline void select_edges(uint64_t seed, int32_t t, uint32_t stream_id, int64_t n, int64_t k, uint64_t* state,
+                         uint32_t* hist, hipStream_t st) {
+  select_kth(AllItemsKey{seed, t, stream_id}, n, k, state, hist, st);
 }
 
 struct Chosen {
@@ -79,18 +39,6 @@ struct Chosen {
   const uint64_t* kth;  // the k-th smallest key (all keys <= it are chosen); null: nothing chosen
   __device__ int64_t operator()(int64_t e) const { return kth && edge_key(seed, t, stream_id, e) <= *kth ? 1 : 0; }
 };
-
-// The k-th smallest key of edges [0, n) into state[0] (8-pass radix select; k >= 1).
-inline void select_kth(uint64_t seed, int32_t t, uint32_t stream_id, int64_t n, int64_t k, uint64_t* state,
-                       uint32_t* hist, hipStream_t st) {
-  hipLaunchKernelGGL(key_init_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, state, hist, (uint64_t)k);
-  const uint32_t grid = (uint32_t)std::max<int64_t>(1, std::min<int64_t>((n + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
-  for (int pass = 0; pass < 8; ++pass) {
-    hipLaunchKernelGGL(key_hist_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, seed, t, stream_id, n, pass,
-                       (const uint64_t*)state, hist);
-    hipLaunchKernelGGL(key_select_kernel, dim3(1), dim3(CLIP_BLOCK), 0, st, pass, state, hist);
-  }
-}
 
 // Partial resample: every chosen edge (key <= *kth) gets the endpoints the full resample of day t
 // gives it (dynamic_endpoints) and weight 1; the other edges are left as they are.
@@ -157,7 +105,7 @@ extern "C" int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float*
   uint32_t* hist = (uint32_t*)(base + align256(16));
   int64_t* tiles = (int64_t*)(base + align256(16) + align256(256 * 4));
   const uint32_t sid = STREAM_CLIP | (uint32_t)slot;
-  if (k > 0) select_kth(seed, t, sid, n, k, state, hist, st);
+  if (k > 0) select_edges(seed, t, sid, n, k, state, hist, st);
   SplitEdges split{p1, p2, beta, keep_p1, keep_p2, move_p1, move_p2, keep_beta, move_beta};
   return scan(Chosen{seed, t, sid, k > 0 ? (const uint64_t*)state : nullptr}, split, n, tiles, nullptr, st);
 }
@@ -183,7 +131,7 @@ extern "C" int cvk_resample_layer_frac(const cvk_layer* layer, int32_t layer_ind
   uint64_t* state = (uint64_t*)work;
   uint32_t* hist = (uint32_t*)((char*)work + align256(16));
   const uint32_t l = (uint32_t)layer_index;
-  select_kth(params->seed, t, STREAM_PARTIAL | l, layer->n_edges, k, state, hist, st);
+  select_edges(params->seed, t, STREAM_PARTIAL | l, layer->n_edges, k, state, hist, st);
   const uint32_t grid =
       (uint32_t)std::max<int64_t>(1, std::min<int64_t>((layer->n_edges + CLIP_BLOCK - 1) / CLIP_BLOCK, 4096));
   hipLaunchKernelGGL(partial_resample_kernel, dim3(grid), dim3(CLIP_BLOCK), 0, st, params->seed, t, l,

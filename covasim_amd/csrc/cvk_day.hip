@@ -266,6 +266,9 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
   __shared__ float s_beta[MAXV];
+  // the launch's span on the device wall clock (work.day_clock, optional)
+  unsigned long long* span = a.work.day_clock ? (unsigned long long*)a.work.day_clock + a.work.npts + 1 + 2 * a.t : nullptr;
+  if (span && threadIdx.x == 0) atomicMin(span, (unsigned long long)wall_clock64());
   const int nbw = MODE == BITS_L2 ? 0 : a.lds_words;
   const WaveQ Q = wave_queues(s_mem + nbw + (threadIdx.x >> 6) * EDGE_WAVE_WORDS, s_beta);
   if (threadIdx.x < MAXV) s_beta[threadIdx.x] = a.sim.beta_v[threadIdx.x];
@@ -280,6 +283,8 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   }
   const bool walked = walk_decision<EDGE_BLOCK>(a.work, a.src_mask, a.src_auto, a.src_max_permille, a.agent_blocks,
                                                 a.sim.n_agents) != 0;
+  if (a.count_src && a.src_mask && !walked && blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd((unsigned long long*)(a.work.ctl + CVK_CTL_SRC_STREAMED), 1ull);
   __syncthreads();
   const uint32_t* bits = s_mem;
   const int64_t n_wt = a.tile_start[a.n_layers];
@@ -347,6 +352,10 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     cur = nxt;
   }
   if (n_x) flush_exact(a.sim, a.seg[q_seg], a.t, Q, n_x);
+  if (span) {  // uniform over the grid
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(span + 1, (unsigned long long)wall_clock64());
+  }
 }
 
 // Launchers of every instantiation, indexed [mode][K == 4][GEN][PF].

@@ -45,6 +45,8 @@ constexpr uint32_t STREAM_PARTIAL = 0xB0000000u;  // | layer: edge choice of a p
 // Testing (| op slot) and contact tracing (| op slot << 8 | layer), cvk_diag.hip.
 constexpr uint32_t STREAM_TEST = 0x90000000u;
 constexpr uint32_t STREAM_TRACE = 0xA0000000u;
+constexpr uint32_t STREAM_ILI = 0xC0000000u;        // | op slot: the agents with influenza-like symptoms
+constexpr uint32_t STREAM_TRACE_CAP = 0xD0000000u;  // | op slot: the cases traced under a capacity
 
 // Per-infection draw slots (counter word c3 of STREAM_INFECT).
 constexpr uint32_t SLOT_EXP2INF = 0;

@@ -2,9 +2,13 @@
 //
 //   test_kernel          test_prob.apply (interventions.py:921-962) + People.test (people.py:589-617):
 //                        one pass over the agents; the test, true-positive and loss draws of agent i
-//                        are Philox words keyed by (i, t, TEST | slot)
+//                        are Philox words keyed by (i, t, TEST | slot). Options: subtarget (a bitmap
+//                        and values), ILI (the day's uniform choice of agents, by a radix select of
+//                        keys, cvk_select.h) and swab delay (swab_hist_kernel + swab_prob_kernel: the
+//                        symptomatic agents by days since onset, then the probability per day count)
 //   trace_cases_kernel   contact_tracing.select_cases (interventions.py:1069-1086): bitmap of the
-//                        agents diagnosed today (or, presumptive, exposed and tested today)
+//                        agents diagnosed today (or, presumptive, exposed and tested today); under a
+//                        capacity only the cases a radix select of keys chose
 //   trace_edges_kernel   Layer.find_contacts (base.py:1808, utils.py:131-147) for every traced layer at once: each
 //                        edge with a case at one end marks the other end's layer bit (atomicOr;
 //                        only the few edges at a case write)
@@ -30,6 +34,7 @@
 
 #include "cvk_host.h"
 #include "cvk_kernels.h"
+#include "cvk_select.h"
 
 namespace cvk {
 namespace {
@@ -91,10 +96,19 @@ __device__ inline bool quar_test(const cvk_diag_op& op, uint32_t f, const cvk_di
   }
 }
 
+// The key of agent i in a uniform choice of agents (ILI, tracing capacity): cvk_select.h's item key.
+__device__ inline uint64_t agent_key(uint64_t seed, int32_t t, uint32_t stream_id, int64_t i) {
+  return item_key(seed, t, stream_id, i);
+}
+
+// ili_thr: the day's k-th smallest ILI key (agents at or below it have influenza-like symptoms), or
+// null; swab_prob: the day's probability by days since onset (swab delay), or null.
 __global__ __launch_bounds__(DIAG_BLOCK) void test_kernel(cvk_people P, cvk_diag D, cvk_diag_op op, uint64_t seed,
-                                                          int32_t t) {
+                                                          int32_t t, const uint64_t* ili_thr, const double* swab_prob) {
   const float tf = (float)t;
   const uint32_t stream = STREAM_TEST | (uint32_t)op.slot;
+  const uint32_t ili_stream = STREAM_ILI | (uint32_t)op.slot;
+  const uint64_t thr = ili_thr ? *ili_thr : 0ull;
   uint32_t c[1] = {0};
   for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
@@ -104,11 +118,25 @@ __global__ __launch_bounds__(DIAG_BLOCK) void test_kernel(cvk_people P, cvk_diag
       if (i < P.n) {
         const uint32_t f = P.flags[i];
         if (!(f & CVK_DIAGNOSED)) {  // test_probs[diag_inds] = 0
-          const bool symp = (f & CVK_SYMPTOMATIC) != 0;
-          const bool q = quar_test(op, f, D, i, t);
-          const double p =
-              q ? (symp ? op.symp_quar_prob : op.asymp_quar_prob) : (symp ? op.symp_prob : op.asymp_prob);
-          if (p > 0.0) {  // binomial_arr: u < p (never for p <= 0)
+          // test_probs set piece by piece in the reference's order: symptomatic, ILI, asymptomatic,
+          // the two quarantine groups (ILI agents are in neither), then the subtarget
+          double p;
+          if (op.sub_mask && ((op.sub_mask[i >> 5] >> (i & 31)) & 1u)) {
+            p = op.sub_vals ? op.sub_vals[i] : op.sub_val;
+          } else if (f & CVK_SYMPTOMATIC) {
+            if (quar_test(op, f, D, i, t))
+              p = op.symp_quar_prob;
+            else if (swab_prob) {
+              const int32_t st = (int32_t)(tf - P.date_symptomatic[i]);  // 0 <= st < t < n_swab_bins
+              p = st >= 0 && st < op.n_swab_bins ? swab_prob[st] : 0.0;
+            } else
+              p = op.symp_prob;
+          } else if (ili_thr && agent_key(seed, t, ili_stream, i) <= thr) {
+            p = op.symp_prob;
+          } else {
+            p = quar_test(op, f, D, i, t) ? op.asymp_quar_prob : op.asymp_prob;
+          }
+          if (p > 0.0) {  // binomial_arr: u < p (never for p <= 0 or NaN)
             const u32x4 w = philox_seeded(seed, (uint32_t)i, (uint32_t)t, stream, 0);
             if (u01(w.x, w.y) < p) {
               tested = true;
@@ -133,18 +161,77 @@ __global__ __launch_bounds__(DIAG_BLOCK) void test_kernel(cvk_people P, cvk_diag
   block_part<1>(c, D.part, cols);
 }
 
+__global__ __launch_bounds__(DIAG_BLOCK) void fill_u32_kernel(uint32_t* p, int32_t n) {
+  for (int32_t k = blockIdx.x * DIAG_BLOCK + threadIdx.x; k < n; k += gridDim.x * DIAG_BLOCK) p[k] = 0u;
+}
+
+// The symptomatic agents by days since onset s = t - date_symptomatic (hist[s]) and all of them
+// (hist[bins]), as np.bincount and len in test_prob.apply (interventions.py:925-927).
+__global__ __launch_bounds__(DIAG_BLOCK) void swab_hist_kernel(cvk_people P, int32_t t, uint32_t* hist, int32_t bins) {
+  __shared__ uint32_t s_total;
+  if (threadIdx.x == 0) s_total = 0;
+  __syncthreads();
+  const float tf = (float)t;
+  uint32_t mine = 0;
+  for (int64_t i = (int64_t)blockIdx.x * DIAG_BLOCK + threadIdx.x; i < P.n; i += (int64_t)gridDim.x * DIAG_BLOCK) {
+    if (!(P.flags[i] & CVK_SYMPTOMATIC)) continue;
+    const int32_t s = (int32_t)(tf - P.date_symptomatic[i]);
+    if (s >= 0 && s < bins) atomicAdd(hist + s, 1u);
+    ++mine;
+  }
+  mine = wave_sum(mine);
+  if (lane_id() == 0 && mine) atomicAdd(&s_total, mine);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_total) atomicAdd(hist + bins, s_total);
+}
+
+// Per day count s: base[s] * (1 / (count[s] / n_symp)), in the reference's operation order
+// (interventions.py:926-933: inv_count = bincount / len, count = 1 / inv_count, pdf * prob * count).
+__global__ __launch_bounds__(DIAG_BLOCK) void swab_prob_kernel(const uint32_t* hist, const double* base, int32_t bins,
+                                                               double* prob) {
+  const double total = (double)hist[bins];
+  for (int32_t s = threadIdx.x; s < bins; s += DIAG_BLOCK) {
+    const uint32_t h = hist[s];
+    prob[s] = h ? base[s] * (1.0 / ((double)h / total)) : 0.0;
+  }
+}
+
 // Bitmap of today's cases, and the coarse map (bit k: some case among agents [k << shift, ...)).
+__device__ inline bool trace_case(const cvk_people& P, const cvk_diag& D, int32_t presumptive, float tf, int64_t i) {
+  return presumptive ? ((P.flags[i] & CVK_EXPOSED) && D.date_tested[i] == tf) : P.date_diagnosed[i] == tf;
+}
+
+// Eligible: today's cases; key: the tracing capacity's choice key.
+struct CaseKey {
+  cvk_people P;
+  cvk_diag D;
+  uint64_t seed;
+  int32_t presumptive, t;
+  uint32_t stream_id;
+  __device__ bool operator()(int64_t i, uint64_t& key) const {
+    if (!trace_case(P, D, presumptive, (float)t, i)) return false;
+    key = agent_key(seed, t, stream_id, i);
+    return true;
+  }
+};
+
+// cap_mode: 0 every case, 1 the cases whose capacity key is at most *cap_thr, 2 none (capacity 0).
 __global__ __launch_bounds__(DIAG_BLOCK) void trace_cases_kernel(cvk_people P, cvk_diag D, int32_t presumptive,
-                                                                 int32_t t, int32_t shift) {
+                                                                 int32_t t, int32_t shift, int32_t cap_mode,
+                                                                 const uint64_t* cap_thr, uint64_t seed,
+                                                                 uint32_t cap_stream) {
   const float tf = (float)t;
   const int64_t n_words = (P.n + 63) / 64;
+  const uint64_t thr = cap_mode == 1 ? *cap_thr : 0ull;
   for (int64_t base = (int64_t)blockIdx.x * DIAG_SPAN; base < P.n; base += span_stride()) {
 #pragma unroll
     for (int r = 0; r < DIAG_APT; ++r) {
       const int64_t i = base + r * DIAG_BLOCK + threadIdx.x;
       bool is_case = false;
-      if (i < P.n)
-        is_case = presumptive ? ((P.flags[i] & CVK_EXPOSED) && D.date_tested[i] == tf) : P.date_diagnosed[i] == tf;
+      if (i < P.n && cap_mode != 2) {
+        is_case = trace_case(P, D, presumptive, tf, i);
+        if (is_case && cap_mode == 1) is_case = agent_key(seed, t, cap_stream, i) <= thr;
+      }
       const uint64_t b = __ballot(is_case);
       const int64_t word = (base + r * DIAG_BLOCK + (threadIdx.x & ~63u)) >> 6;
       if (lane_id() == 0 && word < n_words) {
@@ -419,7 +506,8 @@ int check_common(const cvk_people* P, const cvk_params* prm, const cvk_work* W, 
   return CVK_OK;
 }
 
-int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, const cvk_layer* layers) {
+int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, const cvk_layer* layers,
+             int64_t n_agents, int32_t npts) {
   if (!op) return fail(CVK_E_INVALID, "null op");
   if (op->slot < 0 || op->slot > 255) return fail(CVK_E_INVALID, "op slot must be 0..255");
   if (op->kind == CVK_OP_TEST) {
@@ -427,9 +515,25 @@ int check_op(const cvk_diag_op* op, const cvk_diag* D, const cvk_params* prm, co
       return fail(CVK_E_INVALID, "unknown quarantine testing policy");
     if (op->n_quar_days < 0 || op->n_quar_days > CVK_MAX_QUAR_DAYS)
       return fail(CVK_E_INVALID, "n_quar_days out of range");
+    if (op->ili_counts) {
+      if (op->n_ili_days < 0) return fail(CVK_E_INVALID, "n_ili_days < 0");
+      for (int r = 0; r < op->n_ili_days; ++r)
+        if (op->ili_counts[r] < 0 || op->ili_counts[r] > n_agents)
+          return fail(CVK_E_INVALID, "ili_counts must lie in [0, n_agents]");
+      if (!D->select) return fail(CVK_E_INVALID, "ILI draws need cvk_diag.select");
+    }
+    if (op->n_swab_bins < 0) return fail(CVK_E_INVALID, "n_swab_bins < 0");
+    if (op->n_swab_bins > 0) {
+      if (!op->swab_base) return fail(CVK_E_INVALID, "swab delay without swab_base");
+      if (op->n_swab_bins < npts) return fail(CVK_E_INVALID, "n_swab_bins must be at least the sim's days");
+      if (op->n_swab_bins > D->swab_bins || !D->swab_hist || !D->swab_prob)
+        return fail(CVK_E_CAPACITY, "cvk_diag.swab_bins too small for the op's swab delay");
+    }
+    if (op->sub_vals && !op->sub_mask) return fail(CVK_E_INVALID, "sub_vals without sub_mask");
     return CVK_OK;
   }
   if (op->kind != CVK_OP_TRACE) return fail(CVK_E_INVALID, "unknown op kind");
+  if (op->capacity > 0 && !D->select) return fail(CVK_E_INVALID, "a tracing capacity needs cvk_diag.select");
   for (int l = 0; l < prm->n_layers; ++l) {
     if (!(op->trace_prob[l] > 0.0)) continue;
     if (!(op->trace_time[l] >= 0.0) || op->trace_time[l] >= (double)INT32_MAX / 2)
@@ -449,7 +553,22 @@ int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* p
   if (t < op->start_day || (op->end_day >= 0 && t > op->end_day)) return CVK_OK;
   const int grid = agent_grid(P->n);
   if (op->kind == CVK_OP_TEST) {
-    test_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t);
+    const uint64_t* ili_thr = nullptr;
+    const int64_t r = (int64_t)t - op->start_day;
+    if (op->ili_counts && r < op->n_ili_days && op->ili_counts[r] > 0) {
+      select_kth(AllItemsKey{prm->seed, t, STREAM_ILI | (uint32_t)op->slot}, P->n, op->ili_counts[r], D->select,
+                 (uint32_t*)(D->select + 2), st);
+      ili_thr = D->select;
+    }
+    const double* swab_prob = nullptr;
+    if (op->n_swab_bins > 0) {
+      const int32_t bins = op->n_swab_bins;
+      fill_u32_kernel<<<1, DIAG_BLOCK, 0, st>>>(D->swab_hist, bins + 1);
+      swab_hist_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, t, D->swab_hist, bins);
+      swab_prob_kernel<<<1, DIAG_BLOCK, 0, st>>>(D->swab_hist, op->swab_base, bins, D->swab_prob);
+      swab_prob = D->swab_prob;
+    }
+    test_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, *op, prm->seed, t, ili_thr, swab_prob);
     return check_launch("test_kernel");
   }
   TraceLayers L{};
@@ -468,7 +587,13 @@ int apply_impl(const cvk_people* P, const cvk_layer* layers, const cvk_params* p
   L.nl = nl;
   const int shift = coarse_shift(P->n);
   const int64_t n_cwords = coarse_words(P->n);
-  trace_cases_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, op->presumptive, t, shift);
+  const uint32_t cap_stream = STREAM_TRACE_CAP | (uint32_t)op->slot;
+  int32_t cap_mode = op->capacity < 0 ? 0 : op->capacity == 0 ? 2 : 1;
+  if (cap_mode == 1)
+    select_kth(CaseKey{*P, *D, prm->seed, op->presumptive, t, cap_stream}, P->n, op->capacity, D->select,
+               (uint32_t*)(D->select + 2), st);
+  trace_cases_kernel<<<grid, DIAG_BLOCK, 0, st>>>(*P, *D, op->presumptive, t, shift, cap_mode, D->select, prm->seed,
+                                                  cap_stream);
   int rc = check_launch("trace_cases_kernel");
   if (rc) return rc;
   const int64_t gx = (quads + TRACE_EDGE_BLOCK - 1) / TRACE_EDGE_BLOCK;
@@ -527,7 +652,7 @@ int check_diag_schedule(const cvk_people* P, const cvk_layer* layers, const cvk_
   if (imports->n_diag_ops < 0 || (imports->n_diag_ops > 0 && !imports->diag_ops))
     return fail(CVK_E_INVALID, "bad diag_ops");
   for (int j = 0; j < imports->n_diag_ops; ++j) {
-    rc = check_op(imports->diag_ops + j, imports->diag, prm, layers);
+    rc = check_op(imports->diag_ops + j, imports->diag, prm, layers, P->n, W->npts);
     if (rc) return rc;
   }
   return CVK_OK;
@@ -543,7 +668,7 @@ extern "C" int cvk_diag_apply(const cvk_people* people, const cvk_layer* layers,
                               cvk_work* work, const cvk_diag* diag, const cvk_diag_op* op, int32_t t, void* stream) {
   int rc = check_common(people, params, work, diag, t);
   if (rc) return rc;
-  rc = check_op(op, diag, params, layers);
+  rc = check_op(op, diag, params, layers, people->n, work->npts);
   if (rc) return rc;
   return apply_impl(people, layers, params, work, diag, op, t, (hipStream_t)stream);
 }

@@ -183,39 +183,153 @@ def _day_or_none(sim, day):
     return process_days(sim, day)[0]
 
 
+def process_daily_data(daily_data, sim, start_day):
+    """Per-day values from the intervention's start (interventions.py:111-147): a number is repeated
+    over the sim's days, a pandas Series / DataFrame with a date index is reindexed from the start
+    date (missing days 0), a sequence is used as it is. Named columns of loaded data are not held by
+    this sim."""
+    if daily_data is None:
+        return None
+    if isinstance(daily_data, str):
+        raise NotImplementedError('per-day data by column name needs data loaded into the sim')
+    if np.isscalar(daily_data):
+        return np.array([daily_data] * sim.npts)
+    try:
+        import pandas as pd
+    except ImportError:  # pragma: no cover
+        pd = None
+    if pd is not None and isinstance(daily_data, (pd.Series, pd.DataFrame)):
+        start_date = pd.Timestamp(sim['start_day']) + pd.Timedelta(days=start_day)
+        dateindex = pd.date_range(start_date, daily_data.index[-1])
+        out = daily_data.reindex(dateindex, fill_value=0).to_numpy()
+        return out.reshape(len(out), -1)[:, 0] if out.ndim == 2 and out.shape[1] == 1 else out
+    return np.asarray(daily_data)
+
+
+def get_subtargets(subtarget, sim):
+    """Indices and values of a subtarget (interventions.py:150-187): a dict with 'inds' and 'vals',
+    or a function of the sim returning one; 'inds' and 'vals' may be functions of the sim."""
+    if callable(subtarget):
+        subtarget = subtarget(sim)
+    if 'inds' not in subtarget:
+        raise ValueError(f'The subtarget dict must have keys "inds" and "vals", but you supplied {subtarget}')
+    inds = subtarget['inds'](sim) if callable(subtarget['inds']) else subtarget['inds']
+    vals = subtarget['vals'](sim) if callable(subtarget['vals']) else subtarget['vals']
+    if np.iterable(vals) and not isinstance(vals, str) and len(vals) != len(inds):
+        raise ValueError(f'Length of subtargeting indices ({len(inds)}) does not match length of values ({len(vals)})')
+    return inds, vals
+
+
+def subtarget_arrays(subtarget, sim, device):
+    """The subtarget of a day as device arrays: a bitmap of the agents ([(n + 31) / 32] words) and
+    their probabilities (float64 [n], assigned as test_probs[inds] = vals is, so a repeated index
+    keeps its last value), or the one value when vals is a number."""
+    import torch
+    n = sim.people.n
+    inds, vals = get_subtargets(subtarget, sim)
+    inds = np.asarray(inds)
+    if inds.dtype == bool:
+        inds = inds.nonzero()[0]
+    inds = inds.astype(np.int64).ravel()
+    if len(inds) and (inds.min() < -n or inds.max() >= n):
+        raise IndexError('subtarget indices out of range')
+    mask = np.zeros(n, dtype=bool)
+    mask[inds] = True
+    words = np.zeros((n + 31) // 32 * 4, dtype=np.uint8)
+    packed = np.packbits(mask, bitorder='little')
+    words[:len(packed)] = packed
+    mask_t = torch.from_numpy(words.view(np.int32).copy()).to(device)
+    if np.isscalar(vals):
+        return mask_t, None, float(vals)
+    probs = np.zeros(n, dtype=np.float64)
+    probs[inds] = np.asarray(vals, dtype=np.float64).ravel()
+    return mask_t, torch.from_numpy(probs).to(device), 0.0
+
+
 class test_prob(Intervention):
     """Test each agent with a probability set by its symptoms and quarantine state; true positives
     among the infectious are diagnosed test_delay days later unless lost to follow-up
-    (interventions.py:858-981, People.test people.py:589-617). Runs on the device (cvk_diag_apply);
-    subtarget, ili_prev and swab_delay are not supported."""
+    (interventions.py:858-981, People.test people.py:589-617). Runs on the device (cvk_diag_apply),
+    with the subtarget, ili_prev and swab_delay options: a subtarget of index / value arrays is
+    uploaded once, one given by functions of the sim is evaluated on the host on each of the
+    intervention's days (those days then run one by one); the ILI agents of a day are a uniform
+    choice on the device; the swab delay's pdf is tabulated by whole days since onset."""
 
     def __init__(self, symp_prob, asymp_prob=0.0, symp_quar_prob=None, asymp_quar_prob=None, quar_policy=None,
                  subtarget=None, ili_prev=None, sensitivity=1.0, loss_prob=0.0, test_delay=0, start_day=0,
                  end_day=None, swab_delay=None, **kwargs):
         super().__init__(**kwargs)
-        for name, val in (('subtarget', subtarget), ('ili_prev', ili_prev), ('swab_delay', swab_delay)):
-            if val is not None:
-                raise NotImplementedError(f'test_prob: {name} is not supported by the device implementation')
+        from .utils import get_pdf
         self._kwargs = dict(symp_prob=symp_prob, asymp_prob=asymp_prob, symp_quar_prob=symp_quar_prob,
-                            asymp_quar_prob=asymp_quar_prob, quar_policy=quar_policy, sensitivity=sensitivity,
-                            loss_prob=loss_prob, test_delay=test_delay, start_day=start_day, end_day=end_day,
-                            label=kwargs.get('label'))
+                            asymp_quar_prob=asymp_quar_prob, quar_policy=quar_policy, subtarget=subtarget,
+                            ili_prev=ili_prev, sensitivity=sensitivity, loss_prob=loss_prob, test_delay=test_delay,
+                            start_day=start_day, end_day=end_day, swab_delay=swab_delay, label=kwargs.get('label'))
         self.symp_prob = symp_prob
         self.asymp_prob = asymp_prob
         self.symp_quar_prob = symp_quar_prob if symp_quar_prob is not None else symp_prob
         self.asymp_quar_prob = asymp_quar_prob if asymp_quar_prob is not None else asymp_prob
         self.quar_policy = quar_policy if quar_policy else 'start'
+        self.subtarget = subtarget
+        self.ili_prev = ili_prev
         self.sensitivity = sensitivity
         self.loss_prob = loss_prob
         self.test_delay = test_delay
         self.start_day = start_day
         self.end_day = end_day
+        self.swab_delay = swab_delay
+        self.pdf = get_pdf(**(swab_delay or {}))
+        self._dev = {}
+
+    @property
+    def dynamic_subtarget(self):
+        """Whether the subtarget is given by functions of the sim (evaluated on each active day)."""
+        st = self.subtarget
+        return st is not None and (callable(st) or callable(st.get('inds')) or callable(st.get('vals')))
+
+    def active(self, t):
+        return t >= self.start_day and (self.end_day is None or t <= self.end_day)
 
     def initialize(self, sim):
         super().initialize(sim)
         self.start_day = _day_or_none(sim, self.start_day)
         self.end_day = _day_or_none(sim, self.end_day)
         self.days = [self.start_day, self.end_day]
+        dev = sim.people.device if hasattr(sim, 'people') else None
+        self._dev = {}
+        self._ili_counts = None
+        if self.ili_prev is not None:
+            daily = process_daily_data(self.ili_prev, sim, self.start_day)
+            pop_size = int(sim['pop_size'])
+            counts = np.array([int(v * pop_size) for v in daily], dtype=np.int64)
+            if len(counts) and (counts.min() < 0 or counts.max() > pop_size):
+                raise ValueError('ili_prev must give between 0 and pop_size agents per day')
+            self._ili_counts = counts
+        self._swab_base = None
+        if self.pdf:
+            # interventions.py:925-933 by whole days since onset s: pdf(s) * (symp_prob / (1 - s * symp_prob)
+            # where s * symp_prob < 1, else 1); the day's counts complete it on the device
+            s = np.arange(sim.npts, dtype=np.int32)
+            sp = np.ones(len(s))
+            inds = 1 > (s * self.symp_prob)
+            sp[inds] = self.symp_prob / (1 - s[inds] * self.symp_prob)
+            self._swab_base = np.asarray(self.pdf.pdf(s), dtype=np.float64) * sp
+            if dev is not None:
+                import torch
+                self._dev['swab_base'] = torch.from_numpy(self._swab_base).to(dev)
+        if self.subtarget is not None and not self.dynamic_subtarget and dev is not None:
+            self._dev['sub'] = subtarget_arrays(self.subtarget, sim, dev)
+
+    def prepare_day(self, sim, op):
+        """A subtarget given by functions of the sim: evaluated for today, into the op."""
+        if self.dynamic_subtarget and self.active(sim.t):
+            self._dev['sub'] = subtarget_arrays(self.subtarget, sim, sim.people.device)
+            self._set_sub(op)
+
+    def _set_sub(self, o):
+        sub = self._dev.get('sub')
+        o.sub_mask = sub[0].data_ptr() if sub is not None else None
+        o.sub_vals = sub[1].data_ptr() if sub is not None and sub[1] is not None else None
+        o.sub_val = sub[2] if sub is not None else 0.0
 
     def op(self, slot, layer_keys):
         """This intervention as a cvk_diag_op (slot: its index among the sim's interventions)."""
@@ -246,6 +360,13 @@ class test_prob(Intervention):
             o.n_quar_days = len(qd)
             for j, q in enumerate(qd):
                 o.quar_days[j] = q
+        self._set_sub(o)
+        if self._ili_counts is not None:
+            o.ili_counts = self._ili_counts.ctypes.data
+            o.n_ili_days = len(self._ili_counts)
+        if self._swab_base is not None:
+            o.n_swab_bins = len(self._swab_base)
+            o.swab_base = self._dev['swab_base'].data_ptr() if 'swab_base' in self._dev else None
         return o
 
     def apply(self, sim):
@@ -256,15 +377,18 @@ class contact_tracing(Intervention):
     """Trace the contacts of the agents diagnosed today (presumptive: exposed and tested today) in
     each layer with probability trace_probs[layer]; the living ones become known contacts and are
     scheduled to quarantine trace_time[layer] days later (interventions.py:984-1151). Runs on the
-    device (cvk_diag_apply); capacity is not supported."""
+    device (cvk_diag_apply); with a capacity, at most int(capacity) of the day's cases, chosen
+    uniformly on the device, are traced (the sims here run unscaled: rescale_vec is 1)."""
 
     def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,
                  quar_period=None, capacity=None, **kwargs):
         super().__init__(**kwargs)
-        if capacity is not None:
-            raise NotImplementedError('contact_tracing: capacity is not supported by the device implementation')
         self._kwargs = dict(trace_probs=trace_probs, trace_time=trace_time, start_day=start_day, end_day=end_day,
-                            presumptive=presumptive, quar_period=quar_period, label=kwargs.get('label'))
+                            presumptive=presumptive, quar_period=quar_period, capacity=capacity,
+                            label=kwargs.get('label'))
+        if capacity is not None and not capacity >= 0:
+            raise ValueError('contact_tracing: capacity must be >= 0')
+        self.capacity = capacity
         self.trace_probs = trace_probs
         self.trace_time = trace_time
         self.start_day = start_day
@@ -307,6 +431,7 @@ class contact_tracing(Intervention):
         o.start_day = self.start_day
         o.end_day = -1 if self.end_day is None else self.end_day
         o.presumptive = 1 if self.presumptive else 0
+        o.capacity = -1 if self.capacity is None else max(0, int(self.capacity / 1.0))  # rescale_vec[t] == 1
         for li, lk in enumerate(layer_keys):
             prob = float(self.trace_probs.get(lk, 0.0))
             o.trace_prob[li] = prob

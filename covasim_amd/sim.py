@@ -214,6 +214,10 @@ class Sim:
         w['diag_coarse'] = torch.zeros(int(_lib.lib().cvk_diag_coarse_words(n)), dtype=torch.int32, device=dev)
         w['diag_contacts'] = torch.zeros(n, dtype=torch.int32, device=dev)
         w['diag_part'] = torch.zeros(4 * _lib.DIAG_MAX_BLOCKS, dtype=torch.int32, device=dev)
+        w['diag_select'] = torch.zeros(130, dtype=torch.int64, device=dev)
+        bins = self.npts if any(isinstance(iv, cviv.test_prob) and iv.pdf for iv in ivs) else 0
+        w['diag_swab_hist'] = torch.zeros(bins + 1, dtype=torch.int32, device=dev)
+        w['diag_swab_prob'] = torch.zeros(max(1, bins), dtype=torch.float64, device=dev)
         d = _lib.DiagS()
         for k in _lib.DIAG_DATE_FIELDS:
             setattr(d, k, people._d[k].data_ptr())
@@ -223,6 +227,10 @@ class Sim:
         d.contacts = w['diag_contacts'].data_ptr()
         d.part = w['diag_part'].data_ptr()
         d.ring = ring
+        d.swab_bins = bins
+        d.select = w['diag_select'].data_ptr()
+        d.swab_hist = w['diag_swab_hist'].data_ptr()
+        d.swab_prob = w['diag_swab_prob'].data_ptr()
         self._diag = d
 
     def _build_params(self):
@@ -431,6 +439,8 @@ class Sim:
                     iv(self)
                     self._refresh_layers()
                 elif id(iv) in ops:
+                    if isinstance(iv, cviv.test_prob):
+                        iv.prepare_day(self, ops[id(iv)])
                     _lib.check(L.cvk_diag_apply(P, self._layers, pr, w, C.byref(self._diag), C.byref(ops[id(iv)]), t,
                                                 st), iv.label)
             if self._diag is not None:
@@ -477,6 +487,9 @@ class Sim:
         # day loop (a captured graph)
         manual_days = set(range(self.t, t1)) if self._custom_update else {
             d for iv in self._edge_interventions for d in iv.days if self.t <= d < t1}
+        for iv in self.pars['interventions']:  # subtargets given by functions of the sim: evaluated each day
+            if isinstance(iv, cviv.test_prob) and iv.dynamic_subtarget:
+                manual_days |= {d for d in range(self.t, t1) if iv.active(d)}
         t = self.t
         while t < t1:
             if t in manual_days:

@@ -13,7 +13,7 @@ import numpy as np
 from . import _lib
 
 __all__ = ['compute_viral_load', 'compute_trans_sus', 'compute_infections', 'set_seed', 'get_seed', 'choose',
-           'choose_r', 'default_int', 'default_float']
+           'choose_r', 'default_int', 'default_float', 'get_pdf']
 
 default_int = np.int32    # defaults.py:26-30 (precision 32)
 default_float = np.float32
@@ -146,3 +146,20 @@ def choose(max_n, n):
 def choose_r(max_n, n):
     """n integers from [0, max_n) with replacement (utils.py:446-459), on the host."""
     return np.random.choice(max_n, n, replace=True)
+
+
+def get_pdf(dist=None, par1=None, par2=None):
+    """Frozen scipy distribution of the delay from symptom onset to swab (utils.py:240-268): 'uniform'
+    on [par1, par1 + par2], or 'lognormal' with mean par1 and variance par2, shifted by -0.5 days;
+    None for no distribution."""
+    import scipy.stats as sps
+    if dist in ('None', 'none', None):
+        return None
+    if dist == 'uniform':
+        return sps.uniform(loc=par1, scale=par2)
+    if dist == 'lognormal':
+        mean = np.log(par1 ** 2 / np.sqrt(par2 + par1 ** 2))  # the underlying normal's mean and sigma
+        sigma = np.sqrt(np.log(par2 / par1 ** 2 + 1))
+        return sps.lognorm(sigma, loc=-0.5, scale=np.exp(mean))
+    raise NotImplementedError(f'The selected distribution "{dist}" is not implemented; choices are: none, uniform, '
+                              f'lognormal')

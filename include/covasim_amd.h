@@ -218,10 +218,13 @@ typedef struct {
                             of rel_sus x quarantine factor, others 0; code = float bits >> 18 rounded up */
   int32_t npts;
   int32_t pad;
-  uint64_t* day_clock;   /* optional [npts + 1] (null: off): the device wall clock when day t's first pass
+  uint64_t* day_clock;   /* optional [3 * npts + 1] (null: off): the device wall clock when day t's first pass
                             starts (entry t), and when the closing pass of a run's last day starts (entry
                             t1); a day's duration in a run is day_clock[t + 1] - day_clock[t] ticks of
-                            cvk_wall_clock_rate Hz. No reference counterpart (measurement). */
+                            cvk_wall_clock_rate Hz. Entries npts + 1 + 2t and npts + 2 + 2t: the earliest
+                            workgroup start (atomic min) and the latest workgroup end (atomic max) of day
+                            t's edge pass; the caller sets them to ~0 and 0 before the run. No reference
+                            counterpart (measurement). */
 } cvk_work;
 
 #define CVK_MAX_REDUCE_BLOCKS 65536
@@ -238,13 +241,15 @@ typedef struct {
 #define CVK_CTL_SRC_SLOTS 12      /* 64-bit count (words 12, 13) of source-list slots walked by edge passes,
                                      when CVK_TUNE_COUNT_SOURCES is on */
 #define CVK_CTL_SRC_AGENTS 14     /* 64-bit count (words 14, 15) of (agent, layer) lists walked */
+#define CVK_CTL_SRC_STREAMED 16   /* 64-bit count (words 16, 17) of edge passes that streamed the layers with
+                                     source lists (walk decision: too many infectious agents) */
 #define CVK_CTL_WORDS 64
 
 /* ---- library ---- */
 const char* cvk_last_error(void);
 /* ABI version: changes whenever a public struct, an entry point's signature or a tunable's meaning
  * changes. Bindings compare it with the value they were written for. */
-#define CVK_ABI_VERSION 7
+#define CVK_ABI_VERSION 8
 int cvk_version(void);
 /* No reference counterpart: the frequency (Hz) of the device wall clock of cvk_work.day_clock. */
 int cvk_wall_clock_rate(int64_t* hz_out);
@@ -375,7 +380,8 @@ int cvk_count_pass(const cvk_people* people, const cvk_params* params, cvk_work*
  * (i, t, 0x90000000 | slot, 0) -- words 0,1: tested (u < probability), words 2,3: true positive
  * (u < sensitivity) -- and (i, t, 0x90000000 | slot, 1) words 0,1: not lost (u < 1 - loss_prob);
  * the tracing of contact i found in layer l uses (i, t, 0xA0000000 | slot << 8 | l, 0), words 0,1
- * (u < trace_prob[l]). Uniforms are 53-bit doubles, compared with the double probabilities. */
+ * (u < trace_prob[l]). Uniforms are 53-bit doubles, compared with the double probabilities. The options
+ * (subtarget, ILI, swab delay, tracing capacity) follow the fields below. */
 #define CVK_OP_TEST 1
 #define CVK_OP_TRACE 2
 /* quarantine testing policies of get_quar_inds (interventions.py:682-710) */
@@ -400,12 +406,35 @@ typedef struct {
   int32_t quar_policy;          /* CVK_QP_* */
   int32_t n_quar_days;
   int32_t quar_days[CVK_MAX_QUAR_DAYS];
+  /* subtarget (interventions.py:150-187, 958-960): the agents in the device bitmap sub_mask
+   * [(n + 31) / 32] test with probability sub_vals[i] (device [n]), or sub_val when sub_vals is null,
+   * in place of the probability above (diagnosed agents still 0); null sub_mask: no subtarget */
+  const uint32_t* sub_mask;
+  const double* sub_vals;
+  double sub_val;
+  /* ili_prev (interventions.py:936-942): on day start_day + r, r < n_ili_days, ili_counts[r] agents
+   * (host array) chosen uniformly without replacement have influenza-like symptoms; those without
+   * COVID symptoms test with symp_prob (quarantined or not). The choice is the ili_counts[r] smallest
+   * keys (Philox (i, t, 0xC0000000 | slot, 0) word 0 << 32) | i. Null ili_counts: no ILI. */
+  const int64_t* ili_counts;
+  int32_t n_ili_days;
+  /* swab_delay (interventions.py:924-933): n_swab_bins > 0 replaces symp_prob, for the symptomatic
+   * agents outside quarantine testing, by swab_base[s] * (n_symp / count[s]), s = t - date_symptomatic,
+   * count[s] = symptomatic agents at s days since onset, n_symp = all symptomatic agents;
+   * swab_base (device [n_swab_bins]) = pdf(s) * (symp_prob / (1 - s * symp_prob) when s * symp_prob < 1,
+   * else 1); n_swab_bins must exceed every s (the sim's days suffice) and fit cvk_diag.swab_bins. */
+  int32_t n_swab_bins;
+  const double* swab_base;
   /* CVK_OP_TRACE */
   double trace_prob[CVK_MAX_LAYERS];      /* 0: the layer is not traced */
   double trace_time[CVK_MAX_LAYERS];      /* >= 0; quarantine starts on day int(t + trace_time) */
   int32_t trace_period[CVK_MAX_LAYERS];   /* int(quar_period - trace_time): days of quarantine */
   int32_t presumptive;          /* trace the exposed agents tested today, not those diagnosed today */
-  int32_t pad;
+  int32_t pad1;
+  /* capacity (interventions.py:1078-1083): >= 0: when more agents are to be traced today, capacity of
+   * them chosen uniformly without replacement are traced -- the smallest keys (Philox (i, t,
+   * 0xD0000000 | slot, 0) word 0 << 32) | i among them; < 0: no limit */
+  int64_t capacity;
 } cvk_diag_op;
 
 /* Testing and quarantine state beside cvk_people (device arrays of n agents). */
@@ -423,7 +452,10 @@ typedef struct {
                           quarantined, isolated); initialise to 0 (cvk_states_post sums them into the
                           day's row and clears them) */
   int32_t ring;        /* > the largest int(trace_time) of every tracing op */
-  int32_t pad;
+  int32_t swab_bins;   /* room of swab_hist / swab_prob: >= every test op's n_swab_bins (0: none) */
+  uint64_t* select;    /* [130] scratch of the uniform choices (ILI agents, traced cases under a capacity) */
+  uint32_t* swab_hist; /* [swab_bins + 1] scratch: symptomatic agents by days since onset, then all of them */
+  double* swab_prob;   /* [swab_bins] scratch: the day's test probability by days since onset */
 } cvk_diag;
 
 #define CVK_DIAG_MAX_BLOCKS 4096

@@ -33,20 +33,35 @@ class change_beta:
                         sim.pars['beta_layer'][lkey] = beta * self.changes[ind]
 
 
-def chosen_edges(seed, t, slot, n, k):
-    """Mask of the k edges of n with the smallest keys (philox(seed; e lo, t, 0x80000000 | slot,
-    e hi).x << 32) | e: a uniform choice without replacement, as cvu.choose in clip_edges."""
+def item_keys(seed, t, stream, items):
+    """Choice keys of items: (philox(seed; i lo, t, stream, i hi).x << 32) | i lo, distinct per item."""
     import numpy as np
     from . import philox as ph
-    mask = np.zeros(n, dtype=bool)
-    if k <= 0:
-        return mask
-    e = np.arange(n, dtype=np.uint64)
+    e = np.asarray(items, dtype=np.uint64)
     k0, k1 = ph.seed_key(seed)
-    w = ph.philox4x32(e & np.uint64(0xFFFFFFFF), t, 0x80000000 | slot, e >> np.uint64(32), k0, k1)[0]
-    keys = (np.asarray(w, dtype=np.uint64) << np.uint64(32)) | (e & np.uint64(0xFFFFFFFF))
+    w = ph.philox4x32(e & np.uint64(0xFFFFFFFF), t, stream, e >> np.uint64(32), k0, k1)[0]
+    return (np.asarray(w, dtype=np.uint64) << np.uint64(32)) | (e & np.uint64(0xFFFFFFFF))
+
+
+def choose_items(seed, t, stream, items, k):
+    """The k of items with the smallest keys (all of them when k >= len(items)): a uniform choice
+    without replacement, as cvu.choose / np.random.choice(replace=False)."""
+    import numpy as np
+    items = np.asarray(items, dtype=np.int64)
+    if k <= 0:
+        return items[:0]
+    if k >= len(items):
+        return items
+    keys = item_keys(seed, t, stream, items)
     kth = np.partition(keys, k - 1)[k - 1]
-    mask[keys <= kth] = True
+    return items[keys <= kth]
+
+
+def chosen_edges(seed, t, slot, n, k):
+    """Mask of the k edges of n with the smallest keys in stream 0x80000000 | slot (clip_edges)."""
+    import numpy as np
+    mask = np.zeros(n, dtype=bool)
+    mask[choose_items(seed, t, 0x80000000 | slot, np.arange(n), k)] = True
     return mask
 
 
@@ -124,26 +139,66 @@ def get_quar_inds(quar_policy, sim):
     return np.unique(np.concatenate([(people.date_quarantined == t - 1 - v).nonzero()[0] for v in q]))
 
 
+def get_pdf(dist=None, par1=None, par2=None):
+    """utils.py:240-268: scipy's uniform on [par1, par1 + par2], or the lognormal of mean par1 and
+    variance par2 shifted by -0.5; None for no distribution."""
+    import numpy as np
+    import scipy.stats as sps
+    if dist in ('None', 'none', None):
+        return None
+    if dist == 'uniform':
+        return sps.uniform(loc=par1, scale=par2)
+    if dist == 'lognormal':
+        mean = np.log(par1 ** 2 / np.sqrt(par2 + par1 ** 2))
+        sigma = np.sqrt(np.log(par2 / par1 ** 2 + 1))
+        return sps.lognorm(sigma, loc=-0.5, scale=np.exp(mean))
+    raise NotImplementedError(dist)
+
+
+def get_subtargets(subtarget, sim):
+    """interventions.py:150-187."""
+    if callable(subtarget):
+        subtarget = subtarget(sim)
+    inds = subtarget['inds'](sim) if callable(subtarget['inds']) else subtarget['inds']
+    vals = subtarget['vals'](sim) if callable(subtarget['vals']) else subtarget['vals']
+    return inds, vals
+
+
 class test_prob:
-    """interventions.py:858-981 (without subtarget, ili_prev and swab_delay): test probabilities
-    set piece by piece in the reference's order, one Bernoulli draw per agent, then People.test.
-    The draws of agent i are philox(i, t, 0x90000000 | slot, 0) (tested, true positive) and
-    philox(i, t, 0x90000000 | slot, 1) (not lost), slot = the intervention's index in the sim."""
+    """interventions.py:858-981: test probabilities set piece by piece in the reference's order, one
+    Bernoulli draw per agent, then People.test. The draws of agent i are philox(i, t, 0x90000000 |
+    slot, 0) (tested, true positive) and philox(i, t, 0x90000000 | slot, 1) (not lost), slot = the
+    intervention's index in the sim. Options: subtarget (interventions.py:958-960), ili_prev (the
+    day's int(ili_prev[t - start_day] * pop_size) agents with the smallest keys in stream 0xC0000000 |
+    slot, interventions.py:936-942) and swab_delay (interventions.py:924-933, with the pdf taken from
+    a table of its values at whole days 0..npts-1, which is where it is evaluated)."""
 
     def __init__(self, symp_prob, asymp_prob=0.0, symp_quar_prob=None, asymp_quar_prob=None, quar_policy=None,
-                 sensitivity=1.0, loss_prob=0.0, test_delay=0, start_day=0, end_day=None):
+                 subtarget=None, ili_prev=None, sensitivity=1.0, loss_prob=0.0, test_delay=0, start_day=0,
+                 end_day=None, swab_delay=None):
         self.symp_prob = symp_prob
         self.asymp_prob = asymp_prob
         self.symp_quar_prob = symp_quar_prob if symp_quar_prob is not None else symp_prob
         self.asymp_quar_prob = asymp_quar_prob if asymp_quar_prob is not None else asymp_prob
         self.quar_policy = quar_policy if quar_policy else 'start'
+        self.subtarget = subtarget
+        self.ili_prev = ili_prev
         self.sensitivity = sensitivity
         self.loss_prob = loss_prob
         self.test_delay = test_delay
         self.start_day = start_day
         self.end_day = end_day
+        self.pdf = get_pdf(**(swab_delay or {}))
+        self.last_ili = None
+        self.last_probs = None
 
     def initialize(self, sim):
+        import numpy as np
+        if self.ili_prev is not None and np.isscalar(self.ili_prev):
+            self.ili_prev = np.array([self.ili_prev] * sim.npts)
+        self.pdf_table = None
+        if self.pdf:
+            self.pdf_table = np.asarray(self.pdf.pdf(np.arange(sim.npts, dtype=np.int32)), dtype=np.float64)
         return self
 
     def __call__(self, sim):
@@ -153,21 +208,45 @@ class test_prob:
         if not _active(self, t):
             return
         n = people.n
+        seed = sim.pars['rand_seed']
         symp_inds = people.symptomatic.nonzero()[0]
-        asymp_inds = np.setdiff1d(np.arange(n), symp_inds)
+        symp_prob = self.symp_prob
+        if self.pdf:
+            symp_time = (t - people.date_symptomatic[symp_inds]).astype(np.int32)
+            inv_count = np.bincount(symp_time) / len(symp_time)
+            count = np.nan * np.ones(inv_count.shape)
+            count[inv_count != 0] = 1 / inv_count[inv_count != 0]
+            symp_prob = np.ones(len(symp_time))
+            inds = 1 > (symp_time * self.symp_prob)
+            symp_prob[inds] = self.symp_prob / (1 - symp_time[inds] * self.symp_prob)
+            symp_prob = self.pdf_table[symp_time] * symp_prob * count[symp_time]
+        ili_inds = np.array([], dtype=np.int64)
+        if self.ili_prev is not None:
+            rel_t = t - self.start_day
+            if rel_t < len(self.ili_prev):
+                n_ili = int(self.ili_prev[rel_t] * n)
+                ili_inds = choose_items(seed, t, px.STREAM_ILI | sim.iv_slot, np.arange(n), n_ili)
+                ili_inds = np.setdiff1d(ili_inds, symp_inds)
+        self.last_ili = ili_inds
+        asymp_inds = np.setdiff1d(np.setdiff1d(np.arange(n), symp_inds), ili_inds)
         quar_test_inds = get_quar_inds(self.quar_policy, sim)
         symp_quar_inds = np.intersect1d(quar_test_inds, symp_inds)
         asymp_quar_inds = np.intersect1d(quar_test_inds, asymp_inds)
         diag_inds = people.diagnosed.nonzero()[0]
         test_probs = np.zeros(n)
-        test_probs[symp_inds] = self.symp_prob
+        test_probs[symp_inds] = symp_prob
+        test_probs[ili_inds] = self.symp_prob
         test_probs[asymp_inds] = self.asymp_prob
         test_probs[symp_quar_inds] = self.symp_quar_prob
         test_probs[asymp_quar_inds] = self.asymp_quar_prob
+        if self.subtarget is not None:
+            sub_inds, sub_vals = get_subtargets(self.subtarget, sim)
+            test_probs[sub_inds] = sub_vals
         test_probs[diag_inds] = 0.0
+        self.last_probs = test_probs
         stream = px.STREAM_TEST | sim.iv_slot
-        u_test, u_pos = px.stream_uniforms(sim.pars['rand_seed'], t, np.arange(n), stream, 0)
-        u_keep, _ = px.stream_uniforms(sim.pars['rand_seed'], t, np.arange(n), stream, 1)
+        u_test, u_pos = px.stream_uniforms(seed, t, np.arange(n), stream, 0)
+        u_keep, _ = px.stream_uniforms(seed, t, np.arange(n), stream, 1)
         test_inds = (u_test < test_probs).nonzero()[0]
         people.test(test_inds, u_pos, u_keep, test_sensitivity=self.sensitivity, loss_prob=self.loss_prob,
                     test_delay=self.test_delay)
@@ -189,19 +268,23 @@ def find_contacts(p1, p2, inds):
 
 
 class contact_tracing:
-    """interventions.py:984-1151 (without capacity): the agents diagnosed today (or, presumptive,
-    exposed and tested today) are traced through each layer with trace_prob > 0; each contact found
-    in layer l is kept when philox(i, t, 0xA0000000 | slot << 8 | l, 0) < trace_prob[l], and the
-    living ones are notified and scheduled to quarantine on day int(t + trace_time[l])."""
+    """interventions.py:984-1151: the agents diagnosed today (or, presumptive, exposed and tested
+    today) are traced through each layer with trace_prob > 0 -- under a capacity, when there are more
+    of them, the int(capacity) with the smallest keys in stream 0xD0000000 | slot (select_cases,
+    :1078-1083); each contact found in layer l is kept when philox(i, t, 0xA0000000 | slot << 8 | l,
+    0) < trace_prob[l], and the living ones are notified and scheduled to quarantine on day
+    int(t + trace_time[l])."""
 
     def __init__(self, trace_probs=None, trace_time=None, start_day=0, end_day=None, presumptive=False,
-                 quar_period=None):
+                 quar_period=None, capacity=None):
         self.trace_probs = trace_probs
         self.trace_time = trace_time
         self.start_day = start_day
         self.end_day = end_day
         self.presumptive = presumptive
         self.quar_period = quar_period
+        self.capacity = capacity
+        self.last_traced = None
 
     def initialize(self, sim):
         keys = list(sim.layers.keys())
@@ -228,6 +311,12 @@ class contact_tracing:
         else:
             just_tested = (people.date_tested == t).nonzero()[0]
             trace_inds = just_tested[people.exposed[just_tested]]
+        if self.capacity is not None:
+            capacity = int(self.capacity / 1.0)  # rescale_vec[t] is 1 in unscaled sims
+            if len(trace_inds) > capacity:
+                trace_inds = np.sort(choose_items(sim.pars['rand_seed'], t, px.STREAM_TRACE_CAP | sim.iv_slot,
+                                                  trace_inds, capacity))
+        self.last_traced = trace_inds
         if not len(trace_inds):
             return {}
         keys = list(sim.layers.keys())

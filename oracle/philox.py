@@ -52,6 +52,8 @@ RESAMPLE_MAX_RETRY = 255
 STREAM_TEST = 0x90000000   # | slot
 STREAM_PARTIAL = 0xB0000000  # | layer: edge choice of a partial resample
 STREAM_TRACE = 0xA0000000  # | slot << 8 | layer
+STREAM_ILI = 0xC0000000    # | slot: the agents with influenza-like symptoms (test_prob ili_prev)
+STREAM_TRACE_CAP = 0xD0000000  # | slot: the cases traced under a capacity (contact_tracing)
 
 # Fixed per-infection draw slots (see People.infect restatement in covasim_oracle.py)
 SLOT_EXP2INF = 0

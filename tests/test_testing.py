@@ -183,10 +183,124 @@ def test_host_ops():
     ct.initialize(FakeSim())
     o = ct.op(4, FakeSim().layer_keys())
     assert list(o.trace_prob)[:4] == [1.0, 0.0, 0.0, 0.3] and o.trace_period[3] == 11 and ct.max_delay() == 2
+    o = cv.contact_tracing(capacity=10.7)
+    o.initialize(FakeSim())
+    assert o.op(1, FakeSim().layer_keys()).capacity == 10
+    assert cv.contact_tracing().op.__func__  # no capacity: -1
+    ct0 = cv.contact_tracing()
+    ct0.initialize(FakeSim())
+    assert ct0.op(0, FakeSim().layer_keys()).capacity == -1
+    with pytest.raises(ValueError):
+        cv.contact_tracing(capacity=-1)
     with pytest.raises(NotImplementedError):
-        cv.test_prob(0.1, ili_prev=0.02)
+        cv.test_prob(0.1, quar_policy=lambda sim: []).op(0, [])
     with pytest.raises(NotImplementedError):
-        cv.contact_tracing(capacity=10)
+        cv.test_prob(0.1, swab_delay=dict(dist='gamma', par1=1, par2=1))
+
+
+def test_choice_by_keys_is_exact_and_uniform():
+    from scipy import stats
+    items = np.arange(3, 3000, 7)
+    for k in (0, 1, 50, len(items) - 1, len(items), len(items) + 5):
+        got = oiv.choose_items(11, 4, 0xC0000001, items, k)
+        assert len(got) == min(k, len(items)) and np.isin(got, items).all() and len(np.unique(got)) == len(got)
+    n, k, days = 200, 30, 600
+    hits = np.zeros(n)
+    for t in range(days):
+        hits[oiv.choose_items(5, t, 0xD0000002, np.arange(n), k)] += 1
+    exp = days * k / n
+    chi2 = ((hits - exp) ** 2 / (exp * (1 - k / n))).sum()
+    assert stats.chi2.sf(chi2, n - 1) > 1e-3
+
+
+def test_ili_agents_are_tested_as_symptomatic():
+    # certain tests, no true positives: on a day without COVID symptoms exactly int(ili_prev * n) agents
+    # test (the day's ILI agents); later days test the union of the symptomatic and the ILI agents
+    pop = make_population(n=3000)
+    iv = oiv.test_prob(symp_prob=1.0, asymp_prob=0.0, sensitivity=0.0, ili_prev=0.05, start_day=2)
+    osim, res = run_oracle(pop, [iv], seed=4, n_days=40, beta=0.03)
+    n_ili = int(0.05 * 3000)
+    assert res['new_tests'][0] == res['new_tests'][1] == 0
+    assert res['new_tests'][2] == n_ili  # nobody is symptomatic on day 2
+    sym = res['n_symptomatic']
+    assert np.all(res['new_tests'][2:] >= n_ili) and np.all(res['new_tests'][2:] <= n_ili + sym[2:])
+    assert sym[-1] > 50 and res['new_tests'][-1] > n_ili
+    iv2 = oiv.test_prob(symp_prob=1.0, ili_prev=[0.0, 0.1, 0.2], start_day=5, sensitivity=0.0)
+    osim, res = run_oracle(pop, [iv2], seed=4, n_days=12, beta=0.0, pop_infected=0)  # nobody is ever symptomatic
+    assert list(res['new_tests'][:9]) == [0, 0, 0, 0, 0, 0, 300, 600, 0]
+
+
+def test_swab_delay_probabilities():
+    from covasim_amd.utils import get_pdf
+    # the pdf's parameters: lognormal of mean par1 and variance par2, shifted by -0.5; uniform on
+    # [par1, par1 + par2]; the oracle's and the library's are the same distribution
+    for pdf in (oiv.get_pdf('lognormal', 10, 170), get_pdf('lognormal', 10, 170)):
+        assert abs(pdf.mean() - 9.5) < 1e-9 and abs(pdf.var() - 170) < 1e-6
+    for pdf in (oiv.get_pdf('uniform', 2, 5), get_pdf('uniform', 2, 5)):
+        assert pdf.pdf(1.99) == 0 and pdf.pdf(2.0) == pdf.pdf(6.9) == 0.2 and pdf.pdf(7.01) == 0
+    assert oiv.get_pdf() is None and get_pdf(None) is None
+    # per day since onset s the symptomatic agents share pdf(s) * f(s) (f: symp_prob / (1 - s * symp_prob)
+    # below 1, else 1) equally: the sum of their probabilities is n_symp * sum over the present s
+
+    class Probe(oiv.test_prob):
+        def __call__(self, sim):
+            out = super().__call__(sim)
+            p = sim.people
+            symp = p.symptomatic.nonzero()[0]
+            self.checks.append((sim.t, symp, (sim.t - p.date_symptomatic[symp]).astype(np.int32),
+                                self.last_probs.copy(), p.diagnosed.copy()))
+            return out
+
+    pop = make_population(n=4000)
+    sp = 0.2
+    iv = Probe(symp_prob=sp, swab_delay=dict(dist='lognormal', par1=3, par2=4), sensitivity=0.0)
+    iv.checks = []
+    osim, res = run_oracle(pop, [iv], seed=5, n_days=40, beta=0.04)
+    assert len(iv.checks) == osim.npts
+    checked = 0
+    for t, symp, s, tp, diag in iv.checks:
+        if not len(symp):
+            continue
+        assert not diag.any()  # sensitivity 0: nobody diagnosed, so no probability was zeroed
+        present = np.unique(s)
+        with np.errstate(divide='ignore'):
+            f = np.where(present * sp < 1, sp / (1 - present * sp), 1.0)
+        want = len(symp) * np.sum(iv.pdf.pdf(present) * f)
+        assert np.isclose(tp[symp].sum(), want, rtol=1e-12), t
+        # agents at the same day since onset share one probability
+        for v in present[:5]:
+            assert len(np.unique(tp[symp[s == v]])) == 1
+        checked += 1
+    assert checked > 20
+
+
+def test_subtarget_overrides_and_capacity_limits_tracing():
+    pop = make_population(n=3000)
+    inds = np.array([5, 17, 17, 900, 2999])
+    vals = np.array([0.0, 0.5, 1.0, 1.0, 1.0])  # index 17 keeps its last value
+    iv = oiv.test_prob(symp_prob=0.0, subtarget=dict(inds=inds, vals=vals), sensitivity=0.0)
+    osim, res = run_oracle(pop, [iv], seed=2, n_days=10, beta=0.0, pop_infected=0)
+    assert list(res['new_tests']) == [3] * 11  # 17, 900, 2999 every day; 5 never
+    assert np.isnan(osim.people.date_tested[5]) and osim.people.date_tested[[17, 900, 2999]].tolist() == [10] * 3
+    # a function of the sim, and a scalar value
+    iv = oiv.test_prob(symp_prob=0.0, subtarget=lambda sim: dict(inds=np.arange(sim.t), vals=1.0), sensitivity=0.0)
+    osim, res = run_oracle(pop, [iv], seed=2, n_days=10, beta=0.0, pop_infected=0)
+    assert list(res['new_tests']) == list(range(11))
+    # capacity: the traced cases are min(capacity, cases) of them
+    tp = oiv.test_prob(symp_prob=0.5, asymp_prob=0.05)
+    traced = []
+
+    class Cap(oiv.contact_tracing):
+        def __call__(self, sim):
+            out = super().__call__(sim)
+            if self.last_traced is not None:
+                cases = (sim.people.date_diagnosed == sim.t).nonzero()[0]
+                traced.append((len(cases), self.last_traced))
+                assert np.isin(self.last_traced, cases).all()
+            return out
+    osim, res = run_oracle(pop, [tp, Cap(trace_probs=1.0, capacity=4)], seed=2, n_days=40, beta=0.04)
+    assert max(c for c, _ in traced) > 4
+    assert all(len(tr) == min(c, 4) for c, tr in traced)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -209,6 +323,19 @@ CASES = {
                                    quar_policy='both', start_day=5, end_day=45),
                        m.contact_tracing(trace_probs=0.6, trace_time=1.5, presumptive=True, quar_period=10)],
         pars=dict(n_days=50, beta=0.03, use_waning=True)),
+    'options': dict(
+        ivs=lambda m: [m.test_prob(symp_prob=0.25, asymp_prob=0.004, ili_prev=0.01, test_delay=1,
+                                   swab_delay=dict(dist='lognormal', par1=3, par2=4),
+                                   subtarget=dict(inds=np.arange(0, 20000, 7), vals=np.linspace(0, 0.1, 2858))),
+                       m.contact_tracing(trace_probs=dict(h=1.0, s=0.4, w=0.4, c=0.1), trace_time=1, capacity=15)],
+        pars=dict(n_days=50, beta=0.03, use_waning=False)),
+    'options_dynamic': dict(
+        ivs=lambda m: [m.test_prob(symp_prob=0.2, ili_prev=[0.0] * 8 + [0.03] * 30, start_day=3,
+                                   swab_delay=dict(dist='uniform', par1=0, par2=6),
+                                   subtarget=lambda sim: dict(inds=np.flatnonzero(sim.people.age > 70), vals=0.05)),
+                       m.contact_tracing(trace_probs=0.8, trace_time=dict(h=0, s=2, w=2, c=2), presumptive=True,
+                                         capacity=3)],
+        pars=dict(n_days=40, beta=0.035, use_waning=True)),
     'policies': dict(
         ivs=lambda m: [m.test_prob(symp_prob=0.15, quar_policy=[0, 4], test_delay=1),
                        m.contact_tracing(trace_probs=dict(h=0.9, w=0.4), trace_time=dict(h=1, w=2), start_day=10),
