@@ -386,11 +386,17 @@ def run_relax(w, args):
     for _ in range(reps):
         r32.forward_np(theta)
     fwd_s = (time.perf_counter() - t1) / reps
+    stream = torch.cuda.current_stream()  # the stream the relaxation's kernels are launched on
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     t2 = time.perf_counter()
+    ev0.record(stream)
     for _ in range(reps):
         r32.forward_np(theta)
         g = r32.backward_np(theta, c)
+    ev1.record(stream)
+    torch.cuda.synchronize()
     fb_s = (time.perf_counter() - t2) / reps
+    fb_ev_s = ev0.elapsed_time(ev1) / 1e3 / reps
     h = 1e-4
     fd = np.zeros_like(theta)
     for k in range(len(theta)):
@@ -406,12 +412,33 @@ def run_relax(w, args):
     obs_t = torch.as_tensor(observed[:r32.n_days])
     loss_truth = float(log_incidence_loss(torch.as_tensor(r32.forward_np(theta)), obs_t))
     n, T = w['pop_size'], r32.n_days
+    # algorithmic HBM bytes of one forward + adjoint (every kernel; the neighbour gathers of tr, lbar and
+    # age bin are served from the caches and not counted): per day t a forward (relax_trans_kernel: rel_trans,
+    # bin, tr and min(t, kmax) past deltas; relax_forward_kernel: indptr (int64) per layer, bin, s read and
+    # written, delta, 4 per neighbour slot), the adjoint's recomputed forward (plus L and s recorded), and
+    # the adjoint (relax_adj_agent_kernel: bin, L, s, sbar r/w, lbar, min(kmax, T-1-t) ibar rows;
+    # relax_pack_kernel: tr, lbar, bin, the 16-byte neighbour record written; relax_adj_edge_kernel:
+    # indptr per layer, bin, tr, lbar, rel_trans, ibar written, 4 per slot)
+    slots = int(sum(nbr.numel() for _, nbr in r32._csr))
+    L_ = r32.n_layers
+    kmax = min(r32.w.shape[1] - 1, T)
+    fwd = sum(n * (9 + 4 * min(t, kmax)) + n * (8 * L_ + 13) + 4 * slots for t in range(T))
+    rec = 8 * n * T
+    adj = sum(n * (21 + 4 * min(kmax, T - 1 - t)) + n * 25 + n * (8 * L_ + 17) + 4 * slots for t in range(T))
+    fb_bytes = 2 * fwd + rec + adj
+    fb_gbs = fb_bytes / fb_ev_s / 1e9
     out = {
         'metric': METRIC, 'value': n * T / fb_s, 'unit': 'agent-days/s', 'n_gpus': 1, 'steps': T, 'warmup': 2 * T,
         'ms_per_step': fb_s / T * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32', 'data': 'synthetic',
         'config': {'workload': w['desc'], 'agents': n, 'days': T, 'params': int(len(theta)),
                    'setup_s': round(setup_s, 1)},
+        'roofline': {'bound': 'hbm', 'kernel': 'relaxation forward + adjoint (every kernel, 60 days)',
+                     'achieved': fb_gbs, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s', 'frac': fb_gbs / HBM_PEAK_GBS,
+                     'traffic': None, 'algorithmic_bytes_per_step': fb_bytes, 'step_ms_hip_events': fb_ev_s * 1e3,
+                     'neighbour_slots': slots, 'note': 'the neighbour gathers (one 16-byte record per slot in the '
+                     'adjoint, one tr value in the forward) are served from L2 / MALL and bound the two edge '
+                     'kernels; they are not in the bytes'},
         'relaxation': {'forward_ms': fwd_s * 1e3, 'forward_adjoint_ms': fb_s * 1e3,
                        'grad_rel_err_vs_fp64_central_fd': grad_err, 'fd_step_relative': h,
                        'calibration_iters': len(hist), 'calibration_s': cal_s,

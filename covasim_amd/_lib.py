@@ -213,7 +213,8 @@ class BatchDesc(C.Structure):
                 ('n_agents_total', C.c_int64), ('agent_offsets_host', C.c_void_p), ('edge_starts_host', C.c_void_p),
                 ('edge_counts_host', C.c_void_p), ('params_host', C.c_void_p), ('dynamic_host', C.c_void_p),
                 ('people', PeopleS), ('p1', C.c_void_p * MAX_LAYERS), ('p2', C.c_void_p * MAX_LAYERS),
-                ('beta', C.c_void_p * MAX_LAYERS), ('beta_ones', C.c_int32 * MAX_LAYERS)]
+                ('beta', C.c_void_p * MAX_LAYERS), ('beta_ones', C.c_int32 * MAX_LAYERS),
+                ('sources', C.POINTER(SourcesS))]
 
 
 class BatchImports(C.Structure):

@@ -120,6 +120,14 @@ class BatchSim:
             desc.p2[li] = self._layers[lk]['p2'].data_ptr()
             desc.beta[li] = self._layers[lk]['beta'].data_ptr()
             desc.beta_ones[li] = beta_ones[li]
+        # the sims' source lists of their static layers (built when each sim initialised; local indices)
+        self._sources = (_lib.SourcesS * max(1, self.S * L))()
+        for si, sim in enumerate(sims):
+            for li in range(L):
+                ptr = sim._layers[li].sources
+                if ptr:
+                    self._sources[si * L + li] = ptr.contents
+        desc.sources = C.cast(self._sources, C.POINTER(_lib.SourcesS))
         self._desc = desc
         self.log_capacity = int(log_capacity)
         handle = C.c_void_p()

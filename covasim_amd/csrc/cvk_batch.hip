@@ -140,8 +140,35 @@ struct BatchEdgeArgs {
   int32_t lds_words;           // bitmap words the LDS stage holds (larger sims read L2)
   int32_t t;
   uint32_t gen_mask;           // bit l: layer l's endpoints of day t come from the resample stream (GEN)
+  uint32_t src_mask;           // bit l: layer l has source lists (every sim); walked sims skip its tiles
+  // source work items (SRC_ITEM_AGENTS agents of one listed segment each) of the walk phase
+  const SrcView* srcs;         // [S * L]
+  const int32_t* item_seg;     // [n_items] segment of each item
+  const int64_t* seg_item_start;  // [S * L + 1] first item of each segment
+  int64_t n_items;             // 0: no walk phase
+  const uint32_t* walk;        // [S] nonzero: sim s walks its listed layers today
   BatchReduce reduce;
 };
+
+// The day's walk decision of every sim (block s): walk its listed layers unless automatic and the agent
+// pass set more than max_permille infectious bits per thousand of its agents (the partial column DC_BITS
+// of the sim's agent-pass blocks), as the single-sim edge pass decides.
+__global__ __launch_bounds__(256) void batch_walk_kernel(BatchCommon C, int32_t automatic, int32_t max_permille,
+                                                         uint32_t* walk) {
+  __shared__ uint32_t s_w[4];
+  const int32_t s = blockIdx.x;
+  const BatchSim bs = C.sims[s];
+  uint32_t cnt = 0;
+  for (uint32_t b = threadIdx.x; b < bs.n_blocks; b += 256)
+    cnt += C.partials[(int64_t)DC_BITS * C.part_stride + bs.b_off + b];
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0) s_w[threadIdx.x >> 6] = cnt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t inf = (uint64_t)s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    walk[s] = !automatic || inf * 1000u <= (uint64_t)max_permille * (uint64_t)bs.n ? 1u : 0u;
+  }
+}
 
 // Workgroup b owns the wave tiles [T*b/G, T*(b+1)/G) of the day, ordered by (sim, layer). It walks
 // them one sim at a time: the sim's infectious bitmap is staged in LDS (when it fits), then the
@@ -156,6 +183,40 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
   const WaveQ Q = wave_queues(s_mem + a.lds_words + (threadIdx.x >> 6) * EDGE_WAVE_WORDS, s_beta);
   if (a.reduce.active)  // the agent pass's counts, one sim per block
     for (int32_t s = blockIdx.x; s < C.n_sims; s += gridDim.x) batch_reduce_sim(C, a.reduce, s);
+  if (a.n_items) {  // walk phase: source items over the grid's waves, items of non-walking sims skipped
+    // the waves walk different sims, so each keeps its current sim's beta_v in its own LDS row
+    __shared__ float s_wbeta[BLOCK / 64][MAXV];
+    WaveQ QW = Q;
+    QW.beta = s_wbeta[threadIdx.x >> 6];
+    uint64_t cnt[2] = {0, 0};
+    uint32_t n_w = 0;
+    int32_t w_seg = -1, w_sim = -1;
+    SimView SW;
+    SegView LW;
+    const int64_t stride = (int64_t)gridDim.x * (BLOCK / 64);
+    for (int64_t it = (int64_t)blockIdx.x * (BLOCK / 64) + (threadIdx.x >> 6); it < a.n_items; it += stride) {
+      const int32_t seg = __builtin_amdgcn_readfirstlane(a.item_seg[it]);
+      const int32_t s = seg / C.n_layers;
+      if (!__builtin_amdgcn_readfirstlane(a.walk[s])) continue;
+      if (seg != w_seg) {
+        if (n_w) flush_exact(SW, LW, a.t, QW, n_w);  // with the queued sim's beta_v still in place
+        n_w = 0;
+        w_seg = seg;
+        SW = uniform_copy(a.views[s]);
+        LW = uniform_copy(a.segs[seg]);
+        if (s != w_sim) {
+          w_sim = s;
+          wave_sync_lds();
+          if (lane_id() < (uint32_t)MAXV) s_wbeta[threadIdx.x >> 6][lane_id()] = gmem(a.views)[s].beta_v[lane_id()];
+          wave_sync_lds();
+        }
+      }
+      const SrcView R = uniform_copy(a.srcs[seg]);
+      source_item_pass<BITS_L2>(SW, LW, R, a.t, nullptr, QW, (it - a.seg_item_start[seg]) * (int64_t)SRC_ITEM_AGENTS,
+                                n_w, cnt);
+    }
+    if (n_w) flush_exact(SW, LW, a.t, QW, n_w);
+  }
   const int64_t first = a.n_tiles * blockIdx.x / gridDim.x, last = a.n_tiles * (blockIdx.x + 1) / gridDim.x;
   int64_t tile0 = first;
   while (tile0 < last) {  // block-uniform: one sim of the range per iteration
@@ -185,6 +246,7 @@ __global__ __launch_bounds__(BLOCK) void batch_edge_kernel(BatchCommon C, BatchE
       const int64_t tile = tile0 + k;
       if (tile >= sub_last) break;
       const int32_t seg = a.tile_seg[tile];
+      if (((a.src_mask >> (seg % C.n_layers)) & 1u) && a.walk[s]) continue;  // walked above
       WaveTile<K> T;
       T.seg = seg;
       T.e0 = (tile - a.seg_tile_start[seg]) * (int64_t)(64 * 4 * K);
@@ -524,6 +586,13 @@ struct Batch {
   uint32_t* d_all = nullptr;
   uint32_t* d_short = nullptr;
   int64_t* d_k = nullptr;        // [k_slots][S] counts for choose calls
+  // source lists of the static layers (walked on quiet days)
+  uint32_t src_mask = 0;
+  int64_t n_items = 0;
+  SrcView* d_srcs = nullptr;           // [S * L]
+  int32_t* d_item_seg = nullptr;       // [n_items]
+  int64_t* d_seg_item_start = nullptr;  // [S * L + 1]
+  uint32_t* d_walk = nullptr;          // [S]
   int64_t k_slots = 0;
   // edge-pass timing (cvk_batch_time_edges): an event pair per day around the edge launch, eager runs only
   bool time_edges = false;
@@ -534,7 +603,7 @@ struct Batch {
     void* ptrs[] = {d_sims, d_params, d_segs, d_views, d_a_off, d_seg_tile_start, d_tile_seg, d_block_sim,
                     d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bound, d_bitmap, d_winner, d_touched, d_ctl,
                     d_results, d_results_f, d_partials, d_partials_f, d_log, d_hist, d_prefix, d_remain,
-                    d_all, d_short, d_k};
+                    d_all, d_short, d_k, d_srcs, d_item_seg, d_seg_item_start, d_walk};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
@@ -677,7 +746,8 @@ static int launch_day_begin(Batch& b, int32_t t, int prologue, int transitions, 
   return check_launch("cvk_batch day_begin");
 }
 
-static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t st, bool gen) {
+// walk: the day's walk decisions are in b.d_walk (batch_walk_kernel ran after the agent pass).
+static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t st, bool gen, bool walk) {
   int64_t budget = 0;
   int rc = edge_budget_for(b, &budget);
   if (rc) return rc;
@@ -697,6 +767,14 @@ static int launch_edge(Batch& b, int32_t t, const BatchReduce& red, hipStream_t 
   a.lds_words = (int32_t)lds_words;
   a.t = t;
   a.reduce = red;
+  if (walk) {
+    a.src_mask = b.src_mask;
+    a.srcs = b.d_srcs;
+    a.item_seg = b.d_item_seg;
+    a.seg_item_start = b.d_seg_item_start;
+    a.n_items = b.n_items;
+    a.walk = b.d_walk;
+  }
   if (gen)
     for (int l = 0; l < b.L; ++l)
       if (b.dynamic[l]) a.gen_mask |= 1u << l;
@@ -735,7 +813,11 @@ static int launch_infect(Batch& b, int32_t t, int count_flows, hipStream_t st, i
   ia.count_flows = count_flows;
   ia.allow_beds = count_flows;
   ia.n_variants = b.V;
-  ia.infect_blocks = (uint32_t)std::max(1, b.n_cu);
+  {  // as the single-sim infect pass: one sweep's workgroups, between one and three per CU
+    const int64_t n_cu = std::max(1, b.n_cu);
+    const int64_t want = (b.N / 32 + INFECT_BLOCK) / INFECT_BLOCK;
+    ia.infect_blocks = (uint32_t)std::min<int64_t>(std::max<int64_t>(want, n_cu), 3 * n_cu);
+  }
   uint32_t rs_blocks = 0;
   if (resample_t >= 0 && b.n_rs_chunks > 0) {
     ia.rs_t = resample_t;
@@ -838,8 +920,16 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       }
     }
     BatchReduce red{1, t, t > t0 ? 1 : 0, 1};
+    const int mode = source_lists_mode();
+    const bool walk = b.n_items > 0 && mode != 0;
+    if (walk) {  // which sims walk their listed layers today
+      hipLaunchKernelGGL(batch_walk_kernel, dim3(b.S), dim3(256), 0, st, b.common(), mode == 2 ? 1 : 0,
+                         source_permille(), b.d_walk);
+      rc = check_launch("cvk_batch walk decision");
+      if (rc) return rc;
+    }
     if (timed && hipEventRecord(b.ev_begin[t], st) != hipSuccess) return fail(CVK_E_HIP, "cvk_batch: event record failed");
-    rc = launch_edge(b, t, red, st, gen);
+    rc = launch_edge(b, t, red, st, gen, walk);
     if (rc) return rc;
     if (timed) {
       if (hipEventRecord(b.ev_end[t], st) != hipSuccess) return fail(CVK_E_HIP, "cvk_batch: event record failed");
@@ -956,6 +1046,39 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
     }
   }
   b->n_rs_chunks = (int64_t)rs_seg.size();
+  // source lists: a static layer is listed when every sim with edges in it has lists
+  std::vector<SrcView> srcs((size_t)b->S * b->L);
+  memset(srcs.data(), 0, srcs.size() * sizeof(SrcView));
+  if (d->sources) {
+    for (int l = 0; l < b->L; ++l) {
+      if (b->dynamic[l]) continue;
+      bool all = true, any = false;
+      for (int32_t s = 0; s < b->S && all; ++s) {
+        const cvk_sources& R = d->sources[(size_t)s * b->L + l];
+        const int64_t ne = b->segs[(size_t)s * b->L + l].n;
+        if (ne == 0) continue;
+        if (!R.indptr || !R.target || !R.edge_dir || R.n_slots < 0 || R.n_slots > 2 * ne) all = false;
+        any = true;
+      }
+      if (!all || !any) continue;
+      b->src_mask |= 1u << l;
+      for (int32_t s = 0; s < b->S; ++s) {
+        const cvk_sources& R = d->sources[(size_t)s * b->L + l];
+        if (b->segs[(size_t)s * b->L + l].n == 0) continue;
+        srcs[(size_t)s * b->L + l] = SrcView{R.indptr, R.target, R.edge_dir};
+      }
+    }
+  }
+  std::vector<int32_t> item_seg;
+  std::vector<int64_t> seg_item_start((size_t)b->S * b->L + 1, 0);
+  for (int32_t seg = 0; seg < b->S * b->L; ++seg) {
+    seg_item_start[seg] = (int64_t)item_seg.size();
+    if (!((b->src_mask >> (seg % b->L)) & 1u) || !srcs[seg].indptr) continue;
+    const int64_t n = b->a_off[seg / b->L + 1] - b->a_off[seg / b->L];
+    for (int64_t k = 0; k < (n + SRC_ITEM_AGENTS - 1) / SRC_ITEM_AGENTS; ++k) item_seg.push_back(seg);
+  }
+  seg_item_start[(size_t)b->S * b->L] = (int64_t)item_seg.size();
+  b->n_items = (int64_t)item_seg.size();
   // device buffers
   int64_t max_blocks = 0;
   for (int32_t s = 0; s < b->S; ++s) max_blocks += (b->sims[s].n + AGENT_BLOCK - 1) / AGENT_BLOCK;
@@ -988,6 +1111,13 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
   rc = rc ? rc : dev_alloc(&b->d_remain, b->S);
   rc = rc ? rc : dev_alloc(&b->d_all, b->S);
   rc = rc ? rc : dev_alloc(&b->d_short, b->S);
+  rc = rc ? rc : dev_alloc(&b->d_srcs, std::max<int64_t>(1, (int64_t)srcs.size()));
+  rc = rc ? rc : upload(b->d_srcs, srcs.data(), (int64_t)srcs.size());
+  rc = rc ? rc : dev_alloc(&b->d_item_seg, std::max<int64_t>(1, b->n_items));
+  rc = rc ? rc : upload(b->d_item_seg, item_seg.data(), b->n_items);
+  rc = rc ? rc : dev_alloc(&b->d_seg_item_start, (int64_t)seg_item_start.size());
+  rc = rc ? rc : upload(b->d_seg_item_start, seg_item_start.data(), (int64_t)seg_item_start.size());
+  rc = rc ? rc : dev_alloc(&b->d_walk, b->S);
   if (rc) return rc;
   // per-sim views for the edge pass (pointers offset to each sim's slices)
   b->views.resize(b->S);

@@ -515,6 +515,8 @@ static int g_overlap_resample = 1;  // tunable: the fused day loop resamples ins
 static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
 static int g_edge_prefetch = 0;     // tunable: issue the next wave tile's endpoint loads before the candidates
 int overlap_resample_enabled() { return g_overlap_resample; }
+int source_lists_mode() { return g_source_lists; }
+int source_permille() { return g_source_permille; }
 bool generate_dynamic_for(int64_t dyn_edges, bool diag) {
   return !diag && dyn_edges > 0 && (g_gen_dynamic == 1 || (g_gen_dynamic == 2 && dyn_edges >= GEN_AUTO_MIN_EDGES));
 }
@@ -985,7 +987,13 @@ static int infect_impl(const cvk_people* people, const cvk_layer* layers, const 
   ia.n_layers = params->n_layers;
   ia.count_flows = count_flows;
   ia.allow_beds = allow_beds;
-  const int grid = std::min(std::max(caps().n_cu, 1), INFECT_PART_BLOCKS);
+  // one touched-bitmap word per thread and sweep: enough workgroups for one sweep, at least one per CU and at
+  // most three (the occupancy the kernel's register budget allows), so large populations keep more waves
+  // in flight through the latency-bound outcome tree
+  const int n_cu = std::max(caps().n_cu, 1);
+  const int64_t want = (people->n / 32 + INFECT_BLOCK) / INFECT_BLOCK;
+  const int grid = (int)std::min<int64_t>(std::min<int64_t>(std::max<int64_t>(want, n_cu), 3 * (int64_t)n_cu),
+                                          INFECT_PART_BLOCKS);
   ia.partial = partial_blocks && count_flows ? 1 : 0;
   if (partial_blocks) *partial_blocks = ia.partial ? (uint32_t)grid : 0u;
   ia.infect_blocks = (uint32_t)grid;

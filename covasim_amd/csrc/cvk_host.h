@@ -17,6 +17,9 @@ int overlap_resample_enabled();
 // CVK_TUNE_GENERATE_DYNAMIC applied to a run with dyn_edges dynamic-layer edges in all (diag: the run
 // tests and traces, which reads each day's contacts, so they are written).
 bool generate_dynamic_for(int64_t dyn_edges, bool diag);
+// CVK_TUNE_SOURCE_LISTS (0 never, 1 always, 2 by the day's infectious share) and CVK_TUNE_SOURCE_PERMILLE.
+int source_lists_mode();
+int source_permille();
 
 // Testing and tracing step of the day loop (cvk_diag.hip): the ops of `imports`, then
 // update_states_post; nothing when imports->diag is null.

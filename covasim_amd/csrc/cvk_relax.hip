@@ -66,6 +66,7 @@ struct RelaxDev {
   R* lbar;               // [n]
   double* epart;         // [n_days][n_blocks] per-block sums of E_t
   double* gpart;         // [RELAX_MAX_PARAMS][n_blocks] per-block gradient sums
+  R* pack;               // [n][4] the adjoint's per-neighbour inputs of the day: tr, lbar * sig(bin), sig(bin), 0
   int32_t n_blocks;
   R seed_frac;
 };
@@ -165,6 +166,17 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_agent_kernel(RelaxDev<R
   D.sbar[a] = dbar * (-r_expm1(L)) + sb_next * e;
 }
 
+// Between the two phases: what phase 2 reads of every neighbour j, packed in one 4-wide record (one
+// gather per neighbour slot instead of three: tr_j, lbar_j and j's age bin).
+template <class R>
+__global__ __launch_bounds__(RELAX_BLOCK) void relax_pack_kernel(RelaxDev<R> D, RelaxTheta<R> th, const R* tr) {
+  typedef R r4 __attribute__((ext_vector_type(4)));
+  const int64_t j = (int64_t)blockIdx.x * RELAX_BLOCK + threadIdx.x;
+  if (j >= D.n) return;
+  const R sig = th.sig[D.bin[j]];
+  ((r4*)D.pack)[j] = r4{tr[j], D.lbar[j] * sig, sig, (R)0};
+}
+
 // Phase 2 of the adjoint of day t: Ibar_a(t) (source role) and the parameter partials (target role).
 template <class R>
 __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_edge_kernel(RelaxDev<R> D, RelaxTheta<R> th, int32_t t,
@@ -192,27 +204,22 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_edge_kernel(RelaxDev<R>
       const int32_t* nbr = D.nbr[l];
       const int64_t k1 = D.indptr[l][a + 1];
       for (int64_t k = D.indptr[l][a]; k < k1; k += RELAX_UNROLL) {  // gathers in flight together, sums in order
+        typedef R r4 __attribute__((ext_vector_type(4)));
         int32_t jj[RELAX_UNROLL];
-        R trj[RELAX_UNROLL], lbj[RELAX_UNROLL];
-        uint8_t bj[RELAX_UNROLL];
+        r4 pj[RELAX_UNROLL];
 #pragma unroll
         for (int u = 0; u < RELAX_UNROLL; ++u) jj[u] = k + u < k1 ? nbr[k + u] : -1;
 #pragma unroll
-        for (int u = 0; u < RELAX_UNROLL; ++u) {
-          const bool ok = jj[u] >= 0;
-          trj[u] = ok ? tr[jj[u]] : (R)0;
-          lbj[u] = ok ? D.lbar[jj[u]] : (R)0;
-          bj[u] = ok ? D.bin[jj[u]] : (uint8_t)0;
-        }
+        for (int u = 0; u < RELAX_UNROLL; ++u) pj[u] = jj[u] >= 0 ? ((const r4*)D.pack)[jj[u]] : r4{0, 0, 0, 0};
 #pragma unroll
         for (int u = 0; u < RELAX_UNROLL; ++u) {
           if (jj[u] < 0) break;
-          const R tr_j = trj[u];
+          const R tr_j = pj[u].x;
           const R x_in = bbs * tr_j;  // j -> a
           if (x_in < xmax) c += tr_j / (x_in - (R)1);
-          const R sig_j = th.sig[bj[u]];
+          const R sig_j = pj[u].z;
           const R x_out = (bb * sig_j) * tr_a;  // a -> j
-          if (x_out < xmax) src += (lbj[u] * sig_j) / (x_out - (R)1);
+          if (x_out < xmax) src += pj[u].y / (x_out - (R)1);  // (lbar_j * sig_j) / (x_out - 1)
         }
       }
       ibar += bb * src;
@@ -374,6 +381,8 @@ struct Relax : RelaxBase {
         const int64_t off = (int64_t)(t - t0) * D.n;
         hipLaunchKernelGGL(relax_adj_agent_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, t, (R)ebar_host[t],
                            (const R*)(seg_L + off), (const R*)(seg_s + off));
+        hipLaunchKernelGGL(relax_pack_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th,
+                           (const R*)(seg_tr + off));
         hipLaunchKernelGGL(relax_adj_edge_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, t,
                            (const R*)(seg_tr + off));
       }
@@ -417,6 +426,7 @@ static int relax_create(const cvk_relax_desc* d, RelaxBase** out) {
   rc = rc ? rc : r->alloc(&D.lbar, D.n);
   rc = rc ? rc : r->alloc(&D.epart, (int64_t)D.n_days * D.n_blocks);
   rc = rc ? rc : r->alloc(&D.gpart, (int64_t)RELAX_MAX_PARAMS * D.n_blocks);
+  rc = rc ? rc : r->alloc(&D.pack, 4 * D.n);
   rc = rc ? rc : r->alloc(&r->seg_L, (int64_t)r->ckpt * D.n);
   rc = rc ? rc : r->alloc(&r->seg_s, (int64_t)r->ckpt * D.n);
   rc = rc ? rc : r->alloc(&r->seg_tr, (int64_t)r->ckpt * D.n);

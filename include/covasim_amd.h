@@ -538,6 +538,11 @@ typedef struct {
   int32_t* p2[CVK_MAX_LAYERS];
   float* beta[CVK_MAX_LAYERS];
   int32_t beta_ones[CVK_MAX_LAYERS];   /* nonzero when every weight of the layer is 1.0f */
+  /* Source lists (cvk_sources, local indices) of sim s's static layer l at sources[s * n_layers + l]
+   * (host array; null: none). A static layer with lists for every sim is walked, for the sims with at
+   * most CVK_TUNE_SOURCE_PERMILLE infectious agents per thousand (the same rule as cvk_edge_pass),
+   * instead of streamed. The results are identical either way. */
+  const cvk_sources* sources;
 } cvk_batch_desc;
 
 /* Import events of a batch: on day days[j], counts[j * n_sims + s] susceptible agents of sim s are

@@ -87,6 +87,32 @@ def test_batch_equals_individual_runs_hybrid_dynamic(gpu, batch_generate):
             assert np.array_equal(got, alone[si].people.contacts['c'][col]), (si, col)
 
 
+@pytest.fixture
+def source_lists(request):
+    """CVK_TUNE_SOURCE_LISTS for the batch (1: the listed layers are always walked, 0: always streamed);
+    restored to the default (2: walked on quiet days) after."""
+    from covasim_amd import _lib
+    yield request.param
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_SOURCE_LISTS, 2), 'tuning')
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('source_lists', [1, 0], indirect=True)
+def test_batch_source_lists_walked_or_streamed(gpu, source_lists):
+    import covasim_amd as cv
+    from covasim_amd import _lib
+    common = dict(use_waning=True, dynam_layer=dict(c=1))
+    alone = run_alone(HYBRID_SPEC, 30, 'hybrid', **common)  # the default rule (walked on quiet days)
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_SOURCE_LISTS, source_lists), 'tuning')
+    batch = cv.BatchSim(make_sims(HYBRID_SPEC, 30, 'hybrid', **common))
+    assert batch.sims[0]._sources  # the static layers have lists
+    batch.run()
+    for si, sim in enumerate(batch.sims):
+        assert sim.results['cum_infections'].values[-1] > 30
+        assert_same_results(sim, alone[si])
+        assert_same_state(batch, si, alone[si])
+
+
 @pytest.mark.gpu
 def test_batch_equals_individual_runs_variants_waning(gpu):
     import covasim_amd as cv
