@@ -48,8 +48,9 @@ WORKLOADS = {
                      desc='256 independent hybrid sims, pop sizes log-uniform over 10k-500k, beta uniform over '
                           '0.010-0.025, per-sim seeds, 120 days, layers as hybrid1m, one launch per phase per day'),
     'relax1m': dict(pop_size=1_000_000, pop_type='hybrid', n_days=60, households='categorical', relax=True,
-                    pars=dict(beta=0.016, use_waning=False, pop_infected=20),
-                    desc='differentiable relaxation on the hybrid 1M population (static layers), 60 days: '
+                    pars=dict(beta=0.016, use_waning=False, pop_infected=20, dynam_layer=dict(c=1)),
+                    desc='differentiable relaxation on the hybrid 1M population (h/s/w held, c resampled daily '
+                         'and taken in expectation), 60 days: '
                          'float32 forward + hand-written adjoint with checkpoints every 10 days, gradient wrt '
                          '4 layer betas, beta and 10 age-bin rel_sus; calibration to a stochastic incidence curve'),
     'interv1m': dict(pop_size=1_000_000, pop_type='hybrid', n_days=180, households='categorical', interventions=True,
@@ -419,12 +420,15 @@ def run_relax(w, args):
     # the adjoint (relax_adj_agent_kernel: bin, L, s, sbar r/w, lbar, min(kmax, T-1-t) ibar rows;
     # relax_pack_kernel: tr, lbar, bin, the 16-byte neighbour record written; relax_adj_edge_kernel:
     # indptr per layer, bin, tr, lbar, rel_trans, ibar written, 4 per slot)
-    slots = int(sum(nbr.numel() for _, nbr in r32._csr))
+    slots = int(sum(lay[1].numel() for lay in r32._csr if lay is not None))
+    n_dyn = sum(1 for c_l in r32.dyn_slots if c_l)
     L_ = r32.n_layers
     kmax = min(r32.w.shape[1] - 1, T)
-    fwd = sum(n * (9 + 4 * min(t, kmax)) + n * (8 * L_ + 13) + 4 * slots for t in range(T))
+    L_ = L_ - n_dyn  # the resampled layers have no lists: their expected term reads tr once (4 B per agent)
+    fwd = sum(n * (9 + 4 * min(t, kmax)) + n * (8 * L_ + 13) + 4 * n * (n_dyn > 0) + 4 * slots for t in range(T))
     rec = 8 * n * T
-    adj = sum(n * (21 + 4 * min(kmax, T - 1 - t)) + n * 25 + n * (8 * L_ + 17) + 4 * slots for t in range(T))
+    adj = sum(n * (21 + 4 * min(kmax, T - 1 - t)) + n * 25 + n * (8 * L_ + 17) + 9 * n * (n_dyn > 0) + 4 * slots
+              for t in range(T))
     fb_bytes = 2 * fwd + rec + adj
     fb_gbs = fb_bytes / fb_ev_s / 1e9
     out = {

@@ -233,7 +233,8 @@ class RelaxDesc(C.Structure):
     _fields_ = [('n_agents', C.c_int64), ('n_layers', C.c_int32), ('n_days', C.c_int32), ('kernel_len', C.c_int32),
                 ('checkpoint_every', C.c_int32), ('precision', C.c_int32), ('pad', C.c_int32),
                 ('seed_frac', C.c_double), ('indptr', C.c_void_p * MAX_LAYERS), ('nbr', C.c_void_p * MAX_LAYERS),
-                ('age_bin', C.c_void_p), ('rel_trans', C.c_void_p), ('w_host', C.c_void_p)]
+                ('age_bin', C.c_void_p), ('rel_trans', C.c_void_p), ('w_host', C.c_void_p),
+                ('dyn_slots', C.c_double * MAX_LAYERS)]
 
 
 class FinalizeDesc(C.Structure):

@@ -25,6 +25,14 @@
 //   dJ/dsigma_{bin(a)} += beta Lbar_a sum_l beta_l C_{a,l}
 // The forward keeps s at every checkpoint day; the backward re-runs each checkpoint segment's
 // forward (storing L, tr, s for its days) and then walks the segment backwards.
+//
+// A layer resampled every day enters in expectation (c_l = 2 E_l / N slots per agent, uniform sources,
+// self-pairs left out): with x_{l,b} = beta beta_l sigma_b, g(y) = log1p(-min(y, XMAX)) and
+// S_{l,b}(t) = sum_j g(x_{l,b} tr_j), its term of L_i is c_l / N (S_{l,bin(i)} - g(x_{l,bin(i)} tr_i)). Its
+// adjoint needs, per day, Lambda_b = sum_{bin(i)=b} Lbar_i, G_{l,b} = sum_j g'(x_{l,b} tr_j) tr_j and
+// H_{l,b} = sum_{bin(i)=b} Lbar_i g'(x_{l,b} tr_i) tr_i (block partials summed in a fixed order):
+//   Ibar_a += rel_trans_a c_l / N sum_b x_{l,b} g'(x_{l,b} tr_a) (Lambda_b - [bin(a) = b] Lbar_a)
+//   xbar_{l,b} = c_l / N (Lambda_b G_{l,b} - H_{l,b}), which feeds dJ/dbeta, dJ/dbeta_l and dJ/dsigma_b.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -69,7 +77,17 @@ struct RelaxDev {
   R* pack;               // [n][4] the adjoint's per-neighbour inputs of the day: tr, lbar * sig(bin), sig(bin), 0
   int32_t n_blocks;
   R seed_frac;
+  // layers resampled every day (in expectation)
+  int32_t n_dyn;
+  int32_t dyn_layer[CVK_MAX_LAYERS];  // their layer indices
+  R dyn_slots[CVK_MAX_LAYERS];        // c_l per layer (0: the layer's neighbour lists)
+  double* dyn_part;   // [(RELAX_BINS + 2 n_dyn RELAX_BINS)][n_blocks] per-block sums
+  double* dyn_fsum;   // [n_dyn][RELAX_BINS] the day's S_{l,b} (forward)
+  double* dyn_red;    // [RELAX_BINS + 2 n_dyn RELAX_BINS]: Lambda_b, then G_{l,b}, then H_{l,b} (adjoint day)
+  double* g_dyn;      // [RELAX_MAX_PARAMS] the dynamic layers' parameter partials, summed over days
 };
+
+
 
 __device__ inline float r_log1p(float x) { return log1pf(x); }
 __device__ inline double r_log1p(double x) { return log1p(x); }
@@ -77,6 +95,12 @@ __device__ inline float r_expm1(float x) { return expm1f(x); }
 __device__ inline double r_expm1(double x) { return expm1(x); }
 __device__ inline float r_exp(float x) { return expf(x); }
 __device__ inline double r_exp(double x) { return exp(x); }
+
+// g(y) = log1p(-min(y, XMAX)) and g'(y) (0 above XMAX)
+template <class R>
+__device__ inline R relax_g(R y, R xmax) { return r_log1p(-(y < xmax ? y : xmax)); }
+template <class R>
+__device__ inline R relax_gp(R y, R xmax) { return y < xmax ? (R)1 / (y - (R)1) : (R)0; }
 
 // Block sum of per-thread doubles into out (thread 0 of the block returns it). Fixed order.
 __device__ inline double block_sum_d(double v, double* s_w) {
@@ -115,6 +139,7 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_forward_kernel(RelaxDev<R> 
     const R xmax = (R)RELAX_XMAX;
     R L = 0;
     for (int l = 0; l < D.n_layers; ++l) {
+      if (D.dyn_slots[l] != (R)0) continue;  // resampled daily: the expected term below
       const R bb = (th.beta * th.bl[l]) * sig;
       const int32_t* nbr = D.nbr[l];
       const int64_t k1 = D.indptr[l][i + 1];
@@ -136,6 +161,12 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_forward_kernel(RelaxDev<R> 
         }
       }
     }
+    for (int k = 0; k < D.n_dyn; ++k) {  // c_l / N (S_{l,b} - g(x_{l,b} tr_i))
+      const int l = D.dyn_layer[k];
+      const int b = D.bin[i];
+      const R own = relax_g<R>(((th.beta * th.bl[l]) * sig) * tr[i], xmax);
+      L += D.dyn_slots[l] * ((R)(D.dyn_fsum[k * RELAX_BINS + b] / (double)D.n) - own / (R)D.n);
+    }
     const R s0 = D.s[i];
     const R inf = s0 * (-r_expm1(L));
     D.s[i] = s0 * r_exp(L);
@@ -146,6 +177,72 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_forward_kernel(RelaxDev<R> 
   }
   const double tot = block_sum_d(e_part, s_w);
   if (threadIdx.x == 0) D.epart[(int64_t)t * D.n_blocks + blockIdx.x] = tot;
+}
+
+// The day's S_{l,b} = sum_j g(x_{l,b} tr_j) of every daily-resampled layer: per-block sums (row k * 10 + b).
+template <class R>
+__global__ __launch_bounds__(RELAX_BLOCK) void relax_dyn_sum_kernel(RelaxDev<R> D, RelaxTheta<R> th, const R* tr) {
+  __shared__ double s_w[RELAX_BLOCK / 64];
+  const int64_t j = (int64_t)blockIdx.x * RELAX_BLOCK + threadIdx.x;
+  const R xmax = (R)RELAX_XMAX;
+  const R trj = j < D.n ? tr[j] : (R)0;
+  for (int k = 0; k < D.n_dyn; ++k) {  // uniform: every thread reaches the barriers
+    const int l = D.dyn_layer[k];
+    for (int b = 0; b < RELAX_BINS; ++b) {
+      const double v = j < D.n ? (double)relax_g<R>(((th.beta * th.bl[l]) * th.sig[b]) * trj, xmax) : 0.0;
+      const double tot = block_sum_d(v, s_w);
+      if (threadIdx.x == 0) D.dyn_part[(int64_t)(k * RELAX_BINS + b) * D.n_blocks + blockIdx.x] = tot;
+    }
+  }
+}
+
+// The adjoint day's sums: Lambda_b (rows 0..9), G_{l,b} (rows 10 + k * 10 + b), H_{l,b} (rows 10 + 10 n_dyn + ...).
+template <class R>
+__global__ __launch_bounds__(RELAX_BLOCK) void relax_dyn_adj_sum_kernel(RelaxDev<R> D, RelaxTheta<R> th,
+                                                                        const R* tr) {
+  __shared__ double s_w[RELAX_BLOCK / 64];
+  const int64_t a = (int64_t)blockIdx.x * RELAX_BLOCK + threadIdx.x;
+  const R xmax = (R)RELAX_XMAX;
+  const bool ok = a < D.n;
+  const int ba = ok ? D.bin[a] : -1;
+  const R tra = ok ? tr[a] : (R)0;
+  const R lb = ok ? D.lbar[a] : (R)0;
+  const int64_t nb = D.n_blocks;
+  for (int b = 0; b < RELAX_BINS; ++b) {
+    const double tot = block_sum_d(ba == b ? (double)lb : 0.0, s_w);
+    if (threadIdx.x == 0) D.dyn_part[(int64_t)b * nb + blockIdx.x] = tot;
+  }
+  for (int k = 0; k < D.n_dyn; ++k) {
+    const int l = D.dyn_layer[k];
+    for (int b = 0; b < RELAX_BINS; ++b) {
+      const R x = (th.beta * th.bl[l]) * th.sig[b];
+      const R gt = ok ? relax_gp<R>(x * tra, xmax) * tra : (R)0;
+      double tot = block_sum_d((double)gt, s_w);
+      if (threadIdx.x == 0) D.dyn_part[(int64_t)(RELAX_BINS + k * RELAX_BINS + b) * nb + blockIdx.x] = tot;
+      tot = block_sum_d(ba == b ? (double)(lb * gt) : 0.0, s_w);
+      if (threadIdx.x == 0)
+        D.dyn_part[(int64_t)(RELAX_BINS + (D.n_dyn + k) * RELAX_BINS + b) * nb + blockIdx.x] = tot;
+    }
+  }
+}
+
+// xbar_{l,b} = c_l / N (Lambda_b G_{l,b} - H_{l,b}) into the parameter partials (one thread, fixed order).
+template <class R>
+__global__ void relax_dyn_grad_kernel(RelaxDev<R> D, RelaxTheta<R> th) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int nl = D.n_layers;
+  for (int k = 0; k < D.n_dyn; ++k) {
+    const int l = D.dyn_layer[k];
+    for (int b = 0; b < RELAX_BINS; ++b) {
+      const double lam = D.dyn_red[b];
+      const double G = D.dyn_red[RELAX_BINS + k * RELAX_BINS + b];
+      const double H = D.dyn_red[RELAX_BINS + (D.n_dyn + k) * RELAX_BINS + b];
+      const double xbar = (double)D.dyn_slots[l] / (double)D.n * (lam * G - H);
+      D.g_dyn[0] += xbar * (double)th.bl[l] * (double)th.sig[b];
+      D.g_dyn[1 + l] += xbar * (double)th.beta * (double)th.sig[b];
+      D.g_dyn[1 + nl + b] += xbar * (double)th.beta * (double)th.bl[l];
+    }
+  }
 }
 
 // Phase 1 of the adjoint of day t: Dbar, Lbar_i(t), sbar_i(t).
@@ -198,6 +295,7 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_edge_kernel(RelaxDev<R>
 #pragma unroll
     for (int l = 0; l < CVK_MAX_LAYERS; ++l) {
       if (l >= D.n_layers) break;
+      if (D.dyn_slots[l] != (R)0) continue;  // resampled daily: below
       const R bb = th.beta * th.bl[l];
       const R bbs = bb * sig_a;
       R c = 0, src = 0;
@@ -225,6 +323,16 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_adj_edge_kernel(RelaxDev<R>
       ibar += bb * src;
       sum_blc += th.bl[l] * c;
       g_bl[l] = (double)(th.beta * lbar_a * sig_a * c);
+    }
+    for (int k = 0; k < D.n_dyn; ++k) {  // the daily-resampled layers' source role
+      const int l = D.dyn_layer[k];
+      R acc = 0;
+      for (int b = 0; b < RELAX_BINS; ++b) {
+        const R x = (th.beta * th.bl[l]) * th.sig[b];
+        const R lam = (R)D.dyn_red[b] - (b == b_a ? lbar_a : (R)0);
+        acc += x * relax_gp<R>(x * tr_a, xmax) * lam;
+      }
+      ibar += D.dyn_slots[l] / (R)D.n * acc;
     }
     D.ibar[(int64_t)t * D.n + a] = (R)D.rt[a] * ibar;
     g_beta = (double)(lbar_a * sig_a * sum_blc);
@@ -327,6 +435,11 @@ struct Relax : RelaxBase {
     for (int32_t t = t0; t < t1; ++t) {
       R* tr = rec ? seg_tr + (int64_t)(t - t0) * D.n : tr_scratch;
       hipLaunchKernelGGL(relax_trans_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, t, tr);
+      if (D.n_dyn) {
+        hipLaunchKernelGGL(relax_dyn_sum_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, (const R*)tr);
+        hipLaunchKernelGGL(relax_reduce_kernel, dim3(D.n_dyn * RELAX_BINS), dim3(256), 0, st, (const double*)D.dyn_part,
+                           (int64_t)D.n_blocks, D.n_dyn * RELAX_BINS, D.dyn_fsum);
+      }
       hipLaunchKernelGGL(relax_forward_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, t, (const R*)tr,
                          rec ? seg_L + (int64_t)(t - t0) * D.n : (R*)nullptr,
                          rec ? seg_s + (int64_t)(t - t0) * D.n : (R*)nullptr);
@@ -367,6 +480,7 @@ struct Relax : RelaxBase {
     if (!rc) {
       hipLaunchKernelGGL(relax_fill_kernel<double>, dim3(256), dim3(256), 0, st, D.gpart,
                          (int64_t)RELAX_MAX_PARAMS * D.n_blocks, 0.0);
+      hipLaunchKernelGGL(relax_fill_kernel<double>, dim3(1), dim3(256), 0, st, D.g_dyn, (int64_t)RELAX_MAX_PARAMS, 0.0);
       rc = check_launch("cvk_relax fill");
     }
     if (rc) return rc;
@@ -381,6 +495,14 @@ struct Relax : RelaxBase {
         const int64_t off = (int64_t)(t - t0) * D.n;
         hipLaunchKernelGGL(relax_adj_agent_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, t, (R)ebar_host[t],
                            (const R*)(seg_L + off), (const R*)(seg_s + off));
+        if (D.n_dyn) {
+          const int rows = RELAX_BINS * (1 + 2 * D.n_dyn);
+          hipLaunchKernelGGL(relax_dyn_adj_sum_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th,
+                             (const R*)(seg_tr + off));
+          hipLaunchKernelGGL(relax_reduce_kernel, dim3(rows), dim3(256), 0, st, (const double*)D.dyn_part,
+                             (int64_t)D.n_blocks, rows, D.dyn_red);
+          hipLaunchKernelGGL(relax_dyn_grad_kernel<R>, dim3(1), dim3(64), 0, st, D, th);
+        }
         hipLaunchKernelGGL(relax_pack_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th,
                            (const R*)(seg_tr + off));
         hipLaunchKernelGGL(relax_adj_edge_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, t,
@@ -392,9 +514,12 @@ struct Relax : RelaxBase {
     hipLaunchKernelGGL(relax_reduce_kernel, dim3(np), dim3(256), 0, st, D.gpart, (int64_t)D.n_blocks, np, g_dev);
     rc = check_launch("cvk_relax reduce");
     if (rc) return rc;
+    std::vector<double> gd(RELAX_MAX_PARAMS, 0.0);
     if (hipMemcpyAsync(grad_host, g_dev, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipMemcpyAsync(gd.data(), D.g_dyn, (size_t)np * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess)
       return fail(CVK_E_HIP, "cvk_relax: copy of the gradient failed");
+    for (int k = 0; k < np; ++k) grad_host[k] += gd[k];  // the daily-resampled layers' partials
     return CVK_OK;
   }
 };
@@ -416,6 +541,11 @@ static int relax_create(const cvk_relax_desc* d, RelaxBase** out) {
   D.rt = d->rel_trans;
   D.seed_frac = (R)d->seed_frac;
   D.n_blocks = (int32_t)((D.n + RELAX_BLOCK - 1) / RELAX_BLOCK);
+  D.n_dyn = 0;
+  for (int l = 0; l < d->n_layers; ++l) {
+    D.dyn_slots[l] = (R)d->dyn_slots[l];
+    if (d->dyn_slots[l] > 0.0) D.dyn_layer[D.n_dyn++] = l;
+  }
   r->ckpt = d->checkpoint_every;
   int rc = CVK_OK;
   rc = rc ? rc : r->alloc(&r->w, (int64_t)RELAX_BINS * (D.kmax + 1));
@@ -427,6 +557,10 @@ static int relax_create(const cvk_relax_desc* d, RelaxBase** out) {
   rc = rc ? rc : r->alloc(&D.epart, (int64_t)D.n_days * D.n_blocks);
   rc = rc ? rc : r->alloc(&D.gpart, (int64_t)RELAX_MAX_PARAMS * D.n_blocks);
   rc = rc ? rc : r->alloc(&D.pack, 4 * D.n);
+  rc = rc ? rc : r->alloc(&D.dyn_part, (int64_t)RELAX_BINS * (1 + 2 * std::max(D.n_dyn, 1)) * D.n_blocks);
+  rc = rc ? rc : r->alloc(&D.dyn_fsum, (int64_t)RELAX_BINS * std::max(D.n_dyn, 1));
+  rc = rc ? rc : r->alloc(&D.dyn_red, (int64_t)RELAX_BINS * (1 + 2 * std::max(D.n_dyn, 1)));
+  rc = rc ? rc : r->alloc(&D.g_dyn, RELAX_MAX_PARAMS);
   rc = rc ? rc : r->alloc(&r->seg_L, (int64_t)r->ckpt * D.n);
   rc = rc ? rc : r->alloc(&r->seg_s, (int64_t)r->ckpt * D.n);
   rc = rc ? rc : r->alloc(&r->seg_tr, (int64_t)r->ckpt * D.n);
@@ -462,8 +596,11 @@ extern "C" int cvk_relax_create(const cvk_relax_desc* d, void** handle_out) {
       d->n_days < 1 || d->kernel_len < 2 || d->checkpoint_every < 1 || !d->w_host || !d->age_bin || !d->rel_trans ||
       d->seed_frac < 0 || d->seed_frac > 1 || (d->precision != 32 && d->precision != 64))
     return fail(CVK_E_INVALID, "cvk_relax_create: invalid descriptor");
-  for (int l = 0; l < d->n_layers; ++l)
-    if (!d->indptr[l] || !d->nbr[l]) return fail(CVK_E_INVALID, "cvk_relax_create: null neighbour list");
+  for (int l = 0; l < d->n_layers; ++l) {
+    if (!(d->dyn_slots[l] >= 0.0)) return fail(CVK_E_INVALID, "cvk_relax_create: dyn_slots must be >= 0");
+    if (d->dyn_slots[l] == 0.0 && (!d->indptr[l] || !d->nbr[l]))
+      return fail(CVK_E_INVALID, "cvk_relax_create: null neighbour list");
+  }
   RelaxBase* r = nullptr;
   const int rc = d->precision == 64 ? relax_create<double>(d, &r) : relax_create<float>(d, &r);
   if (rc) return rc;

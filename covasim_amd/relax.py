@@ -2,7 +2,8 @@
 
 ``Relaxation(sim)`` takes an initialized ``Sim`` and uses:
 - its agents (ages -> prognosis age bins, ``rel_trans``);
-- its contact layers as they stand on day 0 (held static);
+- its static contact layers as they stand on day 0, and its daily-resampled layers (``dynam_layer``) in
+  expectation: 2 E_l / N contact slots per agent, each with a uniformly drawn source;
 - its disease parameters.
 
 ``Relaxation.expected(theta)`` returns the expected new infections of each day, E_t, as
@@ -142,8 +143,12 @@ class Relaxation:
         self.bins_np = age_bins(sim.people.age, pars)
         self._bins = torch.from_numpy(self.bins_np).to(dev)
         self._rt = sim.people._d['rel_trans'].clone()
-        self._csr = [neighbour_lists(sim.people.contacts[lk].tensor('p1'), sim.people.contacts[lk].tensor('p2'),
-                                     self.n) for lk in self.layer_keys]
+        dyn = pars.get('dynam_layer') or {}
+        self.dyn_slots = [2.0 * len(sim.people.contacts[lk]) / self.n if dyn.get(lk, 0) else 0.0
+                          for lk in self.layer_keys]
+        self._csr = [None if self.dyn_slots[li] else
+                     neighbour_lists(sim.people.contacts[lk].tensor('p1'), sim.people.contacts[lk].tensor('p2'), self.n)
+                     for li, lk in enumerate(self.layer_keys)]
         self.seed_frac = float(pars['pop_infected']) / self.n
         d = _lib.RelaxDesc()
         d.n_agents, d.n_layers, d.n_days = self.n, self.n_layers, self.n_days
@@ -151,9 +156,12 @@ class Relaxation:
         d.checkpoint_every = int(checkpoint_every)
         d.precision = int(precision)
         d.seed_frac = self.seed_frac
-        for li, (indptr, nbr) in enumerate(self._csr):
-            d.indptr[li] = indptr.data_ptr()
-            d.nbr[li] = nbr.data_ptr()
+        for li, lay in enumerate(self._csr):
+            if lay is None:
+                d.dyn_slots[li] = self.dyn_slots[li]
+                continue
+            d.indptr[li] = lay[0].data_ptr()
+            d.nbr[li] = lay[1].data_ptr()
         d.age_bin = self._bins.data_ptr()
         d.rel_trans = self._rt.data_ptr()
         self._w_host = np.ascontiguousarray(self.w, dtype=np.float64)
@@ -183,8 +191,8 @@ class Relaxation:
                         + list(np.asarray(p['prognoses']['sus_ORs'], dtype=np.float64)), dtype=np.float64)
 
     def csr_host(self):
-        """Neighbour lists as host arrays (for the CPU reference)."""
-        return [(indptr.cpu().numpy(), nbr.cpu().numpy()) for indptr, nbr in self._csr]
+        """Neighbour lists as host arrays (for the CPU reference); None for a daily-resampled layer."""
+        return [None if lay is None else (lay[0].cpu().numpy(), lay[1].cpu().numpy()) for lay in self._csr]
 
     def forward_np(self, theta):
         theta = np.ascontiguousarray(theta, dtype=np.float64)

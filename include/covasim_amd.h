@@ -780,7 +780,10 @@ int cvk_move_edges(const int32_t* p1, const int32_t* p2, const float* beta, int6
  * D_j(t-d) w[bin_j][d], log-survival L_i(t) = sum over layers l and neighbours j of
  * log1p(-min(beta * beta_l * sigma[bin_i] * tr_j, 0.999999)), D_i(t) = s_i(t)(1 - e^{L_i}),
  * s_i(t+1) = s_i(t) e^{L_i}; E_t = sum_i s_i(t)(1 - e^{L_i}). Parameters theta (host doubles):
- * [beta, beta_l for each layer, sigma_b for the 10 age bins]. */
+ * [beta, beta_l for each layer, sigma_b for the 10 age bins]. A layer resampled every day
+ * (base.py:1849-1876) enters in expectation: with c_l = dyn_slots[l] = 2 E_l / n_agents contact slots
+ * per agent, each with a uniformly drawn source, its term of L_i(t) is
+ * c_l / n_agents * sum_{j != i} log1p(-min(beta * beta_l * sigma[bin_i] * tr_j, 0.999999)). */
 typedef struct {
   int64_t n_agents;
   int32_t n_layers, n_days;
@@ -794,6 +797,8 @@ typedef struct {
   const uint8_t* age_bin;      /* device [n_agents]: prognosis age bin 0..9 */
   const float* rel_trans;      /* device [n_agents] */
   const double* w_host;        /* host [10][kernel_len]: mean infectiousness d days after infection */
+  double dyn_slots[CVK_MAX_LAYERS];  /* > 0: layer l is resampled every day and enters in expectation with
+                                        this many contact slots per agent (indptr / nbr unused); 0: its lists */
 } cvk_relax_desc;
 
 int cvk_relax_create(const cvk_relax_desc* desc, void** handle_out);

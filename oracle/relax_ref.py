@@ -12,7 +12,13 @@ theta = [beta, beta_l..., sigma_b (10)]. The model, day by day:
 
 This is the mean-field reading of the transmission step (utils.py:88-123: a susceptible
 target escapes every infectious neighbour independently) with indicator states replaced by
-probabilities.
+probabilities. A layer resampled every day (base.py:1849-1876: E_l pairs with uniform endpoints)
+enters in expectation: agent i has 2 E_l / N contact slots on average, each with a uniformly drawn
+source, and self-pairs do not transmit, so its term is
+
+    L_i(t) += c_l / N * sum_{j != i} log1p(-min(beta * beta_l * sigma[bin_i] * tr_j, XMAX)),  c_l = 2 E_l / N
+
+(the mean over resampled layers of that layer's term, by linearity).
 """
 
 import numpy as np
@@ -20,8 +26,46 @@ import numpy as np
 XMAX = 0.999999
 
 
-def expected_infections(theta, csr, bins, rel_trans, w, n_days, seed_frac):
-    """E (torch float64 [n_days]) as a differentiable function of theta (torch float64)."""
+def log_escape(beta, bl, sig, bins, slots, dyn_slots, tr):
+    """L_i of one day (torch float64 [n]) from tr: every held layer's neighbour slots (tgt, src), and the
+    expected term of every daily-resampled layer (slots[l] None)."""
+    import torch
+    n = len(bins)
+    sig_i = sig[bins]
+    L = torch.zeros(n, dtype=torch.float64)
+    for li, sl in enumerate(slots):
+        if sl is None:
+            xb = (beta * bl[li]) * sig  # [10]
+            g = torch.log1p(-torch.clamp(xb[:, None] * tr[None, :], max=XMAX))  # [10][n]
+            own = g[bins, torch.arange(n)]
+            L = L + dyn_slots[li] * (g.sum(dim=1)[bins] - own) / n
+            continue
+        tgt, src = sl
+        x = ((beta * bl[li]) * sig_i[tgt]) * tr[src]
+        x = torch.clamp(x, max=XMAX)
+        L = L.index_add(0, tgt, torch.log1p(-x))
+    return L
+
+
+def layer_slots(csr, n):
+    """(tgt, src) torch index pairs of each held layer's neighbour lists; None for a resampled layer."""
+    import torch
+    slots = []
+    for lay in csr:
+        if lay is None:
+            slots.append(None)
+            continue
+        indptr, nbr = lay
+        indptr = np.asarray(indptr, dtype=np.int64)
+        tgt = torch.as_tensor(np.repeat(np.arange(n), np.diff(indptr)))
+        slots.append((tgt, torch.as_tensor(np.asarray(nbr, dtype=np.int64))))
+    return slots
+
+
+def expected_infections(theta, csr, bins, rel_trans, w, n_days, seed_frac, dyn_slots=None):
+    """E (torch float64 [n_days]) as a differentiable function of theta (torch float64). csr[l] is
+    (indptr, nbr) for a layer held as it is, or None for a layer resampled every day, whose expected
+    slots per agent c_l are dyn_slots[l]."""
     import torch
     bins = torch.as_tensor(np.asarray(bins, dtype=np.int64))
     rt = torch.as_tensor(np.asarray(rel_trans, dtype=np.float64))
@@ -31,12 +75,7 @@ def expected_infections(theta, csr, bins, rel_trans, w, n_days, seed_frac):
     beta = theta[0]
     bl = theta[1:1 + n_layers]
     sig = theta[1 + n_layers:]
-    sig_i = sig[bins]
-    slots = []
-    for indptr, nbr in csr:
-        indptr = np.asarray(indptr, dtype=np.int64)
-        tgt = torch.as_tensor(np.repeat(np.arange(n), np.diff(indptr)))
-        slots.append((tgt, torch.as_tensor(np.asarray(nbr, dtype=np.int64))))
+    slots = layer_slots(csr, n)
     kmax = min(w.shape[1] - 1, n_days)
     s = torch.full((n,), 1.0 - seed_frac, dtype=torch.float64)
     hist = []
@@ -46,11 +85,7 @@ def expected_infections(theta, csr, bins, rel_trans, w, n_days, seed_frac):
         for d in range(1, min(t, kmax) + 1):
             inf_t = inf_t + hist[t - d] * w[bins, d]
         tr = rt * inf_t
-        L = torch.zeros(n, dtype=torch.float64)
-        for li, (tgt, src) in enumerate(slots):
-            x = ((beta * bl[li]) * sig_i[tgt]) * tr[src]
-            x = torch.clamp(x, max=XMAX)
-            L = L.index_add(0, tgt, torch.log1p(-x))
+        L = log_escape(beta, bl, sig, bins, slots, dyn_slots, tr)
         inf = s * (-torch.expm1(L))
         s = s * torch.exp(L)
         hist.append(inf + (seed_frac if t == 0 else 0.0))

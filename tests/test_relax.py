@@ -144,6 +144,69 @@ def test_relax_reference_gradient_matches_central_differences():
     assert np.linalg.norm(g - fd) <= 1e-7 * np.linalg.norm(fd)
 
 
+def _csr_of_pairs(p1, p2, n):
+    src = np.concatenate([p1, p2])
+    tgt = np.concatenate([p2, p1])
+    keep = src != tgt
+    src, tgt = src[keep], tgt[keep]
+    order = np.lexsort((src, tgt))
+    indptr = np.zeros(n + 1, dtype=np.int64)
+    indptr[1:] = np.cumsum(np.bincount(tgt, minlength=n))
+    return indptr, src[order].astype(np.int32)
+
+
+def test_relax_reference_resampled_layer_is_the_mean_over_layers():
+    """A daily-resampled layer's term is the exact mean of the held-layer term over every layer of E
+    uniform pairs (all N^(2E) of them enumerated): c_l = 2E/N slots per agent, uniform sources, self-pairs
+    left out."""
+    import itertools
+    import torch
+    n, E = 3, 2
+    bins = torch.as_tensor([2, 5, 5])
+    tr = torch.as_tensor([0.7, 1.9, 0.4], dtype=torch.float64)
+    sig = torch.linspace(0.5, 1.4, 10, dtype=torch.float64)
+    beta, bl = torch.tensor(0.2, dtype=torch.float64), torch.as_tensor([1.3], dtype=torch.float64)
+    total = torch.zeros(n, dtype=torch.float64)
+    count = 0
+    for ends in itertools.product(range(n), repeat=2 * E):
+        p1, p2 = np.array(ends[:E]), np.array(ends[E:])
+        slots = relax_ref.layer_slots([_csr_of_pairs(p1, p2, n)], n)
+        total = total + relax_ref.log_escape(beta, bl, sig, bins, slots, [0.0], tr)
+        count += 1
+    mean = (total / count).numpy()
+    dyn = relax_ref.log_escape(beta, bl, sig, bins, [None], [2.0 * E / n], tr).numpy()
+    np.testing.assert_allclose(dyn, mean, rtol=1e-13)
+    # and by hand for agent 0: 2E/N slots, each from agent 1 or 2 with probability 1/N
+    x = beta * bl[0] * sig[2]
+    want = 2 * E / n / n * (np.log1p(-float(x * tr[1])) + np.log1p(-float(x * tr[2])))
+    assert dyn[0] == pytest.approx(want, rel=1e-14)
+
+
+def test_relax_reference_resampled_layer_gradient_matches_central_differences():
+    import torch
+    csr, bins, rt, w = _random_graph(50, 2, 8, 5)
+    csr = [csr[0], None]  # the second layer resampled every day
+    dyn = [0.0, 7.5]
+    theta = np.concatenate([[0.03, 1.5, 0.7], np.linspace(0.5, 1.5, 10)])
+    c = np.random.default_rng(6).uniform(0.5, 1.5, 10)
+    th = torch.tensor(theta, requires_grad=True)
+    J = (torch.as_tensor(c) * relax_ref.expected_infections(th, csr, bins, rt, w, 10, 0.05, dyn)).sum()
+    J.backward()
+    g = th.grad.numpy()
+    h = 1e-6
+    fd = np.zeros_like(theta)
+
+    def Jof(tv):
+        return c @ relax_ref.expected_infections(torch.as_tensor(tv), csr, bins, rt, w, 10, 0.05, dyn).numpy()
+    for k in range(len(theta)):
+        tp, tm = theta.copy(), theta.copy()
+        tp[k] *= 1 + h
+        tm[k] *= 1 - h
+        fd[k] = (Jof(tp) - Jof(tm)) / (2 * h * theta[k])
+    assert np.linalg.norm(g - fd) <= 1e-7 * np.linalg.norm(fd)
+    assert abs(g[2]) > 1e-3 * np.linalg.norm(g)  # the resampled layer's beta matters
+
+
 # ------------------------------------------------------------------ GPU
 
 def _sim(n, n_days, seed=1, rand_seed=None, **pars):
@@ -156,7 +219,7 @@ def _sim(n, n_days, seed=1, rand_seed=None, **pars):
 
 def _cpu_inputs(relax):
     return relax.csr_host(), relax.bins_np, relax._rt.cpu().numpy().astype(np.float64), relax.w, relax.n_days, \
-        relax.seed_frac
+        relax.seed_frac, relax.dyn_slots
 
 
 def _rel(a, b):
@@ -164,18 +227,20 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-def test_relax_forward_and_adjoint_match_cpu_reference(gpu):
+@pytest.mark.parametrize('dynamic', [False, True])
+def test_relax_forward_and_adjoint_match_cpu_reference(gpu, dynamic):
     import torch
     from covasim_amd.relax import Relaxation
-    sim = _sim(1500, 25)
+    sim = _sim(1500, 25, **(dict(dynam_layer=dict(c=1)) if dynamic else {}))
     r64 = Relaxation(sim, precision=64, w_samples=20000)
     r32 = Relaxation(sim, precision=32, w_samples=20000)
+    assert (r64.dyn_slots[3] > 0) == dynamic and r64.dyn_slots[:3] == [0.0, 0.0, 0.0]
     theta = r64.theta0() * np.linspace(0.8, 1.2, r64.n_params)
     theta[0] = 0.05  # a faster epidemic than the default, so every day carries weight
-    csr, bins, rt, w, T, f = _cpu_inputs(r64)
+    csr, bins, rt, w, T, f, dyn = _cpu_inputs(r64)
     c = np.random.default_rng(4).uniform(0.5, 1.5, T)
     th = torch.tensor(theta, requires_grad=True)
-    e_ref = relax_ref.expected_infections(th, csr, bins, rt, w, T, f)
+    e_ref = relax_ref.expected_infections(th, csr, bins, rt, w, T, f, dyn)
     (torch.as_tensor(c) * e_ref).sum().backward()
     g_ref = th.grad.numpy()
     e_ref = e_ref.detach().numpy()
