@@ -214,7 +214,8 @@ class BatchDesc(C.Structure):
                 ('edge_counts_host', C.c_void_p), ('params_host', C.c_void_p), ('dynamic_host', C.c_void_p),
                 ('people', PeopleS), ('p1', C.c_void_p * MAX_LAYERS), ('p2', C.c_void_p * MAX_LAYERS),
                 ('beta', C.c_void_p * MAX_LAYERS), ('beta_ones', C.c_int32 * MAX_LAYERS),
-                ('sources', C.POINTER(SourcesS))]
+                ('sources', C.POINTER(SourcesS)), ('n_beta_days', C.c_int32), ('beta_v_days', C.c_void_p),
+                ('beta_layer_days', C.c_void_p)]
 
 
 class BatchImports(C.Structure):

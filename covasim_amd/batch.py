@@ -26,6 +26,7 @@ import numpy as np
 
 from . import _lib
 from . import results as cvres
+from .interventions import change_beta
 
 __all__ = ['BatchSim']
 
@@ -51,8 +52,9 @@ class BatchSim:
                 raise ValueError('BatchSim takes sims that have not started running')
             if sim._custom_update:
                 raise NotImplementedError('custom Layer.update subclasses are not supported in a batch')
-            if sim.pars['interventions']:
-                raise NotImplementedError('interventions are supported on single sims, not in a batch')
+            if any(not isinstance(iv, change_beta) for iv in sim.pars['interventions']):
+                raise NotImplementedError('a batch takes change_beta interventions; other interventions run on '
+                                          'single sims (multi_run does this by itself)')
         s0 = sims[0]
         lkeys = s0.layer_keys()
         for sim in sims[1:]:
@@ -128,6 +130,20 @@ class BatchSim:
                 if ptr:
                     self._sources[si * L + li] = ptr.contents
         desc.sources = C.cast(self._sources, C.POINTER(_lib.SourcesS))
+        # per-day rates of the sims with change_beta schedules (the others repeat their own rates)
+        if any(sim._beta_days is not None for sim in sims):
+            bv = np.zeros((self.S, self.npts, _lib.MAX_VARIANTS), np.float32)
+            bl = np.zeros((self.S, self.npts, _lib.MAX_LAYERS), np.float32)
+            for si, sim in enumerate(sims):
+                if sim._beta_days is not None:
+                    bv[si], bl[si] = sim._beta_days[0][:self.npts], sim._beta_days[1][:self.npts]
+                else:
+                    bv[si] = np.asarray(sim._params.beta_v, np.float32)
+                    bl[si] = np.asarray(sim._params.beta_layer, np.float32)
+            self._rate_tables = (np.ascontiguousarray(bv), np.ascontiguousarray(bl))
+            desc.n_beta_days = self.npts
+            desc.beta_v_days = self._rate_tables[0].ctypes.data
+            desc.beta_layer_days = self._rate_tables[1].ctypes.data
         self._desc = desc
         self.log_capacity = int(log_capacity)
         handle = C.c_void_p()

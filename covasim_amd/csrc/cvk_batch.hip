@@ -150,6 +150,18 @@ struct BatchEdgeArgs {
   BatchReduce reduce;
 };
 
+// Day t's rates of every sim (block s) into the views the edge pass reads: beta_v of the sim, beta_layer of
+// its segments (change_beta schedules; the kernels read no other copy of them).
+__global__ __launch_bounds__(64) void batch_rates_kernel(SimView* views, SegView* segs, const float* bv_days,
+                                                         const float* bl_days, int32_t n_days, int32_t n_layers,
+                                                         int32_t t) {
+  const int32_t s = blockIdx.x;
+  const int64_t row = (int64_t)s * n_days + (t < n_days ? t : n_days - 1);
+  if (threadIdx.x < MAXV) views[s].beta_v[threadIdx.x] = bv_days[row * CVK_MAX_VARIANTS + threadIdx.x];
+  if ((int)threadIdx.x < n_layers)
+    segs[(int64_t)s * n_layers + threadIdx.x].beta_layer = bl_days[row * CVK_MAX_LAYERS + threadIdx.x];
+}
+
 // The day's walk decision of every sim (block s): walk its listed layers unless automatic and the agent
 // pass set more than max_permille infectious bits per thousand of its agents (the partial column DC_BITS
 // of the sim's agent-pass blocks), as the single-sim edge pass decides.
@@ -593,6 +605,10 @@ struct Batch {
   int32_t* d_item_seg = nullptr;       // [n_items]
   int64_t* d_seg_item_start = nullptr;  // [S * L + 1]
   uint32_t* d_walk = nullptr;          // [S]
+  // per-day rates (change_beta): [S][n_beta_days][CVK_MAX_VARIANTS] and [S][n_beta_days][CVK_MAX_LAYERS]
+  int32_t n_beta_days = 0;
+  float* d_bv_days = nullptr;
+  float* d_bl_days = nullptr;
   int64_t k_slots = 0;
   // edge-pass timing (cvk_batch_time_edges): an event pair per day around the edge launch, eager runs only
   bool time_edges = false;
@@ -603,7 +619,7 @@ struct Batch {
     void* ptrs[] = {d_sims, d_params, d_segs, d_views, d_a_off, d_seg_tile_start, d_tile_seg, d_block_sim,
                     d_rs_chunk_seg, d_rs_chunk_first, d_rec, d_bound, d_bitmap, d_winner, d_touched, d_ctl,
                     d_results, d_results_f, d_partials, d_partials_f, d_log, d_hist, d_prefix, d_remain,
-                    d_all, d_short, d_k, d_srcs, d_item_seg, d_seg_item_start, d_walk};
+                    d_all, d_short, d_k, d_srcs, d_item_seg, d_seg_item_start, d_walk, d_bv_days, d_bl_days};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     for (hipEvent_t e : ev_begin) (void)hipEventDestroy(e);
@@ -920,6 +936,12 @@ static int run_days_impl(Batch& b, int32_t t0, int32_t t1, bool seeds, const cvk
       }
     }
     BatchReduce red{1, t, t > t0 ? 1 : 0, 1};
+    if (b.n_beta_days > 0) {  // the day's rates (change_beta)
+      hipLaunchKernelGGL(batch_rates_kernel, dim3(b.S), dim3(64), 0, st, b.d_views, b.d_segs,
+                         (const float*)b.d_bv_days, (const float*)b.d_bl_days, b.n_beta_days, b.L, t);
+      rc = check_launch("cvk_batch rates");
+      if (rc) return rc;
+    }
     const int mode = source_lists_mode();
     const bool walk = b.n_items > 0 && mode != 0;
     if (walk) {  // which sims walk their listed layers today
@@ -1118,6 +1140,15 @@ extern "C" int cvk_batch_create(const cvk_batch_desc* d, int32_t npts, int64_t l
   rc = rc ? rc : dev_alloc(&b->d_seg_item_start, (int64_t)seg_item_start.size());
   rc = rc ? rc : upload(b->d_seg_item_start, seg_item_start.data(), (int64_t)seg_item_start.size());
   rc = rc ? rc : dev_alloc(&b->d_walk, b->S);
+  if (!rc && d->n_beta_days > 0) {
+    if (!d->beta_v_days || !d->beta_layer_days) return fail(CVK_E_INVALID, "cvk_batch_create: null rate tables");
+    b->n_beta_days = d->n_beta_days;
+    const int64_t rows = (int64_t)b->S * d->n_beta_days;
+    rc = dev_alloc(&b->d_bv_days, rows * CVK_MAX_VARIANTS);
+    rc = rc ? rc : upload(b->d_bv_days, d->beta_v_days, rows * CVK_MAX_VARIANTS);
+    rc = rc ? rc : dev_alloc(&b->d_bl_days, rows * CVK_MAX_LAYERS);
+    rc = rc ? rc : upload(b->d_bl_days, d->beta_layer_days, rows * CVK_MAX_LAYERS);
+  }
   if (rc) return rc;
   // per-sim views for the edge pass (pointers offset to each sim's slices)
   b->views.resize(b->S);

@@ -3,7 +3,7 @@
 The reference runs the copies of a sim (or a given list of sims) one per process with
 ``sc.parallelize``. Here they share the GPU: ``multi_run`` groups the sims a ``BatchSim`` can advance
 together (same days, layer keys, variant count, waning setting and dynamic layers, no interventions
-and no custom ``Layer.update``) and runs each group with one launch per phase per day; any other sim
+other than change_beta and no custom ``Layer.update``) and runs each group with one launch per phase per day; any other sim
 runs on its own (``Sim.run``). Every sim's results are those of running it alone, whichever way it
 ran. ``MultiSim.reduce`` / ``mean`` / ``median`` summarise the finished sims on the device
 (``cvk_reduce``).
@@ -13,6 +13,7 @@ import numpy as np
 
 from . import results as cvres
 from .batch import BatchSim
+from .interventions import change_beta
 from .sim import Sim
 
 __all__ = ['MultiSim', 'single_run', 'multi_run']
@@ -71,7 +72,8 @@ def batchable(sim):
     """Whether a sim can run inside a BatchSim."""
     if not sim.initialized:
         sim.initialize()
-    return not sim.pars['interventions'] and not sim._custom_update and sim.t == 0
+    return all(isinstance(iv, change_beta) for iv in sim.pars['interventions']) and not sim._custom_update \
+        and sim.t == 0
 
 
 def batch_key(sim):

@@ -543,6 +543,13 @@ typedef struct {
    * most CVK_TUNE_SOURCE_PERMILLE infectious agents per thousand (the same rule as cvk_edge_pass),
    * instead of streamed. The results are identical either way. */
   const cvk_sources* sources;
+  /* Per-day rates of each sim (change_beta, interventions.py:530-586), as cvk_imports holds them for one sim:
+   * day t of sim s uses row min(t, n_beta_days - 1) of beta_v_days [n_sims][n_beta_days][CVK_MAX_VARIANTS]
+   * and beta_layer_days [n_sims][n_beta_days][CVK_MAX_LAYERS] (host arrays); n_beta_days = 0 keeps the
+   * params' values. */
+  int32_t n_beta_days;
+  const float* beta_v_days;
+  const float* beta_layer_days;
 } cvk_batch_desc;
 
 /* Import events of a batch: on day days[j], counts[j * n_sims + s] susceptible agents of sim s are

@@ -114,6 +114,35 @@ def test_batch_source_lists_walked_or_streamed(gpu, source_lists):
 
 
 @pytest.mark.gpu
+def test_batch_with_change_beta_schedules(gpu):
+    import covasim_amd as cv
+    common = dict(use_waning=False, dynam_layer=dict(c=1))
+
+    def ivs(i):  # per-sim schedules: overall beta, a layer's beta, both, none
+        return [[cv.change_beta(days=[8, 20], changes=[0.4, 1.2])],
+                [cv.change_beta(days=5, changes=0.2, layers='c')],
+                [cv.change_beta(days=12, changes=0.5), cv.change_beta(days=3, changes=2.0, layers=['h', 's'])],
+                [], []][i]
+
+    def make():
+        sims = make_sims(HYBRID_SPEC, 30, 'hybrid', **common)
+        for i, sim in enumerate(sims):
+            sim.pars['interventions'] = ivs(i)
+        return sims
+    alone = [s.run() for s in make()]
+    batch = cv.BatchSim(make()).run()
+    for si, sim in enumerate(batch.sims):
+        assert_same_results(sim, alone[si])
+        assert_same_state(batch, si, alone[si])
+    plain = cv.BatchSim(make_sims(HYBRID_SPEC, 30, 'hybrid', **common)).run()
+    assert plain.sims[0].results['cum_infections'].values[-1] != batch.sims[0].results['cum_infections'].values[-1]
+    with pytest.raises(NotImplementedError):
+        sims = make_sims(HYBRID_SPEC[:2], 10, 'hybrid', **common)
+        sims[0].pars['interventions'] = [cv.test_prob(symp_prob=0.1)]
+        cv.BatchSim(sims)
+
+
+@pytest.mark.gpu
 def test_batch_equals_individual_runs_variants_waning(gpu):
     import covasim_amd as cv
     spec = [(2000, 0.03, 11, [dict(rel_beta=1.7, days=[5], n_imports=20)]),

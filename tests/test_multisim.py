@@ -4,7 +4,8 @@ CPU: the set-up follows single_run (run.py:1358-1394): copies of one sim get see
 labels 'Sim i' and leave the base sim as it was; iterpars set one value per run; a list of sims keeps
 its seeds unless reseeded; noise multiplies beta by 1 + x or 1 / (1 - x), x = noise * N(0, 1).
 GPU: every sim of a multi_run equals the same sim run alone, bit for bit, whether it ran inside a
-BatchSim (compatible sims) or alone (a sim with an intervention, one with other days); reduce()
+BatchSim (compatible sims, change_beta schedules included) or alone (a sim that tests and traces, one with
+other days); reduce()
 gives the median and the 10 / 90 % quantiles of the sims' rows (np.quantile), mean() the mean and
 2 standard deviations.
 """
@@ -80,7 +81,9 @@ def test_multi_run_mixed_list(gpu):
                 cv.Sim(rand_seed=3, n_days=25, popdict=copy_pop(pop),
                        interventions=[cv.change_beta(days=10, changes=0.3)], **kw),
                 cv.Sim(rand_seed=4, n_days=18, popdict=copy_pop(pop), **kw),
-                cv.Sim(rand_seed=5, n_days=25, popdict=copy_pop(pop), **kw)]
+                cv.Sim(rand_seed=5, n_days=25, popdict=copy_pop(pop), **kw),
+                cv.Sim(rand_seed=6, n_days=25, popdict=copy_pop(pop),
+                       interventions=[cv.test_prob(symp_prob=0.2), cv.contact_tracing(trace_probs=0.5)], **kw)]
     sims = cv.multi_run(make())
     for sim, alone in zip(sims, [s.run() for s in make()]):
         same(sim, alone)
