@@ -26,14 +26,15 @@ class variant:
         self.n_imports = int(n_imports)
         self.rescale = rescale
         self.index = None
-        if isinstance(variant, str):
+        if isinstance(variant, str):  # a known variant or an alias of one (immunity.py:53-67)
             label = variant.lower()
             for txt in ['.', ' ', 'variant', 'voc']:
                 label = label.replace(txt, '')
-            known = cvpar.get_variant_pars()
-            if label not in known:
-                raise NotImplementedError(f'The selected variant "{variant}" is not implemented; choices are {list(known)}')
-            self.p = dict(known[label])
+            choices, mapping = cvpar.get_variant_choices()
+            if label not in mapping:
+                raise NotImplementedError(f'The selected variant "{variant}" is not implemented; choices are {choices}')
+            label = mapping[label]
+            self.p = dict(cvpar.get_variant_pars()[label])
         elif isinstance(variant, dict):
             p = dict(variant)
             label = p.pop('label', label) or 'custom'

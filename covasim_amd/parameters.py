@@ -152,6 +152,19 @@ def get_prognoses(by_age=True):
     return out
 
 
+def get_variant_choices():
+    """Names of the known variants and their aliases (parameters.py:317-330): (choices, alias -> name)."""
+    choices = {
+        'wild': ['wild', 'default', 'pre-existing', 'original'],
+        'alpha': ['alpha', 'b117', 'uk', 'united kingdom', 'kent'],
+        'beta': ['beta', 'b1351', 'sa', 'south africa'],
+        'gamma': ['gamma', 'p1', 'b11248', 'brazil'],
+        'delta': ['delta', 'b16172', 'india'],
+    }
+    mapping = {alias: name for name, aliases in choices.items() for alias in aliases}
+    return choices, mapping
+
+
 def get_variant_pars(default=False, variant=None):
     """Known variant parameters (parameters.py:372-419)."""
     pars = dict(

@@ -118,18 +118,21 @@ class Sim:
             raise ValueError('n_days must be non-negative')
 
     def init_variants(self):
-        labels = ['wild']
-        vpars = [dict(self.pars['variant_pars']['wild'])]
-        for i, var in enumerate(self.pars['variants'], start=1):
+        """sim.py:477-495 with variant.initialize (immunity.py:107-114): each variant's parameters are stored
+        under its label and its index is that label's position, so a later variant with the same label (two
+        dicts without one are both 'custom') replaces the parameters and shares the index."""
+        vpars = {'wild': dict(self.pars['variant_pars']['wild'])}
+        for i, var in enumerate(self.pars['variants']):
             if not isinstance(var, cvimm.variant):
                 raise TypeError(f'Variant {i} ({var}) is not a covasim_amd.variant object')
-            var.index = i
-            labels.append(var.label)
-            vpars.append(dict(var.p))
+            vpars[var.label] = dict(var.p)
+        labels = list(vpars)
+        for var in self.pars['variants']:
+            var.index = labels.index(var.label)
         if len(labels) > _lib.MAX_VARIANTS:
             raise ValueError(f'At most {_lib.MAX_VARIANTS} variants are supported')
         self.pars['variant_map'] = dict(enumerate(labels))
-        self.pars['variant_pars'] = {lab: vp for lab, vp in zip(labels, vpars)}
+        self.pars['variant_pars'] = vpars
         self.pars['n_variants'] = len(labels)
 
     def initialize(self, reset=False):

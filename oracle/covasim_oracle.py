@@ -116,12 +116,18 @@ def make_pars(**kwargs):
     # variant_pars['wild'] is captured from the defaults before kwargs apply (parameters.py:131-134)
     wild = {k: pars[k] for k in ['rel_beta', 'rel_symp_prob', 'rel_severe_prob', 'rel_crit_prob', 'rel_death_prob']}
     pars.update(kwargs)
-    pars['variant_pars'] = [wild]
+    # immunity.py:107-114 (variant.initialize): a variant's parameters are stored under its label (a dict
+    # without one is 'custom', immunity.py:78-80) and its index is the label's position, so a later variant
+    # with the same label replaces the parameters and shares the index
+    vpars = {'wild': wild}
     for var in pars['variants']:
         vp = dict(rel_beta=1.0, rel_symp_prob=1.0, rel_severe_prob=1.0, rel_crit_prob=1.0, rel_death_prob=1.0)
         vp.update({k: v for k, v in var.items() if k in vp})
-        pars['variant_pars'].append(vp)
-    pars['n_variants'] = len(pars['variant_pars'])
+        vpars[var.get('label') or 'custom'] = vp
+    labels = list(vpars)
+    pars['variant_index'] = [labels.index(var.get('label') or 'custom') for var in pars['variants']]
+    pars['variant_pars'] = [vpars[lab] for lab in labels]
+    pars['n_variants'] = len(labels)
     ld = LAYER_DEFAULTS[pars['pop_type']]
     for key in ['beta_layer', 'contacts', 'dynam_layer', 'iso_factor', 'quar_factor']:
         merged = dict(ld[key])
@@ -825,7 +831,7 @@ class Sim:
                 lay[2] = np.ones(len(lay[0]), dtype=F32)
         hosp_max = people.severe.sum() > pars['n_beds_hosp'] if pars['n_beds_hosp'] is not None else False
         icu_max = people.critical.sum() > pars['n_beds_icu'] if pars['n_beds_icu'] is not None else False
-        for vi, var in enumerate(pars['variants'], start=1):  # immunity.py:117-130
+        for var, vi in zip(pars['variants'], pars['variant_index']):  # immunity.py:117-130
             if t in var.get('days', []):
                 sus_inds = people.susceptible.nonzero()[0]
                 n_imports = int(var.get('n_imports', 1))

@@ -133,7 +133,7 @@ class PortSim:
         self.scratch_i = np.zeros(n + 2 * e_max + 1, np.int32)
         self.prm = self._params(list(layers.keys()), seed)
         days, vars_, counts = [], [], []
-        for vi, var in enumerate(pars['variants'], start=1):
+        for var, vi in zip(pars['variants'], pars['variant_index']):
             for d in var.get('days', []):
                 days.append(d)
                 vars_.append(vi)

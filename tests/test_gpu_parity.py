@@ -122,7 +122,8 @@ def test_infections_philox_bit_exact(gpu):
 def run_pair(pop, n_days=40, seed=1, variants=(), use_graph=None, **pars):
     import covasim_amd as cv
     gpars = dict(pars)
-    gvars = [cv.variant(dict(rel_beta=v['rel_beta']), days=v['days'], n_imports=v['n_imports']) for v in variants]
+    gvars = [cv.variant({k: v[k] for k in ('rel_beta', 'label') if k in v}, days=v['days'], n_imports=v['n_imports'])
+             for v in variants]
     sim = cv.Sim(pop_size=len(pop['age']), n_days=n_days, rand_seed=seed, popdict=copy_pop(pop), variants=gvars,
                  **gpars)
     sim.run(use_graph=use_graph)
@@ -219,6 +220,23 @@ CONFIGS = {
     'weighted_sources_coarse': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
                                     pars=dict(use_waning=True, beta=0.05, dynam_layer=dict(c=1)),
                                     tuning=dict(edge_lds=0, edge_coarse=1, source_lists=1)),
+    # three variants (two imported), waning: default tuning, and streamed with the coarse map
+    'three_variants_waning': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                  pars=dict(use_waning=True, beta=0.02, dynam_layer=dict(c=1),
+                                            variants=[dict(label='v1', rel_beta=1.7, days=[8], n_imports=30),
+                                                      dict(label='v2', rel_beta=1.3, days=[15], n_imports=30)])),
+    'three_variants_streamed_coarse': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                           pars=dict(use_waning=True, beta=0.02, dynam_layer=dict(c=1),
+                                                     variants=[dict(label='v1', rel_beta=1.7, days=[8], n_imports=30),
+                                                               dict(label='v2', rel_beta=1.3, days=[15],
+                                                                    n_imports=30)]),
+                                           tuning=dict(source_lists=0, edge_lds=0)),
+    # two variants without labels are both 'custom': one index, the second's parameters (immunity.py:107-114)
+    'same_label_variants': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                pars=dict(use_waning=True, beta=0.02, dynam_layer=dict(c=1),
+                                          variants=[dict(rel_beta=1.7, days=[8], n_imports=30),
+                                                    dict(rel_beta=1.3, days=[15], n_imports=30)]),
+                                n_variants=2),
     'beds_limited': dict(pop=dict(n=3000, pop_type='random'),
                          pars=dict(use_waning=False, beta=0.08, n_beds_hosp=2, n_beds_icu=1,
                                    rel_severe_prob=20.0, rel_crit_prob=5.0)),
@@ -246,7 +264,9 @@ def test_full_sim_matches_oracle(gpu, name):
         for key, default in keys.values():
             _lib.lib().cvk_set_tuning(key, default)
     assert sim.results['cum_infections'].values[-1] > 50  # the epidemic took off
-    assert_results_equal(sim, ores, nv=1 + len(variants))
+    nv = cfg.get('n_variants', 1 + len(variants))
+    assert sim.pars['n_variants'] == osim.pars['n_variants'] == nv
+    assert_results_equal(sim, ores, nv=nv)
     assert_people_equal(sim, osim)
 
 

@@ -103,6 +103,32 @@ def test_variant_parsing():
         cvimm.variant(dict(rel_betta=2), days=1)
     m = cvimm.init_immunity_matrix(['wild', 'alpha', 'custom'])
     assert m[1, 0] == np.float32(0.5) and m[2, 2] == 1 and m[0, 2] == 1
+    # aliases of known variants (parameters.py:317-330), after the same clean-up of the name
+    for name, label in [('B.1.1.7', 'alpha'), ('UK', 'alpha'), ('b1351', 'beta'), ('P1', 'gamma'),
+                        ('Delta variant', 'delta'), ('india', 'delta'), ('original', 'wild')]:
+        assert cvimm.variant(name, days=5).label == label, name
+    with pytest.raises(NotImplementedError):
+        cvimm.variant('united kingdom', days=5)  # the space is removed before the lookup, as in the reference
+    with pytest.raises(NotImplementedError):
+        cvimm.variant('omicron', days=5)
+
+
+def test_variant_labels_set_the_indices():
+    """variant.initialize (immunity.py:107-114): parameters stored by label, index = the label's position; two
+    dicts without a label are both 'custom', so the second replaces the first's parameters and shares index 1;
+    the oracle follows the same rule."""
+    import covasim_amd as cv
+    vs = [cv.variant(dict(rel_beta=1.7), days=8), cv.variant('alpha', days=9), cv.variant(dict(rel_beta=1.3), days=15),
+          cv.variant(dict(rel_beta=2.0, label='x'), days=20)]
+    sim = cv.Sim(pop_size=1000, n_days=30, variants=vs)
+    sim.init_variants()
+    assert sim.pars['n_variants'] == 4 and [v.index for v in vs] == [1, 2, 1, 3]
+    assert sim.pars['variant_map'] == {0: 'wild', 1: 'custom', 2: 'alpha', 3: 'x'}
+    assert sim.pars['variant_pars']['custom']['rel_beta'] == 1.3
+    opars = co.make_pars(variants=[dict(rel_beta=1.7, days=[8]), dict(rel_beta=1.3, days=[15]),
+                                   dict(label='x', rel_beta=2.0, days=[20])])
+    assert opars['n_variants'] == 3 and opars['variant_index'] == [1, 1, 2]
+    assert [vp['rel_beta'] for vp in opars['variant_pars']] == [1.0, 1.3, 2.0]
 
 
 def test_sim_rejects_out_of_path_parameters():
