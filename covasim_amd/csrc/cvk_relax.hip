@@ -81,7 +81,8 @@ struct RelaxDev {
   int32_t n_dyn;
   int32_t dyn_layer[CVK_MAX_LAYERS];  // their layer indices
   R dyn_slots[CVK_MAX_LAYERS];        // c_l per layer (0: the layer's neighbour lists)
-  double* dyn_part;   // [(RELAX_BINS + 2 n_dyn RELAX_BINS)][n_blocks] per-block sums
+  double* dyn_part;   // [(RELAX_BINS + 2 n_dyn RELAX_BINS)][n_dyn_blocks] per-workgroup sums
+  int32_t n_dyn_blocks;  // workgroups of the per-bin sums (RELAX_DYN_SPAN agents each)
   double* dyn_fsum;   // [n_dyn][RELAX_BINS] the day's S_{l,b} (forward)
   double* dyn_red;    // [RELAX_BINS + 2 n_dyn RELAX_BINS]: Lambda_b, then G_{l,b}, then H_{l,b} (adjoint day)
   double* g_dyn;      // [RELAX_MAX_PARAMS] the dynamic layers' parameter partials, summed over days
@@ -179,20 +180,56 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_forward_kernel(RelaxDev<R> 
   if (threadIdx.x == 0) D.epart[(int64_t)t * D.n_blocks + blockIdx.x] = tot;
 }
 
-// The day's S_{l,b} = sum_j g(x_{l,b} tr_j) of every daily-resampled layer: per-block sums (row k * 10 + b).
+// Per-bin sums run over RELAX_DYN_ITEMS agents per thread (agent blockIdx.x * RELAX_DYN_SPAN + u * RELAX_BLOCK
+// + threadIdx.x for u = 0, 1, ...): each thread sums its agents in registers, then one reduction per group of
+// rows (wave sums, then the waves in order) gives the workgroup's partial of every row. Fixed order.
+constexpr int RELAX_DYN_ITEMS = 4;
+constexpr int RELAX_DYN_SPAN = RELAX_BLOCK * RELAX_DYN_ITEMS;
+
+// part[(row0 + q) * stride + blockIdx.x] = the workgroup's sum of v[q], q < NV.
+template <int NV>
+__device__ inline void block_rows_store(const double (&v)[NV], double* s_w, double* part, int64_t row0,
+                                        int64_t stride) {
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const double w = wave_sum_d(v[q]);
+    if (lane_id() == 0) s_w[(threadIdx.x >> 6) * NV + q] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double tot = 0.0;
+    for (int k = 0; k < RELAX_BLOCK / 64; ++k) tot += s_w[k * NV + threadIdx.x];
+    part[(row0 + threadIdx.x) * stride + blockIdx.x] = tot;
+  }
+  __syncthreads();
+}
+
+// The day's S_{l,b} = sum_j g(x_{l,b} tr_j) of every daily-resampled layer: per-workgroup sums (row k * 10 + b).
 template <class R>
 __global__ __launch_bounds__(RELAX_BLOCK) void relax_dyn_sum_kernel(RelaxDev<R> D, RelaxTheta<R> th, const R* tr) {
-  __shared__ double s_w[RELAX_BLOCK / 64];
-  const int64_t j = (int64_t)blockIdx.x * RELAX_BLOCK + threadIdx.x;
+  __shared__ double s_w[(RELAX_BLOCK / 64) * RELAX_BINS];
   const R xmax = (R)RELAX_XMAX;
-  const R trj = j < D.n ? tr[j] : (R)0;
+  R trv[RELAX_DYN_ITEMS];
+  bool ok[RELAX_DYN_ITEMS];
+#pragma unroll
+  for (int u = 0; u < RELAX_DYN_ITEMS; ++u) {
+    const int64_t j = (int64_t)blockIdx.x * RELAX_DYN_SPAN + u * RELAX_BLOCK + threadIdx.x;
+    ok[u] = j < D.n;
+    trv[u] = ok[u] ? tr[j] : (R)0;
+  }
   for (int k = 0; k < D.n_dyn; ++k) {  // uniform: every thread reaches the barriers
     const int l = D.dyn_layer[k];
+    double acc[RELAX_BINS];
+#pragma unroll
     for (int b = 0; b < RELAX_BINS; ++b) {
-      const double v = j < D.n ? (double)relax_g<R>(((th.beta * th.bl[l]) * th.sig[b]) * trj, xmax) : 0.0;
-      const double tot = block_sum_d(v, s_w);
-      if (threadIdx.x == 0) D.dyn_part[(int64_t)(k * RELAX_BINS + b) * D.n_blocks + blockIdx.x] = tot;
+      const R x = (th.beta * th.bl[l]) * th.sig[b];
+      double v = 0.0;
+#pragma unroll
+      for (int u = 0; u < RELAX_DYN_ITEMS; ++u)
+        if (ok[u]) v += (double)relax_g<R>(x * trv[u], xmax);
+      acc[b] = v;
     }
+    block_rows_store<RELAX_BINS>(acc, s_w, D.dyn_part, (int64_t)k * RELAX_BINS, D.n_dyn_blocks);
   }
 }
 
@@ -200,29 +237,48 @@ __global__ __launch_bounds__(RELAX_BLOCK) void relax_dyn_sum_kernel(RelaxDev<R> 
 template <class R>
 __global__ __launch_bounds__(RELAX_BLOCK) void relax_dyn_adj_sum_kernel(RelaxDev<R> D, RelaxTheta<R> th,
                                                                         const R* tr) {
-  __shared__ double s_w[RELAX_BLOCK / 64];
-  const int64_t a = (int64_t)blockIdx.x * RELAX_BLOCK + threadIdx.x;
+  __shared__ double s_w[(RELAX_BLOCK / 64) * RELAX_BINS];
   const R xmax = (R)RELAX_XMAX;
-  const bool ok = a < D.n;
-  const int ba = ok ? D.bin[a] : -1;
-  const R tra = ok ? tr[a] : (R)0;
-  const R lb = ok ? D.lbar[a] : (R)0;
-  const int64_t nb = D.n_blocks;
-  for (int b = 0; b < RELAX_BINS; ++b) {
-    const double tot = block_sum_d(ba == b ? (double)lb : 0.0, s_w);
-    if (threadIdx.x == 0) D.dyn_part[(int64_t)b * nb + blockIdx.x] = tot;
+  int ba[RELAX_DYN_ITEMS];
+  R tra[RELAX_DYN_ITEMS], lb[RELAX_DYN_ITEMS];
+#pragma unroll
+  for (int u = 0; u < RELAX_DYN_ITEMS; ++u) {
+    const int64_t a = (int64_t)blockIdx.x * RELAX_DYN_SPAN + u * RELAX_BLOCK + threadIdx.x;
+    const bool ok = a < D.n;
+    ba[u] = ok ? D.bin[a] : -1;
+    tra[u] = ok ? tr[a] : (R)0;
+    lb[u] = ok ? D.lbar[a] : (R)0;
   }
+  const int64_t nb = D.n_dyn_blocks;
+  double acc[RELAX_BINS];
+#pragma unroll
+  for (int b = 0; b < RELAX_BINS; ++b) {
+    double v = 0.0;
+#pragma unroll
+    for (int u = 0; u < RELAX_DYN_ITEMS; ++u)
+      if (ba[u] == b) v += (double)lb[u];
+    acc[b] = v;
+  }
+  block_rows_store<RELAX_BINS>(acc, s_w, D.dyn_part, 0, nb);
   for (int k = 0; k < D.n_dyn; ++k) {
     const int l = D.dyn_layer[k];
+    double hsum[RELAX_BINS];
+#pragma unroll
     for (int b = 0; b < RELAX_BINS; ++b) {
       const R x = (th.beta * th.bl[l]) * th.sig[b];
-      const R gt = ok ? relax_gp<R>(x * tra, xmax) * tra : (R)0;
-      double tot = block_sum_d((double)gt, s_w);
-      if (threadIdx.x == 0) D.dyn_part[(int64_t)(RELAX_BINS + k * RELAX_BINS + b) * nb + blockIdx.x] = tot;
-      tot = block_sum_d(ba == b ? (double)(lb * gt) : 0.0, s_w);
-      if (threadIdx.x == 0)
-        D.dyn_part[(int64_t)(RELAX_BINS + (D.n_dyn + k) * RELAX_BINS + b) * nb + blockIdx.x] = tot;
+      double g = 0.0, h = 0.0;
+#pragma unroll
+      for (int u = 0; u < RELAX_DYN_ITEMS; ++u) {
+        if (ba[u] < 0) continue;
+        const R gt = relax_gp<R>(x * tra[u], xmax) * tra[u];
+        g += (double)gt;
+        if (ba[u] == b) h += (double)(lb[u] * gt);
+      }
+      acc[b] = g;
+      hsum[b] = h;
     }
+    block_rows_store<RELAX_BINS>(acc, s_w, D.dyn_part, (int64_t)RELAX_BINS + k * RELAX_BINS, nb);
+    block_rows_store<RELAX_BINS>(hsum, s_w, D.dyn_part, (int64_t)RELAX_BINS + (D.n_dyn + k) * RELAX_BINS, nb);
   }
 }
 
@@ -436,9 +492,10 @@ struct Relax : RelaxBase {
       R* tr = rec ? seg_tr + (int64_t)(t - t0) * D.n : tr_scratch;
       hipLaunchKernelGGL(relax_trans_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, t, tr);
       if (D.n_dyn) {
-        hipLaunchKernelGGL(relax_dyn_sum_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, (const R*)tr);
+        hipLaunchKernelGGL(relax_dyn_sum_kernel<R>, dim3(D.n_dyn_blocks), dim3(RELAX_BLOCK), 0, st, D, th,
+                           (const R*)tr);
         hipLaunchKernelGGL(relax_reduce_kernel, dim3(D.n_dyn * RELAX_BINS), dim3(256), 0, st, (const double*)D.dyn_part,
-                           (int64_t)D.n_blocks, D.n_dyn * RELAX_BINS, D.dyn_fsum);
+                           (int64_t)D.n_dyn_blocks, D.n_dyn * RELAX_BINS, D.dyn_fsum);
       }
       hipLaunchKernelGGL(relax_forward_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th, t, (const R*)tr,
                          rec ? seg_L + (int64_t)(t - t0) * D.n : (R*)nullptr,
@@ -497,10 +554,10 @@ struct Relax : RelaxBase {
                            (const R*)(seg_L + off), (const R*)(seg_s + off));
         if (D.n_dyn) {
           const int rows = RELAX_BINS * (1 + 2 * D.n_dyn);
-          hipLaunchKernelGGL(relax_dyn_adj_sum_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th,
+          hipLaunchKernelGGL(relax_dyn_adj_sum_kernel<R>, dim3(D.n_dyn_blocks), dim3(RELAX_BLOCK), 0, st, D, th,
                              (const R*)(seg_tr + off));
           hipLaunchKernelGGL(relax_reduce_kernel, dim3(rows), dim3(256), 0, st, (const double*)D.dyn_part,
-                             (int64_t)D.n_blocks, rows, D.dyn_red);
+                             (int64_t)D.n_dyn_blocks, rows, D.dyn_red);
           hipLaunchKernelGGL(relax_dyn_grad_kernel<R>, dim3(1), dim3(64), 0, st, D, th);
         }
         hipLaunchKernelGGL(relax_pack_kernel<R>, dim3(grid()), dim3(RELAX_BLOCK), 0, st, D, th,
@@ -541,6 +598,7 @@ static int relax_create(const cvk_relax_desc* d, RelaxBase** out) {
   D.rt = d->rel_trans;
   D.seed_frac = (R)d->seed_frac;
   D.n_blocks = (int32_t)((D.n + RELAX_BLOCK - 1) / RELAX_BLOCK);
+  D.n_dyn_blocks = (int32_t)((D.n + RELAX_DYN_SPAN - 1) / RELAX_DYN_SPAN);
   D.n_dyn = 0;
   for (int l = 0; l < d->n_layers; ++l) {
     D.dyn_slots[l] = (R)d->dyn_slots[l];
