@@ -612,6 +612,9 @@ def main():
                     help='static layers walked through source lists: 0 never, 1 always, 2 (default) on quiet days')
     ap.add_argument('--source-permille', type=int, default=None,
                     help='with --source-lists 2: most infectious agents per thousand for which lists are walked')
+    ap.add_argument('--target-bits', type=int, default=None,
+                    help='1 (default): with the bitmap out of LDS, mixed edges read target bits instead of target '
+                         'codes; 0: codes')
     ap.add_argument('--eager-only', action='store_true',
                     help='counter-collection mode: only the eager pass over every day (no graphs, no CPU baseline)')
     ap.add_argument('--copy-probe', type=int, default=0,
@@ -652,6 +655,9 @@ def main():
     if args.edge_lds is not None:
         _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, args.edge_lds), 'tuning')
         tuning['edge_lds'] = args.edge_lds
+    if args.target_bits is not None:
+        _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_TARGET_BITS, args.target_bits), 'tuning')
+        tuning['target_bits'] = args.target_bits
     if 'n_sims' in w:
         return run_batch(w, args)
     if w.get('popgen'):

@@ -43,6 +43,7 @@ TUNE_EDGE_PREFETCH = 7
 TUNE_SOURCE_LISTS = 8
 TUNE_COUNT_SOURCES = 9
 TUNE_SOURCE_PERMILLE = 10
+TUNE_TARGET_BITS = 11
 
 LAYER_SEED = 14
 LAYER_IMPORT = 15
@@ -124,13 +125,13 @@ class WorkS(C.Structure):
                 ('results', C.c_void_p), ('results_f', C.c_void_p), ('log', C.c_void_p),
                 ('log_capacity', C.c_int64), ('select_hist', C.c_void_p), ('coarse', C.c_void_p),
                 ('rec', C.c_void_p), ('partials', C.c_void_p), ('partials_f', C.c_void_p), ('bound', C.c_void_p),
-                ('npts', C.c_int32), ('pad', C.c_int32), ('day_clock', C.c_void_p)]
+                ('npts', C.c_int32), ('pad', C.c_int32), ('day_clock', C.c_void_p), ('aux', C.c_void_p)]
 
 
 MAX_REDUCE_BLOCKS = 65536
 PARTIAL_COLS = 64
 WORK_PTR_FIELDS = ['winner', 'touched', 'bitmap', 'ctl', 'results', 'results_f', 'log', 'select_hist', 'coarse',
-                   'rec', 'partials', 'partials_f', 'bound', 'day_clock']
+                   'rec', 'partials', 'partials_f', 'bound', 'day_clock', 'aux']
 
 
 def make_workspace(n, npts, log_capacity, device):
@@ -152,6 +153,7 @@ def make_workspace(n, npts, log_capacity, device):
         partials_f=torch.zeros((MAX_REDUCE_BLOCKS, RF_COLS), dtype=torch.float64, device=device),
         bound=torch.zeros(n, dtype=torch.int16, device=device),
         day_clock=torch.zeros(3 * npts + 1, dtype=torch.int64, device=device),
+        aux=torch.zeros(words, dtype=torch.int32, device=device),
     )
     s = WorkS()
     for k in WORK_PTR_FIELDS:
@@ -164,7 +166,7 @@ def make_workspace(n, npts, log_capacity, device):
 def reset_workspace(w):
     """Clear a workspace for a fresh run (device-side fills, no host copies)."""
     w['winner'].fill_(-1)
-    for k in ('touched', 'bitmap', 'coarse', 'rec', 'ctl', 'results', 'results_f', 'select_hist'):
+    for k in ('touched', 'bitmap', 'coarse', 'rec', 'ctl', 'results', 'results_f', 'select_hist', 'aux'):
         w[k].zero_()
 
 

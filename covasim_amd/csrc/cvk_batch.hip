@@ -102,6 +102,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void batch_day_begin_kernel(BatchCommo
   S.coarse = nullptr;  // the batched edge pass stages sims' whole bitmaps
   S.rec = C.rec + B.a_off;
   S.bound = C.bound + B.a_off;
+  S.aux = nullptr;
   S.partials = C.partials;
   S.partials_f = C.partials_f;
   S.part_stride = C.part_stride;
@@ -115,10 +116,10 @@ struct BatchReduce {
 
 // Sums sim s's partial rows and writes its day rows (block-wide).
 __device__ void batch_reduce_sim(const BatchCommon& C, const BatchReduce& r, int32_t s) {
-  __shared__ uint32_t s_tot[N_AGENT_COLS];
+  __shared__ uint32_t s_tot[N_SUM_COLS];
   __shared__ double s_totf[N_AGENT_FCOLS];
   const BatchSim& B = C.sims[s];
-  sum_partials<N_AGENT_COLS, N_AGENT_FCOLS>(C.partials, C.partials_f, C.part_stride, B.b_off, B.n_blocks, s_tot,
+  sum_partials<N_SUM_COLS, N_AGENT_FCOLS>(C.partials, C.partials_f, C.part_stride, B.b_off, B.n_blocks, s_tot,
                                             s_totf);
   write_agent_rows(C.results + (int64_t)s * C.npts * CVK_R_COLS, C.results_f + (int64_t)s * C.npts * CVK_RF_COLS,
                    s_tot, s_totf, r.t, r.prologue, r.transitions);

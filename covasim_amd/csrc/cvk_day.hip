@@ -83,6 +83,7 @@ __global__ __launch_bounds__(AGENT_BLOCK) void day_begin_kernel(cvk_people P, cv
   S.coarse = W.coarse;  // one word per block: AGENT_BLOCK * AGENTS_PER_THREAD == 512
   S.rec = W.rec;
   S.bound = W.bound;
+  S.aux = (uint64_t*)W.aux;  // null unless the day's edge pass reads target bits
   S.partials = W.partials;
   S.partials_f = W.partials_f;
   S.part_stride = CVK_MAX_REDUCE_BLOCKS;
@@ -106,11 +107,12 @@ struct DayReduce {
   int32_t infect_t;
 };
 
+static_assert(N_INFECT_COLS <= N_SUM_COLS, "day_reduce sums the infect columns into s_tot");
 __device__ void day_reduce(const cvk_work& W, const DayReduce& r) {
-  __shared__ uint32_t s_tot[N_AGENT_COLS];
+  __shared__ uint32_t s_tot[N_SUM_COLS];
   __shared__ double s_totf[N_AGENT_FCOLS];
   if (r.n_blocks) {
-    sum_partials<N_AGENT_COLS, N_AGENT_FCOLS>(W.partials, W.partials_f, CVK_MAX_REDUCE_BLOCKS, 0, r.n_blocks, s_tot,
+    sum_partials<N_SUM_COLS, N_AGENT_FCOLS>(W.partials, W.partials_f, CVK_MAX_REDUCE_BLOCKS, 0, r.n_blocks, s_tot,
                                               s_totf);
     write_agent_rows(W.results, W.results_f, s_tot, s_totf, r.t, r.prologue, r.transitions);
   }
@@ -229,6 +231,20 @@ __device__ inline uint32_t walk_decision(const cvk_work& W, uint32_t mask, int32
   return (uint64_t)s_inf * 1000u > (uint64_t)max_permille * n_agents ? 0u : mask;
 }
 
+// The day's largest target code: the maximum of the agent pass's DC_TMAX column over its agent_blocks blocks
+// (block-wide; s_max zeroed before a barrier that precedes this call, read after the next one).
+template <int BLOCK>
+__device__ inline void day_target_max(const cvk_work& W, uint32_t agent_blocks, uint32_t* s_max) {
+  const uint32_t* col = W.partials + (int64_t)DC_TMAX * CVK_MAX_REDUCE_BLOCKS;
+  uint32_t m = 0;
+  for (uint32_t b0 = threadIdx.x; b0 < agent_blocks; b0 += 4 * BLOCK) {  // 4 loads in flight
+#pragma unroll
+    for (int u = 0; u < 4; ++u) m = max(m, b0 + u * BLOCK < agent_blocks ? col[b0 + u * BLOCK] : 0u);
+  }
+  m = wave_max(m);
+  if (lane_id() == 0 && m) atomicMax(s_max, m);
+}
+
 struct EdgeArgs {
   SimView sim;
   SegView seg[CVK_MAX_LAYERS];
@@ -247,7 +263,8 @@ struct EdgeArgs {
   int32_t n_layers;
   int32_t lds_words;  // words of the LDS map (fine bitmap or coarse map, a multiple of 4)
   int32_t t;
-  int32_t pad;
+  int32_t target_bits;  // 1: mixed edges read the targets' aux bits, bounded by the day's largest target code
+                        // (the agent pass of day t has just written both)
   cvk_work work;     // for the fused day reduction
   DayReduce reduce;  // day_begin partial rows to reduce in block 0 (n_blocks == 0: none)
 };
@@ -260,12 +277,15 @@ struct EdgeArgs {
 // MODE: BITS_LDS / BITS_COARSE / BITS_L2 (the LDS map is a.lds_words words of the fine bitmap or of
 // the coarse map). GEN: some layer's endpoints are generated (CVK_TUNE_GENERATE_DYNAMIC); the other
 // instantiations do not carry that code. PF: the next tile's endpoint loads are issued before this
-// tile's candidates are processed.
-template <int MODE, int K, bool GEN, bool PF>
+// tile's candidates are processed. TB: target bits (a.target_bits; instantiated only where the bitmap is
+// not in LDS).
+template <int MODE, int K, bool GEN, bool PF, bool TB>
 __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   extern __shared__ uint32_t s_mem[];
   __shared__ uint32_t s_next;
   __shared__ float s_beta[MAXV];
+  __shared__ uint32_t s_tmax;  // target bits: the day's largest target code
+  if (threadIdx.x == 0) s_tmax = 0u;
   // the launch's span on the device wall clock (work.day_clock, optional)
   unsigned long long* span = a.work.day_clock ? (unsigned long long*)a.work.day_clock + a.work.npts + 1 + 2 * a.t : nullptr;
   if (span && threadIdx.x == 0) atomicMin(span, (unsigned long long)wall_clock64());
@@ -285,7 +305,13 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
                                                 a.sim.n_agents) != 0;
   if (a.count_src && a.src_mask && !walked && blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd((unsigned long long*)(a.work.ctl + CVK_CTL_SRC_STREAMED), 1ull);
+  constexpr bool tbits = TB && MODE != BITS_LDS;
+  if (tbits) {  // uniform over the grid
+    __syncthreads();  // s_tmax zeroed
+    day_target_max<EDGE_BLOCK>(a.work, a.agent_blocks, &s_tmax);
+  }
   __syncthreads();
+  const uint32_t tmax = s_tmax;  // 0 with target bits: no agent can be infected today
   const uint32_t* bits = s_mem;
   const int64_t n_wt = a.tile_start[a.n_layers];
   const uint32_t skip = reducing && gridDim.x > 1 ? 1u : 0u;
@@ -342,12 +368,13 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
       q_seg = l;
     }
     TileBounds<K> B;
-    tile_bits_and_bounds<MODE, K>(a.sim, bits, cur, B);
+    tile_bits_and_bounds<MODE, K, tbits>(a.sim, bits, cur, B);
     // the next tile's endpoints are issued after this tile's bound loads (loads complete in order, so
     // waiting for the bounds does not wait for them) and arrive while the candidates are processed
     grab(nxt);
     if (PF && nxt.seg >= 0) load_wave_edges<K, GEN>(a.sim, a.seg[nxt.seg], a.t, nxt);
-    tile_candidates<K>(a.sim, a.seg[l], a.t, Q, cur, B, n_x);
+    if (!tbits) tile_candidates<K>(a.sim, a.seg[l], a.t, Q, cur, B, n_x);
+    else if (tmax) tile_candidates<K, true>(a.sim, a.seg[l], a.t, Q, cur, B, n_x, tmax);
     if (!PF && nxt.seg >= 0) load_wave_edges<K, GEN>(a.sim, a.seg[nxt.seg], a.t, nxt);
     cur = nxt;
   }
@@ -358,26 +385,32 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   }
 }
 
-// Launchers of every instantiation, indexed [mode][K == 4][GEN][PF].
-template <int MODE, int K, bool GEN, bool PF>
+// Launchers of every instantiation, indexed [mode][K == 4][GEN][PF][TB] (TB only where the bitmap is not in
+// LDS: the LDS rows hold the TB = false kernel twice).
+template <int MODE, int K, bool GEN, bool PF, bool TB>
 static void edge_launch(dim3 g, dim3 b, size_t shmem, hipStream_t st, const EdgeArgs& a) {
-  hipLaunchKernelGGL((edge_pass_kernel<MODE, K, GEN, PF>), g, b, shmem, st, a);
+  hipLaunchKernelGGL((edge_pass_kernel<MODE, K, GEN, PF, TB>), g, b, shmem, st, a);
 }
 typedef void (*EdgeLaunchFn)(dim3, dim3, size_t, hipStream_t, const EdgeArgs&);
+#define CVK_EDGE_T(M, K, G, P) {edge_launch<M, K, G, P, false>, edge_launch<M, K, G, P, (M != BITS_LDS)>}
 #define CVK_EDGE_K(M, K) \
-  {{edge_launch<M, K, false, false>, edge_launch<M, K, false, true>}, \
-   {edge_launch<M, K, true, false>, edge_launch<M, K, true, true>}}
+  {{CVK_EDGE_T(M, K, false, false), CVK_EDGE_T(M, K, false, true)}, \
+   {CVK_EDGE_T(M, K, true, false), CVK_EDGE_T(M, K, true, true)}}
 #define CVK_EDGE_ROW(M) {CVK_EDGE_K(M, 2), CVK_EDGE_K(M, 4)}
-static const EdgeLaunchFn kEdgeLaunch[3][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
-                                                     CVK_EDGE_ROW(BITS_L2)};
+static const EdgeLaunchFn kEdgeLaunch[3][2][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                                        CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_T
 #undef CVK_EDGE_K
 #undef CVK_EDGE_ROW
-#define CVK_EDGE_K(M, K)                                                                                        \
-  {{(const void*)edge_pass_kernel<M, K, false, false>, (const void*)edge_pass_kernel<M, K, false, true>},     \
-   {(const void*)edge_pass_kernel<M, K, true, false>, (const void*)edge_pass_kernel<M, K, true, true>}}
+#define CVK_EDGE_T(M, K, G, P) \
+  {(const void*)edge_pass_kernel<M, K, G, P, false>, (const void*)edge_pass_kernel<M, K, G, P, (M != BITS_LDS)>}
+#define CVK_EDGE_K(M, K) \
+  {{CVK_EDGE_T(M, K, false, false), CVK_EDGE_T(M, K, false, true)}, \
+   {CVK_EDGE_T(M, K, true, false), CVK_EDGE_T(M, K, true, true)}}
 #define CVK_EDGE_ROW(M) {CVK_EDGE_K(M, 2), CVK_EDGE_K(M, 4)}
-static const void* const kEdgeFns[3][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
-                                                 CVK_EDGE_ROW(BITS_L2)};
+static const void* const kEdgeFns[3][2][2][2][2] = {CVK_EDGE_ROW(BITS_LDS), CVK_EDGE_ROW(BITS_COARSE),
+                                                    CVK_EDGE_ROW(BITS_L2)};
+#undef CVK_EDGE_T
 #undef CVK_EDGE_K
 #undef CVK_EDGE_ROW
 
@@ -514,6 +547,8 @@ static int g_count_sources = 0;  // tunable: count the slots and lists the edge 
 static int g_overlap_resample = 1;  // tunable: the fused day loop resamples inside the previous infect pass
 static int g_edge_coarse = 1;       // tunable: stage the coarse map in LDS when the bitmap does not fit
 static int g_edge_prefetch = 0;     // tunable: issue the next wave tile's endpoint loads before the candidates
+static int g_target_bits = 1;       // tunable: the fused day loop's edge pass reads target bits instead of target
+                                    // codes when the bitmap is not staged in LDS (1) or never (0)
 int overlap_resample_enabled() { return g_overlap_resample; }
 int source_lists_mode() { return g_source_lists; }
 int source_permille() { return g_source_permille; }
@@ -618,6 +653,8 @@ static int fill_resample(DayBeginArgs& a, const cvk_layer* layers, const cvk_par
   return CVK_OK;
 }
 
+static bool target_bits_for(int64_t n, const cvk_work* work);
+
 static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, const cvk_params* params,
                             cvk_work* work, int32_t t, int prologue, int transitions, int resample, hipStream_t st,
                             DayReduce* reduce_out) {
@@ -631,8 +668,10 @@ static int day_begin_launch(const cvk_people* people, const cvk_layer* layers, c
   uint32_t rs_blocks = 0;
   int rc = fill_resample(a, layers, params, resample, &rs_blocks);
   if (rc) return rc;
+  cvk_work w = *work;
+  if (!(transitions && target_bits_for(people->n, work))) w.aux = nullptr;  // the target bits are for the edge pass
   hipLaunchKernelGGL(day_begin_kernel, dim3(a.agent_blocks + rs_blocks), dim3(AGENT_BLOCK), 0, st, *people, *params,
-                     *work, a);
+                     w, a);
   reduce_out->n_blocks = a.agent_blocks;
   reduce_out->t = t;
   reduce_out->prologue = prologue;
@@ -680,6 +719,9 @@ extern "C" int cvk_set_tuning(int32_t key, int32_t value) {
       return CVK_OK;
     case CVK_TUNE_COUNT_SOURCES:
       g_count_sources = value ? 1 : 0;
+      return CVK_OK;
+    case CVK_TUNE_TARGET_BITS:
+      g_target_bits = value ? 1 : 0;
       return CVK_OK;
     default:
       return fail(CVK_E_INVALID, "cvk_set_tuning: unknown key");
@@ -839,9 +881,9 @@ static int edge_dynamic_lds(int64_t* budget) {
   static thread_local int configured_device = -1;
   static thread_local int64_t value = 0;
   if (configured_device != c.device) {
-    const void* const* fns = &kEdgeFns[0][0][0][0];
+    const void* const* fns = &kEdgeFns[0][0][0][0][0];
     int64_t b_min = -1;
-    for (int j = 0; j < 24; ++j) {
+    for (int j = 0; j < 48; ++j) {
       hipFuncAttributes attr;
       if (hipFuncGetAttributes(&attr, fns[j]) != hipSuccess)
         return fail(CVK_E_HIP, "cannot query the edge pass attributes");
@@ -856,6 +898,19 @@ static int edge_dynamic_lds(int64_t* budget) {
   *budget = value;
   return CVK_OK;
 }
+
+// Whether the fused day loop's edge pass over n agents reads target bits (CVK_TUNE_TARGET_BITS), and so
+// whether its agent pass writes them: with cvk_work.aux, when the infectious bitmap is not staged in LDS
+// (the same rule as edge_pass_impl's choice of the bitmap's place; with it in LDS the codes fit in L2).
+namespace cvk {
+static bool target_bits_for(int64_t n, const cvk_work* work) {
+  if (!work->aux || !g_target_bits) return false;
+  int64_t budget = 0;
+  if (edge_dynamic_lds(&budget)) return false;
+  const int64_t queue_bytes = (int64_t)EDGE_WAVES * EDGE_WAVE_WORDS * 4;
+  return !(g_edge_lds && bitmap_words_for(n) * 4 + queue_bytes <= budget);
+}
+}  // namespace cvk
 
 // generate: dynamic layers' endpoints of day t are computed from the resample stream (fused day loop,
 // which writes them to the layer only when it closes its last day).
@@ -885,6 +940,7 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   S.vl_lo = params->vl_lo;
   for (int v = 0; v < MAXV; ++v) S.beta_v[v] = params->beta_v[v];
   S.coarse = work->coarse;
+  S.aux = work->aux;
   a.t = t;
 
   DeviceCaps& c = caps();
@@ -946,6 +1002,10 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   a.listed_tiles = 0;
   for (int q = 0; q < p; ++q)
     if ((a.src_mask >> a.pos_layer[q]) & 1u) a.listed_tiles = a.tile_start[q + 1];
+  // the fused day loop (reduce given) runs the agent pass of day t right before, which then writes the target
+  // bits and the largest target code (day_begin_launch makes the same choice); a standalone edge pass may
+  // follow anything
+  a.target_bits = reduce && reduce->n_blocks && reduce->transitions && target_bits_for(people->n, work) ? 1 : 0;
   a.src_auto = g_source_lists == 2 ? 1 : 0;
   a.src_max_permille = g_source_permille;
   a.agent_blocks = (uint32_t)agent_blocks_for(people->n);
@@ -956,7 +1016,8 @@ static int edge_pass_impl(const cvk_people* people, const cvk_layer* layers, con
   const size_t shmem = (size_t)((int64_t)a.lds_words * 4 + queue_bytes);
   bool any_gen = false;
   for (int l = 0; l < params->n_layers; ++l) any_gen = any_gen || a.seg[l].gen;
-  kEdgeLaunch[mode][g_edge_chunks == 4 ? 1 : 0][any_gen ? 1 : 0][g_edge_prefetch ? 1 : 0](dim3(grid), dim3(EDGE_BLOCK),
+  kEdgeLaunch[mode][g_edge_chunks == 4 ? 1 : 0][any_gen ? 1 : 0][g_edge_prefetch ? 1 : 0][a.target_bits ? 1 : 0](
+      dim3(grid), dim3(EDGE_BLOCK),
                                                                                        shmem, st, a);
   return check_launch("cvk_edge_pass");
 }

@@ -116,7 +116,10 @@ constexpr int SC_ALIVE = 13, SC_EXPV = 14, SC_INFV = 14 + MAXV, N_STOCK_COLS = 1
 constexpr int DC_BASE = N_STOCK_COLS, N_DAY_COLS = 9 + MAXV;
 // column of the agents whose infectious bit this pass set (the edge pass's choice between walking
 // source lists and streaming reads it)
-constexpr int DC_BITS = N_STOCK_COLS + N_DAY_COLS, N_AGENT_COLS = DC_BITS + 1;
+constexpr int DC_BITS = N_STOCK_COLS + N_DAY_COLS, N_SUM_COLS = DC_BITS + 1;
+// column of the block's largest target code (not summed: the edge pass takes the maximum over the blocks),
+// written with the target bits (AgentSpan.aux)
+constexpr int DC_TMAX = N_SUM_COLS, N_AGENT_COLS = DC_TMAX + 1;
 constexpr int N_AGENT_FCOLS = 3;
 static_assert(N_AGENT_COLS <= CVK_PARTIAL_COLS, "partial row too small");
 
@@ -151,6 +154,19 @@ __device__ inline uint16_t edge_bound(uint32_t f, float rt, float rs, const cvk_
     bc = bound_code((f & CVK_QUARANTINED) ? rs * quar_max : rs);
   }
   return (uint16_t)bc;
+}
+
+// The largest code a susceptible agent's edge_bound can take before the edge pass: quarantine at
+// max(1, its largest factor), whether or not the agent is quarantined (testing and tracing change that
+// flag, and rewrite the code, after the agent pass). 0 for other agents.
+__device__ inline uint32_t target_bound_any(uint32_t f, float rs, float quar_any) {
+  return ((f & CVK_SUSCEPTIBLE) && !(f & CVK_INFECTIOUS) && rs != 0.0f) ? bound_code(rs * quar_any) : 0u;
+}
+
+__device__ inline uint32_t wave_max(uint32_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (uint32_t)__shfl_xor(v, off, 64));
+  return v;
 }
 
 // Per-agent inputs, loaded in two round trips: the state words (and the NAb fields), then only the
@@ -217,6 +233,8 @@ struct AgentSpan {
                        // written (needs AGENT_BLOCK * APT == 512 and li0 a multiple of 512)
   uint64_t* rec;       // the sim's edge records (local indexing)
   uint16_t* bound;     // the sim's upper-bound codes (local indexing)
+  uint64_t* aux;       // the sim's target bits (susceptible with rel_sus != 0, not infectious) and, in the
+                       // partial column DC_TMAX, the block's largest target code; null: neither is written
   uint32_t* partials;  // column-major partial rows; this block writes entry `part_col` of each column
   double* partials_f;
   int64_t part_stride;
@@ -248,6 +266,8 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
     quar_max = fmaxf(quar_max, prm.quar_factor[l]);
     iso_max = fmaxf(iso_max, prm.iso_factor[l]);
   }
+  const float quar_any = fmaxf(1.0f, quar_max);
+  uint32_t tmax = 0;  // this lane's largest target code (target bits only)
 
   int64_t idx[APT];
   AgentIn in[APT];
@@ -270,6 +290,7 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
     const int64_t g = S.g0 + li;
     const bool valid = li < S.n_local;
     bool inf_bit = false;
+    bool tgt_bit = false;
     uint32_t f_close = 0, vs_close = 0;  // state counted as the previous day's stocks
     uint32_t ev = 0;                     // this day's transition events (bits as the day columns)
     uint32_t ev_var = CVK_VAR_NONE;      // variant of a new infectious agent
@@ -368,6 +389,11 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
         inf_bit = (f & CVK_INFECTIOUS) && x.rt != 0.0f;
         S.rec[li] = edge_record(f, x.rt, x.rs);
         S.bound[li] = edge_bound(f, x.rt, x.rs, prm, quar_max, iso_max);
+        if (S.aux) {  // uniform
+          const uint32_t tb = target_bound_any(f, x.rs, quar_any);
+          tgt_bit = tb != 0u;
+          tmax = max(tmax, tb);
+        }
         if (f & CVK_SEVERE) ev |= 1u << 7;
         if (f & CVK_CRITICAL) ev |= 1u << 8;
       }
@@ -399,12 +425,18 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
       c[DC_BITS] += (uint32_t)__popcll(ib);
       const int64_t word = li >> 6;
       if (lane_id() == 0 && word < n_words) S.bitmap[word] = ib;
+      if (S.aux) {  // uniform
+        const uint64_t tw = __ballot(tgt_bit);
+        if (lane_id() == 0 && word < n_words) S.aux[word] = tw;
+      }
       // coarse bits of this ballot's four 16-agent groups, at their place in the block's 512 agents
       const uint32_t nib = (uint32_t)((ib & 0xFFFFull) != 0) | ((uint32_t)(((ib >> 16) & 0xFFFFull) != 0) << 1) |
                            ((uint32_t)(((ib >> 32) & 0xFFFFull) != 0) << 2) | ((uint32_t)((ib >> 48) != 0) << 3);
       coarse_word |= nib << (uint32_t)((((li - S.li0) & ~63ll) >> 4) & 31);
     }
   }
+
+  c[DC_TMAX] = (S.aux && transitions) ? wave_max(tmax) : 0u;  // uniform
 
   // block partial row: per-wave counts through LDS (no atomics), float sums in wave order
   __shared__ uint32_t s_wc[AGENT_BLOCK / 64][N_AGENT_COLS];
@@ -424,11 +456,18 @@ __device__ void agent_block(const cvk_people& P, const cvk_params& prm, const Ag
     }
   }
   __syncthreads();
-  if (threadIdx.x < N_AGENT_COLS) {
+  if (threadIdx.x < N_SUM_COLS) {
     uint32_t tot = 0;
 #pragma unroll
     for (int j = 0; j < AGENT_BLOCK / 64; ++j) tot += s_wc[j][threadIdx.x];
     S.partials[(size_t)threadIdx.x * S.part_stride + S.part_col] = tot;
+  } else if (threadIdx.x == DC_TMAX) {
+    if (S.aux) {
+      uint32_t m = 0;
+#pragma unroll
+      for (int j = 0; j < AGENT_BLOCK / 64; ++j) m = max(m, s_wc[j][DC_TMAX]);
+      S.partials[(size_t)DC_TMAX * S.part_stride + S.part_col] = m;
+    }
   } else if (threadIdx.x >= 64 && threadIdx.x < 64 + N_AGENT_FCOLS) {
     const int k = threadIdx.x - 64;
     double tot = 0.0;
@@ -556,6 +595,7 @@ struct SimView {
   int64_t sus_stride;
   const uint32_t* bitmap;
   const uint32_t* coarse;   // one bit per 16 agents (single-sim day loop; see AgentSpan), or null
+  const uint32_t* aux;      // target bits (see AgentSpan), or null
   uint32_t* touched;        // bitmap over storage indices (a_off + local): targets holding a key
   int64_t a_off;
   uint64_t seed;
@@ -848,7 +888,8 @@ struct TileBounds {
 
 // Phase 1 of a wave tile: infectious bits of every endpoint, then the bound loads of the mixed edges
 // (issued, not waited for). lds_bits: the LDS map of MODE (fine or coarse; unused for BITS_L2).
-template <int MODE, int K>
+// TB: the targets' words of the aux bits (bt) instead of their codes (tile_candidates<K, true>).
+template <int MODE, int K, bool TB = false>
 __device__ inline void tile_bits_and_bounds(const SimView& S, const uint32_t* lds_bits, const WaveTile<K>& T,
                                             TileBounds<K>& B) {
   const CVK_GLOBAL uint16_t* bound = gmem(S.bound);
@@ -901,16 +942,19 @@ __device__ inline void tile_bits_and_bounds(const SimView& S, const uint32_t* ld
       const uint32_t xi = (uint32_t)T.x[j >> 2][j & 3], yi = (uint32_t)T.y[j >> 2][j & 3];
       const bool sx = (from_x >> j) & 1u;
       B.bs[j] = bound[sx ? xi : yi];
-      B.bt[j] = bound[sx ? yi : xi];
+      B.bt[j] = TB ? gmem(S.aux)[(sx ? yi : xi) >> 5] : (uint32_t)bound[sx ? yi : xi];
     }
   }
 }
 
 // Phase 2 of a wave tile: candidates onto the LDS stack, rounds of draws and prefilter tests, exact
 // tests queued. n_x: the wave's exact-queue fill, whose slots all belong to segment L.
-template <int K>
+// TB: B.bt holds the targets' aux words; a target is a candidate if its bit is set, and its bound is tmax
+// (nonzero), the day's largest target code (at or above the target's own code, so p_ub stays >= p).
+template <int K, bool TB = false>
 __device__ inline void tile_candidates(const SimView& S, const SegView& L, int32_t t, const WaveQ& Q,
-                                       const WaveTile<K>& T, const TileBounds<K>& B, uint32_t& n_x) {
+                                       const WaveTile<K>& T, const TileBounds<K>& B, uint32_t& n_x,
+                                       uint32_t tmax = 0u) {
   const uint32_t lane = lane_id();
   const uint32_t mixed = B.mixed, from_x = B.from_x;
   const uint32_t* bs = B.bs;
@@ -926,9 +970,17 @@ __device__ inline void tile_candidates(const SimView& S, const SegView& L, int32
       has[q] = false;
       ent[q] = 0;
       const uint32_t vq = bs[j] >> 13;
-      // a non-infectious target's code is its susceptibility bound, or 0 if it cannot be infected
-      if (((mixed >> j) & 1u) && vq < (uint32_t)S.n_variants && bt[j] != 0u) {
-        const uint32_t pc = candidate_pcode(L, Q, bs[j], bt[j], vq);
+      // a non-infectious target's code is its susceptibility bound, or 0 if it cannot be infected; with
+      // target bits, its bit says whether it can be, and tmax bounds it
+      bool can;
+      if (TB) {
+        const uint32_t tgt = (uint32_t)(((from_x >> j) & 1u) ? T.y[c][q] : T.x[c][q]);
+        can = ((bt[j] >> (tgt & 31u)) & 1u) != 0u;
+      } else {
+        can = bt[j] != 0u;
+      }
+      if (((mixed >> j) & 1u) && vq < (uint32_t)S.n_variants && can) {
+        const uint32_t pc = candidate_pcode(L, Q, bs[j], TB ? tmax : bt[j], vq);
         if (pc) {
           has[q] = true;
           const uint32_t dir = ((from_x >> j) & 1u) ? 0u : 1u;

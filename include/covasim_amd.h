@@ -225,6 +225,9 @@ typedef struct {
                             workgroup start (atomic min) and the latest workgroup end (atomic max) of day
                             t's edge pass; the caller sets them to ~0 and 0 before the run. No reference
                             counterpart (measurement). */
+  uint32_t* aux;         /* optional [bitmap_words] (null: not kept): bit i = agent i is susceptible with rel_sus != 0
+                            and not infectious (it can be a target); written by the agent pass of the fused day
+                            loop when its edge pass reads it (CVK_TUNE_TARGET_BITS). No reference counterpart. */
 } cvk_work;
 
 #define CVK_MAX_REDUCE_BLOCKS 65536
@@ -249,7 +252,7 @@ typedef struct {
 const char* cvk_last_error(void);
 /* ABI version: changes whenever a public struct, an entry point's signature or a tunable's meaning
  * changes. Bindings compare it with the value they were written for. */
-#define CVK_ABI_VERSION 8
+#define CVK_ABI_VERSION 9
 int cvk_version(void);
 /* No reference counterpart: the frequency (Hz) of the device wall clock of cvk_work.day_clock. */
 int cvk_wall_clock_rate(int64_t* hz_out);
@@ -288,6 +291,12 @@ int cvk_abi_layout(int64_t* out, int32_t n);
 /* CVK_TUNE_EDGE_PREFETCH: 1 = each edge-pass wave issues its next tile's endpoint loads after this
  * tile's bound loads, before its candidates are processed; 0 (default) = after. */
 #define CVK_TUNE_EDGE_PREFETCH 7
+/* CVK_TUNE_TARGET_BITS: inside cvk_run_days / graphs, with cvk_work.aux given and the infectious bitmap not
+ * staged in LDS (large populations, whose codes miss L2), 1 (default) = a mixed edge of a wave tile reads its
+ * target's aux bit (L2-resident, N/8 bytes) instead of the target's bound code, and its prefilter bound takes
+ * the day's largest target code (the agent pass's maximum) with the source's own code; 0 = both endpoints'
+ * codes are gathered. The results are identical. */
+#define CVK_TUNE_TARGET_BITS 11
 int cvk_set_tuning(int32_t key, int32_t value);
 /* No reference counterpart: compute units and LDS bytes per workgroup of the current device. */
 int cvk_device_info(int32_t* n_cu, int32_t* max_lds_per_block);

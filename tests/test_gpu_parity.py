@@ -231,6 +231,27 @@ CONFIGS = {
                                                                dict(label='v2', rel_beta=1.3, days=[15],
                                                                     n_imports=30)]),
                                            tuning=dict(source_lists=0, edge_lds=0)),
+    # target bits (the bitmap not in LDS: each mixed edge reads its target's bit instead of its code, bounded by
+    # the day's largest target code): three variants; dense exact tests; walk switching; bitmap in L2; and the
+    # codes path in coarse mode with the bits off
+    'target_bits_three_variants': dict(pop=dict(n=4000, pop_type='hybrid', households='categorical'),
+                                       pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1),
+                                                 variants=[dict(label='v1', rel_beta=1.7, days=[8], n_imports=30),
+                                                           dict(label='v2', rel_beta=1.3, days=[15],
+                                                                n_imports=30)]),
+                                       tuning=dict(edge_lds=0, source_lists=0)),
+    'target_bits_dense': dict(pop=dict(n=3000, pop_type='random'), pars=dict(use_waning=False, beta=0.9),
+                              tuning=dict(edge_lds=0)),
+    'target_bits_walk_switching': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                       pars=dict(use_waning=True, beta=0.03, dynam_layer=dict(c=1)),
+                                       tuning=dict(edge_lds=0, source_lists=2, source_permille=5)),
+    'target_bits_l2_weighted': dict(pop=dict(n=3000, pop_type='hybrid', weighted=True),
+                                    pars=dict(use_waning=False, beta=0.05, dynam_layer=dict(c=1)),
+                                    tuning=dict(edge_lds=0, edge_coarse=0, source_lists=0)),
+    'codes_coarse_variants': dict(pop=dict(n=4000, pop_type='hybrid'),
+                                  pars=dict(use_waning=True, beta=0.02, dynam_layer=dict(c=1),
+                                            variants=[dict(rel_beta=1.7, days=[10], n_imports=30)]),
+                                  tuning=dict(edge_lds=0, source_lists=0, target_bits=0)),
     # two variants without labels are both 'custom': one index, the second's parameters (immunity.py:107-114)
     'same_label_variants': dict(pop=dict(n=4000, pop_type='hybrid'),
                                 pars=dict(use_waning=True, beta=0.02, dynam_layer=dict(c=1),
@@ -255,7 +276,7 @@ def test_full_sim_matches_oracle(gpu, name):
                 generate_dynamic=(_lib.TUNE_GENERATE_DYNAMIC, 2),
                 overlap_resample=(_lib.TUNE_OVERLAP_RESAMPLE, 1), edge_coarse=(_lib.TUNE_EDGE_COARSE, 1),
                 edge_prefetch=(_lib.TUNE_EDGE_PREFETCH, 0), source_lists=(_lib.TUNE_SOURCE_LISTS, 2),
-                source_permille=(_lib.TUNE_SOURCE_PERMILLE, 30))
+                source_permille=(_lib.TUNE_SOURCE_PERMILLE, 30), target_bits=(_lib.TUNE_TARGET_BITS, 1))
     for k, v in cfg.get('tuning', {}).items():
         _lib.check(_lib.lib().cvk_set_tuning(keys[k][0], v), 'tuning')
     try:

@@ -342,6 +342,14 @@ CASES = {
                        m.test_prob(symp_prob=0.05, asymp_prob=0.02, quar_policy='daily', sensitivity=0.8)],
         pars=dict(n_days=45, beta=0.035, use_waning=True),
         variants=[dict(rel_beta=1.7, days=[12], n_imports=30)]),
+    # quarantine and isolation that raise transmission (factors above 1): the codes testing and tracing
+    # rewrite exceed the ones the agent pass wrote
+    'factors_above_one': dict(
+        ivs=lambda m: [m.test_prob(symp_prob=0.3, asymp_prob=0.02, test_delay=1),
+                       m.contact_tracing(trace_probs=dict(h=1.0, s=0.6, w=0.6, c=0.2), trace_time=1)],
+        pars=dict(n_days=45, beta=0.03, use_waning=True, quar_factor=dict(h=1.8, s=1.2, w=1.2, c=1.5),
+                  iso_factor=dict(h=2.0, s=1.1, w=1.1, c=1.3)),
+        variants=[dict(rel_beta=1.5, days=[8], n_imports=40)]),
 }
 
 
@@ -385,6 +393,32 @@ def test_testing_tracing_matches_oracle(case, mode):
     compare(sim, osim, ores)
     for key in ('test_yield', 'rel_test_yield'):
         assert np.all(np.isfinite(sim.results[key].values)), key
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('case', ['factors_above_one', 'policies', 'options'])
+def test_testing_tracing_target_bits_matches_oracle(case):
+    """The fused day loop with the bitmap out of LDS, so the edge pass reads target bits bounded by the day's
+    largest target code: that bound must cover the codes testing and tracing rewrite after the agent pass
+    (quarantine / isolation factors up to 2)."""
+    import covasim_amd as cv
+    from covasim_amd import _lib
+    print(f'testing/tracing target bits case={case}', flush=True)
+    spec = CASES[case]
+    pop = make_population(n=20000, pop_type='hybrid')
+    kw = dict(pop_type='hybrid', dynam_layer=dict(c=1), **spec['pars'])
+    variants = spec.get('variants', [])
+    gvars = [cv.variant(dict(rel_beta=v['rel_beta']), days=v['days'], n_imports=v['n_imports']) for v in variants]
+    sim = cv.Sim(pop_size=20000, rand_seed=7, popdict=copy_pop(pop), interventions=spec['ivs'](cv),
+                 variants=gvars, **kw)
+    _lib.check(_lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, 0), 'tuning')
+    try:
+        sim.run(use_graph=True)
+    finally:
+        _lib.lib().cvk_set_tuning(_lib.TUNE_EDGE_LDS, 1)
+    osim, ores = run_oracle(pop, spec['ivs'](oiv), seed=7, variants=variants, **kw)
+    assert ores['new_diagnoses'].sum() > 0 and ores['new_quarantined'].sum() > 0
+    compare(sim, osim, ores)
 
 
 @pytest.mark.gpu
