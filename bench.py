@@ -616,7 +616,8 @@ def main():
                     help='1 (default): with the bitmap out of LDS, mixed edges read target bits instead of target '
                          'codes; 0: codes')
     ap.add_argument('--eager-only', action='store_true',
-                    help='counter-collection mode: only the eager pass over every day (no graphs, no CPU baseline)')
+                    help='counter-collection mode: the whole run once through the uncaptured day loop (the graph\'s '
+                         'kernels, launch by launch; no CPU baseline)')
     ap.add_argument('--copy-probe', type=int, default=0,
                     help='MiB of one device-to-device copy after the run (a known-size kernel for counter scaling)')
     args = ap.parse_args()
@@ -771,16 +772,15 @@ def run_single(w, args, tuning):
     hz = C.c_int64(0)
     _lib.check(L.cvk_wall_clock_rate(C.byref(hz)), 'wall clock rate')
     tick_s = 1.0 / hz.value
-    if args.eager_only:  # counter collection: only the eager pass (2) below, every launch in the trace
+    if args.eager_only:  # counter collection: the whole run once through the day loop, launch by launch
+        # (cvk_run_days without capture: the same kernels and tunables as the captured graph, every launch in
+        # the trace)
         with torch.cuda.stream(stream):
             reset()
-            edge_ev = []
-            for t in range(npts):
-                eager_day(t, edge_ev)
+            _lib.check(L.cvk_run_days(P, sim._layers, pr, wk, 0, npts, pop_inf, sim._imports, 0, st), 'run days')
             torch.cuda.synchronize()
-        edge_ms = np.array([a.elapsed_time(b) for a, b in edge_ev])
-        line = {'mode': 'eager pass only (counter collection)', 'days': npts, 'edge_avg_launch_us':
-                float(edge_ms.mean()) * 1e3, 'source_hash': f'{L.cvk_source_hash():016x}',
+        line = {'mode': 'the day loop, uncaptured (counter collection)', 'days': npts,
+                'source_hash': f'{L.cvk_source_hash():016x}',
                 'cum_infections': int(counts()[:, _lib.R_COL['new_infections']].sum()) + pop_inf}
         if args.copy_probe:
             nbytes = args.copy_probe << 20
