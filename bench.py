@@ -622,8 +622,7 @@ def main():
                     help='MiB of one device-to-device copy after the run (a known-size kernel for counter scaling)')
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({'error': 'this build runs on one GPU; --gpus above 1 is not supported (the sharded '
-                                   'design is described in DESIGN.md)'}))
+        print(json.dumps({'error': 'this build runs on one GPU; --gpus above 1 is not supported'}))
         sys.exit(2)
 
     import torch
