@@ -271,7 +271,8 @@ struct EdgeArgs {
 
 // On days the source lists are walked (walk_decision), every wave first walks its source items (item i
 // goes to wave i mod the grid's working waves) and the listed layers' tiles are skipped. Then workgroup b
-// takes its share [T*b/G, T*(b+1)/G) of the (other) layers' tiles (G workgroups); its waves take the next
+// takes its share of the (other) layers' tiles (G workgroups: [T*b/G, T*(b+1)/G) with the bitmap in LDS,
+// else every G-th tile from b); its waves take the next
 // tile from an LDS counter, so no wave waits for another and the global memory system sees no tile
 // atomics. A wave's exact-test queue is drained when its work changes layer and at the end.
 // MODE: BITS_LDS / BITS_COARSE / BITS_L2 (the LDS map is a.lds_words words of the fine bitmap or of
@@ -317,11 +318,18 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
   const uint32_t skip = reducing && gridDim.x > 1 ? 1u : 0u;
   const int64_t workers = (int64_t)(gridDim.x - skip);
   const int64_t wb = (int64_t)blockIdx.x - skip;
+  // This workgroup's tiles. With the bitmap in LDS (up to ~1.06M agents): its listed share [l0, l1) (empty on
+  // walked days) and its other share [o0, o1), contiguous. Above that: every workers-th tile from base + wb,
+  // base = 0, or on walked days the first tile after the listed layers' (whose tiles are skipped), so each
+  // workgroup meets every part of every layer (measured: the 10M edge pass 856 -> 842 us; at 1M the
+  // contiguous shares are faster, 40.8 against 42.9 us).
+  constexpr bool STRIDED = MODE != BITS_LDS;
   const int64_t tl = a.listed_tiles, to = n_wt - tl;
-  // this workgroup's listed share [l0, l1) (empty on walked days) and other share [o0, o1)
   const int64_t l0 = wb < 0 || walked ? 0 : tl * wb / workers, l1 = wb < 0 || walked ? 0 : tl * (wb + 1) / workers;
   const int64_t o0 = wb < 0 ? 0 : tl + to * wb / workers, o1 = wb < 0 ? 0 : tl + to * (wb + 1) / workers;
-  const int64_t n_mine = (l1 - l0) + (o1 - o0);
+  const int64_t base = walked ? tl : 0;
+  const int64_t n_mine = !STRIDED ? (l1 - l0) + (o1 - o0)
+                                  : (wb < 0 || n_wt - base <= wb ? 0 : (n_wt - base - wb + workers - 1) / workers);
   uint32_t n_x = 0;
   int32_t q_seg = a.pos_layer[0];
   if (walked && wb >= 0) {  // source items, one wave at a time over the grid's working waves
@@ -351,7 +359,8 @@ __global__ __launch_bounds__(EDGE_BLOCK) void edge_pass_kernel(EdgeArgs a) {
     k = __builtin_amdgcn_readfirstlane(__shfl(k, 0, 64));
     T.seg = -1;
     if ((int64_t)k >= n_mine) return;
-    const int64_t tile = (int64_t)k < l1 - l0 ? l0 + k : o0 + (k - (l1 - l0));
+    const int64_t tile = STRIDED ? base + wb + (int64_t)k * workers
+                                 : ((int64_t)k < l1 - l0 ? l0 + k : o0 + (k - (l1 - l0)));
     int p = 0;
     while (p + 1 < a.n_layers && tile >= a.tile_start[p + 1]) ++p;
     T.seg = a.pos_layer[p];
