@@ -137,3 +137,17 @@ def test_sim_rejects_out_of_path_parameters():
         cv.Sim(pop_scale=10).validate_pars()
     with pytest.raises(NotImplementedError):
         cv.Sim(interventions=[object()]).validate_pars()
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """The bench runs on one GPU: --gpus 2 prints one JSON line with the reason and exits with status 2."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    out = subprocess.run([sys.executable, os.path.join(root, 'bench.py'), '--gpus', '2'], capture_output=True,
+                         text=True, timeout=120)
+    assert out.returncode == 2
+    line = json.loads(out.stdout.strip().splitlines()[-1])
+    assert 'one GPU' in line['error']
